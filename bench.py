@@ -1,0 +1,175 @@
+"""Throughput of the SIMPLE iteration on one MI355X.
+
+Workload: the 301x701 differentially heated tall cavity (tanh beta = 1.2
+mesh, H/W = 2.33, Ra = 1e5, Pr = 0.71, dt = 0.003), all fields resident in
+HBM.  A step is one SIMPLE iteration (one time step with single-pass inner
+loops): the u, v, p', T line sweeps, the correction, the residual and the
+norms.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver pcr|thomas]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+
+HBM_PEAK_GBS = 8000.0
+
+# arrays each kernel must read or write per launch (every array holds
+# (mi+1)(nj+1) FP64 values); T is read by the v sweeps only with buoyancy
+ALG_ARRAYS = {
+    "sweep_u_y": 5,   # u, u_ant, v, p -> u
+    "sweep_u_x": 6,   # u, u_ant, v, p -> u, a_u
+    "sweep_v_y": 6,   # v, v_ant, u, p, T -> v
+    "sweep_v_x": 8,   # v, v_ant, u, p, T, v at pass start -> v, a_v
+    "sweep_p_y": 5,   # u, v, a_u, a_v -> p'
+    "sweep_p_x": 7,   # u, v, a_u, a_v, p' -> p', b_o
+    "correct": 9,     # p', a_u, a_v, u, v, p -> u, v, p
+    "sweep_t_y": 5,   # T, T_ant, u, v -> T
+    "sweep_t_x": 6,   # T, T_ant, u, v, T at pass start -> T
+    "residual_u": 5,  # u, u_ant, v, p -> Res_u
+    "finalize": 0,
+}
+
+
+def build_case():
+    import cases as K
+    return K.nc301x701()
+
+
+def cpu_baseline(case, budget_s: float = 20.0):
+    """Upstream's serial CPU build (oracle/_ref) when present, else the C
+    oracle with OpenMP, on a bounded number of iterations of the same case."""
+    m = case.mesh
+    cells = m.mi * m.nj
+    ref = os.path.join(ROOT, "oracle", "_ref", f"ixchel2d_ref_m{m.mi}_n{m.nj}")
+    if os.path.exists(ref):
+        from ixchel2d_amd import case as C
+        work = tempfile.mkdtemp(prefix="ixh_cpu_")
+        def run(iters):
+            case.params.itermax = iters
+            case.params.paq_itera = 10
+            C.write_reference_case(case, work, postprocesar=False)
+            t = time.perf_counter()
+            subprocess.run([ref], cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+            return time.perf_counter() - t
+        t0 = run(0)
+        t1 = run(10)
+        per = max((t1 - t0) / 10.0, 1e-6)
+        n = int(max(10, min(400, (budget_s / per) // 10 * 10)))
+        tn = run(n)
+        per = (tn - t0) / n
+        return {"value": cells / per / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": 1, "kind": "reference",
+                "sample": f"upstream serial flang build, {n} iterations of the 301x701 case minus a "
+                          f"0-iteration run (setup and file reads)"}
+    from oracle.pyoracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    o = Oracle(case)
+    t = time.perf_counter()
+    o.run(5)
+    per = (time.perf_counter() - t) / 5
+    n = int(max(5, min(500, budget_s / per)))
+    t = time.perf_counter()
+    o.run(n)
+    per = (time.perf_counter() - t) / n
+    return {"value": cells / per / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": threads, "kind": "port",
+            "sample": f"C oracle (OpenMP), {n} iterations of the 301x701 case"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--solver", default="pcr", choices=["pcr", "thomas"])
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
+              "(the cavities of a batch would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ixchel2d_amd.simple import Cavity
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    case = build_case()
+    m = case.mesh
+    cells = m.mi * m.nj
+    npts = (m.mi + 1) * (m.nj + 1)
+    gpu = Cavity(case, solver=args.solver)
+    gpu.run(args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu.run(args.steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / args.steps
+    value = cells * args.steps / dt / 1e6
+
+    # per-kernel durations: HIP events around every launch on the library's
+    # stream, over a second run of the same number of steps
+    prof_steps = min(args.steps, 500)
+    gpu.set_profiling(True)
+    gpu.run(prof_steps)
+    kt = gpu.kernel_times()
+    gpu.set_profiling(False)
+    ri_y = case.params.derived()["Riy"]
+    arrays = dict(ALG_ARRAYS)
+    if ri_y == 0.0:
+        arrays["sweep_v_y"] -= 1
+        arrays["sweep_v_x"] -= 1
+    per_kernel = {}
+    for name, (ms, n) in kt.items():
+        if n == 0:
+            continue
+        avg_us = ms * 1e3 / n
+        b = arrays.get(name, 0) * npts * 8
+        per_kernel[name] = {"avg_us": round(avg_us, 3), "launches": n,
+                            "alg_bytes": b, "GB/s": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches"])
+    dk = per_kernel[dom]
+    achieved = dk["GB/s"]
+    iter_bytes = sum(arrays[k] for k in arrays) * npts * 8
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"],
+                "iteration_alg_bytes": iter_bytes,
+                "iteration_GB/s": round(iter_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                "iteration_frac": round(iter_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    cpu = None if args.no_cpu else cpu_baseline(build_case())
+    out = {
+        "metric": "Mcell-SIMPLE-iters/s @301x701",
+        "value": round(value, 2),
+        "unit": "Mcell-SIMPLE-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "natural-convection tall cavity 301x701 (tanh beta=1.2, H/W=2.33, Ra=1e5, "
+                               "Pr=0.71, dt=0.003), FP64 plain path, one SIMPLE iteration per step",
+                   "mesh": [m.mi, m.nj], "cells": cells, "solver": args.solver, "path": "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": per_kernel,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * ixchel2d.h -- C-ABI of the MI355X (gfx950) SIMPLE-iteration library.
+ *
+ * Drop-in boundary for the time-step loop body of Ixchel2D
+ * (source/main.f90:391-1271).  Upstream has no FFI: its operator API is a set
+ * of Fortran module procedures called per cell (ensambla_*, residuo_u) or per
+ * line (tridiagonal) from loop nests in main.f90, with all state in module
+ * arrays inside an OpenACC data region (main.f90:356-389).  Calls per cell
+ * across an FFI are not workable, so each entry point below replaces one
+ * main.f90 loop nest or one inner-loop pass; the comment on each names what
+ * it replaces.  The Fortran binding (ixchel2d_amd/fortran/mod_ixchel_hip.f90)
+ * and the Python binding (ixchel2d_amd/_lib.py) bind exactly these symbols.
+ *
+ * Conventions
+ *  - Every call returns 0 on success or a negative IXH_ERR_* code; the text
+ *    of the last error on the calling thread is in ixh_last_error().
+ *  - Arrays cross the boundary in upstream's Fortran layout: element (i,j),
+ *    1-based, of an array with leading dimension ld sits at (i-1)+(j-1)*ld.
+ *    Extents: u(mi,nj+1), v(mi+1,nj), T/p/p'/b_o(mi+1,nj+1), a_u(mi,nj+1),
+ *    a_v(mi+1,nj) (source/mod_ec_momento.f90:27-41, mod_ec_energia.f90:23-24).
+ *  - Host pointers in, host pointers out; device residency is owned by the
+ *    state object.  One HIP stream per state; calls on one state are not
+ *    thread-safe, separate states are independent.
+ */
+#ifndef IXCHEL2D_H
+#define IXCHEL2D_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IXH_OK 0
+#define IXH_ERR_ARG (-1)      /* invalid argument or state */
+#define IXH_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define IXH_ERR_NOMEM (-3)    /* device allocation failed */
+#define IXH_ERR_UNSUPP (-4)   /* size or option outside what is built */
+
+/* fields (ixh_upload_field / ixh_download_field) */
+enum {
+  IXH_U = 0, IXH_V = 1, IXH_T = 2, IXH_P = 3,
+  IXH_U_ANT = 4, IXH_V_ANT = 5, IXH_T_ANT = 6,
+  IXH_PCORR = 7, IXH_BO = 8, IXH_AU = 9, IXH_AV = 10, IXH_RESU = 11,
+  IXH_NFIELDS = 12
+};
+/* variables for boundary conditions and sources */
+enum { IXH_VAR_U = 0, IXH_VAR_V = 1, IXH_VAR_T = 2 };
+/* sides: a = west (x=0), b = south, c = east, d = north */
+enum { IXH_SIDE_A = 0, IXH_SIDE_B = 1, IXH_SIDE_C = 2, IXH_SIDE_D = 3 };
+/* boundary node types (tipo_cond_front % tipo_condi) */
+enum { IXH_BC_NONE = 0, IXH_BC_DIRI = 1, IXH_BC_NEUM = 2 };
+/* sweep directions: Y = lines along y (the `_y` assemblies), X = along x */
+enum { IXH_DIR_Y = 0, IXH_DIR_X = 1 };
+/* line solvers */
+enum {
+  IXH_SOLVER_PCR = 0,    /* parallel cyclic reduction in LDS (default) */
+  IXH_SOLVER_THOMAS = 1  /* one lane per line, upstream's Thomas order */
+};
+/* arithmetic of the sweeps */
+enum { IXH_FP64 = 0, IXH_FP32_MIXED = 1 };
+
+typedef struct ixh_state ixh_state;
+
+/* mesh: node coordinates as read by lectura_mallas_escalonadas
+ * (source/mod_malla.f90:62-162): xu(mi), yv(nj), xp(mi+1), yp(nj+1) */
+typedef struct {
+  int32_t mi, nj;
+  const double *xu, *yv, *xp, *yp;
+} ixh_grid;
+
+/* the scalars the loop body reads (source/main.f90:164-244) */
+typedef struct {
+  double dt, rel_pres, rel_vel, rel_ener;
+  double gamma_momen, gamma_energ; /* constant diffusivities (main.f90:221-244) */
+  double ri_x, ri_y;               /* Ri (u equation) and Riy (v equation) */
+  double conv_u, conv_t, conv_p, conv_resi, conv_paso;
+  int32_t iter_ecuaci_max, iter_simple_max;
+  int32_t solver;                  /* IXH_SOLVER_* */
+  int32_t precision;               /* IXH_FP64 | IXH_FP32_MIXED */
+} ixh_params;
+
+/* one record per SIMPLE iteration: sqrt of the squared-difference sums of
+ * the momentum (v only), p' and energy passes, sqrt(sum b_o^2) and the u
+ * residual norm (main.f90:716-725, 884-906, 1128-1136, 1192-1199); pass
+ * counts; the SIMPLE counter after the exit test and whether upstream writes
+ * the line to fort.102 (main.f90:1202-1215) */
+typedef struct {
+  double err_v, err_p, maxbo, err_t, residuo;
+  int32_t passes_mom, passes_p, passes_t, simple_iter, logged, pad_;
+} ixh_hist;
+
+const char *ixh_last_error(void);
+const char *ixh_version(void);
+
+/* Create the device state: geometry factors (mod_malla.f90:167-217), field
+ * buffers, a_u = a_v = 1e40 (main.f90:309-310), everything else 0.
+ * Replaces the `!$acc data` region (main.f90:356-389). */
+int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out);
+int ixh_destroy(ixh_state *s);
+int ixh_set_params(ixh_state *s, const ixh_params *params);
+/* run the state's work on an existing HIP stream (NULL = its own stream) */
+int ixh_set_stream(ixh_state *s, void *hip_stream);
+int ixh_synchronize(ixh_state *s);
+
+/* field transfer, Fortran layout with leading dimension ld (>= the field's
+ * first extent).  Replaces the data-region copy/copyin/update clauses. */
+int ixh_upload_field(ixh_state *s, int32_t field, const double *src, int32_t ld);
+int ixh_download_field(ixh_state *s, int32_t field, double *dst, int32_t ld);
+
+/* Boundary conditions per node of one side, flattened from tipo_cond_front
+ * (source/mod_cond_frontera.f90:25-38) by the lectura_cond_frontera index
+ * rule (:125-149).  type/value have one entry per node along the side: u sides
+ * a/c nj+1, b/d mi; v sides a/c nj, b/d mi+1; T sides a/c nj+1, b/d mi+1.
+ * Nodes 2..n-1 must all be typed. */
+int ixh_set_bc(ixh_state *s, int32_t var, int32_t side, const int32_t *type, const double *value);
+
+/* Source terms S = Sc + Sp*phi of one variable in its own extent (NULL,NULL
+ * clears them).  Immersed bodies enter here as definir_cuerpo's large-source
+ * penalty (source/mod_front_inmersa.f90:54-179). */
+int ixh_set_sources(ixh_state *s, int32_t var, const double *sc, const double *sp);
+
+/* ---- one directional sweep: assembly + boundary rows + line solves +
+ * copy-back.  dir = IXH_DIR_Y replaces main.f90:420-484 (u), 570-640 (v);
+ * dir = IXH_DIR_X replaces 492-556 (u), 647-711 (v).  The Y sweep of v takes
+ * the snapshot of v (main.f90:559-564); the X sweep of v returns the momentum
+ * error sqrt(sum (v-fv)^2) (main.f90:716-725) in *err_v when err_v != NULL. */
+int ixh_momentum_u(ixh_state *s, int32_t dir);
+int ixh_momentum_v(ixh_state *s, int32_t dir, double *err_v);
+/* p' = 0 (main.f90:752-758) */
+int ixh_pcorr_zero(ixh_state *s);
+/* p' sweeps (main.f90:762-879); the X sweep stores b_o and returns the p'
+ * error and maxbo = sqrt(sum b_o^2) (main.f90:884-906) when non-NULL */
+int ixh_pcorr(ixh_state *s, int32_t dir, double *err_p, double *maxbo);
+/* p += 0.6 p'; u, v corrections (main.f90:931-962) */
+int ixh_correct(ixh_state *s);
+/* T sweeps (main.f90:974-1121); the X sweep returns the energy error
+ * (main.f90:1128-1136) when err_t != NULL */
+int ixh_energy(ixh_state *s, int32_t dir, double *err_t);
+/* residuo_u over the interior and its norm (mod_ec_momento.f90:1107-1266,
+ * main.f90:1169-1199) */
+int ixh_residual_u(ixh_state *s, double *residuo);
+/* lagged copies T_ant = T, u_ant = u, v_ant = v (main.f90:1251-1270) */
+int ixh_lag(ixh_state *s);
+
+/* nsteps time steps of main.f90:391-1270 with upstream's inner-loop and
+ * SIMPLE exit rules.  When every loop is single-pass (iter_ecuaci_max < 0 and
+ * iter_simple_max < 0) the steps run from a captured HIP graph with no host
+ * synchronisation; otherwise the errors are read back after every pass.  One
+ * record per SIMPLE iteration goes to hist (up to cap); *nrec receives the
+ * count (may be NULL). */
+int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t cap, int32_t *nrec);
+
+/* nusselt_promedio_y on the current T (source/mod_postproceso.f90:100-215):
+ * integrals over ii = i0..i1 (step 2) on the walls j=1 and j=nj+1 */
+int ixh_nusselt_y(ixh_state *s, int32_t i0, int32_t i1, double *nu0, double *nu1);
+/* mean Nusselt number on the vertical walls (no upstream counterpart; the
+ * definition is in ixh_nusselt_x's implementation and in the oracle) */
+int ixh_nusselt_x(ixh_state *s, double *nu_west, double *nu_east);
+
+/* tridiagonal(a,b,c,r,n) (source/mod_solucionador.f90:25-47) for nsys
+ * independent systems stored one after another (stride n), solved on the
+ * device; b and r are overwritten as upstream does (r = solution). */
+int ixh_tridiagonal(const double *a, double *b, const double *c, double *r, int32_t n,
+                    int32_t nsys, int32_t solver);
+
+/* per-kernel timing of the last ixh_simple_iterations call made with
+ * profiling on (ixh_set_profiling): for each kernel id the total milliseconds
+ * and the launch count, measured with HIP events on the state's stream */
+int ixh_set_profiling(ixh_state *s, int32_t on);
+int ixh_kernel_times(ixh_state *s, double *ms, int32_t *launches, int32_t cap, int32_t *nkernels);
+const char *ixh_kernel_name(int32_t id);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IXCHEL2D_H */

@@ -1,0 +1,1044 @@
+// Host side of the C-ABI (include/ixchel2d.h): device state, buffer
+// rotation, phase launches, the SIMPLE control loop and HIP-graph replay.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ixchel2d.h"
+#include "simple_device.hpp"
+
+using namespace ixh;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(IXH_ERR_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                \
+  } while (0)
+
+#define CHK(expr)           \
+  do {                      \
+    int rc_ = (expr);       \
+    if (rc_ != 0) return rc_; \
+  } while (0)
+
+enum KernelId {
+  K_U_Y = 0, K_U_X, K_V_Y, K_V_X, K_P_Y, K_P_X, K_CORR, K_T_Y, K_T_X, K_RES, K_FIN, K_NUM
+};
+const char *kKernelNames[K_NUM] = {"sweep_u_y", "sweep_u_x", "sweep_v_y", "sweep_v_x", "sweep_p_y",
+                                   "sweep_p_x", "correct",   "sweep_t_y", "sweep_t_x", "residual_u",
+                                   "finalize"};
+
+constexpr int kNbuf = 4;  // rotation buffers per solved field
+
+}  // namespace
+
+struct ixh_state {
+  int mi = 0, nj = 0, P = 0;
+  size_t fsize = 0;  // doubles per field buffer
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  ixh_params prm{};
+  // geometry (one allocation, 1-based arrays)
+  double *d_geo = nullptr;
+  Geo geo{};
+  // solved fields with rotation buffers
+  double *ub[kNbuf]{}, *vb[kNbuf]{}, *tb[kNbuf]{}, *pcb[kNbuf]{};
+  int u_cur = 0, u_ant = 0, v_cur = 0, v_ant = 0, t_cur = 0, t_ant = 0, pc_cur = 0;
+  int v_snap = -1, t_snap = -1, pc_snap = -1;  // -1: snapshot is zero
+  bool pc_zero = true;
+  double *p = nullptr, *bo = nullptr, *au = nullptr, *av = nullptr, *resu = nullptr;
+  double *src_c[3]{}, *src_p[3]{};
+  bool has_src[3]{};
+  int8_t *bct[3][4]{};
+  double *bcv[3][4]{};
+  int bclen[3][4]{};
+  bool bc_set[3][4]{};
+  double *part = nullptr;  // 5 partial arrays
+  int plen = 0;
+  double *d_out = nullptr;   // 8 doubles: last folded norms
+  double *d_hist = nullptr;  // 8 doubles per recorded iteration
+  int hist_cap = 0;
+  int *d_slot = nullptr;
+  // profiling
+  bool profiling = false;
+  double kms[K_NUM]{};
+  int kcount[K_NUM]{};
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // graphs replaying one single-pass time step, keyed by rotation state
+  struct GraphEntry {
+    int key[7];
+    int next[7];
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+};
+
+namespace {
+
+double *pc_of(ixh_state *s, int idx) { return idx < 0 ? nullptr : s->pcb[idx]; }
+
+int pick(int excl0, int excl1, int excl2 = -1) {
+  for (int k = 0; k < kNbuf; ++k)
+    if (k != excl0 && k != excl1 && k != excl2) return k;
+  return -1;
+}
+
+int dmalloc(double **p, size_t n) {
+  hipError_t e = hipMalloc((void **)p, n * sizeof(double) + 256);
+  if (e != hipSuccess) return fail(IXH_ERR_NOMEM, "hipMalloc(%zu doubles): %s", n, hipGetErrorString(e));
+  e = hipMemset(*p, 0, n * sizeof(double) + 256);
+  if (e != hipSuccess) return fail(IXH_ERR_HIP, "hipMemset: %s", hipGetErrorString(e));
+  return 0;
+}
+
+void field_extent(const ixh_state *s, int f, int *m, int *n) {
+  switch (f) {
+    case IXH_U: case IXH_U_ANT: case IXH_AU: case IXH_RESU: *m = s->mi; *n = s->nj + 1; break;
+    case IXH_V: case IXH_V_ANT: case IXH_AV: *m = s->mi + 1; *n = s->nj; break;
+    default: *m = s->mi + 1; *n = s->nj + 1; break;
+  }
+}
+
+int side_len(const ixh_state *s, int var, int side) {
+  const bool ac = side == IXH_SIDE_A || side == IXH_SIDE_C;
+  if (var == IXH_VAR_U) return ac ? s->nj + 1 : s->mi;
+  if (var == IXH_VAR_V) return ac ? s->nj : s->mi + 1;
+  return ac ? s->nj + 1 : s->mi + 1;
+}
+
+int check_params(const ixh_params *p) {
+  if (!p) return fail(IXH_ERR_ARG, "params is NULL");
+  if (!(p->dt > 0) || !(p->rel_pres > 0) || !(p->rel_vel > 0) || !(p->rel_ener > 0))
+    return fail(IXH_ERR_ARG, "dt and relaxation factors must be positive");
+  if (!(p->gamma_momen > 0) || !(p->gamma_energ > 0))
+    return fail(IXH_ERR_ARG, "diffusivities must be positive");
+  if (p->solver != IXH_SOLVER_PCR && p->solver != IXH_SOLVER_THOMAS)
+    return fail(IXH_ERR_ARG, "unknown solver %d", p->solver);
+  if (p->precision != IXH_FP64)
+    return fail(IXH_ERR_UNSUPP, "precision %d is not built in this library", p->precision);
+  return 0;
+}
+
+Phys phys_of(const ixh_state *s) {
+  const ixh_params &p = s->prm;
+  return Phys{p.dt, p.rel_pres, p.rel_vel, p.rel_ener, p.gamma_momen, p.gamma_energ, p.ri_x, p.ri_y};
+}
+
+int ept_for(int n) {
+  const int e = (n + kBlock - 1) / kBlock;
+  if (e <= 1) return 1;
+  if (e <= 2) return 2;
+  if (e <= 3) return 3;
+  if (e <= 4) return 4;
+  if (e <= 6) return 6;
+  return 8;
+}
+
+// ---- kernel launch helpers (with optional event timing)
+struct Timed {
+  ixh_state *s;
+  int id;
+  Timed(ixh_state *s_, int id_) : s(s_), id(id_) {
+    if (s->profiling) (void)hipEventRecord(s->ev0, s->stream);
+  }
+  int done() {
+    if (!s->profiling) return 0;
+    HIPCHK(hipEventRecord(s->ev1, s->stream));
+    HIPCHK(hipEventSynchronize(s->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+    s->kms[id] += ms;
+    s->kcount[id] += 1;
+    return 0;
+  }
+};
+
+template <int EQ, int DIR, int SOLVER>
+int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
+  dim3 grid(A.nlines), block(kBlock);
+  switch (ept_for(n)) {
+    case 1: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 1>), grid, block, 0, s->stream, A); break;
+    case 2: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 2>), grid, block, 0, s->stream, A); break;
+    case 3: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 3>), grid, block, 0, s->stream, A); break;
+    case 4: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 4>), grid, block, 0, s->stream, A); break;
+    case 6: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 6>), grid, block, 0, s->stream, A); break;
+    default: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 8>), grid, block, 0, s->stream, A); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <int EQ, int DIR>
+int launch_sweep(ixh_state *s, const SweepArgs &A, int n) {
+  if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_t<EQ, DIR, SOLVE_THOMAS>(s, A, n);
+  return launch_sweep_t<EQ, DIR, SOLVE_PCR>(s, A, n);
+}
+
+SweepArgs base_args(ixh_state *s) {
+  SweepArgs A;
+  memset(&A, 0, sizeof A);
+  A.mi = s->mi;
+  A.nj = s->nj;
+  A.P = s->P;
+  A.g = s->geo;
+  A.ph = phys_of(s);
+  A.u = s->ub[s->u_cur];
+  A.v = s->vb[s->v_cur];
+  A.t = s->tb[s->t_cur];
+  A.p = s->p;
+  A.uant = s->ub[s->u_ant];
+  A.vant = s->vb[s->v_ant];
+  A.tant = s->tb[s->t_ant];
+  A.pc = s->pcb[s->pc_cur];
+  A.au = s->au;
+  A.av = s->av;
+  return A;
+}
+
+void set_bc(ixh_state *s, SweepArgs &A, int var) {
+  for (int sd = 0; sd < 4; ++sd) {
+    A.bc[sd].type = s->bct[var][sd];
+    A.bc[sd].val = s->bcv[var][sd];
+  }
+  A.sc = s->has_src[var] ? s->src_c[var] : nullptr;
+  A.sp = s->has_src[var] ? s->src_p[var] : nullptr;
+}
+
+int check_bcs(ixh_state *s, int var) {
+  for (int sd = 0; sd < 4; ++sd)
+    if (!s->bc_set[var][sd])
+      return fail(IXH_ERR_ARG, "boundary conditions of variable %d side %d are not set", var, sd);
+  return 0;
+}
+
+double *part_of(ixh_state *s, int k) { return s->part + (size_t)k * s->plen; }
+
+// ---- phases
+int sweep_u(ixh_state *s, int dir) {
+  CHK(check_bcs(s, IXH_VAR_U));
+  SweepArgs A = base_args(s);
+  set_bc(s, A, IXH_VAR_U);
+  const int out = pick(s->u_cur, s->u_ant);
+  A.out = s->ub[out];
+  A.acoef = s->au;
+  Timed tm(s, dir == IXH_DIR_Y ? K_U_Y : K_U_X);
+  if (dir == IXH_DIR_Y) {
+    A.line0 = 2;
+    A.nlines = s->mi - 2;
+    CHK((launch_sweep<EQ_U, DIR_Y>(s, A, s->nj + 1)));
+  } else {
+    A.line0 = 2;
+    A.nlines = s->nj - 1;
+    CHK((launch_sweep<EQ_U, DIR_X>(s, A, s->mi)));
+  }
+  CHK(tm.done());
+  s->u_cur = out;
+  return 0;
+}
+
+int sweep_v(ixh_state *s, int dir) {
+  CHK(check_bcs(s, IXH_VAR_V));
+  SweepArgs A = base_args(s);
+  set_bc(s, A, IXH_VAR_V);
+  A.acoef = s->av;
+  if (dir == IXH_DIR_Y) {
+    s->v_snap = s->v_cur;  // fv = v (main.f90:559-564)
+    const int out = pick(s->v_cur, s->v_ant);
+    A.out = s->vb[out];
+    A.line0 = 2;
+    A.nlines = s->mi - 1;
+    Timed tm(s, K_V_Y);
+    CHK((launch_sweep<EQ_V, DIR_Y>(s, A, s->nj)));
+    CHK(tm.done());
+    s->v_cur = out;
+  } else {
+    const int out = pick(s->v_cur, s->v_ant, s->v_snap);
+    A.out = s->vb[out];
+    A.snap = s->v_snap >= 0 ? s->vb[s->v_snap] : s->vb[s->v_cur];
+    A.part0 = part_of(s, 0);
+    A.line0 = 2;
+    A.nlines = s->nj - 2;
+    Timed tm(s, K_V_X);
+    CHK((launch_sweep<EQ_V, DIR_X>(s, A, s->mi + 1)));
+    CHK(tm.done());
+    s->v_cur = out;
+  }
+  return 0;
+}
+
+int sweep_p(ixh_state *s, int dir) {
+  SweepArgs A = base_args(s);
+  A.in_zero = s->pc_zero ? 1 : 0;
+  if (dir == IXH_DIR_Y) {
+    s->pc_snap = s->pc_zero ? -1 : s->pc_cur;  // fcorr = p' (main.f90:762-767)
+    const int out = pick(s->pc_cur, s->pc_snap);
+    A.out = s->pcb[out];
+    A.line0 = 2;
+    A.nlines = s->mi - 1;
+    Timed tm(s, K_P_Y);
+    CHK((launch_sweep<EQ_P, DIR_Y>(s, A, s->nj + 1)));
+    CHK(tm.done());
+    s->pc_cur = out;
+    s->pc_zero = false;
+  } else {
+    const int out = pick(s->pc_cur, s->pc_snap);
+    A.out = s->pcb[out];
+    A.snap = pc_of(s, s->pc_snap);
+    A.bo = s->bo;
+    A.part0 = part_of(s, 1);
+    A.part1 = part_of(s, 2);
+    A.line0 = 2;
+    A.nlines = s->nj - 1;
+    Timed tm(s, K_P_X);
+    CHK((launch_sweep<EQ_P, DIR_X>(s, A, s->mi + 1)));
+    CHK(tm.done());
+    s->pc_cur = out;
+    s->pc_zero = false;
+  }
+  return 0;
+}
+
+int sweep_t(ixh_state *s, int dir) {
+  CHK(check_bcs(s, IXH_VAR_T));
+  SweepArgs A = base_args(s);
+  set_bc(s, A, IXH_VAR_T);
+  if (dir == IXH_DIR_Y) {
+    s->t_snap = s->t_cur;  // ftemp = T (main.f90:974-979)
+    const int out = pick(s->t_cur, s->t_ant);
+    A.out = s->tb[out];
+    A.line0 = 2;
+    A.nlines = s->mi - 1;
+    Timed tm(s, K_T_Y);
+    CHK((launch_sweep<EQ_T, DIR_Y>(s, A, s->nj + 1)));
+    CHK(tm.done());
+    s->t_cur = out;
+  } else {
+    const int out = pick(s->t_cur, s->t_ant, s->t_snap);
+    A.out = s->tb[out];
+    A.snap = s->t_snap >= 0 ? s->tb[s->t_snap] : s->tb[s->t_cur];
+    A.part0 = part_of(s, 3);
+    A.line0 = 2;
+    A.nlines = s->nj - 1;
+    Timed tm(s, K_T_X);
+    CHK((launch_sweep<EQ_T, DIR_X>(s, A, s->mi + 1)));
+    CHK(tm.done());
+    s->t_cur = out;
+  }
+  return 0;
+}
+
+// move field f (IXH_U/V/T or their _ANT) to a buffer of its own when the
+// current and lagged copies share one
+int unalias(ixh_state *s, int f) {
+  double **b = nullptr;
+  int *cur = nullptr, *ant = nullptr;
+  if (f == IXH_U || f == IXH_U_ANT) { b = s->ub; cur = &s->u_cur; ant = &s->u_ant; }
+  else if (f == IXH_V || f == IXH_V_ANT) { b = s->vb; cur = &s->v_cur; ant = &s->v_ant; }
+  else if (f == IXH_T || f == IXH_T_ANT) { b = s->tb; cur = &s->t_cur; ant = &s->t_ant; }
+  else return 0;
+  if (*cur != *ant) return 0;
+  const int k = pick(*cur, -1);
+  HIPCHK(hipMemcpyAsync(b[k], b[*cur], s->fsize * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  if (f == IXH_U || f == IXH_V || f == IXH_T) *cur = k; else *ant = k;
+  return 0;
+}
+
+int do_correct(ixh_state *s) {
+  CorrArgs A;
+  A.mi = s->mi;
+  A.nj = s->nj;
+  A.P = s->P;
+  A.g = s->geo;
+  A.u = s->ub[s->u_cur];
+  A.v = s->vb[s->v_cur];
+  A.p = s->p;
+  A.pc = s->pcb[s->pc_cur];
+  A.au = s->au;
+  A.av = s->av;
+  if (s->pc_zero) return 0;  // p' = 0: nothing changes
+  // the correction updates u, v in place; give them their own buffers when
+  // they still share one with the lagged copies
+  CHK(unalias(s, IXH_U));
+  CHK(unalias(s, IXH_V));
+  A.u = s->ub[s->u_cur];
+  A.v = s->vb[s->v_cur];
+  dim3 block(kBlock), grid((s->mi + kBlock) / kBlock, s->nj + 1);
+  Timed tm(s, K_CORR);
+  hipLaunchKernelGGL(k_correct, grid, block, 0, s->stream, A);
+  HIPCHK(hipGetLastError());
+  CHK(tm.done());
+  return 0;
+}
+
+int do_residual(ixh_state *s) {
+  ResArgs A;
+  A.mi = s->mi;
+  A.nj = s->nj;
+  A.P = s->P;
+  A.g = s->geo;
+  A.ph = phys_of(s);
+  A.u = s->ub[s->u_cur];
+  A.v = s->vb[s->v_cur];
+  A.t = s->tb[s->t_cur];
+  A.p = s->p;
+  A.uant = s->ub[s->u_ant];
+  A.sc = s->has_src[0] ? s->src_c[0] : nullptr;
+  A.sp = s->has_src[0] ? s->src_p[0] : nullptr;
+  A.resu = s->resu;
+  A.part = part_of(s, 4);
+  Timed tm(s, K_RES);
+  hipLaunchKernelGGL(k_residual_u, dim3(s->nj - 1), dim3(kBlock), 0, s->stream, A);
+  HIPCHK(hipGetLastError());
+  CHK(tm.done());
+  return 0;
+}
+
+int do_finalize(ixh_state *s, int which, double *out, int *slot) {
+  FinArgs F;
+  F.nj = s->nj;
+  F.pv = part_of(s, 0);
+  F.pp = part_of(s, 1);
+  F.pb = part_of(s, 2);
+  F.pt = part_of(s, 3);
+  F.pr = part_of(s, 4);
+  F.out = out;
+  F.slot = slot;
+  F.which = which;
+  Timed tm(s, K_FIN);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, F);
+  HIPCHK(hipGetLastError());
+  CHK(tm.done());
+  return 0;
+}
+
+// fold the selected norms and copy them to the host
+int read_norms(ixh_state *s, int which, double out[8]) {
+  CHK(do_finalize(s, which, s->d_out, nullptr));
+  HIPCHK(hipMemcpyAsync(out, s->d_out, 8 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+// one single-pass time step without host synchronisation; norms land in
+// d_hist[slot] through the device counter
+int fast_step(ixh_state *s) {
+  CHK(sweep_u(s, IXH_DIR_Y));
+  CHK(sweep_u(s, IXH_DIR_X));
+  CHK(sweep_v(s, IXH_DIR_Y));
+  CHK(sweep_v(s, IXH_DIR_X));
+  s->pc_zero = true;
+  CHK(sweep_p(s, IXH_DIR_Y));
+  CHK(sweep_p(s, IXH_DIR_X));
+  CHK(do_correct(s));
+  CHK(sweep_t(s, IXH_DIR_Y));
+  CHK(sweep_t(s, IXH_DIR_X));
+  CHK(do_residual(s));
+  CHK(do_finalize(s, 31, s->d_hist, s->d_slot));
+  s->u_ant = s->u_cur;
+  s->v_ant = s->v_cur;
+  s->t_ant = s->t_cur;
+  return 0;
+}
+
+void rot_key(const ixh_state *s, int k[7]) {
+  k[0] = s->u_cur; k[1] = s->u_ant; k[2] = s->v_cur; k[3] = s->v_ant;
+  k[4] = s->t_cur; k[5] = s->t_ant; k[6] = s->pc_cur;
+}
+
+void set_rot(ixh_state *s, const int k[7]) {
+  s->u_cur = k[0]; s->u_ant = k[1]; s->v_cur = k[2]; s->v_ant = k[3];
+  s->t_cur = k[4]; s->t_ant = k[5]; s->pc_cur = k[6];
+}
+
+void drop_graphs(ixh_state *s) {
+  for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.exec);
+  s->graphs.clear();
+}
+
+// replay one time step from a graph captured for the current rotation state
+int graph_step(ixh_state *s) {
+  int key[7];
+  rot_key(s, key);
+  for (auto &g : s->graphs) {
+    if (memcmp(g.key, key, sizeof key) == 0) {
+      HIPCHK(hipGraphLaunch(g.exec, s->stream));
+      set_rot(s, g.next);
+      s->pc_zero = false;
+      s->v_snap = -1;
+      s->t_snap = -1;
+      return 0;
+    }
+  }
+  hipGraph_t graph;
+  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
+  int rc = fast_step(s);
+  hipError_t e = hipStreamEndCapture(s->stream, &graph);
+  if (rc != 0) return rc;
+  if (e != hipSuccess) return fail(IXH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  ixh_state::GraphEntry ge;
+  memcpy(ge.key, key, sizeof key);
+  rot_key(s, ge.next);
+  e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(IXH_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  s->graphs.push_back(ge);
+  HIPCHK(hipGraphLaunch(ge.exec, s->stream));
+  s->pc_zero = false;
+  return 0;
+}
+
+int ensure_hist(ixh_state *s, int n) {
+  if (n <= s->hist_cap) return 0;
+  drop_graphs(s);  // graphs hold the old pointer
+  if (s->d_hist) HIPCHK(hipFree(s->d_hist));
+  s->d_hist = nullptr;
+  CHK(dmalloc(&s->d_hist, (size_t)8 * n));
+  s->hist_cap = n;
+  return 0;
+}
+
+}  // namespace
+
+// =================================================================== C-ABI
+extern "C" {
+
+const char *ixh_last_error(void) { return g_err.c_str(); }
+const char *ixh_version(void) { return "ixchel2d-amd 0.1 (gfx950, fp64)"; }
+const char *ixh_kernel_name(int32_t id) { return (id >= 0 && id < K_NUM) ? kKernelNames[id] : ""; }
+
+int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) {
+  if (!grid || !out) return fail(IXH_ERR_ARG, "grid/out is NULL");
+  *out = nullptr;
+  CHK(check_params(params));
+  const int mi = grid->mi, nj = grid->nj;
+  if (mi < 4 || nj < 4) return fail(IXH_ERR_ARG, "mesh needs mi, nj >= 4 (got %d, %d)", mi, nj);
+  if (mi + 1 > kNmax || nj + 1 > kNmax)
+    return fail(IXH_ERR_UNSUPP, "lines longer than %d nodes are not built (mi=%d nj=%d)", kNmax, mi, nj);
+  if (!grid->xu || !grid->yv || !grid->xp || !grid->yp) return fail(IXH_ERR_ARG, "grid coordinates are NULL");
+  ixh_state *s = new ixh_state();
+  s->mi = mi;
+  s->nj = nj;
+  s->P = ((mi + 1) + 15) / 16 * 16;  // rows start on 128-byte boundaries
+  s->fsize = (size_t)s->P * (nj + 1);
+  s->prm = *params;
+  int rc = 0;
+  auto bail = [&](int code) {
+    ixh_destroy(s);
+    return code;
+  };
+  if (hipStreamCreate(&s->stream) != hipSuccess) return bail(fail(IXH_ERR_HIP, "hipStreamCreate failed"));
+  s->own_stream = true;
+  if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return bail(fail(IXH_ERR_HIP, "hipEventCreate failed"));
+  // geometry on the host (mod_malla.f90:167-217), 1-based
+  const int gx = mi + 2, gy = nj + 2;
+  std::vector<double> xu(gx, 0.0), yv(gy, 0.0), xp(gx, 0.0), yp(gy, 0.0);
+  for (int i = 1; i <= mi; ++i) xu[i] = grid->xu[i - 1];
+  for (int j = 1; j <= nj; ++j) yv[j] = grid->yv[j - 1];
+  for (int i = 1; i <= mi + 1; ++i) xp[i] = grid->xp[i - 1];
+  for (int j = 1; j <= nj + 1; ++j) yp[j] = grid->yp[j - 1];
+  std::vector<double> dxp(gx, 0.0), dyp(gy, 0.0), dxu(gx, 0.0), dyu(gy, 0.0), dxv(gx, 0.0), dyv(gy, 0.0);
+  std::vector<double> fexp(gx, 0.0), feyp(gy, 0.0), fexu(gx, 0.0), feyv(gy, 0.0);
+  for (int i = 2; i <= mi; ++i) dxp[i] = xu[i] - xu[i - 1];
+  for (int j = 2; j <= nj; ++j) dyp[j] = yv[j] - yv[j - 1];
+  for (int i = 1; i <= mi; ++i) dxu[i] = xp[i + 1] - xp[i];
+  for (int j = 2; j <= nj; ++j) dyu[j] = yv[j] - yv[j - 1];
+  for (int i = 2; i <= mi; ++i) dxv[i] = xu[i] - xu[i - 1];
+  for (int j = 1; j <= nj; ++j) dyv[j] = yp[j + 1] - yp[j];
+  for (int i = 1; i <= mi; ++i) fexp[i] = (xp[i + 1] - xu[i]) / (xp[i + 1] - xp[i]);
+  for (int j = 1; j <= nj; ++j) feyp[j] = (yp[j + 1] - yv[j]) / (yp[j + 1] - yp[j]);
+  for (int i = 1; i <= mi - 1; ++i) fexu[i] = (xu[i + 1] - xp[i + 1]) / (xu[i + 1] - xu[i]);
+  for (int j = 1; j <= nj - 1; ++j) feyv[j] = (yv[j + 1] - yp[j + 1]) / (yv[j + 1] - yv[j]);
+  const std::vector<double> *xs[] = {&dxp, &dxu, &dxv, &fexp, &fexu, &xu, &xp};
+  const std::vector<double> *ys[] = {&dyp, &dyu, &dyv, &feyp, &feyv, &yv, &yp};
+  const size_t ngeo = 7 * (size_t)gx + 7 * (size_t)gy;
+  std::vector<double> host(ngeo);
+  size_t off = 0;
+  for (auto *v : xs) { memcpy(&host[off], v->data(), gx * sizeof(double)); off += gx; }
+  for (auto *v : ys) { memcpy(&host[off], v->data(), gy * sizeof(double)); off += gy; }
+  if ((rc = dmalloc(&s->d_geo, ngeo))) return bail(rc);
+  if (hipMemcpy(s->d_geo, host.data(), ngeo * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(IXH_ERR_HIP, "geometry upload failed"));
+  const double *d = s->d_geo;
+  s->geo.dxp = d; s->geo.dxu = d + gx; s->geo.dxv = d + 2 * gx; s->geo.fexp = d + 3 * gx;
+  s->geo.fexu = d + 4 * gx; s->geo.xu = d + 5 * gx; s->geo.xp = d + 6 * gx;
+  d += 7 * (size_t)gx;
+  s->geo.dyp = d; s->geo.dyu = d + gy; s->geo.dyv = d + 2 * gy; s->geo.feyp = d + 3 * gy;
+  s->geo.feyv = d + 4 * gy; s->geo.yv = d + 5 * gy; s->geo.yp = d + 6 * gy;
+  // fields
+  for (int k = 0; k < kNbuf; ++k) {
+    if ((rc = dmalloc(&s->ub[k], s->fsize)) || (rc = dmalloc(&s->vb[k], s->fsize)) ||
+        (rc = dmalloc(&s->tb[k], s->fsize)) || (rc = dmalloc(&s->pcb[k], s->fsize)))
+      return bail(rc);
+  }
+  if ((rc = dmalloc(&s->p, s->fsize)) || (rc = dmalloc(&s->bo, s->fsize)) ||
+      (rc = dmalloc(&s->au, s->fsize)) || (rc = dmalloc(&s->av, s->fsize)) ||
+      (rc = dmalloc(&s->resu, s->fsize)))
+    return bail(rc);
+  {
+    std::vector<double> big(s->fsize, 1.e40);  // a_u = a_v = 1e40 (main.f90:309-310)
+    if (hipMemcpy(s->au, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->av, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(IXH_ERR_HIP, "a_u/a_v initialisation failed"));
+  }
+  s->plen = (mi > nj ? mi : nj) + 4;
+  if ((rc = dmalloc(&s->part, (size_t)5 * s->plen)) || (rc = dmalloc(&s->d_out, 8))) return bail(rc);
+  if (hipMalloc((void **)&s->d_slot, sizeof(int)) != hipSuccess) return bail(fail(IXH_ERR_NOMEM, "slot"));
+  for (int v = 0; v < 3; ++v)
+    for (int sd = 0; sd < 4; ++sd) {
+      const int n = side_len(s, v, sd);
+      s->bclen[v][sd] = n;
+      if (hipMalloc((void **)&s->bct[v][sd], n + 2) != hipSuccess) return bail(fail(IXH_ERR_NOMEM, "bc"));
+      if (hipMemset(s->bct[v][sd], 0, n + 2) != hipSuccess) return bail(fail(IXH_ERR_HIP, "bc memset"));
+      if ((rc = dmalloc(&s->bcv[v][sd], n + 2))) return bail(rc);
+    }
+  *out = s;
+  return 0;
+}
+
+int ixh_destroy(ixh_state *s) {
+  if (!s) return 0;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  drop_graphs(s);
+  double *ptrs[] = {s->d_geo, s->p, s->bo, s->au, s->av, s->resu, s->part, s->d_out, s->d_hist};
+  for (double *p : ptrs)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < kNbuf; ++k) {
+    if (s->ub[k]) (void)hipFree(s->ub[k]);
+    if (s->vb[k]) (void)hipFree(s->vb[k]);
+    if (s->tb[k]) (void)hipFree(s->tb[k]);
+    if (s->pcb[k]) (void)hipFree(s->pcb[k]);
+  }
+  for (int v = 0; v < 3; ++v) {
+    if (s->src_c[v]) (void)hipFree(s->src_c[v]);
+    if (s->src_p[v]) (void)hipFree(s->src_p[v]);
+    for (int sd = 0; sd < 4; ++sd) {
+      if (s->bct[v][sd]) (void)hipFree(s->bct[v][sd]);
+      if (s->bcv[v][sd]) (void)hipFree(s->bcv[v][sd]);
+    }
+  }
+  if (s->d_slot) (void)hipFree(s->d_slot);
+  if (s->ev0) (void)hipEventDestroy(s->ev0);
+  if (s->ev1) (void)hipEventDestroy(s->ev1);
+  if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return 0;
+}
+
+int ixh_set_params(ixh_state *s, const ixh_params *params) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  CHK(check_params(params));
+  s->prm = *params;
+  drop_graphs(s);  // kernel arguments carry the scalars
+  return 0;
+}
+
+int ixh_set_stream(ixh_state *s, void *stream) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  if (s->own_stream) HIPCHK(hipStreamDestroy(s->stream));
+  s->own_stream = false;
+  if (!stream) {
+    HIPCHK(hipStreamCreate(&s->stream));
+    s->own_stream = true;
+  } else {
+    s->stream = (hipStream_t)stream;
+  }
+  drop_graphs(s);
+  return 0;
+}
+
+int ixh_synchronize(ixh_state *s) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+static double *field_ptr(ixh_state *s, int f) {
+  switch (f) {
+    case IXH_U: return s->ub[s->u_cur];
+    case IXH_V: return s->vb[s->v_cur];
+    case IXH_T: return s->tb[s->t_cur];
+    case IXH_P: return s->p;
+    case IXH_U_ANT: return s->ub[s->u_ant];
+    case IXH_V_ANT: return s->vb[s->v_ant];
+    case IXH_T_ANT: return s->tb[s->t_ant];
+    case IXH_PCORR: return s->pcb[s->pc_cur];
+    case IXH_BO: return s->bo;
+    case IXH_AU: return s->au;
+    case IXH_AV: return s->av;
+    case IXH_RESU: return s->resu;
+  }
+  return nullptr;
+}
+
+int ixh_upload_field(ixh_state *s, int32_t f, const double *src, int32_t ld) {
+  if (!s || !src || f < 0 || f >= IXH_NFIELDS) return fail(IXH_ERR_ARG, "bad upload arguments");
+  int m, n;
+  field_extent(s, f, &m, &n);
+  if (ld < m) return fail(IXH_ERR_ARG, "ld %d < first extent %d", ld, m);
+  // a field that shares its buffer with its lagged copy gets its own first
+  CHK(unalias(s, f));
+  if (f == IXH_PCORR) s->pc_zero = false;
+  drop_graphs(s);
+  HIPCHK(hipMemcpy2DAsync(field_ptr(s, f), (size_t)s->P * 8, src, (size_t)ld * 8, (size_t)m * 8, n,
+                          hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ixh_download_field(ixh_state *s, int32_t f, double *dst, int32_t ld) {
+  if (!s || !dst || f < 0 || f >= IXH_NFIELDS) return fail(IXH_ERR_ARG, "bad download arguments");
+  int m, n;
+  field_extent(s, f, &m, &n);
+  if (ld < m) return fail(IXH_ERR_ARG, "ld %d < first extent %d", ld, m);
+  if (f == IXH_PCORR && s->pc_zero) {
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int j = 0; j < n; ++j) memset(dst + (size_t)j * ld, 0, (size_t)m * 8);
+    return 0;
+  }
+  HIPCHK(hipMemcpy2DAsync(dst, (size_t)ld * 8, field_ptr(s, f), (size_t)s->P * 8, (size_t)m * 8, n,
+                          hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+int ixh_set_bc(ixh_state *s, int32_t var, int32_t side, const int32_t *type, const double *value) {
+  if (!s || var < 0 || var > 2 || side < 0 || side > 3 || !type || !value)
+    return fail(IXH_ERR_ARG, "bad boundary-condition arguments");
+  const int n = s->bclen[var][side];
+  std::vector<int8_t> t(n + 2, 0);
+  std::vector<double> v(n + 2, 0.0);
+  for (int k = 1; k <= n; ++k) {
+    const int32_t ty = type[k - 1];
+    if (ty < IXH_BC_NONE || ty > IXH_BC_NEUM) return fail(IXH_ERR_ARG, "unknown boundary type %d", ty);
+    if (ty == IXH_BC_NONE && k >= 2 && k <= n - 1)
+      return fail(IXH_ERR_ARG, "node %d of side %d of variable %d has no boundary condition", k, side, var);
+    t[k] = (int8_t)ty;
+    v[k] = value[k - 1];
+  }
+  HIPCHK(hipMemcpy(s->bct[var][side], t.data(), n + 2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(s->bcv[var][side], v.data(), (n + 2) * sizeof(double), hipMemcpyHostToDevice));
+  s->bc_set[var][side] = true;
+  drop_graphs(s);
+  return 0;
+}
+
+int ixh_set_sources(ixh_state *s, int32_t var, const double *sc, const double *sp) {
+  if (!s || var < 0 || var > 2) return fail(IXH_ERR_ARG, "bad source arguments");
+  drop_graphs(s);
+  if (!sc && !sp) {
+    s->has_src[var] = false;
+    return 0;
+  }
+  if (!sc || !sp) return fail(IXH_ERR_ARG, "Sc and Sp must both be given or both be NULL");
+  int rc;
+  if (!s->src_c[var] && ((rc = dmalloc(&s->src_c[var], s->fsize)) || (rc = dmalloc(&s->src_p[var], s->fsize))))
+    return rc;
+  const int fid = var == 0 ? IXH_U : var == 1 ? IXH_V : IXH_T;
+  int m, n;
+  field_extent(s, fid, &m, &n);
+  HIPCHK(hipMemcpy2D(s->src_c[var], (size_t)s->P * 8, sc, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy2D(s->src_p[var], (size_t)s->P * 8, sp, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
+  s->has_src[var] = true;
+  return 0;
+}
+
+int ixh_momentum_u(ixh_state *s, int32_t dir) {
+  if (!s || (dir != IXH_DIR_X && dir != IXH_DIR_Y)) return fail(IXH_ERR_ARG, "bad arguments");
+  return sweep_u(s, dir);
+}
+
+int ixh_momentum_v(ixh_state *s, int32_t dir, double *err_v) {
+  if (!s || (dir != IXH_DIR_X && dir != IXH_DIR_Y)) return fail(IXH_ERR_ARG, "bad arguments");
+  CHK(sweep_v(s, dir));
+  if (dir == IXH_DIR_X && err_v) {
+    double o[8];
+    CHK(read_norms(s, 1, o));
+    *err_v = o[0];
+  }
+  return 0;
+}
+
+int ixh_pcorr_zero(ixh_state *s) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  s->pc_zero = true;
+  return 0;
+}
+
+int ixh_pcorr(ixh_state *s, int32_t dir, double *err_p, double *maxbo) {
+  if (!s || (dir != IXH_DIR_X && dir != IXH_DIR_Y)) return fail(IXH_ERR_ARG, "bad arguments");
+  CHK(sweep_p(s, dir));
+  if (dir == IXH_DIR_X && (err_p || maxbo)) {
+    double o[8];
+    CHK(read_norms(s, 2 | 4, o));
+    if (err_p) *err_p = o[1];
+    if (maxbo) *maxbo = o[2];
+  }
+  return 0;
+}
+
+int ixh_correct(ixh_state *s) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  return do_correct(s);
+}
+
+int ixh_energy(ixh_state *s, int32_t dir, double *err_t) {
+  if (!s || (dir != IXH_DIR_X && dir != IXH_DIR_Y)) return fail(IXH_ERR_ARG, "bad arguments");
+  CHK(sweep_t(s, dir));
+  if (dir == IXH_DIR_X && err_t) {
+    double o[8];
+    CHK(read_norms(s, 8, o));
+    *err_t = o[3];
+  }
+  return 0;
+}
+
+int ixh_residual_u(ixh_state *s, double *residuo) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  CHK(do_residual(s));
+  if (residuo) {
+    double o[8];
+    CHK(read_norms(s, 16, o));
+    *residuo = o[4];
+  }
+  return 0;
+}
+
+int ixh_lag(ixh_state *s) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  s->u_ant = s->u_cur;
+  s->v_ant = s->v_cur;
+  s->t_ant = s->t_cur;
+  return 0;
+}
+
+int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t cap, int32_t *nrec) {
+  if (!s || nsteps < 0 || cap < 0 || (cap > 0 && !hist)) return fail(IXH_ERR_ARG, "bad arguments");
+  for (int v = 0; v < 3; ++v) CHK(check_bcs(s, v));
+  const ixh_params &p = s->prm;
+  int32_t count = 0;
+  const bool single = p.iter_ecuaci_max < 0 && p.iter_simple_max < 0;
+  if (single) {
+    // every inner loop and the SIMPLE loop run exactly once per step
+    CHK(ensure_hist(s, nsteps > 0 ? nsteps : 1));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->d_slot, 0);
+    HIPCHK(hipGetLastError());
+    for (int k = 0; k < nsteps; ++k) {
+      if (s->profiling)
+        CHK(fast_step(s));
+      else
+        CHK(graph_step(s));
+    }
+    const int keep = nsteps < cap ? nsteps : cap;
+    std::vector<double> h((size_t)8 * (keep > 0 ? keep : 1));
+    if (keep > 0)
+      HIPCHK(hipMemcpyAsync(h.data(), s->d_hist, (size_t)8 * keep * sizeof(double), hipMemcpyDeviceToHost,
+                            s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int k = 0; k < keep; ++k) {
+      ixh_hist &r = hist[k];
+      r.err_v = h[8 * k + 0];
+      r.err_p = h[8 * k + 1];
+      r.maxbo = h[8 * k + 2];
+      r.err_t = h[8 * k + 3];
+      r.residuo = h[8 * k + 4];
+      r.passes_mom = r.passes_p = r.passes_t = 1;
+      r.simple_iter = 0;
+      r.logged = (r.maxbo < p.conv_paso && r.residuo < p.conv_resi) ? 1 : 0;
+      r.pad_ = 0;
+    }
+    count = nsteps;
+  } else {
+    // data-dependent pass counts: read the norms back after every pass
+    for (int step = 0; step < nsteps; ++step) {
+      int iter_simple = 0;
+      for (;;) {
+        ixh_hist r;
+        memset(&r, 0, sizeof r);
+        int it = 0;
+        for (;;) {
+          CHK(sweep_u(s, IXH_DIR_Y));
+          CHK(sweep_u(s, IXH_DIR_X));
+          CHK(sweep_v(s, IXH_DIR_Y));
+          CHK(sweep_v(s, IXH_DIR_X));
+          double o[8];
+          CHK(read_norms(s, 1, o));
+          r.err_v = o[0];
+          r.passes_mom++;
+          if (r.err_v < p.conv_u || it > p.iter_ecuaci_max) break;
+          it++;
+        }
+        it = 0;
+        s->pc_zero = true;
+        for (;;) {
+          CHK(sweep_p(s, IXH_DIR_Y));
+          CHK(sweep_p(s, IXH_DIR_X));
+          double o[8];
+          CHK(read_norms(s, 2 | 4, o));
+          r.err_p = o[1];
+          r.maxbo = o[2];
+          r.passes_p++;
+          if (r.err_p < p.conv_p || it > p.iter_ecuaci_max) break;
+          it++;
+        }
+        it = 0;
+        CHK(do_correct(s));
+        for (;;) {
+          CHK(sweep_t(s, IXH_DIR_Y));
+          CHK(sweep_t(s, IXH_DIR_X));
+          double o[8];
+          CHK(read_norms(s, 8, o));
+          r.err_t = o[3];
+          r.passes_t++;
+          if (r.err_t < p.conv_t || it > p.iter_ecuaci_max) break;
+          it++;
+        }
+        CHK(do_residual(s));
+        double o[8];
+        CHK(read_norms(s, 16, o));
+        r.residuo = o[4];
+        bool done = false;
+        if (r.maxbo < p.conv_paso && r.residuo < p.conv_resi) {
+          iter_simple = 0;
+          done = true;
+          r.logged = 1;
+        } else if (iter_simple > p.iter_simple_max) {
+          iter_simple = 0;
+          done = true;
+          r.logged = 0;
+        } else {
+          iter_simple++;
+          r.logged = 1;
+        }
+        r.simple_iter = iter_simple;
+        if (count < cap) hist[count] = r;
+        count++;
+        if (done) break;
+      }
+      CHK(ixh_lag(s));
+    }
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  if (nrec) *nrec = count;
+  return 0;
+}
+
+int ixh_nusselt_y(ixh_state *s, int32_t i0, int32_t i1, double *nu0, double *nu1) {
+  if (!s || !nu0 || !nu1) return fail(IXH_ERR_ARG, "bad arguments");
+  if (i0 < 1 || i1 < i0 || i1 + 2 > s->mi + 1)
+    return fail(IXH_ERR_ARG, "Nusselt range %d..%d outside 1..%d", i0, i1, s->mi - 1);
+  NuArgs A;
+  A.mi = s->mi; A.nj = s->nj; A.P = s->P; A.i0 = i0; A.i1 = i1;
+  A.g = s->geo; A.t = s->tb[s->t_cur]; A.out = s->d_out; A.vertical = 0;
+  hipLaunchKernelGGL(k_nusselt, dim3(1), dim3(kBlock), 0, s->stream, A);
+  HIPCHK(hipGetLastError());
+  double o[2];
+  HIPCHK(hipMemcpyAsync(o, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *nu0 = o[0];
+  *nu1 = o[1];
+  return 0;
+}
+
+int ixh_nusselt_x(ixh_state *s, double *nu_w, double *nu_e) {
+  if (!s || !nu_w || !nu_e) return fail(IXH_ERR_ARG, "bad arguments");
+  NuArgs A;
+  A.mi = s->mi; A.nj = s->nj; A.P = s->P; A.i0 = 0; A.i1 = 0;
+  A.g = s->geo; A.t = s->tb[s->t_cur]; A.out = s->d_out; A.vertical = 1;
+  hipLaunchKernelGGL(k_nusselt, dim3(1), dim3(kBlock), 0, s->stream, A);
+  HIPCHK(hipGetLastError());
+  double o[2];
+  HIPCHK(hipMemcpyAsync(o, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  *nu_w = o[0];
+  *nu_e = o[1];
+  return 0;
+}
+
+int ixh_tridiagonal(const double *a, double *b, const double *c, double *r, int32_t n, int32_t nsys,
+                    int32_t solver) {
+  if (!a || !b || !c || !r || n < 1 || nsys < 0) return fail(IXH_ERR_ARG, "bad tridiagonal arguments");
+  if (n > kNmax) return fail(IXH_ERR_UNSUPP, "systems longer than %d are not built", kNmax);
+  if (solver != IXH_SOLVER_PCR && solver != IXH_SOLVER_THOMAS) return fail(IXH_ERR_ARG, "unknown solver");
+  if (nsys == 0) return 0;
+  const size_t tot = (size_t)n * nsys;
+  double *d = nullptr;
+  HIPCHK(hipMalloc((void **)&d, 4 * tot * sizeof(double)));
+  double *da = d, *db = d + tot, *dc = d + 2 * tot, *dr = d + 3 * tot;
+  int rc = 0;
+  hipError_t e = hipSuccess;
+  if ((e = hipMemcpy(da, a, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(db, b, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dc, c, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dr, r, tot * 8, hipMemcpyHostToDevice)) != hipSuccess) {
+    (void)hipFree(d);
+    return fail(IXH_ERR_HIP, "tridiagonal upload: %s", hipGetErrorString(e));
+  }
+  const int ept = ept_for(n);
+#define IXH_TRI(S, E) hipLaunchKernelGGL((k_tridiag<S, E>), dim3(nsys), dim3(kBlock), 0, 0, da, db, dc, dr, n)
+  if (solver == IXH_SOLVER_THOMAS) {
+    switch (ept) {
+      case 1: IXH_TRI(SOLVE_THOMAS, 1); break; case 2: IXH_TRI(SOLVE_THOMAS, 2); break;
+      case 3: IXH_TRI(SOLVE_THOMAS, 3); break; case 4: IXH_TRI(SOLVE_THOMAS, 4); break;
+      case 6: IXH_TRI(SOLVE_THOMAS, 6); break; default: IXH_TRI(SOLVE_THOMAS, 8); break;
+    }
+  } else {
+    switch (ept) {
+      case 1: IXH_TRI(SOLVE_PCR, 1); break; case 2: IXH_TRI(SOLVE_PCR, 2); break;
+      case 3: IXH_TRI(SOLVE_PCR, 3); break; case 4: IXH_TRI(SOLVE_PCR, 4); break;
+      case 6: IXH_TRI(SOLVE_PCR, 6); break; default: IXH_TRI(SOLVE_PCR, 8); break;
+    }
+  }
+#undef IXH_TRI
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(r, dr, tot * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(b, db, tot * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) rc = fail(IXH_ERR_HIP, "tridiagonal: %s", hipGetErrorString(e));
+  (void)hipFree(d);
+  return rc;
+}
+
+int ixh_set_profiling(ixh_state *s, int32_t on) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  s->profiling = on != 0;
+  for (int k = 0; k < K_NUM; ++k) {
+    s->kms[k] = 0.0;
+    s->kcount[k] = 0;
+  }
+  return 0;
+}
+
+int ixh_kernel_times(ixh_state *s, double *ms, int32_t *launches, int32_t cap, int32_t *nk) {
+  if (!s) return fail(IXH_ERR_ARG, "state is NULL");
+  for (int k = 0; k < K_NUM && k < cap; ++k) {
+    if (ms) ms[k] = s->kms[k];
+    if (launches) launches[k] = s->kcount[k];
+  }
+  if (nk) *nk = K_NUM;
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,693 @@
+// Device code of the SIMPLE iteration for gfx950 (CDNA4), FP64 path.
+//
+// One fused kernel per directional sweep: each workgroup owns one line (an
+// x-line = fixed jj, or a y-line = fixed ii), assembles the tridiagonal rows
+// of that line in registers (the ensambla_* arithmetic of
+// source/mod_ec_momento.f90 / mod_ec_energia.f90), writes the boundary rows
+// (impone_cond_frontera, source/mod_cond_frontera.f90:254-420), solves the
+// line in LDS and writes the new field into a separate buffer (every line of
+// a sweep reads the field as it was before the sweep, as upstream's
+// assemble-all-then-solve-all order does).  Per-line partial sums of the
+// convergence norms are written by the same kernel and folded by one small
+// kernel in a fixed order, so results do not depend on scheduling.
+//
+// Arithmetic follows upstream term by term (same association; the library is
+// built with -ffp-contract=off), so with the Thomas line solver the fields
+// equal the upstream CPU build's bit for bit; the PCR solver differs only by
+// the rounding of the line solve.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ixh {
+
+constexpr int kBlock = 256;   // threads per line workgroup (4 wave64)
+constexpr int kNmax = 2048;   // longest line the LDS solvers hold
+
+enum Eq { EQ_U = 0, EQ_V = 1, EQ_P = 2, EQ_T = 3 };
+enum Dir { DIR_Y = 0, DIR_X = 1 };
+enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1 };
+
+// 1-based geometry arrays (index 0 unused), built as in
+// source/mod_malla.f90:167-217
+struct Geo {
+  const double *dxp, *dyp, *dxu, *dyu, *dxv, *dyv;
+  const double *fexp, *feyp, *fexu, *feyv;
+  const double *xu, *yv, *xp, *yp;
+};
+
+struct Phys {
+  double dt, rel_pres, rel_vel, rel_ener, gm, gt, ri_x, ri_y;
+};
+
+// boundary nodes of one side: type (0 none, 1 Dirichlet, 2 Neumann) and value,
+// indexed by the 1-based node number along the side
+struct Side {
+  const int8_t *type;
+  const double *val;
+};
+
+struct SweepArgs {
+  int mi, nj, P;          // mesh extents and the common row pitch of all fields
+  int line0, nlines;      // first line (1-based) and number of lines
+  Geo g;
+  Phys ph;
+  const double *u, *v, *t, *p, *uant, *vant, *tant, *pc, *au, *av;
+  const double *sc, *sp;  // sources of this equation (nullptr = zero)
+  Side bc[4];             // a, b, c, d of this equation
+  const double *snap;     // field at the start of the pass (nullptr = zero)
+  double *out;            // solved field
+  double *acoef;          // a_u / a_v written by the momentum sweeps
+  double *bo;             // mass residual written by the p' x-sweep
+  double *part0, *part1;  // per-line partial sums (indexed by line number)
+  int in_zero;            // p' input identically zero (first pass)
+};
+
+#define AT(ptr, i, j) (ptr)[((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P]
+
+__device__ __forceinline__ double pow5(double t) { return t * (t * (t * (t * t))); }
+__device__ __forceinline__ double dmax0(double x) { return x < 0.0 ? 0.0 : x; }
+
+// power-law neighbour coefficient with upstream's sign and association:
+// -(G*L/d * max(0,(1-0.1|w d/G|)^5) + max(0, +-w L))
+__device__ __forceinline__ double nbc(double g, double len, double d, double w, bool east) {
+  double conv = w * len;
+  if (east) conv = -conv;
+  return -(g * len / d * dmax0(pow5(1.0 - 0.1 * fabs(w * d / g))) + dmax0(conv));
+}
+
+__device__ __forceinline__ double harm(double g1, double g2, double f) {
+  return (g1 * g2) / (g1 * (1.0 - f) + g2 * f);
+}
+
+struct Coef {
+  double ai, ad, bs, bn, ac;
+};
+
+// ---- u momentum, ensambla_velu_x/_y (source/mod_ec_momento.f90:365-724)
+__device__ __forceinline__ Coef u_coef(const SweepArgs &A, int ii, int jj) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  const double gm = A.ph.gm;
+  Coef c;
+  double ud = g.fexu[ii] * AT(A.u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(A.u, ii, jj);
+  double ui = g.fexu[ii - 1] * AT(A.u, ii, jj) + (1.0 - g.fexu[ii - 1]) * AT(A.u, ii - 1, jj);
+  double vn = g.fexp[ii] * AT(A.v, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj);
+  double vs = g.fexp[ii] * AT(A.v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj - 1);
+  double gd = harm(gm, gm, g.feyp[jj]);
+  double gi = harm(gm, gm, g.feyp[jj]);
+  double gn = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
+  gd = harm(gm, gm, g.feyp[jj - 1]);
+  gi = harm(gm, gm, g.feyp[jj - 1]);
+  double gs = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
+  double di = g.dxp[ii], dd = g.dxp[ii + 1], ds = g.dyv[jj - 1], dn = g.dyv[jj];
+  double dx = g.dxu[ii], dy = g.dyu[jj];
+  c.ai = nbc(gm, dy, di, ui, false);
+  c.ad = nbc(gm, dy, dd, ud, true);
+  c.bs = nbc(gs, dx, ds, vs, false);
+  c.bn = nbc(gn, dx, dn, vn, true);
+  double acc = -c.ai - c.ad - c.bs - c.bn;
+  if (A.sp) acc = acc - dx * dy * AT(A.sp, ii, jj);
+  c.ac = (acc + dx * dy / A.ph.dt) / A.ph.rel_vel;
+  return c;
+}
+
+__device__ __forceinline__ double u_rhs(const SweepArgs &A, int ii, int jj, double head, double ac) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  double dx = g.dxu[ii], dy = g.dyu[jj];
+  double r = head;
+  if (A.ph.ri_x != 0.0) {
+    double tint = g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.t, ii, jj);
+    r = r - dx * dy * A.ph.ri_x * tint;
+  }
+  if (A.sc) r = r + dx * dy * AT(A.sc, ii, jj);
+  r = r + dx * dy * AT(A.uant, ii, jj) / A.ph.dt;
+  r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * dy;
+  r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.u, ii, jj);
+  return r;
+}
+
+// ---- v momentum, ensambla_velv_x/_y (source/mod_ec_momento.f90:736-1098)
+__device__ __forceinline__ Coef v_coef(const SweepArgs &A, int ii, int jj) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  const double gm = A.ph.gm;
+  Coef c;
+  double ud = g.feyp[jj] * AT(A.u, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.u, ii, jj);
+  double ui = g.feyp[jj] * AT(A.u, ii - 1, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.u, ii - 1, jj);
+  double vn = g.feyv[jj] * AT(A.v, ii, jj + 1) + (1.0 - g.feyv[jj]) * AT(A.v, ii, jj);
+  double vs = g.feyv[jj - 1] * AT(A.v, ii, jj) + (1.0 - g.feyv[jj - 1]) * AT(A.v, ii, jj - 1);
+  double gs = harm(gm, gm, g.fexp[ii]);
+  double gn = harm(gm, gm, g.fexp[ii]);
+  double gd = (gn * gs) / (gn * (1.0 - g.feyp[jj]) + gs * g.feyp[jj]);
+  gs = harm(gm, gm, g.fexp[ii - 1]);
+  gn = harm(gm, gm, g.fexp[ii - 1]);
+  double gi = (gs * gn) / (gn * (1.0 - g.feyp[jj]) + gs * g.feyp[jj]);
+  double di = g.dxu[ii - 1], dd = g.dxu[ii], ds = g.dyp[jj], dn = g.dyp[jj + 1];
+  double dxv = g.dxv[ii], dyv = g.dyv[jj];
+  c.ai = nbc(gi, dyv, di, ui, false);
+  c.ad = nbc(gd, dyv, dd, ud, true);
+  c.bs = nbc(gm, dxv, ds, vs, false);
+  c.bn = nbc(gm, dxv, dn, vn, true);
+  double acc = -c.ai - c.ad - c.bs - c.bn;
+  if (A.sp) acc = acc - dxv * dyv * AT(A.sp, ii, jj);
+  c.ac = (acc + dxv * dyv / A.ph.dt) / A.ph.rel_vel;
+  return c;
+}
+
+__device__ __forceinline__ double v_rhs(const SweepArgs &A, int ii, int jj, double head, double ac) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  double dxv = g.dxv[ii], dyv = g.dyv[jj];
+  double r = head;
+  if (A.ph.ri_y != 0.0) {
+    double tint = g.feyp[jj] * AT(A.t, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.t, ii, jj);
+    r = r - dxv * dyv * A.ph.ri_y * tint;
+  }
+  if (A.sc) r = r + dxv * dyv * AT(A.sc, ii, jj);
+  r = r + dxv * dyv * AT(A.vant, ii, jj) / A.ph.dt;
+  r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * dxv;
+  r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.v, ii, jj);
+  return r;
+}
+
+// ---- pressure correction, ensambla_corr_pres_x/_y (mod_ec_momento.f90:145-355)
+struct PCoef {
+  double ai, ad, bs, bn, ac, b;
+};
+__device__ __forceinline__ PCoef p_coef(const SweepArgs &A, int ii, int jj) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  PCoef c;
+  double ud = AT(A.u, ii, jj), ui = AT(A.u, ii - 1, jj);
+  double vn = AT(A.v, ii, jj), vs = AT(A.v, ii, jj - 1);
+  double ai = AT(A.au, ii - 1, jj), ad = AT(A.au, ii, jj);
+  double as = AT(A.av, ii, jj - 1), an = AT(A.av, ii, jj);
+  double dyp = g.dyp[jj], dxp = g.dxp[ii];
+  c.ai = -dyp * dyp / ai;
+  c.ad = -dyp * dyp / ad;
+  c.bs = -dxp * dxp / as;
+  c.bn = -dxp * dxp / an;
+  c.ac = (-c.ai - c.ad - c.bs - c.bn) / A.ph.rel_pres;
+  c.b = (ui - ud) * dyp + (vs - vn) * dxp;
+  return c;
+}
+
+// ---- energy, ensambla_energia_x/_y (mod_ec_energia.f90:119-443); the
+// driver hands fexu/feyv to the routine's interpolation-weight dummies
+// (main.f90:986,1057), so those weight the interface diffusivities
+__device__ __forceinline__ Coef t_coef(const SweepArgs &A, int ii, int jj) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  const double gt = A.ph.gt;
+  Coef c;
+  double ud = AT(A.u, ii, jj), ui = AT(A.u, ii - 1, jj);
+  double vn = AT(A.v, ii, jj), vs = AT(A.v, ii, jj - 1);
+  double di = g.dxu[ii - 1], dd = g.dxu[ii], ds = g.dyv[jj - 1], dn = g.dyv[jj];
+  double gd = harm(gt, gt, g.fexu[ii]);
+  double gi = harm(gt, gt, g.fexu[ii - 1]);
+  double gn = harm(gt, gt, g.feyv[jj]);
+  double gs = harm(gt, gt, g.feyv[jj - 1]);
+  double dxp = g.dxp[ii], dyp = g.dyp[jj];
+  c.ai = nbc(gi, dyp, di, ui, false);
+  c.ad = nbc(gd, dyp, dd, ud, true);
+  c.bs = nbc(gs, dxp, ds, vs, false);
+  c.bn = nbc(gn, dxp, dn, vn, true);
+  double acc = -c.ai - c.ad - c.bs - c.bn;
+  if (A.sp) acc = acc - dxp * dyp * AT(A.sp, ii, jj);
+  c.ac = (acc + dxp * dyp / A.ph.dt) / A.ph.rel_ener;
+  return c;
+}
+
+__device__ __forceinline__ double t_rhs(const SweepArgs &A, int ii, int jj, double head, double ac) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  double dxp = g.dxp[ii], dyp = g.dyp[jj];
+  double r = head;
+  if (A.sc) r = r + dxp * dyp * AT(A.sc, ii, jj);
+  r = r + dxp * dyp * AT(A.tant, ii, jj) / A.ph.dt;
+  r = r + ac * (1.0 - A.ph.rel_ener) * AT(A.t, ii, jj);
+  return r;
+}
+
+// ---------------------------------------------------------------- solvers
+// Thomas without pivoting in upstream's order (mod_solucionador.f90:25-47),
+// one lane; a[0] and c[n-1] are not read.
+__device__ __forceinline__ void thomas_lds(double *a, double *b, double *c, double *r, int n) {
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < n; ++k) {
+      double m = a[k] / b[k - 1];
+      r[k] = r[k] - m * r[k - 1];
+      b[k] = b[k] - m * c[k - 1];
+    }
+    r[n - 1] = r[n - 1] / b[n - 1];
+    for (int k = n - 2; k >= 0; --k) r[k] = (r[k] - c[k] * r[k + 1]) / b[k];
+  }
+  __syncthreads();
+}
+
+// Parallel cyclic reduction (Hockney) over the whole line in LDS; EPT rows
+// per thread.  Row k couples to k-s and k+s after step s; a[0] and c[n-1]
+// are set to zero first.  On exit r[k] holds x[k].
+template <int EPT>
+__device__ __forceinline__ void pcr_lds(double *a, double *b, double *c, double *r, int n) {
+  if (threadIdx.x == 0) {
+    a[0] = 0.0;
+    c[n - 1] = 0.0;
+  }
+  __syncthreads();
+  for (int s = 1; s < n; s <<= 1) {
+    double na[EPT], nb[EPT], nc[EPT], nr[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      int k = threadIdx.x + e * kBlock;
+      if (k < n) {
+        double ak = a[k], bk = b[k], ck = c[k], rk = r[k];
+        double ta = 0.0, tc = 0.0;
+        if (k - s >= 0) {
+          double al = -ak / b[k - s];
+          ta = al * a[k - s];
+          bk = bk + al * c[k - s];
+          rk = rk + al * r[k - s];
+        }
+        if (k + s < n) {
+          double ga = -ck / b[k + s];
+          tc = ga * c[k + s];
+          bk = bk + ga * a[k + s];
+          rk = rk + ga * r[k + s];
+        }
+        na[e] = ta;
+        nb[e] = bk;
+        nc[e] = tc;
+        nr[e] = rk;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      int k = threadIdx.x + e * kBlock;
+      if (k < n) {
+        a[k] = na[e];
+        b[k] = nb[e];
+        c[k] = nc[e];
+        r[k] = nr[e];
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    int k = threadIdx.x + e * kBlock;
+    if (k < n) r[k] = r[k] / b[k];
+  }
+  __syncthreads();
+}
+
+template <int SOLVER, int EPT>
+__device__ __forceinline__ void solve_line(double *a, double *b, double *c, double *r, int n) {
+  if (SOLVER == SOLVE_THOMAS)
+    thomas_lds(a, b, c, r, n);
+  else
+    pcr_lds<EPT>(a, b, c, r, n);
+}
+
+// deterministic block sum (fixed tree)
+__device__ __forceinline__ double block_sum(double x, double *scratch) {
+  scratch[threadIdx.x] = x;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) scratch[threadIdx.x] = scratch[threadIdx.x] + scratch[threadIdx.x + w];
+    __syncthreads();
+  }
+  double s = scratch[0];
+  __syncthreads();
+  return s;
+}
+
+// boundary row of a sweep: side low (a/b) or high (c/d); returns false when
+// the node is untyped
+__device__ __forceinline__ bool bc_row(const Side &sd, int node, bool high, double &a0, double &a1,
+                                       double &a2, double &rr) {
+  int t = sd.type[node];
+  if (t == 0) return false;
+  if (t == 1) {
+    a0 = 0.0; a1 = 1.0; a2 = 0.0;
+  } else if (!high) {
+    a0 = 0.0; a1 = -1.0; a2 = 1.0;
+  } else {
+    a0 = -1.0; a1 = 1.0; a2 = 0.0;
+  }
+  rr = sd.val[node];
+  return true;
+}
+
+// ---------------------------------------------------------- sweep kernel
+// Extents per equation:  line length n, interior range [lo, hi] along the
+// line, first/last line, and the boundary side pair.
+template <int EQ, int DIR>
+struct Layout {
+  __device__ static int n(int mi, int nj) {
+    if (DIR == DIR_X) return EQ == EQ_U ? mi : mi + 1;
+    return EQ == EQ_V ? nj : nj + 1;
+  }
+  __device__ static int hi(int mi, int nj) {  // last interior position
+    if (DIR == DIR_X) return EQ == EQ_U ? mi - 1 : mi;
+    return EQ == EQ_V ? nj - 1 : nj;
+  }
+  __device__ static int last_line(int mi, int nj) {
+    if (DIR == DIR_X) return EQ == EQ_V ? nj - 1 : nj;
+    return EQ == EQ_U ? mi - 1 : mi;
+  }
+};
+
+template <int EQ, int DIR, int SOLVER, int EPT>
+__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
+  __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
+  __shared__ double red[kBlock];
+  const int P = A.P;
+  const int mi = A.mi, nj = A.nj;
+  const int line = A.line0 + (int)blockIdx.x;
+  const int n = Layout<EQ, DIR>::n(mi, nj);
+  const int hi = Layout<EQ, DIR>::hi(mi, nj);
+  const int lastl = Layout<EQ, DIR>::last_line(mi, nj);
+  // sides holding the two ends of this line: x-lines end on a (low) and c,
+  // y-lines on b (low) and d
+  const Side &lo_side = DIR == DIR_X ? A.bc[0] : A.bc[1];
+  const Side &hi_side = DIR == DIR_X ? A.bc[2] : A.bc[3];
+
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int pos = k + 1;                 // 1-based position along the line
+    const int ii = DIR == DIR_X ? pos : line;
+    const int jj = DIR == DIR_X ? line : pos;
+    double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
+    if (pos >= 2 && pos <= hi) {
+      if (EQ == EQ_P) {
+        PCoef c = p_coef(A, ii, jj);
+        double pcP = A.in_zero ? 0.0 : AT(A.pc, ii, jj);
+        double head;
+        if (DIR == DIR_X) {
+          a0 = c.ai; a2 = c.ad;
+          double pS = A.in_zero ? 0.0 : AT(A.pc, ii, jj - 1);
+          double pN = A.in_zero ? 0.0 : AT(A.pc, ii, jj + 1);
+          head = -c.bs * pS - c.bn * pN;
+          A.bo[((size_t)ii - 1) + ((size_t)jj - 1) * P] = c.b;
+        } else {
+          a0 = c.bs; a2 = c.bn;
+          double pW = A.in_zero ? 0.0 : AT(A.pc, ii - 1, jj);
+          double pE = A.in_zero ? 0.0 : AT(A.pc, ii + 1, jj);
+          head = -c.ai * pW - c.ad * pE;
+        }
+        a1 = c.ac;
+        rr = head + c.b + (1.0 - A.ph.rel_pres) * c.ac * pcP;
+      } else {
+        Coef c = EQ == EQ_U ? u_coef(A, ii, jj) : EQ == EQ_V ? v_coef(A, ii, jj) : t_coef(A, ii, jj);
+        const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
+        double head;
+        if (DIR == DIR_X) {
+          a0 = c.ai; a2 = c.ad;
+          head = -c.bs * AT(phi, ii, jj - 1) - c.bn * AT(phi, ii, jj + 1);
+        } else {
+          a0 = c.bs; a2 = c.bn;
+          head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
+        }
+        a1 = c.ac;
+        rr = EQ == EQ_U ? u_rhs(A, ii, jj, head, c.ac)
+           : EQ == EQ_V ? v_rhs(A, ii, jj, head, c.ac) : t_rhs(A, ii, jj, head, c.ac);
+        if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X)
+          A.acoef[((size_t)ii - 1) + ((size_t)jj - 1) * P] = c.ac * A.ph.rel_vel;
+      }
+    } else {
+      const bool high = pos != 1;
+      if (EQ == EQ_P) {
+        // p' ghost rows: Dirichlet 0 (main.f90:789-800, 845-856)
+        a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = 0.0;
+      } else {
+        // the node number along the side is the line index
+        bool typed = bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
+        if (typed && (EQ == EQ_U || EQ == EQ_V))
+          A.acoef[((size_t)ii - 1) + ((size_t)jj - 1) * P] = 1.e40;
+      }
+    }
+    sa[k] = a0;
+    sb[k] = a1;
+    sc_[k] = a2;
+    sr[k] = rr;
+  }
+  __syncthreads();
+  solve_line<SOLVER, EPT>(sa, sb, sc_, sr, n);
+
+  // write the line, accumulate the x-sweep norms
+  double acc0 = 0.0, acc1 = 0.0;
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int pos = k + 1;
+    const int ii = DIR == DIR_X ? pos : line;
+    const int jj = DIR == DIR_X ? line : pos;
+    const double x = sr[k];
+    AT(A.out, ii, jj) = x;
+    if (DIR == DIR_X && EQ != EQ_U && pos >= 2 && pos <= mi) {
+      double old = A.snap ? AT(A.snap, ii, jj) : 0.0;
+      double d = x - old;
+      acc0 = acc0 + d * d;
+      if (EQ == EQ_P) {
+        double b = AT(A.bo, ii, jj);
+        acc1 = acc1 + b * b;
+      }
+    }
+  }
+  if (DIR == DIR_X && EQ != EQ_U) {
+    double s0 = block_sum(acc0, red);
+    if (EQ == EQ_P) {
+      double s1 = block_sum(acc1, red);
+      if (threadIdx.x == 0) A.part1[line] = s1;
+    }
+    if (threadIdx.x == 0) A.part0[line] = s0;
+  }
+
+  // carry the lines this sweep does not solve into the output buffer
+  const double *src = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc;
+  for (int side = 0; side < 2; ++side) {
+    if (line != (side == 0 ? A.line0 : lastl)) continue;
+    const int other = side == 0 ? A.line0 - 1 : lastl + 1;
+    for (int k = threadIdx.x; k < n; k += kBlock) {
+      const int pos = k + 1;
+      const int ii = DIR == DIR_X ? pos : other;
+      const int jj = DIR == DIR_X ? other : pos;
+      AT(A.out, ii, jj) = (EQ == EQ_P && A.in_zero) ? 0.0 : AT(src, ii, jj);
+    }
+  }
+}
+
+// ------------------------------------------------------------- correction
+// p += 0.6 p' on ii 2..mi, jj 2..nj; u += dyu (p'_P - p'_E)/a_u on ii 2..mi-1,
+// jj 2..nj; v += dxv (p'_P - p'_N)/a_v on ii 2..mi, jj 2..nj-1
+// (source/main.f90:931-962), in place.
+struct CorrArgs {
+  int mi, nj, P;
+  Geo g;
+  double *u, *v, *p;
+  const double *pc, *au, *av;
+};
+
+__global__ void __launch_bounds__(kBlock) k_correct(CorrArgs A) {
+  const int P = A.P;
+  const int ii = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+  const int jj = 1 + (int)blockIdx.y;
+  if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) return;
+  const double pcP = AT(A.pc, ii, jj);
+  AT(A.p, ii, jj) = AT(A.p, ii, jj) + 0.6 * pcP;
+  if (ii <= A.mi - 1)
+    AT(A.u, ii, jj) = AT(A.u, ii, jj) + A.g.dyu[jj] * (pcP - AT(A.pc, ii + 1, jj)) / AT(A.au, ii, jj);
+  if (jj <= A.nj - 1)
+    AT(A.v, ii, jj) = AT(A.v, ii, jj) + A.g.dxv[ii] * (pcP - AT(A.pc, ii, jj + 1)) / AT(A.av, ii, jj);
+}
+
+// ---------------------------------------------------------------- residual
+// residuo_u (source/mod_ec_momento.f90:1107-1266): unrelaxed u residual on
+// ii 2..mi-1, jj 2..nj; one workgroup per row, per-row sum of squares.  The
+// south diffusivity helper keeps upstream's numerator Gamma(ii,jj+1)Gamma(ii,jj)
+// (:1208-1209).
+struct ResArgs {
+  int mi, nj, P;
+  Geo g;
+  Phys ph;
+  const double *u, *v, *t, *p, *uant, *sc, *sp;
+  double *resu, *part;
+};
+
+__global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
+  __shared__ double red[kBlock];
+  const int P = A.P;
+  const Geo &g = A.g;
+  const double gm = A.ph.gm;
+  const int jj = 2 + (int)blockIdx.x;
+  double acc = 0.0;
+  for (int ii = 2 + (int)threadIdx.x; ii <= A.mi - 1; ii += kBlock) {
+    double ud = g.fexu[ii] * AT(A.u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(A.u, ii, jj);
+    double ui = g.fexu[ii - 1] * AT(A.u, ii, jj) + (1.0 - g.fexu[ii - 1]) * AT(A.u, ii - 1, jj);
+    double vn = g.fexp[ii] * AT(A.v, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj);
+    double vs = g.fexp[ii] * AT(A.v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj - 1);
+    double gd = harm(gm, gm, g.feyp[jj]);
+    double gi = harm(gm, gm, g.feyp[jj]);
+    double gn = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
+    gd = harm(gm, gm, g.feyp[jj - 1]);
+    gi = (gm * gm) / (gm * (1.0 - g.feyp[jj - 1]) + gm * g.feyp[jj - 1]);
+    double gs = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
+    double di = g.dxp[ii], dd = g.dxp[ii + 1], ds = g.dyv[jj - 1], dn = g.dyv[jj];
+    double dx = g.dxu[ii], dy = g.dyu[jj];
+    double ai = -nbc(gm, dy, di, ui, false);
+    double ad = -nbc(gm, dy, dd, ud, true);
+    double bs = -nbc(gs, dx, ds, vs, false);
+    double bn = -nbc(gn, dx, dn, vn, true);
+    double ac = ai + ad + bs + bn;
+    if (A.sp) ac = ac - dx * dy * AT(A.sp, ii, jj);
+    ac = ac + dx * dy / A.ph.dt;
+    double r = ai * AT(A.u, ii - 1, jj) - ac * AT(A.u, ii, jj) + ad * AT(A.u, ii + 1, jj) +
+               bs * AT(A.u, ii, jj - 1) + bn * AT(A.u, ii, jj + 1);
+    if (A.ph.ri_x != 0.0) {
+      double tint = g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.t, ii, jj);
+      r = r - dx * dy * A.ph.ri_x * tint;
+    }
+    if (A.sc) r = r + dx * dy * AT(A.sc, ii, jj);
+    r = r + dx * dy * AT(A.uant, ii, jj) / A.ph.dt;
+    r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * dy;
+    AT(A.resu, ii, jj) = r;
+    acc = acc + r * r;
+  }
+  double s = block_sum(acc, red);
+  if (threadIdx.x == 0) A.part[jj] = s;
+}
+
+// ---------------------------------------------------------------- finalize
+// Fold per-line partials into the five norms of one SIMPLE iteration:
+// out[0] err_v (lines 2..nj-1), out[1] err_p, out[2] maxbo, out[3] err_t
+// (lines 2..nj), out[4] residuo (lines 2..nj).  `which` masks the ones to
+// fold; `slot` (if not null) is a device counter selecting out + 8*slot and
+// incremented afterwards, so one graph can record a whole history.
+struct FinArgs {
+  int nj;
+  const double *pv, *pp, *pb, *pt, *pr;
+  double *out;
+  int *slot;
+  int which;
+};
+
+__device__ __forceinline__ double fold(const double *part, int lo, int hi, double *red) {
+  double acc = 0.0;
+  for (int j = lo + (int)threadIdx.x; j <= hi; j += kBlock) acc = acc + part[j];
+  return block_sum(acc, red);
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) {
+  __shared__ double red[kBlock];
+  double *o = A.out;
+  if (A.slot) o = A.out + 8 * (size_t)(*A.slot);
+  double r0 = (A.which & 1) ? fold(A.pv, 2, A.nj - 1, red) : 0.0;
+  double r1 = (A.which & 2) ? fold(A.pp, 2, A.nj, red) : 0.0;
+  double r2 = (A.which & 4) ? fold(A.pb, 2, A.nj, red) : 0.0;
+  double r3 = (A.which & 8) ? fold(A.pt, 2, A.nj, red) : 0.0;
+  double r4 = (A.which & 16) ? fold(A.pr, 2, A.nj, red) : 0.0;
+  if (threadIdx.x == 0) {
+    if (A.which & 1) o[0] = sqrt(r0);
+    if (A.which & 2) o[1] = sqrt(r1);
+    if (A.which & 4) o[2] = sqrt(r2);
+    if (A.which & 8) o[3] = sqrt(r3);
+    if (A.which & 16) o[4] = sqrt(r4);
+    if (A.slot) *A.slot = *A.slot + 1;
+  }
+}
+
+__global__ void k_set_int(int *p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+// ----------------------------------------------------------------- Nusselt
+// nusselt_promedio_y (source/mod_postproceso.f90:100-215) and the
+// vertical-wall mean Nusselt numbers; a single workgroup, sums in upstream
+// order by one lane so the result equals the CPU value.
+struct NuArgs {
+  int mi, nj, P, i0, i1;
+  Geo g;
+  const double *t;
+  double *out;  // [nu0, nu1] (horizontal) or [nu_w, nu_e] (vertical)
+  int vertical;
+};
+
+__global__ void __launch_bounds__(kBlock) k_nusselt(NuArgs A) {
+  __shared__ double der[kNmax + 4];
+  const int P = A.P;
+  const Geo &g = A.g;
+  if (!A.vertical) {
+    for (int w = 0; w < 2; ++w) {
+      const int jj = w == 0 ? 1 : A.nj - 1;
+      for (int k = threadIdx.x; k < A.mi + 4; k += kBlock) der[k] = 0.0;
+      __syncthreads();
+      for (int ii = A.i0 + (int)threadIdx.x; ii <= A.i1; ii += kBlock) {
+        double dy1 = AT(A.t, ii, jj + 1) - AT(A.t, ii, jj);
+        double dy2 = AT(A.t, ii, jj + 2) - AT(A.t, ii, jj + 1);
+        double dx1 = g.yp[jj + 1] - g.yp[jj];
+        double dx2 = g.yp[jj + 2] - g.yp[jj + 1];
+        double a = (dy2 / dx2 - dy1 / dx1) / (dx1 + dx2);
+        double b = dy2 / dx2 - a * (g.yp[jj + 1] + g.yp[jj + 2]);
+        double ye = w == 0 ? g.yp[jj] : g.yp[jj + 2];
+        der[ii] = 2.0 * a * ye + b;
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int ii = A.i0; ii <= A.i1; ii += 2) {
+          double x0 = g.xp[ii], x1 = g.xp[ii + 1], x2 = g.xp[ii + 2];
+          double y0 = der[ii], y1 = der[ii + 1], y2 = der[ii + 2];
+          double al = ((y2 - y1) / (x2 - x1) - (y1 - y0) / (x1 - x0)) / (x2 - x0);
+          double be = (y2 - y1) / (x2 - x1) - al * (x1 + x2);
+          double ga = y0 - al * x0 * x0 - be * x0;
+          acc = acc + al / 3.0 * (x2 * x2 * x2 - x0 * x0 * x0) + be / 2.0 * (x2 * x2 - x0 * x0) +
+                ga * (x2 - x0);
+        }
+        A.out[w] = -acc;
+      }
+      __syncthreads();
+    }
+  } else {
+    const double hgt = g.yv[A.nj] - g.yv[1];
+    for (int w = 0; w < 2; ++w) {
+      const int i1 = w == 0 ? 1 : A.mi - 1;
+      const double xw = w == 0 ? g.xu[1] : g.xu[A.mi];
+      for (int jj = 2 + (int)threadIdx.x; jj <= A.nj; jj += kBlock) {
+        double x1 = g.xp[i1], x2 = g.xp[i1 + 1], x3 = g.xp[i1 + 2];
+        double d1 = (AT(A.t, i1 + 1, jj) - AT(A.t, i1, jj)) / (x2 - x1);
+        double d2 = (AT(A.t, i1 + 2, jj) - AT(A.t, i1 + 1, jj)) / (x3 - x2);
+        double a = (d2 - d1) / (x3 - x1);
+        der[jj] = (d1 + a * (2.0 * xw - x1 - x2)) * g.dyp[jj];
+      }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int jj = 2; jj <= A.nj; ++jj) acc = acc + der[jj];
+        A.out[w] = -acc / hgt;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// --------------------------------------------------- batched tridiagonal
+template <int SOLVER, int EPT>
+__global__ void __launch_bounds__(kBlock) k_tridiag(double *a, double *b, double *c, double *r, int n) {
+  __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
+  const size_t base = (size_t)blockIdx.x * n;
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    sa[k] = a[base + k];
+    sb[k] = b[base + k];
+    sc_[k] = c[base + k];
+    sr[k] = r[base + k];
+  }
+  __syncthreads();
+  solve_line<SOLVER, EPT>(sa, sb, sc_, sr, n);
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    r[base + k] = sr[k];
+    b[base + k] = sb[k];
+  }
+}
+
+}  // namespace ixh

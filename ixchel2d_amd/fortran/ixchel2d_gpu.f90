@@ -1,0 +1,193 @@
+!
+! IXCHEL2D on one MI355X: the driver of upstream's source/main.f90 with the
+! time-step loop body on the GPU.
+!
+! Reads the same case directory upstream reads (parametros.dat, the three
+! mesh/initial-field files, cond_frontera{u,v,t}.dat) and writes the same
+! outputs: fort.102 ('SIMPLE', iteration, maxbo, residuo), the nuss_sim file
+! every 1000 iterations, and per package the out_*.dat restart files and the
+! exact .bin dumps when postprocesar is true.  The mesh extents come from the
+! files (no recompilation per mesh).  Environment: IXCHEL2D_SOLVER = pcr
+! (default) | thomas selects the line solver.
+!
+program ixchel2d_gpu
+  use iso_c_binding
+  use malla
+  use cond_frontera
+  use ec_momento
+  use ec_energia
+  use residuos
+  use frontera_inmersa
+  use postproceso
+  use ixchel_hip
+  implicit none
+
+  character(len=28) :: entrada_u, entrada_v, entrada_tp
+  character(len=8) :: adimen, flujo_ini, tempe_ini
+  character(len=16) :: solver_env
+  character(len=6) :: Rec
+  character(len=64) :: prefijo, nussf
+  logical :: postprocesar
+  real(DBL) :: Ra, Pr, Ri_1, dt, conv_resi, ao, tiempo_inicial
+  real(DBL) :: nusselt0, nusselt1, temp_med, residuo_ult, maxbo_ult
+  integer :: paq_itera, itermax, iter_ecuaci_max, iter_simple_max
+  integer :: placa_min, placa_max, itera_inicial, itera, itera_total, ll, resto, n, k, cap, un, ic
+  type(ixh_grid_t) :: grid
+  type(ixh_params_t) :: prm
+  type(ixh_hist_t), allocatable :: hist(:)
+  integer(c_int32_t) :: nrec
+
+  write(*,*) "------------------------------------------"
+  write(*,*) "          IXCHEL2D (MI355X, HIP)"
+  write(*,*) "------------------------------------------"
+  postprocesar = .false.
+  open(newunit=un, file='parametros.dat', status='old', action='read')
+  read(un, *) adimen
+  read(un, *) Ra
+  read(un, *) Pr
+  read(un, *) Ri_1
+  read(un, *) dt
+  read(un, *) paq_itera
+  read(un, *) itermax
+  read(un, *) rel_pres
+  read(un, *) rel_vel
+  read(un, *) rel_ener
+  read(un, *) conv_u
+  read(un, *) conv_t
+  read(un, *) conv_p
+  read(un, *) conv_resi
+  read(un, *) conv_paso
+  read(un, *) iter_ecuaci_max
+  read(un, *) iter_simple_max
+  read(un, *) entrada_u
+  read(un, *) entrada_v
+  read(un, *) entrada_tp
+  read(un, *) flujo_ini
+  read(un, *) tempe_ini
+  read(un, *) postprocesar
+  close(un)
+
+  ! non-dimensionalisation (main.f90:221-244)
+  if (trim(adimen) == 'natural') then
+     gamma_momen = sqrt(Pr/Ra)
+     gamma_energ = sqrt(1._DBL/(Pr*Ra))
+     Ri = 0.0_DBL
+     Riy = -1.0_DBL
+     Rec = entero_caracter(ceiling(sqrt(Ra)))
+  else if (trim(adimen) == 'mixta') then
+     gamma_momen = 1.0_DBL/Ra
+     gamma_energ = 1.0_DBL/(Ra*Pr)
+     Ri = 0.0_DBL
+     Riy = Ri_1
+     Rec = entero_caracter(ceiling(Ra))
+  else
+     error stop 'adimen must be natural or mixta'
+  end if
+  if (paq_itera < 1) error stop 'paq_itera must be positive'
+
+  call dimensiones_malla(entrada_u, entrada_tp)
+  mic = entero_caracter(mi)
+  njc = entero_caracter(nj)
+  call reserva_momento()
+  call reserva_energia()
+  call lectura_mallas_escalonadas(entrada_u, entrada_v, entrada_tp, u_ant, v_ant, pres, temp_ant, &
+       ao, placa_min, placa_max, itera_inicial)
+  tiempo_inicial = itera_inicial * dt
+  call condicion_inicial_flujo(flujo_ini)
+  call condicion_inicial_tempe(tempe_ini)
+  u = u_ant
+  v = v_ant
+  temp = temp_ant
+
+  ! device state (replaces the !$acc data region, main.f90:356-389)
+  grid % mi = mi
+  grid % nj = nj
+  grid % xu = c_loc(xu)
+  grid % yv = c_loc(yv)
+  grid % xp = c_loc(xp)
+  grid % yp = c_loc(yp)
+  prm % dt = dt
+  prm % rel_pres = rel_pres
+  prm % rel_vel = rel_vel
+  prm % rel_ener = rel_ener
+  prm % gamma_momen = gamma_momen
+  prm % gamma_energ = gamma_energ
+  prm % ri_x = Ri
+  prm % ri_y = Riy
+  prm % conv_u = conv_u
+  prm % conv_t = conv_t
+  prm % conv_p = conv_p
+  prm % conv_resi = conv_resi
+  prm % conv_paso = conv_paso
+  prm % iter_ecuaci_max = iter_ecuaci_max
+  prm % iter_simple_max = iter_simple_max
+  prm % solver = IXH_SOLVER_PCR
+  prm % precision = IXH_FP64
+  call get_environment_variable('IXCHEL2D_SOLVER', solver_env)
+  if (trim(solver_env) == 'thomas') prm % solver = IXH_SOLVER_THOMAS
+  call ixh_check(ixh_create(grid, prm, estado), 'ixh_create')
+  call ixh_check(ixh_upload_field(estado, IXH_F_U_ANT, u_ant, mi), 'upload u_ant')
+  call ixh_check(ixh_upload_field(estado, IXH_F_V_ANT, v_ant, mi+1), 'upload v_ant')
+  call ixh_check(ixh_upload_field(estado, IXH_F_T_ANT, temp_ant, mi+1), 'upload temp_ant')
+  call ixh_check(ixh_upload_field(estado, IXH_F_U, u, mi), 'upload u')
+  call ixh_check(ixh_upload_field(estado, IXH_F_V, v, mi+1), 'upload v')
+  call ixh_check(ixh_upload_field(estado, IXH_F_T, temp, mi+1), 'upload temp')
+  call ixh_check(ixh_upload_field(estado, IXH_F_P, pres, mi+1), 'upload pres')
+  call ini_frontera_uv()
+  call ini_frontera_t()
+  call definir_cuerpo('sincu')
+  write(*,'(1X,A,I6,A,I6,A,A)') 'mi=', mi, ', nj=', nj, ', Re=', Rec
+
+  ! SIMPLE records per time step are at most iter_simple_max + 2
+  cap = 1000 * max(1, iter_simple_max + 2)
+  allocate(hist(cap))
+  open(unit=102, file='fort.102', status='replace')
+  nussf = 'nuss_sim_n'//trim(njc)//'m'//trim(mic)//'_R'//trim(Rec)//'.dat'
+  itera = 0
+  residuo_ult = 0.0_DBL
+  maxbo_ult = 0.0_DBL
+  do ll = 1, itermax / paq_itera
+     resto = paq_itera
+     do while (resto > 0)
+        ! stop at every multiple of 500 (progress line) and 1000 (Nusselt)
+        n = min(resto, 500 - mod(itera, 500), cap / max(1, iter_simple_max + 2))
+        call ixh_check(ixh_simple_iterations(estado, int(n, c_int32_t), hist, int(cap, c_int32_t), nrec), &
+             'ixh_simple_iterations')
+        do k = 1, min(int(nrec), cap)
+           if (hist(k) % logged /= 0) write(102, *) 'SIMPLE', hist(k) % simple_iter, hist(k) % maxbo, &
+                hist(k) % residuo
+        end do
+        if (nrec > 0) then
+           residuo_ult = hist(min(int(nrec), cap)) % residuo
+           maxbo_ult = hist(min(int(nrec), cap)) % maxbo
+        end if
+        itera = itera + n
+        resto = resto - n
+        if (mod(itera, 500) == 0) write(*,*) 'ITERACION: ', itera, residuo_ult, maxbo_ult
+        if (mod(itera, 1000) == 0) then
+           call nusselt_promedio_y(nusselt0, nusselt1, placa_min, placa_max)
+           call ixh_check(ixh_download_field(estado, IXH_F_T, temp, mi+1), 'download temp')
+           ic = (placa_min + placa_max) / 2
+           temp_med = (temp(ic, nj/2+1) + temp(ic+1, nj/2+1)) / 2._DBL
+           open(newunit=un, file=trim(nussf), position='append')
+           write(un, form26) tiempo_inicial + itera*dt, nusselt0, -nusselt1, temp_med, 0.0_DBL, 0.0_DBL
+           close(un)
+        end if
+     end do
+     ! end of the package: fields back to the host (main.f90:1277-1354)
+     call ixh_check(ixh_download_field(estado, IXH_F_U, u, mi), 'download u')
+     call ixh_check(ixh_download_field(estado, IXH_F_V, v, mi+1), 'download v')
+     call ixh_check(ixh_download_field(estado, IXH_F_T, temp, mi+1), 'download temp')
+     call ixh_check(ixh_download_field(estado, IXH_F_P, pres, mi+1), 'download pres')
+     itera_total = itera_inicial + itera
+     write(*,*) 'itera_total=', itera_total
+     write(*,'(1X,A,D23.15,A,D23.15)') 'b_o  =', maxbo_ult, ', Res_u=', residuo_ult
+     if (postprocesar) then
+        prefijo = 'out_n'//trim(njc)//'m'//trim(mic)//'_R'//trim(Rec)
+        call postproceso_dat(trim(prefijo), u, v, temp, pres, placa_min, placa_max, itera_total, ao)
+        call postproceso_bin(trim(prefijo), u, v, temp, pres)
+     end if
+  end do
+  close(102)
+  call ixh_check(ixh_destroy(estado), 'ixh_destroy')
+end program ixchel2d_gpu

@@ -1,0 +1,206 @@
+"""Python host for the SIMPLE iteration on one MI355X.
+
+``Cavity`` owns one device state (``ixh_state``) and exposes the phases of
+upstream's time-step loop body (source/main.f90:391-1271) under the names of
+the C-ABI; ``Cavity.run`` is the whole loop with upstream's convergence
+control.  Every method goes through libixchel2d.so; nothing here computes a
+field on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Dict, Optional
+
+import numpy as np
+
+from . import _lib
+from .case import Case
+
+_DP = C.POINTER(C.c_double)
+
+
+def _f64(a) -> np.ndarray:
+    return np.require(np.asarray(a, dtype=np.float64), requirements=["F", "A", "W"])
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(_DP)
+
+
+def params_of(case: Case, solver: str = "pcr", precision: str = "fp64") -> _lib.Params:
+    p = case.params
+    d = p.derived()
+    return _lib.Params(p.dt, p.rel_pres, p.rel_vel, p.rel_ener, d["gamma_momen"], d["gamma_energ"],
+                       d["Ri"], d["Riy"], p.conv_u, p.conv_t, p.conv_p, p.conv_resi, p.conv_paso,
+                       p.iter_ecuaci_max, p.iter_simple_max, _lib.SOLVER_ID[solver],
+                       _lib.PRECISION_ID[precision])
+
+
+class Cavity:
+    """One cavity resident on the GPU."""
+
+    def __init__(self, case: Case, solver: str = "pcr", precision: str = "fp64", upload: bool = True):
+        L = _lib.load()
+        self.L = L
+        self.case = case
+        m = case.mesh
+        self.mi, self.nj = m.mi, m.nj
+        self._geo = [np.ascontiguousarray(x, dtype=np.float64) for x in (m.xu, m.yv, m.xp, m.yp)]
+        grid = _lib.Grid(m.mi, m.nj, *[_ptr(x) for x in self._geo])
+        self._params = params_of(case, solver, precision)
+        h = C.c_void_p()
+        _lib.check(L.ixh_create(C.byref(grid), C.byref(self._params), C.byref(h)))
+        self.h = h
+        for var, bset in case.bcs.items():
+            for side, (typ, val) in bset.expand(m).items():
+                t = np.ascontiguousarray(typ, dtype=np.int32)
+                v = np.ascontiguousarray(val, dtype=np.float64)
+                _lib.check(L.ixh_set_bc(self.h, _lib.VAR_ID[var], _lib.SIDE_ID[side],
+                                        t.ctypes.data_as(C.POINTER(C.c_int32)), _ptr(v)))
+        for var, (sc, sp) in case.sources.items():
+            self.set_sources(var, sc, sp)
+        if upload:
+            u, v, t, p = case.initial_fields()
+            for name, arr in (("u_ant", u), ("v_ant", v), ("t_ant", t), ("u", u), ("v", v), ("t", t), ("p", p)):
+                self.set(name, arr)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ixh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------------- state
+    def shape(self, name: str):
+        mi, nj = self.mi, self.nj
+        if name in ("u", "u_ant", "au", "resu"):
+            return (mi, nj + 1)
+        if name in ("v", "v_ant", "av"):
+            return (mi + 1, nj)
+        return (mi + 1, nj + 1)
+
+    def set(self, name: str, arr) -> None:
+        a = _f64(arr)
+        if a.shape != self.shape(name):
+            raise ValueError(f"{name}: shape {a.shape} != {self.shape(name)}")
+        _lib.check(self.L.ixh_upload_field(self.h, _lib.FIELD_ID[name], _ptr(a), a.shape[0]))
+
+    def get(self, name: str) -> np.ndarray:
+        out = np.zeros(self.shape(name), dtype=np.float64, order="F")
+        _lib.check(self.L.ixh_download_field(self.h, _lib.FIELD_ID[name], _ptr(out), out.shape[0]))
+        return out
+
+    def fields(self) -> Dict[str, np.ndarray]:
+        return {n: self.get(n) for n in _lib.FIELD_ID}
+
+    def set_sources(self, var: str, sc, sp) -> None:
+        if sc is None and sp is None:
+            _lib.check(self.L.ixh_set_sources(self.h, _lib.VAR_ID[var], None, None))
+            return
+        a, b = _f64(sc), _f64(sp)
+        _lib.check(self.L.ixh_set_sources(self.h, _lib.VAR_ID[var], _ptr(a), _ptr(b)))
+
+    def set_params(self, **kw) -> None:
+        for k, v in kw.items():
+            if k == "solver":
+                v = _lib.SOLVER_ID[v]
+            setattr(self._params, k, v)
+        _lib.check(self.L.ixh_set_params(self.h, C.byref(self._params)))
+
+    def set_stream(self, stream_handle: Optional[int]) -> None:
+        _lib.check(self.L.ixh_set_stream(self.h, C.c_void_p(stream_handle) if stream_handle else None))
+
+    def synchronize(self) -> None:
+        _lib.check(self.L.ixh_synchronize(self.h))
+
+    # --------------------------------------------------------------- phases
+    def momentum_u(self, direction: int) -> None:
+        _lib.check(self.L.ixh_momentum_u(self.h, direction))
+
+    def momentum_v(self, direction: int) -> Optional[float]:
+        e = C.c_double()
+        _lib.check(self.L.ixh_momentum_v(self.h, direction, C.byref(e) if direction == _lib.DIR_X else None))
+        return e.value if direction == _lib.DIR_X else None
+
+    def pcorr_zero(self) -> None:
+        _lib.check(self.L.ixh_pcorr_zero(self.h))
+
+    def pcorr(self, direction: int):
+        e, b = C.c_double(), C.c_double()
+        if direction == _lib.DIR_X:
+            _lib.check(self.L.ixh_pcorr(self.h, direction, C.byref(e), C.byref(b)))
+            return e.value, b.value
+        _lib.check(self.L.ixh_pcorr(self.h, direction, None, None))
+        return None
+
+    def correct(self) -> None:
+        _lib.check(self.L.ixh_correct(self.h))
+
+    def energy(self, direction: int) -> Optional[float]:
+        e = C.c_double()
+        _lib.check(self.L.ixh_energy(self.h, direction, C.byref(e) if direction == _lib.DIR_X else None))
+        return e.value if direction == _lib.DIR_X else None
+
+    def residual_u(self) -> float:
+        r = C.c_double()
+        _lib.check(self.L.ixh_residual_u(self.h, C.byref(r)))
+        return r.value
+
+    def lag(self) -> None:
+        _lib.check(self.L.ixh_lag(self.h))
+
+    def run(self, nsteps: int, cap: Optional[int] = None) -> Dict[str, np.ndarray]:
+        """``nsteps`` time steps of the upstream loop; one record per SIMPLE
+        iteration."""
+        cap = max(1, nsteps * 4 if cap is None else cap)
+        hist = (_lib.Hist * cap)()
+        n = C.c_int32()
+        _lib.check(self.L.ixh_simple_iterations(self.h, nsteps, hist, cap, C.byref(n)))
+        k = min(n.value, cap)
+        rec = {f: np.array([getattr(hist[i], f) for i in range(k)])
+               for f in ("err_v", "err_p", "maxbo", "err_t", "residuo")}
+        for f in ("passes_mom", "passes_p", "passes_t", "simple_iter", "logged"):
+            rec[f] = np.array([getattr(hist[i], f) for i in range(k)], dtype=np.int32)
+        return rec
+
+    def nusselt_y(self, i0: Optional[int] = None, i1: Optional[int] = None):
+        m = self.case.mesh
+        a, b = C.c_double(), C.c_double()
+        _lib.check(self.L.ixh_nusselt_y(self.h, m.placa_min if i0 is None else i0,
+                                        m.placa_max if i1 is None else i1, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def nusselt_x(self):
+        a, b = C.c_double(), C.c_double()
+        _lib.check(self.L.ixh_nusselt_x(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    # -------------------------------------------------------------- timing
+    def set_profiling(self, on: bool) -> None:
+        _lib.check(self.L.ixh_set_profiling(self.h, 1 if on else 0))
+
+    def kernel_times(self) -> Dict[str, tuple]:
+        ms = (C.c_double * 32)()
+        cnt = (C.c_int32 * 32)()
+        nk = C.c_int32()
+        _lib.check(self.L.ixh_kernel_times(self.h, ms, cnt, 32, C.byref(nk)))
+        return {self.L.ixh_kernel_name(k).decode(): (ms[k], cnt[k]) for k in range(nk.value)}
+
+
+def tridiagonal(a, b, c, r, solver: str = "pcr"):
+    """Batched ``tridiagonal(a,b,c,r,n)`` on the device; the last axis is the
+    system.  Returns (solution, overwritten b)."""
+    L = _lib.load()
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64).copy()
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64).copy()
+    n = r.shape[-1]
+    nsys = r.size // n if n else 0
+    _lib.check(L.ixh_tridiagonal(_ptr(a), _ptr(b), _ptr(c), _ptr(r), n, nsys, _lib.SOLVER_ID[solver]))
+    return r, b

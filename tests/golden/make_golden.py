@@ -1,0 +1,77 @@
+"""Write the golden fixtures by running the upstream Ixchel2D program.
+
+Runs in the build container only (it needs /root/reference to build the
+upstream binaries with oracle/build_ref.sh).  For each case of
+tests/golden/cases.py it writes the case files in upstream's formats into a
+scratch directory, runs the serial flang build there, and stores the exact
+FP64 fields (from upstream's stream-access .bin dumps), the fort.102 history
+(maxbo, residuo per logged SIMPLE iteration) and the Nusselt records as
+tests/golden/<case>.npz.  The full-size case stores a subsample of each field
+plus its centre lines and norms.
+
+usage: python tests/golden/make_golden.py [case ...]
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+import cases as K  # noqa: E402
+
+ROOT = K.ROOT
+from ixchel2d_amd import case as C  # noqa: E402
+
+
+def ref_binary(mi: int, nj: int) -> str:
+    out = subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh"), str(mi), str(nj)],
+                         check=True, stdout=subprocess.PIPE, text=True).stdout.strip().splitlines()[-1]
+    return out
+
+
+def run_case(c: C.Case, workdir: str) -> dict:
+    C.write_reference_case(c, workdir)
+    exe = ref_binary(c.mesh.mi, c.mesh.nj)
+    t0 = time.time()
+    with open(os.path.join(workdir, "stdout.txt"), "w") as log:
+        subprocess.run([exe], cwd=workdir, stdout=log, stderr=subprocess.STDOUT, check=True)
+    out = C.read_reference_outputs(c, workdir)
+    out["wall_seconds"] = np.array(time.time() - t0)
+    return out
+
+
+def make(name: str) -> None:
+    c = K.CASES[name]()
+    work = tempfile.mkdtemp(prefix=f"golden_{name}_")
+    try:
+        out = run_case(c, work)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    store = {"history": out["history"], "nusselt": out["nusselt"], "wall_seconds": out["wall_seconds"],
+             "itermax": np.array(c.params.itermax)}
+    if name == "nc301x701":
+        for f in ("u", "v", "t", "p"):
+            a = out[f]
+            store[f + "_sub"] = K.subsample(a)
+            store[f + "_xmid"] = np.ascontiguousarray(a[a.shape[0] // 2, :])
+            store[f + "_ymid"] = np.ascontiguousarray(a[:, a.shape[1] // 2])
+            store[f + "_l2"] = np.array(np.sqrt(np.sum(a * a)))
+            store[f + "_sum"] = np.array(np.sum(a))
+    else:
+        for f in ("u", "v", "t", "p"):
+            store[f] = out[f]
+    np.savez_compressed(K.fixture_path(name), **store)
+    print(f"{name}: {K.fixture_path(name)} ({float(out['wall_seconds']):.1f} s upstream)", flush=True)
+
+
+if __name__ == "__main__":
+    names = sys.argv[1:] or [n for n in K.CASES if n != "nc301x701"]
+    for n in names:
+        make(n)

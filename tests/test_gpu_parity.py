@@ -1,0 +1,186 @@
+"""GPU parity: the HIP path through the C-ABI (libixchel2d.so) against the
+CPU oracle and the upstream golden fixtures.
+
+Tolerances: with the Thomas line solver the device arithmetic follows
+upstream operation by operation (no FMA contraction, IEEE division), so
+fields must be bit-identical; with the PCR line solver only the line-solve
+rounding differs and fields must agree to 1e-10 relative (the FP64 target),
+norms to 1e-9.
+"""
+import numpy as np
+import pytest
+
+import cases as K
+from ixchel2d_amd import case as C
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+
+
+def _cavity(case, solver):
+    from ixchel2d_amd.simple import Cavity
+    return Cavity(case, solver=solver)
+
+
+def relerr(a, b):
+    scale = max(float(np.max(np.abs(b))), 1e-300)
+    return float(np.max(np.abs(a - b))) / scale
+
+
+PHASES = [
+    ("momentum_u_y", lambda g: g.momentum_u(0), ["u", "au"]),
+    ("momentum_u_x", lambda g: g.momentum_u(1), ["u", "au"]),
+    ("momentum_v_y", lambda g: g.momentum_v(0), ["v", "av"]),
+    ("momentum_v_x", lambda g: g.momentum_v(1), ["v", "av"]),
+    ("pcorr_zero", lambda g: g.pcorr_zero(), []),
+    ("pcorr_y", lambda g: g.pcorr(0), ["pcorr"]),
+    ("pcorr_x", lambda g: g.pcorr(1), ["pcorr", "bo"]),
+    ("correct", lambda g: g.correct(), ["u", "v", "p"]),
+    ("energy_y", lambda g: g.energy(0), ["t"]),
+    ("energy_x", lambda g: g.energy(1), ["t"]),
+    ("residual_u", lambda g: g.residual_u(), ["resu"]),
+    ("lag", lambda g: g.lag(), ["u_ant", "v_ant", "t_ant"]),
+]
+
+
+def _warm_states(case, steps):
+    """Oracle and GPU states with the same non-trivial fields: the oracle
+    runs `steps` iterations and the GPU state receives its fields."""
+    o = Oracle(case)
+    if steps:
+        o.run(steps)
+    return o
+
+
+@pytest.mark.parametrize("name,steps", [("nc18x14", 3), ("mixed22x17", 2)])
+@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+def test_phase_by_phase(name, steps, solver):
+    case = K.CASES[name]()
+    o = _warm_states(case, steps)
+    g = _cavity(case, solver)
+    for f in ("u", "v", "t", "p", "u_ant", "v_ant", "t_ant", "au", "av"):
+        g.set(f, o.get(f))
+    for pname, call, outs in PHASES:
+        o.phase(pname if pname != "residual_u" else "residual_u")
+        call(g)
+        for f in outs:
+            a, b = g.get(f), o.get(f)
+            if solver == "thomas":
+                assert np.array_equal(a, b), f"{name}/{pname}: {f} not bit-identical ({relerr(a, b):.2e})"
+            else:
+                assert relerr(a, b) <= 1e-12, f"{name}/{pname}: {f} rel err {relerr(a, b):.2e}"
+            g.set(f, b)  # resynchronise so each phase is judged on its own
+
+
+@pytest.mark.parametrize("name", ["nc18x14", "mixed22x17", "ctrl18x14", "lid66", "nc34"])
+@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+def test_run_against_upstream_fixture(name, solver):
+    case = K.CASES[name]()
+    gold = K.load_fixture(name)
+    g = _cavity(case, solver)
+    h = g.run(case.params.itermax)
+    for f in "uvtp":
+        a = g.get(f)
+        if solver == "thomas":
+            assert np.array_equal(a, gold[f]), f"{name}: {f} differs from upstream ({relerr(a, gold[f]):.2e})"
+        else:
+            assert relerr(a, gold[f]) <= REL, f"{name}: {f} rel err {relerr(a, gold[f]):.2e}"
+    logged = h["logged"].astype(bool)
+    hist = np.c_[h["maxbo"], h["residuo"]][logged]
+    assert hist.shape == gold["history"].shape, "SIMPLE iteration count differs from upstream"
+    np.testing.assert_allclose(hist, gold["history"], rtol=1e-9, atol=1e-15)
+
+
+def test_control_flow_pass_counts_match_oracle():
+    case = K.ctrl18x14()
+    g = _cavity(case, "thomas")
+    o = Oracle(case)
+    hg = g.run(case.params.itermax)
+    ho = o.run(case.params.itermax)
+    for k in ("passes_mom", "passes_p", "passes_t", "simple_iter", "logged"):
+        assert np.array_equal(hg[k], ho[k]), k
+    for k in ("err_v", "err_p", "maxbo", "err_t", "residuo"):
+        np.testing.assert_allclose(hg[k], ho[k], rtol=1e-9, atol=1e-15)
+
+
+def test_nusselt_against_oracle_and_upstream():
+    case = K.nc34()
+    gold = K.load_fixture("nc34")
+    g = _cavity(case, "thomas")
+    g.run(1000)
+    o = Oracle(case)
+    o.run(1000)
+    assert g.nusselt_y() == o.nusselt_y()
+    assert g.nusselt_x() == o.nusselt_x()
+    n0, n1 = g.nusselt_y()
+    assert n0 == pytest.approx(gold["nusselt"][0, 1], rel=1e-14, abs=1e-17)
+    assert -n1 == pytest.approx(gold["nusselt"][0, 2], rel=1e-14, abs=1e-17)
+
+
+def test_immersed_cylinder_sources():
+    case = C.natural_convection_case(40, 48, height=1.2, Ra=1e5, iters=30, dt=0.01, seed=5)
+    C.cylinder_sources(case, 0.55, 0.6, 0.15, t_body=1.0)
+    g = _cavity(case, "thomas")
+    o = Oracle(case)
+    g.run(30)
+    o.run(30)
+    for f in "uvtp":
+        assert np.array_equal(g.get(f), o.get(f)), f
+    # inside the body the penalty pins T to the body temperature
+    m = case.mesh
+    X, Y = np.meshgrid(m.xp, m.yp, indexing="ij")
+    inside = (X - 0.55) ** 2 + (Y - 0.6) ** 2 <= 0.15 ** 2
+    t = g.get("t")
+    assert np.max(np.abs(t[inside] - 1.0)) < 1e-12
+
+
+@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+def test_tridiagonal_against_dense_solve(solver):
+    from ixchel2d_amd.simple import tridiagonal
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 7, 256, 257, 702, 2048):
+        nsys = 5
+        a = rng.uniform(-1, 0, (nsys, n))
+        c = rng.uniform(-1, 0, (nsys, n))
+        b = 2.2 + rng.uniform(0, 1, (nsys, n))
+        r = rng.standard_normal((nsys, n))
+        x, _ = tridiagonal(a, b, c, r, solver=solver)
+        for k in range(nsys):
+            if n == 1:
+                np.testing.assert_allclose(x[k], r[k] / b[k], rtol=1e-15)
+                continue
+            A = np.diag(b[k]) + np.diag(a[k, 1:], -1) + np.diag(c[k, :-1], 1)
+            np.testing.assert_allclose(x[k], np.linalg.solve(A, r[k]), rtol=1e-12, atol=1e-13)
+        if solver == "thomas":
+            for k in range(nsys):
+                from oracle.pyoracle import tridiagonal as otri
+                xo, bo = otri(a[k], b[k], c[k], r[k])
+                assert np.array_equal(x[k], xo)
+
+
+def test_full_size_5000_iterations_against_upstream():
+    """301x701, 5000 SIMPLE iterations (tanh mesh, Ra=1e5): PCR path vs the
+    upstream CPU build, on the stored subsample, centre lines and norms."""
+    case = K.nc301x701()
+    gold = K.load_fixture("nc301x701")
+    g = _cavity(case, "pcr")
+    h = g.run(5000)
+    for f in "uvtp":
+        a = g.get(f)
+        sub = K.subsample(a)
+        assert relerr(sub, gold[f + "_sub"]) <= REL, f"{f}: {relerr(sub, gold[f + '_sub']):.2e}"
+        assert relerr(a[a.shape[0] // 2, :], gold[f + "_xmid"]) <= REL
+        assert relerr(a[:, a.shape[1] // 2], gold[f + "_ymid"]) <= REL
+        assert float(np.sqrt(np.sum(a * a))) == pytest.approx(float(gold[f + "_l2"]), rel=REL)
+    hist = np.c_[h["maxbo"], h["residuo"]]
+    np.testing.assert_allclose(hist, gold["history"], rtol=1e-8, atol=1e-15)
