@@ -92,6 +92,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--solver", default="pcr", choices=["pcr", "thomas"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--calibrate-copy-mb", type=int, default=0,
+                    help="first run one device-to-device copy of this many MB (reference kernel for "
+                         "the FETCH_SIZE / WRITE_SIZE counter scale)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -103,6 +106,14 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.calibrate_copy_mb:
+        n = args.calibrate_copy_mb * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     case = build_case()
     m = case.mesh
     cells = m.mi * m.nj

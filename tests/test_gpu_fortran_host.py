@@ -1,0 +1,41 @@
+"""The Fortran host (ixchel2d_amd/fortran/ixchel2d_gpu) as a drop-in for the
+upstream program: the same case directory in, the same fort.102 history and
+the same exact .bin field dumps out (Thomas line solver: bit-identical; PCR:
+1e-10 relative)."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases as K
+from ixchel2d_amd import case as C
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "ixchel2d_amd", "fortran", "ixchel2d_gpu")
+
+
+@pytest.mark.parametrize("name", ["nc18x14", "mixed22x17", "ctrl18x14", "nc34"])
+@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+def test_fortran_host_reproduces_upstream_outputs(name, solver, tmp_path):
+    assert os.path.exists(EXE), "Fortran host not built"
+    c = K.CASES[name]()
+    gold = K.load_fixture(name)
+    C.write_reference_case(c, str(tmp_path))
+    env = dict(os.environ, IXCHEL2D_SOLVER=solver)
+    r = subprocess.run([EXE], cwd=tmp_path, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:]
+    out = C.read_reference_outputs(c, str(tmp_path))
+    for f in "uvtp":
+        if solver == "thomas":
+            assert np.array_equal(out[f], gold[f]), f
+        else:
+            rel = np.max(np.abs(out[f] - gold[f])) / max(np.max(np.abs(gold[f])), 1e-300)
+            assert rel <= 1e-10, (f, rel)
+    assert out["history"].shape == gold["history"].shape
+    np.testing.assert_allclose(out["history"], gold["history"], rtol=1e-9, atol=1e-15)
+    if gold["nusselt"].size:
+        np.testing.assert_allclose(out["nusselt"][:, :3], gold["nusselt"][:, :3], rtol=1e-12, atol=1e-16)
