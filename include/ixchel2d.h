@@ -53,8 +53,9 @@ enum { IXH_BC_NONE = 0, IXH_BC_DIRI = 1, IXH_BC_NEUM = 2 };
 enum { IXH_DIR_Y = 0, IXH_DIR_X = 1 };
 /* line solvers */
 enum {
-  IXH_SOLVER_PCR = 0,    /* parallel cyclic reduction in LDS (default) */
-  IXH_SOLVER_THOMAS = 1  /* one lane per line, upstream's Thomas order */
+  IXH_SOLVER_PCR = 0,      /* PCR in LDS, division-free assembly (default) */
+  IXH_SOLVER_THOMAS = 1,   /* upstream's Thomas and assembly order: bit-faithful */
+  IXH_SOLVER_PCR_EXACT = 2 /* PCR with upstream-order assembly */
 };
 /* arithmetic of the sweeps */
 enum { IXH_FP64 = 0, IXH_FP32_MIXED = 1 };

@@ -63,6 +63,9 @@ struct ixh_state {
   // geometry (one allocation, 1-based arrays)
   double *d_geo = nullptr;
   Geo geo{};
+  std::vector<double> h_dx[6];  // host copies of dxp, dyp, dxu, dyu, dxv, dyv (1-based)
+  double *d_fast = nullptr;     // FastGeo tables
+  FastGeo fast{};
   // solved fields with rotation buffers
   double *ub[kNbuf]{}, *vb[kNbuf]{}, *tb[kNbuf]{}, *pcb[kNbuf]{};
   int u_cur = 0, u_ant = 0, v_cur = 0, v_ant = 0, t_cur = 0, t_ant = 0, pc_cur = 0;
@@ -134,7 +137,7 @@ int check_params(const ixh_params *p) {
     return fail(IXH_ERR_ARG, "dt and relaxation factors must be positive");
   if (!(p->gamma_momen > 0) || !(p->gamma_energ > 0))
     return fail(IXH_ERR_ARG, "diffusivities must be positive");
-  if (p->solver != IXH_SOLVER_PCR && p->solver != IXH_SOLVER_THOMAS)
+  if (p->solver != IXH_SOLVER_PCR && p->solver != IXH_SOLVER_THOMAS && p->solver != IXH_SOLVER_PCR_EXACT)
     return fail(IXH_ERR_ARG, "unknown solver %d", p->solver);
   if (p->precision != IXH_FP64)
     return fail(IXH_ERR_UNSUPP, "precision %d is not built in this library", p->precision);
@@ -193,7 +196,49 @@ int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
 template <int EQ, int DIR>
 int launch_sweep(ixh_state *s, const SweepArgs &A, int n) {
   if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_t<EQ, DIR, SOLVE_THOMAS>(s, A, n);
+  if (s->prm.solver == IXH_SOLVER_PCR_EXACT) return launch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT>(s, A, n);
   return launch_sweep_t<EQ, DIR, SOLVE_PCR>(s, A, n);
+}
+
+// FastGeo tables for the current diffusivities, dt and relaxation factors
+int build_fast(ixh_state *s) {
+  const int mi = s->mi, nj = s->nj, gx = mi + 2, gy = nj + 2;
+  const std::vector<double> &dxp = s->h_dx[0], &dyp = s->h_dx[1], &dxu = s->h_dx[2], &dyv = s->h_dx[5];
+  const double gm = s->prm.gamma_momen, gt = s->prm.gamma_energ;
+  const size_t per = 4 * (size_t)gx + 4 * (size_t)gy;
+  std::vector<double> h(3 * per, 0.0);
+  auto q = [](double a, double b) { return b != 0.0 ? a / b : 0.0; };
+  for (int e = 0; e < 3; ++e) {
+    double *cw = &h[e * per], *pw = cw + gx, *ce = pw + gx, *pe = ce + gx;
+    double *cs = pe + gx, *ps = cs + gy, *cn = ps + gy, *pn = cn + gy;
+    const double G = e == 2 ? gt : gm;
+    for (int i = 2; i <= mi; ++i) {
+      // west/east distances: u between u-nodes (dxp), v and T between p-nodes (dxu)
+      const double dw = e == 0 ? dxp[i] : dxu[i - 1];
+      const double de = e == 0 ? (i + 1 <= mi ? dxp[i + 1] : 0.0) : dxu[i];
+      cw[i] = q(G, dw); pw[i] = q(dw, G);
+      ce[i] = q(G, de); pe[i] = q(de, G);
+    }
+    for (int j = 2; j <= nj; ++j) {
+      // south/north distances: u and T between p-rows (dyv), v between v-rows (dyp)
+      const double ds = e == 1 ? dyp[j] : dyv[j - 1];
+      const double dn = e == 1 ? (j + 1 <= nj ? dyp[j + 1] : 0.0) : dyv[j];
+      cs[j] = q(G, ds); ps[j] = q(ds, G);
+      cn[j] = q(G, dn); pn[j] = q(dn, G);
+    }
+  }
+  if (!s->d_fast) CHK(dmalloc(&s->d_fast, 3 * per));
+  HIPCHK(hipMemcpy(s->d_fast, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  for (int e = 0; e < 3; ++e) {
+    const double *b = s->d_fast + e * per;
+    s->fast.cw[e] = b; s->fast.pw[e] = b + gx; s->fast.ce[e] = b + 2 * gx; s->fast.pe[e] = b + 3 * gx;
+    b += 4 * (size_t)gx;
+    s->fast.cs[e] = b; s->fast.ps[e] = b + gy; s->fast.cn[e] = b + 2 * gy; s->fast.pn[e] = b + 3 * gy;
+  }
+  s->fast.inv_dt = 1.0 / s->prm.dt;
+  s->fast.inv_rel_vel = 1.0 / s->prm.rel_vel;
+  s->fast.inv_rel_ener = 1.0 / s->prm.rel_ener;
+  return 0;
 }
 
 SweepArgs base_args(ixh_state *s) {
@@ -204,6 +249,7 @@ SweepArgs base_args(ixh_state *s) {
   A.P = s->P;
   A.g = s->geo;
   A.ph = phys_of(s);
+  A.fg = s->fast;
   A.u = s->ub[s->u_cur];
   A.v = s->vb[s->v_cur];
   A.t = s->tb[s->t_cur];
@@ -394,6 +440,7 @@ int do_correct(ixh_state *s) {
 
 int do_residual(ixh_state *s) {
   ResArgs A;
+  memset(&A, 0, sizeof A);
   A.mi = s->mi;
   A.nj = s->nj;
   A.P = s->P;
@@ -571,6 +618,7 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   for (int j = 1; j <= nj; ++j) feyp[j] = (yp[j + 1] - yv[j]) / (yp[j + 1] - yp[j]);
   for (int i = 1; i <= mi - 1; ++i) fexu[i] = (xu[i + 1] - xp[i + 1]) / (xu[i + 1] - xu[i]);
   for (int j = 1; j <= nj - 1; ++j) feyv[j] = (yv[j + 1] - yp[j + 1]) / (yv[j + 1] - yv[j]);
+  s->h_dx[0] = dxp; s->h_dx[1] = dyp; s->h_dx[2] = dxu; s->h_dx[3] = dyu; s->h_dx[4] = dxv; s->h_dx[5] = dyv;
   const std::vector<double> *xs[] = {&dxp, &dxu, &dxv, &fexp, &fexu, &xu, &xp};
   const std::vector<double> *ys[] = {&dyp, &dyu, &dyv, &feyp, &feyv, &yv, &yp};
   const size_t ngeo = 7 * (size_t)gx + 7 * (size_t)gy;
@@ -606,6 +654,7 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   s->plen = (mi > nj ? mi : nj) + 4;
   if ((rc = dmalloc(&s->part, (size_t)5 * s->plen)) || (rc = dmalloc(&s->d_out, 8))) return bail(rc);
   if (hipMalloc((void **)&s->d_slot, sizeof(int)) != hipSuccess) return bail(fail(IXH_ERR_NOMEM, "slot"));
+  if ((rc = build_fast(s))) return bail(rc);
   for (int v = 0; v < 3; ++v)
     for (int sd = 0; sd < 4; ++sd) {
       const int n = side_len(s, v, sd);
@@ -640,6 +689,7 @@ int ixh_destroy(ixh_state *s) {
     }
   }
   if (s->d_slot) (void)hipFree(s->d_slot);
+  if (s->d_fast) (void)hipFree(s->d_fast);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
@@ -652,7 +702,7 @@ int ixh_set_params(ixh_state *s, const ixh_params *params) {
   CHK(check_params(params));
   s->prm = *params;
   drop_graphs(s);  // kernel arguments carry the scalars
-  return 0;
+  return build_fast(s);
 }
 
 int ixh_set_stream(ixh_state *s, void *stream) {

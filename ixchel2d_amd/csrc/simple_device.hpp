@@ -26,7 +26,10 @@ constexpr int kNmax = 2048;   // longest line the LDS solvers hold
 
 enum Eq { EQ_U = 0, EQ_V = 1, EQ_P = 2, EQ_T = 3 };
 enum Dir { DIR_Y = 0, DIR_X = 1 };
-enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1 };
+// PCR: LDS cyclic reduction with the division-free assembly below (default);
+// THOMAS: upstream's Thomas order with upstream-order assembly (bit-faithful);
+// PCR_EXACT: PCR with upstream-order assembly (isolates the solver rounding)
+enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1, SOLVE_PCR_EXACT = 2 };
 
 // 1-based geometry arrays (index 0 unused), built as in
 // source/mod_malla.f90:167-217
@@ -47,11 +50,22 @@ struct Side {
   const double *val;
 };
 
+// Geometry-only factors of the division-free assembly, one set per equation
+// (0 = u, 1 = v, 2 = T): for each face the conductance G/d and the Peclet
+// factor d/G (constant G), west/east faces indexed by ii, south/north by jj.
+// Upstream computes G*L/d and w*d/G with divisions in every cell; these
+// tables leave only multiplications (results move by a few ulp).
+struct FastGeo {
+  const double *cw[3], *pw[3], *ce[3], *pe[3], *cs[3], *ps[3], *cn[3], *pn[3];
+  double inv_dt, inv_rel_vel, inv_rel_ener;
+};
+
 struct SweepArgs {
   int mi, nj, P;          // mesh extents and the common row pitch of all fields
   int line0, nlines;      // first line (1-based) and number of lines
   Geo g;
   Phys ph;
+  FastGeo fg;
   const double *u, *v, *t, *p, *uant, *vant, *tant, *pc, *au, *av;
   const double *sc, *sp;  // sources of this equation (nullptr = zero)
   Side bc[4];             // a, b, c, d of this equation
@@ -231,6 +245,90 @@ __device__ __forceinline__ double t_rhs(const SweepArgs &A, int ii, int jj, doub
   return r;
 }
 
+// ---- division-free assembly (PCR path): same discretisation, geometry
+// quotients from FastGeo, products instead of quotients by dt and the
+// relaxation factor; the interface diffusivities of a constant G are G.
+__device__ __forceinline__ double nbf(double cond, double len, double pec, double w, bool east) {
+  double conv = w * len;
+  if (east) conv = -conv;
+  const double t = 1.0 - 0.1 * fabs(w * pec);
+  const double t2 = t * t;
+  return -(cond * len * dmax0(t2 * t2 * t) + dmax0(conv));
+}
+
+template <int EQ>
+__device__ __forceinline__ Coef fast_coef(const SweepArgs &A, int ii, int jj, double &area) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  const FastGeo &F = A.fg;
+  constexpr int e = EQ == EQ_U ? 0 : EQ == EQ_V ? 1 : 2;
+  double ui, ud, vs, vn, lx, ly;
+  if (EQ == EQ_U) {
+    ud = g.fexu[ii] * AT(A.u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(A.u, ii, jj);
+    ui = g.fexu[ii - 1] * AT(A.u, ii, jj) + (1.0 - g.fexu[ii - 1]) * AT(A.u, ii - 1, jj);
+    vn = g.fexp[ii] * AT(A.v, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj);
+    vs = g.fexp[ii] * AT(A.v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj - 1);
+    lx = g.dxu[ii];
+    ly = g.dyu[jj];
+  } else if (EQ == EQ_V) {
+    ud = g.feyp[jj] * AT(A.u, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.u, ii, jj);
+    ui = g.feyp[jj] * AT(A.u, ii - 1, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.u, ii - 1, jj);
+    vn = g.feyv[jj] * AT(A.v, ii, jj + 1) + (1.0 - g.feyv[jj]) * AT(A.v, ii, jj);
+    vs = g.feyv[jj - 1] * AT(A.v, ii, jj) + (1.0 - g.feyv[jj - 1]) * AT(A.v, ii, jj - 1);
+    lx = g.dxv[ii];
+    ly = g.dyv[jj];
+  } else {
+    ud = AT(A.u, ii, jj);
+    ui = AT(A.u, ii - 1, jj);
+    vn = AT(A.v, ii, jj);
+    vs = AT(A.v, ii, jj - 1);
+    lx = g.dxp[ii];
+    ly = g.dyp[jj];
+  }
+  Coef c;
+  c.ai = nbf(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
+  c.ad = nbf(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
+  c.bs = nbf(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
+  c.bn = nbf(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
+  area = lx * ly;
+  double acc = -c.ai - c.ad - c.bs - c.bn;
+  if (A.sp) acc = acc - area * AT(A.sp, ii, jj);
+  c.ac = (acc + area * F.inv_dt) * (EQ == EQ_T ? F.inv_rel_ener : F.inv_rel_vel);
+  return c;
+}
+
+template <int EQ>
+__device__ __forceinline__ double fast_rhs(const SweepArgs &A, int ii, int jj, double head, double ac,
+                                           double area) {
+  const int P = A.P;
+  const Geo &g = A.g;
+  double r = head;
+  if (EQ == EQ_U) {
+    if (A.ph.ri_x != 0.0) {
+      double tint = g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.t, ii, jj);
+      r = r - area * A.ph.ri_x * tint;
+    }
+    if (A.sc) r = r + area * AT(A.sc, ii, jj);
+    r = r + area * AT(A.uant, ii, jj) * A.fg.inv_dt;
+    r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * g.dyu[jj];
+    r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.u, ii, jj);
+  } else if (EQ == EQ_V) {
+    if (A.ph.ri_y != 0.0) {
+      double tint = g.feyp[jj] * AT(A.t, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.t, ii, jj);
+      r = r - area * A.ph.ri_y * tint;
+    }
+    if (A.sc) r = r + area * AT(A.sc, ii, jj);
+    r = r + area * AT(A.vant, ii, jj) * A.fg.inv_dt;
+    r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * g.dxv[ii];
+    r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.v, ii, jj);
+  } else {
+    if (A.sc) r = r + area * AT(A.sc, ii, jj);
+    r = r + area * AT(A.tant, ii, jj) * A.fg.inv_dt;
+    r = r + ac * (1.0 - A.ph.rel_ener) * AT(A.t, ii, jj);
+  }
+  return r;
+}
+
 // ---------------------------------------------------------------- solvers
 // Thomas without pivoting in upstream's order (mod_solucionador.f90:25-47),
 // one lane; a[0] and c[n-1] are not read.
@@ -304,25 +402,184 @@ __device__ __forceinline__ void pcr_lds(double *a, double *b, double *c, double 
   __syncthreads();
 }
 
+// Wave-partitioned solve (one wave64, no workgroup barriers inside):
+//  1. lane l owns rows s = l*M .. e = s+M-1 (M = ceil(n/64), rows past n are
+//     identity rows) and eliminates them in registers, downward then upward,
+//     so that every row reads  L_i x[s-1] + B_i x_i + C_i x[e+1] = R_i ;
+//  2. the chunk end rows give two equations per lane in the chunk-boundary
+//     unknowns z[2l] = x_s, z[2l+1] = x_e; combined without divisions they
+//     form a tridiagonal system of 128 rows, solved by parallel cyclic
+//     reduction across lanes with shuffles (7 steps);
+//  3. each lane back-substitutes its interior rows from x[s-1] = z[2l-1] and
+//     x[e+1] = z[2l+2].
+// MMAX bounds M at compile time (registers); M itself is uniform per wave.
+template <int MMAX>
+__device__ __forceinline__ void wpt_solve(double *a, double *b, double *c, double *r, int n) {
+  if (threadIdx.x < 64) {
+    const int l = threadIdx.x;
+    const int M = (n + 63) >> 6 < 2 ? 2 : (n + 63) >> 6;
+    const int s0 = l * M;
+    double A[MMAX], B[MMAX], C[MMAX], R[MMAX], L[MMAX], Cp[MMAX];
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i) {
+      if (i < M) {
+        const int k = s0 + i;
+        if (k < n) {
+          A[i] = k == 0 ? 0.0 : a[k];
+          B[i] = b[k];
+          C[i] = k == n - 1 ? 0.0 : c[k];
+          R[i] = r[k];
+        } else {
+          A[i] = 0.0; B[i] = 1.0; C[i] = 0.0; R[i] = 0.0;
+        }
+      }
+    }
+    const bool hasL = A[0] != 0.0;
+    // downward: L_i = spike on x[s-1]
+    L[0] = A[0];
+#pragma unroll
+    for (int i = 1; i < MMAX; ++i) {
+      if (i < M) {
+        const double f = A[i] / B[i - 1];
+        B[i] = B[i] - f * C[i - 1];
+        R[i] = R[i] - f * R[i - 1];
+        L[i] = -f * L[i - 1];
+      }
+    }
+    // upward: Cp_i = spike on x[e+1]
+#pragma unroll
+    for (int i = MMAX - 1; i >= 0; --i) {
+      if (i == M - 1) {
+        Cp[i] = C[i];
+      } else if (i < M - 1) {
+        const double g = C[i] / B[i + 1];
+        R[i] = R[i] - g * R[i + 1];
+        L[i] = L[i] - g * L[i + 1];
+        Cp[i] = -g * Cp[i + 1];
+      }
+    }
+    double Ls = L[0], Bs = B[0], Cs = Cp[0], Rs = R[0];
+    double Le = 0.0, Be = 0.0, Ce = 0.0, Re = 0.0;
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i)
+      if (i == M - 1) {
+        Le = L[i]; Be = B[i]; Ce = Cp[i]; Re = R[i];
+      }
+    const bool hasC = Ce != 0.0;
+    // reduced rows: p = 0 -> z[2l] (couples 2l-1, 2l, 2l+1), p = 1 -> z[2l+1]
+    double ra[2], rb[2], rc[2], rr[2];
+    if (hasC) {
+      ra[0] = Ce * Ls - Cs * Le; rb[0] = Ce * Bs; rc[0] = -(Cs * Be); rr[0] = Ce * Rs - Cs * Re;
+    } else {
+      ra[0] = Ls; rb[0] = Bs; rc[0] = 0.0; rr[0] = Rs;
+    }
+    if (hasL) {
+      ra[1] = -(Le * Bs); rb[1] = Ls * Be; rc[1] = Ls * Ce - Le * Cs; rr[1] = Ls * Re - Le * Rs;
+    } else {
+      ra[1] = 0.0; rb[1] = Be; rc[1] = Ce; rr[1] = Re;
+    }
+    // PCR over the 128 reduced rows k = 2l + p
+    for (int st = 1; st < 128; st <<= 1) {
+      const double ib0 = 1.0 / rb[0], ib1 = 1.0 / rb[1];
+      double na[2], nb[2], nc[2], nr[2];
+      if (st == 1) {
+        // row 2l: k-1 = 2l-1 (lane l-1, p=1), k+1 = 2l+1 (own p=1)
+        // row 2l+1: k-1 = 2l (own p=0), k+1 = 2l+2 (lane l+1, p=0)
+        const double pa = __shfl_up(ra[1], 1, 64), pc = __shfl_up(rc[1], 1, 64);
+        const double pr = __shfl_up(rr[1], 1, 64), pib = __shfl_up(ib1, 1, 64);
+        const double qa = __shfl_down(ra[0], 1, 64), qc = __shfl_down(rc[0], 1, 64);
+        const double qr = __shfl_down(rr[0], 1, 64), qib = __shfl_down(ib0, 1, 64);
+        {
+          const double al = l > 0 ? -ra[0] * pib : 0.0;
+          const double ga = -rc[0] * ib1;
+          na[0] = l > 0 ? al * pa : 0.0;
+          nc[0] = ga * rc[1];
+          nb[0] = rb[0] + (l > 0 ? al * pc : 0.0) + ga * ra[1];
+          nr[0] = rr[0] + (l > 0 ? al * pr : 0.0) + ga * rr[1];
+        }
+        {
+          const double al = -ra[1] * ib0;
+          const double ga = l < 63 ? -rc[1] * qib : 0.0;
+          na[1] = al * ra[0];
+          nc[1] = l < 63 ? ga * qc : 0.0;
+          nb[1] = rb[1] + al * rc[0] + (l < 63 ? ga * qa : 0.0);
+          nr[1] = rr[1] + al * rr[0] + (l < 63 ? ga * qr : 0.0);
+        }
+      } else {
+        const int d = st >> 1;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+          const double ib = p ? ib1 : ib0;
+          const double pa = __shfl_up(ra[p], d, 64), pc = __shfl_up(rc[p], d, 64);
+          const double pr = __shfl_up(rr[p], d, 64), pib = __shfl_up(ib, d, 64);
+          const double qa = __shfl_down(ra[p], d, 64), qc = __shfl_down(rc[p], d, 64);
+          const double qr = __shfl_down(rr[p], d, 64), qib = __shfl_down(ib, d, 64);
+          const bool lo = l - d >= 0, hi = l + d <= 63;
+          const double al = lo ? -ra[p] * pib : 0.0;
+          const double ga = hi ? -rc[p] * qib : 0.0;
+          na[p] = lo ? al * pa : 0.0;
+          nc[p] = hi ? ga * qc : 0.0;
+          nb[p] = rb[p] + (lo ? al * pc : 0.0) + (hi ? ga * qa : 0.0);
+          nr[p] = rr[p] + (lo ? al * pr : 0.0) + (hi ? ga * qr : 0.0);
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < 2; ++p) {
+        ra[p] = na[p]; rb[p] = nb[p]; rc[p] = nc[p]; rr[p] = nr[p];
+      }
+    }
+    const double zs = rr[0] / rb[0], ze = rr[1] / rb[1];
+    const double xl = __shfl_up(ze, 1, 64);    // x[s-1]
+    const double xr = __shfl_down(zs, 1, 64);  // x[e+1]
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i) {
+      if (i < M) {
+        const int k = s0 + i;
+        double x;
+        if (i == 0) x = zs;
+        else if (i == M - 1) x = ze;
+        else x = (R[i] - (l > 0 ? L[i] * xl : 0.0) - (l < 63 ? Cp[i] * xr : 0.0)) / B[i];
+        if (k < n) r[k] = x;
+      }
+    }
+  }
+  __syncthreads();
+}
+
 template <int SOLVER, int EPT>
 __device__ __forceinline__ void solve_line(double *a, double *b, double *c, double *r, int n) {
   if (SOLVER == SOLVE_THOMAS)
     thomas_lds(a, b, c, r, n);
+  else if (EPT <= 4)
+    wpt_solve<4 * EPT>(a, b, c, r, n);  // SOLVE_PCR and SOLVE_PCR_EXACT
   else
-    pcr_lds<EPT>(a, b, c, r, n);
+    pcr_lds<EPT>(a, b, c, r, n);        // lines longer than 1024
 }
 
-// deterministic block sum (fixed tree)
+// deterministic block sum: butterfly within each wave64, then the four wave
+// sums in a fixed order (a + b and b + a round alike, so every lane agrees)
+__device__ __forceinline__ double wave_sum(double x) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) x = x + __shfl_xor(x, o, 64);
+  return x;
+}
+
 __device__ __forceinline__ double block_sum(double x, double *scratch) {
-  scratch[threadIdx.x] = x;
+  static_assert(kBlock == 256, "block_sum folds four waves");
+  x = wave_sum(x);
+  if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = x;
   __syncthreads();
-  for (int w = kBlock / 2; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) scratch[threadIdx.x] = scratch[threadIdx.x] + scratch[threadIdx.x + w];
-    __syncthreads();
-  }
-  double s = scratch[0];
+  const double s = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
   __syncthreads();
   return s;
+}
+
+// Workgroup w runs on XCD w % 8; give every XCD a contiguous block of lines
+// so that neighbouring lines, which read each other's cache lines, share
+// one L2.  A bijection of [0, n).
+__device__ __forceinline__ int xcd_line(int w, int n) {
+  const int x = w & 7, k = w >> 3, q = n >> 3, r = n & 7;
+  return x * q + (x < r ? x : r) + k;
 }
 
 // boundary row of a sweep: side low (a/b) or high (c/d); returns false when
@@ -367,7 +624,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
   __shared__ double red[kBlock];
   const int P = A.P;
   const int mi = A.mi, nj = A.nj;
-  const int line = A.line0 + (int)blockIdx.x;
+  const int line = A.line0 + xcd_line((int)blockIdx.x, A.nlines);
   const int n = Layout<EQ, DIR>::n(mi, nj);
   const int hi = Layout<EQ, DIR>::hi(mi, nj);
   const int lastl = Layout<EQ, DIR>::last_line(mi, nj);
@@ -401,7 +658,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
         a1 = c.ac;
         rr = head + c.b + (1.0 - A.ph.rel_pres) * c.ac * pcP;
       } else {
-        Coef c = EQ == EQ_U ? u_coef(A, ii, jj) : EQ == EQ_V ? v_coef(A, ii, jj) : t_coef(A, ii, jj);
+        constexpr bool fast = SOLVER == SOLVE_PCR;
+        double area = 0.0;
+        Coef c = fast ? fast_coef<EQ>(A, ii, jj, area)
+               : EQ == EQ_U ? u_coef(A, ii, jj) : EQ == EQ_V ? v_coef(A, ii, jj) : t_coef(A, ii, jj);
         const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
         double head;
         if (DIR == DIR_X) {
@@ -412,8 +672,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
           head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
         }
         a1 = c.ac;
-        rr = EQ == EQ_U ? u_rhs(A, ii, jj, head, c.ac)
-           : EQ == EQ_V ? v_rhs(A, ii, jj, head, c.ac) : t_rhs(A, ii, jj, head, c.ac);
+        if (fast)
+          rr = fast_rhs<EQ>(A, ii, jj, head, c.ac, area);
+        else
+          rr = EQ == EQ_U ? u_rhs(A, ii, jj, head, c.ac)
+             : EQ == EQ_V ? v_rhs(A, ii, jj, head, c.ac) : t_rhs(A, ii, jj, head, c.ac);
         if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X)
           A.acoef[((size_t)ii - 1) + ((size_t)jj - 1) * P] = c.ac * A.ph.rel_vel;
       }
@@ -520,7 +783,7 @@ __global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
   const int P = A.P;
   const Geo &g = A.g;
   const double gm = A.ph.gm;
-  const int jj = 2 + (int)blockIdx.x;
+  const int jj = 2 + xcd_line((int)blockIdx.x, A.nj - 1);
   double acc = 0.0;
   for (int ii = 2 + (int)threadIdx.x; ii <= A.mi - 1; ii += kBlock) {
     double ud = g.fexu[ii] * AT(A.u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(A.u, ii, jj);
@@ -560,10 +823,12 @@ __global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
 
 // ---------------------------------------------------------------- finalize
 // Fold per-line partials into the five norms of one SIMPLE iteration:
-// out[0] err_v (lines 2..nj-1), out[1] err_p, out[2] maxbo, out[3] err_t
-// (lines 2..nj), out[4] residuo (lines 2..nj).  `which` masks the ones to
-// fold; `slot` (if not null) is a device counter selecting out + 8*slot and
-// incremented afterwards, so one graph can record a whole history.
+// out[0] err_v (lines 2..nj-1), out[1] err_p, out[2] maxbo, out[3] err_t,
+// out[4] residuo (lines 2..nj).  `which` masks the ones to fold; `slot` (if
+// not null) is a device counter selecting out + 8*slot and incremented
+// afterwards, so one graph can record a whole history.  Every thread loads
+// its share of all five arrays at once, so the fold costs one memory latency
+// and one barrier; the order of the additions is fixed.
 struct FinArgs {
   int nj;
   const double *pv, *pp, *pb, *pt, *pr;
@@ -572,27 +837,32 @@ struct FinArgs {
   int which;
 };
 
-__device__ __forceinline__ double fold(const double *part, int lo, int hi, double *red) {
-  double acc = 0.0;
-  for (int j = lo + (int)threadIdx.x; j <= hi; j += kBlock) acc = acc + part[j];
-  return block_sum(acc, red);
-}
-
 __global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) {
-  __shared__ double red[kBlock];
-  double *o = A.out;
-  if (A.slot) o = A.out + 8 * (size_t)(*A.slot);
-  double r0 = (A.which & 1) ? fold(A.pv, 2, A.nj - 1, red) : 0.0;
-  double r1 = (A.which & 2) ? fold(A.pp, 2, A.nj, red) : 0.0;
-  double r2 = (A.which & 4) ? fold(A.pb, 2, A.nj, red) : 0.0;
-  double r3 = (A.which & 8) ? fold(A.pt, 2, A.nj, red) : 0.0;
-  double r4 = (A.which & 16) ? fold(A.pr, 2, A.nj, red) : 0.0;
+  __shared__ double red[5][kBlock / 64];
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
+  for (int j = 2 + (int)threadIdx.x; j <= A.nj; j += kBlock) {
+    if ((A.which & 1) && j <= A.nj - 1) a0 = a0 + A.pv[j];
+    if (A.which & 2) a1 = a1 + A.pp[j];
+    if (A.which & 4) a2 = a2 + A.pb[j];
+    if (A.which & 8) a3 = a3 + A.pt[j];
+    if (A.which & 16) a4 = a4 + A.pr[j];
+  }
+  a0 = wave_sum(a0);
+  a1 = wave_sum(a1);
+  a2 = wave_sum(a2);
+  a3 = wave_sum(a3);
+  a4 = wave_sum(a4);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[0][w] = a0; red[1][w] = a1; red[2][w] = a2; red[3][w] = a3; red[4][w] = a4;
+  }
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (A.which & 1) o[0] = sqrt(r0);
-    if (A.which & 2) o[1] = sqrt(r1);
-    if (A.which & 4) o[2] = sqrt(r2);
-    if (A.which & 8) o[3] = sqrt(r3);
-    if (A.which & 16) o[4] = sqrt(r4);
+    double *o = A.out + (A.slot ? 8 * (size_t)(*A.slot) : 0);
+    double r[5];
+    for (int k = 0; k < 5; ++k) r[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
+    for (int k = 0; k < 5; ++k)
+      if (A.which & (1 << k)) o[k] = sqrt(r[k]);
     if (A.slot) *A.slot = *A.slot + 1;
   }
 }

@@ -3,9 +3,10 @@ CPU oracle and the upstream golden fixtures.
 
 Tolerances: with the Thomas line solver the device arithmetic follows
 upstream operation by operation (no FMA contraction, IEEE division), so
-fields must be bit-identical; with the PCR line solver only the line-solve
-rounding differs and fields must agree to 1e-10 relative (the FP64 target),
-norms to 1e-9.
+fields must be bit-identical.  "pcr_exact" keeps upstream's assembly and
+changes only the line-solve rounding (1e-12 per phase); "pcr" (the default
+path) also uses the division-free assembly (1e-11 per phase).  Whole runs
+must agree with upstream to 1e-10 relative (the FP64 target), norms to 1e-9.
 """
 import numpy as np
 import pytest
@@ -62,8 +63,11 @@ def _warm_states(case, steps):
     return o
 
 
+PHASE_TOL = {"pcr_exact": 1e-12, "pcr": 1e-11}
+
+
 @pytest.mark.parametrize("name,steps", [("nc18x14", 3), ("mixed22x17", 2)])
-@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+@pytest.mark.parametrize("solver", ["thomas", "pcr_exact", "pcr"])
 def test_phase_by_phase(name, steps, solver):
     case = K.CASES[name]()
     o = _warm_states(case, steps)
@@ -78,12 +82,13 @@ def test_phase_by_phase(name, steps, solver):
             if solver == "thomas":
                 assert np.array_equal(a, b), f"{name}/{pname}: {f} not bit-identical ({relerr(a, b):.2e})"
             else:
-                assert relerr(a, b) <= 1e-12, f"{name}/{pname}: {f} rel err {relerr(a, b):.2e}"
+                tol = PHASE_TOL[solver]
+                assert relerr(a, b) <= tol, f"{name}/{pname}: {f} rel err {relerr(a, b):.2e}"
             g.set(f, b)  # resynchronise so each phase is judged on its own
 
 
 @pytest.mark.parametrize("name", ["nc18x14", "mixed22x17", "ctrl18x14", "lid66", "nc34"])
-@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+@pytest.mark.parametrize("solver", ["thomas", "pcr_exact", "pcr"])
 def test_run_against_upstream_fixture(name, solver):
     case = K.CASES[name]()
     gold = K.load_fixture(name)
