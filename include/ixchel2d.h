@@ -145,7 +145,7 @@ int ixh_lag(ixh_state *s);
 
 /* nsteps time steps of main.f90:391-1270 with upstream's inner-loop and
  * SIMPLE exit rules.  When every loop is single-pass (iter_ecuaci_max < 0 and
- * iter_simple_max < 0) the steps run from a captured HIP graph with no host
+ * iter_simple_max < 0) the steps run as a pipelined launch sequence on three streams with no host
  * synchronisation; otherwise the errors are read back after every pass.  One
  * record per SIMPLE iteration goes to hist (up to cap); *nrec receives the
  * count (may be NULL). */

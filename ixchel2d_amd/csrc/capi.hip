@@ -1,5 +1,6 @@
 // Host side of the C-ABI (include/ixchel2d.h): device state, buffer
-// rotation, phase launches, the SIMPLE control loop and HIP-graph replay.
+// rotation, phase launches, the SIMPLE control loop and the pipelined
+// multi-stream time step.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -7,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <algorithm>
 #include <vector>
 
 #include "../../include/ixchel2d.h"
@@ -55,10 +57,16 @@ constexpr int kNbuf = 4;  // rotation buffers per solved field
 }  // namespace
 
 struct ixh_state {
-  int mi = 0, nj = 0, P = 0;
-  size_t fsize = 0;  // doubles per field buffer
+  int mi = 0, nj = 0, P = 0, Q = 0;  // row pitch P >= mi+1, transposed pitch Q >= nj+1
+  size_t fsize = 0;  // doubles per field buffer (either layout)
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // kernels go to `launch` (normally `stream`); the pipelined time step
+  // forks two more branches onto aux[0], aux[1] and joins them back
+  hipStream_t launch = nullptr;
+  hipStream_t aux[2] = {nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_t = nullptr, ev_j[2] = {nullptr, nullptr};
+  int u_hold = -1;  // u buffer still read by an in-flight residual
   ixh_params prm{};
   // geometry (one allocation, 1-based arrays)
   double *d_geo = nullptr;
@@ -68,11 +76,14 @@ struct ixh_state {
   FastGeo fast{};
   // solved fields with rotation buffers
   double *ub[kNbuf]{}, *vb[kNbuf]{}, *tb[kNbuf]{}, *pcb[kNbuf]{};
+  // transposed twins (y contiguous), rotated together with the row buffers
+  double *ubT[kNbuf]{}, *vbT[kNbuf]{}, *tbT[kNbuf]{}, *pcbT[kNbuf]{};
   int u_cur = 0, u_ant = 0, v_cur = 0, v_ant = 0, t_cur = 0, t_ant = 0, pc_cur = 0;
   int v_snap = -1, t_snap = -1, pc_snap = -1;  // -1: snapshot is zero
   bool pc_zero = true;
   double *p = nullptr, *bo = nullptr, *au = nullptr, *av = nullptr, *resu = nullptr;
-  double *src_c[3]{}, *src_p[3]{};
+  double *pT = nullptr, *auT = nullptr, *avT = nullptr;
+  double *src_c[3]{}, *src_p[3]{}, *src_cT[3]{}, *src_pT[3]{};
   bool has_src[3]{};
   int8_t *bct[3][4]{};
   double *bcv[3][4]{};
@@ -89,13 +100,6 @@ struct ixh_state {
   double kms[K_NUM]{};
   int kcount[K_NUM]{};
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  // graphs replaying one single-pass time step, keyed by rotation state
-  struct GraphEntry {
-    int key[7];
-    int next[7];
-    hipGraphExec_t exec;
-  };
-  std::vector<GraphEntry> graphs;
 };
 
 namespace {
@@ -182,12 +186,12 @@ template <int EQ, int DIR, int SOLVER>
 int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
   dim3 grid(A.nlines), block(kBlock);
   switch (ept_for(n)) {
-    case 1: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 1>), grid, block, 0, s->stream, A); break;
-    case 2: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 2>), grid, block, 0, s->stream, A); break;
-    case 3: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 3>), grid, block, 0, s->stream, A); break;
-    case 4: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 4>), grid, block, 0, s->stream, A); break;
-    case 6: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 6>), grid, block, 0, s->stream, A); break;
-    default: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 8>), grid, block, 0, s->stream, A); break;
+    case 1: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 1>), grid, block, 0, s->launch, A); break;
+    case 2: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 2>), grid, block, 0, s->launch, A); break;
+    case 3: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 3>), grid, block, 0, s->launch, A); break;
+    case 4: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 4>), grid, block, 0, s->launch, A); break;
+    case 6: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 6>), grid, block, 0, s->launch, A); break;
+    default: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 8>), grid, block, 0, s->launch, A); break;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -241,35 +245,45 @@ int build_fast(ixh_state *s) {
   return 0;
 }
 
-SweepArgs base_args(ixh_state *s) {
+// arguments of a sweep: the y-sweeps read the transposed copies
+SweepArgs base_args(ixh_state *s, int dir) {
+  const bool tr = dir == IXH_DIR_Y;
   SweepArgs A;
   memset(&A, 0, sizeof A);
   A.mi = s->mi;
   A.nj = s->nj;
-  A.P = s->P;
+  A.P = tr ? s->Q : s->P;
+  A.P2 = tr ? s->P : s->Q;
   A.g = s->geo;
   A.ph = phys_of(s);
   A.fg = s->fast;
-  A.u = s->ub[s->u_cur];
-  A.v = s->vb[s->v_cur];
-  A.t = s->tb[s->t_cur];
-  A.p = s->p;
-  A.uant = s->ub[s->u_ant];
-  A.vant = s->vb[s->v_ant];
-  A.tant = s->tb[s->t_ant];
-  A.pc = s->pcb[s->pc_cur];
-  A.au = s->au;
-  A.av = s->av;
+  A.u = (tr ? s->ubT : s->ub)[s->u_cur];
+  A.v = (tr ? s->vbT : s->vb)[s->v_cur];
+  A.t = (tr ? s->tbT : s->tb)[s->t_cur];
+  A.p = tr ? s->pT : s->p;
+  A.uant = (tr ? s->ubT : s->ub)[s->u_ant];
+  A.vant = (tr ? s->vbT : s->vb)[s->v_ant];
+  A.tant = (tr ? s->tbT : s->tb)[s->t_ant];
+  A.pc = (tr ? s->pcbT : s->pcb)[s->pc_cur];
+  A.au = tr ? s->auT : s->au;
+  A.av = tr ? s->avT : s->av;
   return A;
 }
 
-void set_bc(ixh_state *s, SweepArgs &A, int var) {
+// output buffers k of a field: the sweep's own layout and the other one
+void set_out(SweepArgs &A, int dir, double *const *rowb, double *const *trb, int k) {
+  A.out = dir == IXH_DIR_Y ? trb[k] : rowb[k];
+  A.out2 = dir == IXH_DIR_Y ? rowb[k] : trb[k];
+}
+
+void set_bc(ixh_state *s, SweepArgs &A, int var, int dir) {
   for (int sd = 0; sd < 4; ++sd) {
     A.bc[sd].type = s->bct[var][sd];
     A.bc[sd].val = s->bcv[var][sd];
   }
-  A.sc = s->has_src[var] ? s->src_c[var] : nullptr;
-  A.sp = s->has_src[var] ? s->src_p[var] : nullptr;
+  const bool tr = dir == IXH_DIR_Y;
+  A.sc = s->has_src[var] ? (tr ? s->src_cT : s->src_c)[var] : nullptr;
+  A.sp = s->has_src[var] ? (tr ? s->src_pT : s->src_p)[var] : nullptr;
 }
 
 int check_bcs(ixh_state *s, int var) {
@@ -284,11 +298,12 @@ double *part_of(ixh_state *s, int k) { return s->part + (size_t)k * s->plen; }
 // ---- phases
 int sweep_u(ixh_state *s, int dir) {
   CHK(check_bcs(s, IXH_VAR_U));
-  SweepArgs A = base_args(s);
-  set_bc(s, A, IXH_VAR_U);
-  const int out = pick(s->u_cur, s->u_ant);
-  A.out = s->ub[out];
-  A.acoef = s->au;
+  SweepArgs A = base_args(s, dir);
+  set_bc(s, A, IXH_VAR_U, dir);
+  const int out = pick(s->u_cur, s->u_ant, s->u_hold);
+  set_out(A, dir, s->ub, s->ubT, out);
+  A.acoef = dir == IXH_DIR_Y ? s->auT : s->au;
+  A.acoef2 = dir == IXH_DIR_Y ? s->au : s->auT;
   Timed tm(s, dir == IXH_DIR_Y ? K_U_Y : K_U_X);
   if (dir == IXH_DIR_Y) {
     A.line0 = 2;
@@ -306,13 +321,14 @@ int sweep_u(ixh_state *s, int dir) {
 
 int sweep_v(ixh_state *s, int dir) {
   CHK(check_bcs(s, IXH_VAR_V));
-  SweepArgs A = base_args(s);
-  set_bc(s, A, IXH_VAR_V);
-  A.acoef = s->av;
+  SweepArgs A = base_args(s, dir);
+  set_bc(s, A, IXH_VAR_V, dir);
+  A.acoef = dir == IXH_DIR_Y ? s->avT : s->av;
+  A.acoef2 = dir == IXH_DIR_Y ? s->av : s->avT;
   if (dir == IXH_DIR_Y) {
     s->v_snap = s->v_cur;  // fv = v (main.f90:559-564)
     const int out = pick(s->v_cur, s->v_ant);
-    A.out = s->vb[out];
+    set_out(A, dir, s->vb, s->vbT, out);
     A.line0 = 2;
     A.nlines = s->mi - 1;
     Timed tm(s, K_V_Y);
@@ -321,7 +337,7 @@ int sweep_v(ixh_state *s, int dir) {
     s->v_cur = out;
   } else {
     const int out = pick(s->v_cur, s->v_ant, s->v_snap);
-    A.out = s->vb[out];
+    set_out(A, dir, s->vb, s->vbT, out);
     A.snap = s->v_snap >= 0 ? s->vb[s->v_snap] : s->vb[s->v_cur];
     A.part0 = part_of(s, 0);
     A.line0 = 2;
@@ -335,12 +351,12 @@ int sweep_v(ixh_state *s, int dir) {
 }
 
 int sweep_p(ixh_state *s, int dir) {
-  SweepArgs A = base_args(s);
+  SweepArgs A = base_args(s, dir);
   A.in_zero = s->pc_zero ? 1 : 0;
   if (dir == IXH_DIR_Y) {
     s->pc_snap = s->pc_zero ? -1 : s->pc_cur;  // fcorr = p' (main.f90:762-767)
     const int out = pick(s->pc_cur, s->pc_snap);
-    A.out = s->pcb[out];
+    set_out(A, dir, s->pcb, s->pcbT, out);
     A.line0 = 2;
     A.nlines = s->mi - 1;
     Timed tm(s, K_P_Y);
@@ -350,7 +366,7 @@ int sweep_p(ixh_state *s, int dir) {
     s->pc_zero = false;
   } else {
     const int out = pick(s->pc_cur, s->pc_snap);
-    A.out = s->pcb[out];
+    set_out(A, dir, s->pcb, s->pcbT, out);
     A.snap = pc_of(s, s->pc_snap);
     A.bo = s->bo;
     A.part0 = part_of(s, 1);
@@ -368,12 +384,12 @@ int sweep_p(ixh_state *s, int dir) {
 
 int sweep_t(ixh_state *s, int dir) {
   CHK(check_bcs(s, IXH_VAR_T));
-  SweepArgs A = base_args(s);
-  set_bc(s, A, IXH_VAR_T);
+  SweepArgs A = base_args(s, dir);
+  set_bc(s, A, IXH_VAR_T, dir);
   if (dir == IXH_DIR_Y) {
     s->t_snap = s->t_cur;  // ftemp = T (main.f90:974-979)
     const int out = pick(s->t_cur, s->t_ant);
-    A.out = s->tb[out];
+    set_out(A, dir, s->tb, s->tbT, out);
     A.line0 = 2;
     A.nlines = s->mi - 1;
     Timed tm(s, K_T_Y);
@@ -382,7 +398,7 @@ int sweep_t(ixh_state *s, int dir) {
     s->t_cur = out;
   } else {
     const int out = pick(s->t_cur, s->t_ant, s->t_snap);
-    A.out = s->tb[out];
+    set_out(A, dir, s->tb, s->tbT, out);
     A.snap = s->t_snap >= 0 ? s->tb[s->t_snap] : s->tb[s->t_cur];
     A.part0 = part_of(s, 3);
     A.line0 = 2;
@@ -398,15 +414,16 @@ int sweep_t(ixh_state *s, int dir) {
 // move field f (IXH_U/V/T or their _ANT) to a buffer of its own when the
 // current and lagged copies share one
 int unalias(ixh_state *s, int f) {
-  double **b = nullptr;
+  double **b = nullptr, **bT = nullptr;
   int *cur = nullptr, *ant = nullptr;
-  if (f == IXH_U || f == IXH_U_ANT) { b = s->ub; cur = &s->u_cur; ant = &s->u_ant; }
-  else if (f == IXH_V || f == IXH_V_ANT) { b = s->vb; cur = &s->v_cur; ant = &s->v_ant; }
-  else if (f == IXH_T || f == IXH_T_ANT) { b = s->tb; cur = &s->t_cur; ant = &s->t_ant; }
+  if (f == IXH_U || f == IXH_U_ANT) { b = s->ub; bT = s->ubT; cur = &s->u_cur; ant = &s->u_ant; }
+  else if (f == IXH_V || f == IXH_V_ANT) { b = s->vb; bT = s->vbT; cur = &s->v_cur; ant = &s->v_ant; }
+  else if (f == IXH_T || f == IXH_T_ANT) { b = s->tb; bT = s->tbT; cur = &s->t_cur; ant = &s->t_ant; }
   else return 0;
   if (*cur != *ant) return 0;
   const int k = pick(*cur, -1);
   HIPCHK(hipMemcpyAsync(b[k], b[*cur], s->fsize * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(hipMemcpyAsync(bT[k], bT[*cur], s->fsize * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
   if (f == IXH_U || f == IXH_V || f == IXH_T) *cur = k; else *ant = k;
   return 0;
 }
@@ -416,10 +433,10 @@ int do_correct(ixh_state *s) {
   A.mi = s->mi;
   A.nj = s->nj;
   A.P = s->P;
+  A.Q = s->Q;
   A.g = s->geo;
-  A.u = s->ub[s->u_cur];
-  A.v = s->vb[s->v_cur];
   A.p = s->p;
+  A.pT = s->pT;
   A.pc = s->pcb[s->pc_cur];
   A.au = s->au;
   A.av = s->av;
@@ -430,9 +447,11 @@ int do_correct(ixh_state *s) {
   CHK(unalias(s, IXH_V));
   A.u = s->ub[s->u_cur];
   A.v = s->vb[s->v_cur];
-  dim3 block(kBlock), grid((s->mi + kBlock) / kBlock, s->nj + 1);
+  A.uT = s->ubT[s->u_cur];
+  A.vT = s->vbT[s->v_cur];
+  dim3 block(kBlock), grid((s->mi + 1 + kTile - 1) / kTile, (s->nj + 1 + kTile - 1) / kTile);
   Timed tm(s, K_CORR);
-  hipLaunchKernelGGL(k_correct, grid, block, 0, s->stream, A);
+  hipLaunchKernelGGL(k_correct, grid, block, 0, s->launch, A);
   HIPCHK(hipGetLastError());
   CHK(tm.done());
   return 0;
@@ -456,7 +475,7 @@ int do_residual(ixh_state *s) {
   A.resu = s->resu;
   A.part = part_of(s, 4);
   Timed tm(s, K_RES);
-  hipLaunchKernelGGL(k_residual_u, dim3(s->nj - 1), dim3(kBlock), 0, s->stream, A);
+  hipLaunchKernelGGL(k_residual_u, dim3(s->nj - 1), dim3(kBlock), 0, s->launch, A);
   HIPCHK(hipGetLastError());
   CHK(tm.done());
   return 0;
@@ -474,7 +493,7 @@ int do_finalize(ixh_state *s, int which, double *out, int *slot) {
   F.slot = slot;
   F.which = which;
   Timed tm(s, K_FIN);
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->stream, F);
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->launch, F);
   HIPCHK(hipGetLastError());
   CHK(tm.done());
   return 0;
@@ -488,8 +507,8 @@ int read_norms(ixh_state *s, int which, double out[8]) {
   return 0;
 }
 
-// one single-pass time step without host synchronisation; norms land in
-// d_hist[slot] through the device counter
+// One single-pass time step in upstream's order, without host
+// synchronisation; the norms land in d_hist[slot] through the device counter.
 int fast_step(ixh_state *s) {
   CHK(sweep_u(s, IXH_DIR_Y));
   CHK(sweep_u(s, IXH_DIR_X));
@@ -509,56 +528,55 @@ int fast_step(ixh_state *s) {
   return 0;
 }
 
-void rot_key(const ixh_state *s, int k[7]) {
-  k[0] = s->u_cur; k[1] = s->u_ant; k[2] = s->v_cur; k[3] = s->v_ant;
-  k[4] = s->t_cur; k[5] = s->t_ant; k[6] = s->pc_cur;
-}
-
-void set_rot(ixh_state *s, const int k[7]) {
-  s->u_cur = k[0]; s->u_ant = k[1]; s->v_cur = k[2]; s->v_ant = k[3];
-  s->t_cur = k[4]; s->t_ant = k[5]; s->pc_cur = k[6];
-}
-
-void drop_graphs(ixh_state *s) {
-  for (auto &g : s->graphs) (void)hipGraphExecDestroy(g.exec);
-  s->graphs.clear();
-}
-
-// replay one time step from a graph captured for the current rotation state
-int graph_step(ixh_state *s) {
-  int key[7];
-  rot_key(s, key);
-  for (auto &g : s->graphs) {
-    if (memcmp(g.key, key, sizeof key) == 0) {
-      HIPCHK(hipGraphLaunch(g.exec, s->stream));
-      set_rot(s, g.next);
-      s->pc_zero = false;
-      s->v_snap = -1;
-      s->t_snap = -1;
-      return 0;
-    }
+// Pipelined time step k: v and p' sweeps and the correction, then three
+// branches that each need only the corrected fields of step k:
+//   stream: T sweeps of step k
+//   aux[1]: residual of step k, then (after the T sweeps) the norm fold
+//   aux[0]: u sweeps of step k+1 (u_ant of step k+1 is u of step k)
+// joined before the v sweeps of step k+1.  The u equation must not read T
+// (Ri = 0), so the branches share no written buffer; the u buffer the
+// residual still reads as u_ant is held out of the rotation.  The u sweeps
+// of the first step run before the first call (prologue).
+int pipe_step(ixh_state *s, bool last) {
+  CHK(sweep_v(s, IXH_DIR_Y));
+  CHK(sweep_v(s, IXH_DIR_X));
+  s->pc_zero = true;
+  CHK(sweep_p(s, IXH_DIR_Y));
+  CHK(sweep_p(s, IXH_DIR_X));
+  CHK(do_correct(s));
+  HIPCHK(hipEventRecord(s->ev_fork, s->stream));
+  HIPCHK(hipStreamWaitEvent(s->aux[0], s->ev_fork, 0));
+  HIPCHK(hipStreamWaitEvent(s->aux[1], s->ev_fork, 0));
+  s->launch = s->aux[1];
+  CHK(do_residual(s));
+  s->launch = s->stream;
+  CHK(sweep_t(s, IXH_DIR_Y));
+  CHK(sweep_t(s, IXH_DIR_X));
+  HIPCHK(hipEventRecord(s->ev_t, s->stream));
+  HIPCHK(hipStreamWaitEvent(s->aux[1], s->ev_t, 0));
+  s->launch = s->aux[1];
+  CHK(do_finalize(s, 31, s->d_hist, s->d_slot));
+  s->launch = s->aux[0];
+  const int held = s->u_ant;
+  s->u_ant = s->u_cur;
+  if (!last) {
+    s->u_hold = held;
+    CHK(sweep_u(s, IXH_DIR_Y));
+    CHK(sweep_u(s, IXH_DIR_X));
+    s->u_hold = -1;
   }
-  hipGraph_t graph;
-  HIPCHK(hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal));
-  int rc = fast_step(s);
-  hipError_t e = hipStreamEndCapture(s->stream, &graph);
-  if (rc != 0) return rc;
-  if (e != hipSuccess) return fail(IXH_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-  ixh_state::GraphEntry ge;
-  memcpy(ge.key, key, sizeof key);
-  rot_key(s, ge.next);
-  e = hipGraphInstantiate(&ge.exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e != hipSuccess) return fail(IXH_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-  s->graphs.push_back(ge);
-  HIPCHK(hipGraphLaunch(ge.exec, s->stream));
-  s->pc_zero = false;
+  s->launch = s->stream;
+  HIPCHK(hipEventRecord(s->ev_j[0], s->aux[0]));
+  HIPCHK(hipEventRecord(s->ev_j[1], s->aux[1]));
+  HIPCHK(hipStreamWaitEvent(s->stream, s->ev_j[0], 0));
+  HIPCHK(hipStreamWaitEvent(s->stream, s->ev_j[1], 0));
+  s->v_ant = s->v_cur;
+  s->t_ant = s->t_cur;
   return 0;
 }
 
 int ensure_hist(ixh_state *s, int n) {
   if (n <= s->hist_cap) return 0;
-  drop_graphs(s);  // graphs hold the old pointer
   if (s->d_hist) HIPCHK(hipFree(s->d_hist));
   s->d_hist = nullptr;
   CHK(dmalloc(&s->d_hist, (size_t)8 * n));
@@ -588,7 +606,8 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   s->mi = mi;
   s->nj = nj;
   s->P = ((mi + 1) + 15) / 16 * 16;  // rows start on 128-byte boundaries
-  s->fsize = (size_t)s->P * (nj + 1);
+  s->Q = ((nj + 1) + 15) / 16 * 16;  // so do the columns of the transposed copies
+  s->fsize = std::max((size_t)s->P * (nj + 1), (size_t)s->Q * (mi + 1));
   s->prm = *params;
   int rc = 0;
   auto bail = [&](int code) {
@@ -598,6 +617,14 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   if (hipStreamCreate(&s->stream) != hipSuccess) return bail(fail(IXH_ERR_HIP, "hipStreamCreate failed"));
   s->own_stream = true;
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
+    return bail(fail(IXH_ERR_HIP, "hipEventCreate failed"));
+  s->launch = s->stream;
+  for (int k = 0; k < 2; ++k)
+    if (hipStreamCreate(&s->aux[k]) != hipSuccess ||
+        hipEventCreateWithFlags(&s->ev_j[k], hipEventDisableTiming) != hipSuccess)
+      return bail(fail(IXH_ERR_HIP, "auxiliary stream/event creation failed"));
+  if (hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&s->ev_t, hipEventDisableTiming) != hipSuccess)
     return bail(fail(IXH_ERR_HIP, "hipEventCreate failed"));
   // geometry on the host (mod_malla.f90:167-217), 1-based
   const int gx = mi + 2, gy = nj + 2;
@@ -638,17 +665,22 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   // fields
   for (int k = 0; k < kNbuf; ++k) {
     if ((rc = dmalloc(&s->ub[k], s->fsize)) || (rc = dmalloc(&s->vb[k], s->fsize)) ||
-        (rc = dmalloc(&s->tb[k], s->fsize)) || (rc = dmalloc(&s->pcb[k], s->fsize)))
+        (rc = dmalloc(&s->tb[k], s->fsize)) || (rc = dmalloc(&s->pcb[k], s->fsize)) ||
+        (rc = dmalloc(&s->ubT[k], s->fsize)) || (rc = dmalloc(&s->vbT[k], s->fsize)) ||
+        (rc = dmalloc(&s->tbT[k], s->fsize)) || (rc = dmalloc(&s->pcbT[k], s->fsize)))
       return bail(rc);
   }
   if ((rc = dmalloc(&s->p, s->fsize)) || (rc = dmalloc(&s->bo, s->fsize)) ||
       (rc = dmalloc(&s->au, s->fsize)) || (rc = dmalloc(&s->av, s->fsize)) ||
-      (rc = dmalloc(&s->resu, s->fsize)))
+      (rc = dmalloc(&s->resu, s->fsize)) || (rc = dmalloc(&s->pT, s->fsize)) ||
+      (rc = dmalloc(&s->auT, s->fsize)) || (rc = dmalloc(&s->avT, s->fsize)))
     return bail(rc);
   {
     std::vector<double> big(s->fsize, 1.e40);  // a_u = a_v = 1e40 (main.f90:309-310)
     if (hipMemcpy(s->au, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->av, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+        hipMemcpy(s->av, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->auT, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->avT, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(IXH_ERR_HIP, "a_u/a_v initialisation failed"));
   }
   s->plen = (mi > nj ? mi : nj) + 4;
@@ -670,8 +702,8 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
 int ixh_destroy(ixh_state *s) {
   if (!s) return 0;
   if (s->stream) (void)hipStreamSynchronize(s->stream);
-  drop_graphs(s);
-  double *ptrs[] = {s->d_geo, s->p, s->bo, s->au, s->av, s->resu, s->part, s->d_out, s->d_hist};
+  double *ptrs[] = {s->d_geo, s->p, s->bo, s->au, s->av, s->resu, s->part, s->d_out, s->d_hist,
+                    s->pT, s->auT, s->avT};
   for (double *p : ptrs)
     if (p) (void)hipFree(p);
   for (int k = 0; k < kNbuf; ++k) {
@@ -679,10 +711,14 @@ int ixh_destroy(ixh_state *s) {
     if (s->vb[k]) (void)hipFree(s->vb[k]);
     if (s->tb[k]) (void)hipFree(s->tb[k]);
     if (s->pcb[k]) (void)hipFree(s->pcb[k]);
+    double *tws[] = {s->ubT[k], s->vbT[k], s->tbT[k], s->pcbT[k]};
+    for (double *p : tws)
+      if (p) (void)hipFree(p);
   }
   for (int v = 0; v < 3; ++v) {
-    if (s->src_c[v]) (void)hipFree(s->src_c[v]);
-    if (s->src_p[v]) (void)hipFree(s->src_p[v]);
+    double *srcs[] = {s->src_c[v], s->src_p[v], s->src_cT[v], s->src_pT[v]};
+    for (double *p : srcs)
+      if (p) (void)hipFree(p);
     for (int sd = 0; sd < 4; ++sd) {
       if (s->bct[v][sd]) (void)hipFree(s->bct[v][sd]);
       if (s->bcv[v][sd]) (void)hipFree(s->bcv[v][sd]);
@@ -692,6 +728,12 @@ int ixh_destroy(ixh_state *s) {
   if (s->d_fast) (void)hipFree(s->d_fast);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  for (int k = 0; k < 2; ++k) {
+    if (s->aux[k]) (void)hipStreamDestroy(s->aux[k]);
+    if (s->ev_j[k]) (void)hipEventDestroy(s->ev_j[k]);
+  }
+  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
+  if (s->ev_t) (void)hipEventDestroy(s->ev_t);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return 0;
@@ -701,7 +743,6 @@ int ixh_set_params(ixh_state *s, const ixh_params *params) {
   if (!s) return fail(IXH_ERR_ARG, "state is NULL");
   CHK(check_params(params));
   s->prm = *params;
-  drop_graphs(s);  // kernel arguments carry the scalars
   return build_fast(s);
 }
 
@@ -716,7 +757,7 @@ int ixh_set_stream(ixh_state *s, void *stream) {
   } else {
     s->stream = (hipStream_t)stream;
   }
-  drop_graphs(s);
+  s->launch = s->stream;
   return 0;
 }
 
@@ -744,6 +785,31 @@ static double *field_ptr(ixh_state *s, int f) {
   return nullptr;
 }
 
+static double *field_ptr_T(ixh_state *s, int f) {
+  switch (f) {
+    case IXH_U: return s->ubT[s->u_cur];
+    case IXH_V: return s->vbT[s->v_cur];
+    case IXH_T: return s->tbT[s->t_cur];
+    case IXH_P: return s->pT;
+    case IXH_U_ANT: return s->ubT[s->u_ant];
+    case IXH_V_ANT: return s->vbT[s->v_ant];
+    case IXH_T_ANT: return s->tbT[s->t_ant];
+    case IXH_PCORR: return s->pcbT[s->pc_cur];
+    case IXH_AU: return s->auT;
+    case IXH_AV: return s->avT;
+  }
+  return nullptr;  // b_o and Res_u live in the row layout only
+}
+
+// host array (extent m x n, leading dimension ld) -> transposed device copy
+static int upload_T(ixh_state *s, double *dst, const double *src, int ld, int m, int n) {
+  std::vector<double> t((size_t)s->Q * m, 0.0);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) t[(size_t)j + (size_t)i * s->Q] = src[(size_t)i + (size_t)j * ld];
+  HIPCHK(hipMemcpy(dst, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  return 0;
+}
+
 int ixh_upload_field(ixh_state *s, int32_t f, const double *src, int32_t ld) {
   if (!s || !src || f < 0 || f >= IXH_NFIELDS) return fail(IXH_ERR_ARG, "bad upload arguments");
   int m, n;
@@ -752,10 +818,10 @@ int ixh_upload_field(ixh_state *s, int32_t f, const double *src, int32_t ld) {
   // a field that shares its buffer with its lagged copy gets its own first
   CHK(unalias(s, f));
   if (f == IXH_PCORR) s->pc_zero = false;
-  drop_graphs(s);
   HIPCHK(hipMemcpy2DAsync(field_ptr(s, f), (size_t)s->P * 8, src, (size_t)ld * 8, (size_t)m * 8, n,
                           hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  if (double *dT = field_ptr_T(s, f)) CHK(upload_T(s, dT, src, ld, m, n));
   return 0;
 }
 
@@ -792,26 +858,27 @@ int ixh_set_bc(ixh_state *s, int32_t var, int32_t side, const int32_t *type, con
   HIPCHK(hipMemcpy(s->bct[var][side], t.data(), n + 2, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(s->bcv[var][side], v.data(), (n + 2) * sizeof(double), hipMemcpyHostToDevice));
   s->bc_set[var][side] = true;
-  drop_graphs(s);
   return 0;
 }
 
 int ixh_set_sources(ixh_state *s, int32_t var, const double *sc, const double *sp) {
   if (!s || var < 0 || var > 2) return fail(IXH_ERR_ARG, "bad source arguments");
-  drop_graphs(s);
   if (!sc && !sp) {
     s->has_src[var] = false;
     return 0;
   }
   if (!sc || !sp) return fail(IXH_ERR_ARG, "Sc and Sp must both be given or both be NULL");
   int rc;
-  if (!s->src_c[var] && ((rc = dmalloc(&s->src_c[var], s->fsize)) || (rc = dmalloc(&s->src_p[var], s->fsize))))
+  if (!s->src_c[var] && ((rc = dmalloc(&s->src_c[var], s->fsize)) || (rc = dmalloc(&s->src_p[var], s->fsize)) ||
+                          (rc = dmalloc(&s->src_cT[var], s->fsize)) || (rc = dmalloc(&s->src_pT[var], s->fsize))))
     return rc;
   const int fid = var == 0 ? IXH_U : var == 1 ? IXH_V : IXH_T;
   int m, n;
   field_extent(s, fid, &m, &n);
   HIPCHK(hipMemcpy2D(s->src_c[var], (size_t)s->P * 8, sc, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy2D(s->src_p[var], (size_t)s->P * 8, sp, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
+  CHK(upload_T(s, s->src_cT[var], sc, m, m, n));
+  CHK(upload_T(s, s->src_pT[var], sp, m, m, n));
   s->has_src[var] = true;
   return 0;
 }
@@ -896,11 +963,17 @@ int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t 
     CHK(ensure_hist(s, nsteps > 0 ? nsteps : 1));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->d_slot, 0);
     HIPCHK(hipGetLastError());
-    for (int k = 0; k < nsteps; ++k) {
-      if (s->profiling)
-        CHK(fast_step(s));
-      else
-        CHK(graph_step(s));
+    if (s->profiling || p.ri_x != 0.0) {
+      // upstream order, one kernel after the other (per-kernel event timing,
+      // or a u equation that reads T)
+      for (int k = 0; k < nsteps; ++k) CHK(fast_step(s));
+    } else if (nsteps > 0) {
+      // direct launches: on this stack a captured HIP graph of the same step
+      // ran slower (145.7 us vs 140.0 us per step sequential, 155.5 us vs
+      // 138.5 us pipelined, 301x701 FP64)
+      CHK(sweep_u(s, IXH_DIR_Y));  // prologue: u sweeps of the first step
+      CHK(sweep_u(s, IXH_DIR_X));
+      for (int k = 0; k < nsteps; ++k) CHK(pipe_step(s, k == nsteps - 1));
     }
     const int keep = nsteps < cap ? nsteps : cap;
     std::vector<double> h((size_t)8 * (keep > 0 ? keep : 1));

@@ -70,14 +70,27 @@ struct SweepArgs {
   const double *sc, *sp;  // sources of this equation (nullptr = zero)
   Side bc[4];             // a, b, c, d of this equation
   const double *snap;     // field at the start of the pass (nullptr = zero)
-  double *out;            // solved field
+  double *out;            // solved field (layout of the sweep)
   double *acoef;          // a_u / a_v written by the momentum sweeps
+  int P2;                 // pitch of the other layout
+  double *out2, *acoef2;  // the same results in the other layout
   double *bo;             // mass residual written by the p' x-sweep
   double *part0, *part1;  // per-line partial sums (indexed by line number)
   int in_zero;            // p' input identically zero (first pass)
 };
 
-#define AT(ptr, i, j) (ptr)[((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P]
+// Field element (i,j), 1-based.  Every field is kept twice: row layout
+// (x contiguous, pitch P >= mi+1) for the x-sweeps and transposed layout
+// (y contiguous, pitch >= nj+1) for the y-sweeps, so that the lanes of a wave
+// always read consecutive addresses.  TR (compile time, in scope) selects the
+// transposed one; P is the pitch of the layout in use.
+#define AT(ptr, i, j)                                                             \
+  (ptr)[TR ? (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P)                 \
+           : (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P)]
+// the other layout (pitch P2), for the second copy a sweep writes
+#define AO(ptr, i, j)                                                             \
+  (ptr)[TR ? (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P2)                \
+           : (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P2)]
 
 __device__ __forceinline__ double pow5(double t) { return t * (t * (t * (t * t))); }
 __device__ __forceinline__ double dmax0(double x) { return x < 0.0 ? 0.0 : x; }
@@ -99,6 +112,7 @@ struct Coef {
 };
 
 // ---- u momentum, ensambla_velu_x/_y (source/mod_ec_momento.f90:365-724)
+template <bool TR>
 __device__ __forceinline__ Coef u_coef(const SweepArgs &A, int ii, int jj) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -126,6 +140,7 @@ __device__ __forceinline__ Coef u_coef(const SweepArgs &A, int ii, int jj) {
   return c;
 }
 
+template <bool TR>
 __device__ __forceinline__ double u_rhs(const SweepArgs &A, int ii, int jj, double head, double ac) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -143,6 +158,7 @@ __device__ __forceinline__ double u_rhs(const SweepArgs &A, int ii, int jj, doub
 }
 
 // ---- v momentum, ensambla_velv_x/_y (source/mod_ec_momento.f90:736-1098)
+template <bool TR>
 __device__ __forceinline__ Coef v_coef(const SweepArgs &A, int ii, int jj) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -170,6 +186,7 @@ __device__ __forceinline__ Coef v_coef(const SweepArgs &A, int ii, int jj) {
   return c;
 }
 
+template <bool TR>
 __device__ __forceinline__ double v_rhs(const SweepArgs &A, int ii, int jj, double head, double ac) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -190,6 +207,7 @@ __device__ __forceinline__ double v_rhs(const SweepArgs &A, int ii, int jj, doub
 struct PCoef {
   double ai, ad, bs, bn, ac, b;
 };
+template <bool TR>
 __device__ __forceinline__ PCoef p_coef(const SweepArgs &A, int ii, int jj) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -211,6 +229,7 @@ __device__ __forceinline__ PCoef p_coef(const SweepArgs &A, int ii, int jj) {
 // ---- energy, ensambla_energia_x/_y (mod_ec_energia.f90:119-443); the
 // driver hands fexu/feyv to the routine's interpolation-weight dummies
 // (main.f90:986,1057), so those weight the interface diffusivities
+template <bool TR>
 __device__ __forceinline__ Coef t_coef(const SweepArgs &A, int ii, int jj) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -234,6 +253,7 @@ __device__ __forceinline__ Coef t_coef(const SweepArgs &A, int ii, int jj) {
   return c;
 }
 
+template <bool TR>
 __device__ __forceinline__ double t_rhs(const SweepArgs &A, int ii, int jj, double head, double ac) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -256,7 +276,7 @@ __device__ __forceinline__ double nbf(double cond, double len, double pec, doubl
   return -(cond * len * dmax0(t2 * t2 * t) + dmax0(conv));
 }
 
-template <int EQ>
+template <int EQ, bool TR>
 __device__ __forceinline__ Coef fast_coef(const SweepArgs &A, int ii, int jj, double &area) {
   const int P = A.P;
   const Geo &g = A.g;
@@ -297,7 +317,7 @@ __device__ __forceinline__ Coef fast_coef(const SweepArgs &A, int ii, int jj, do
   return c;
 }
 
-template <int EQ>
+template <int EQ, bool TR>
 __device__ __forceinline__ double fast_rhs(const SweepArgs &A, int ii, int jj, double head, double ac,
                                            double area) {
   const int P = A.P;
@@ -622,7 +642,9 @@ template <int EQ, int DIR, int SOLVER, int EPT>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
   __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
   __shared__ double red[kBlock];
+  constexpr bool TR = DIR == DIR_Y;
   const int P = A.P;
+  const int P2 = A.P2;
   const int mi = A.mi, nj = A.nj;
   const int line = A.line0 + xcd_line((int)blockIdx.x, A.nlines);
   const int n = Layout<EQ, DIR>::n(mi, nj);
@@ -640,7 +662,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
     double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
     if (pos >= 2 && pos <= hi) {
       if (EQ == EQ_P) {
-        PCoef c = p_coef(A, ii, jj);
+        PCoef c = p_coef<TR>(A, ii, jj);
         double pcP = A.in_zero ? 0.0 : AT(A.pc, ii, jj);
         double head;
         if (DIR == DIR_X) {
@@ -648,7 +670,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
           double pS = A.in_zero ? 0.0 : AT(A.pc, ii, jj - 1);
           double pN = A.in_zero ? 0.0 : AT(A.pc, ii, jj + 1);
           head = -c.bs * pS - c.bn * pN;
-          A.bo[((size_t)ii - 1) + ((size_t)jj - 1) * P] = c.b;
+          AT(A.bo, ii, jj) = c.b;  // x-sweep: row layout
         } else {
           a0 = c.bs; a2 = c.bn;
           double pW = A.in_zero ? 0.0 : AT(A.pc, ii - 1, jj);
@@ -660,8 +682,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
       } else {
         constexpr bool fast = SOLVER == SOLVE_PCR;
         double area = 0.0;
-        Coef c = fast ? fast_coef<EQ>(A, ii, jj, area)
-               : EQ == EQ_U ? u_coef(A, ii, jj) : EQ == EQ_V ? v_coef(A, ii, jj) : t_coef(A, ii, jj);
+        Coef c = fast ? fast_coef<EQ, TR>(A, ii, jj, area)
+               : EQ == EQ_U ? u_coef<TR>(A, ii, jj) : EQ == EQ_V ? v_coef<TR>(A, ii, jj) : t_coef<TR>(A, ii, jj);
         const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
         double head;
         if (DIR == DIR_X) {
@@ -673,12 +695,14 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
         }
         a1 = c.ac;
         if (fast)
-          rr = fast_rhs<EQ>(A, ii, jj, head, c.ac, area);
+          rr = fast_rhs<EQ, TR>(A, ii, jj, head, c.ac, area);
         else
-          rr = EQ == EQ_U ? u_rhs(A, ii, jj, head, c.ac)
-             : EQ == EQ_V ? v_rhs(A, ii, jj, head, c.ac) : t_rhs(A, ii, jj, head, c.ac);
-        if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X)
-          A.acoef[((size_t)ii - 1) + ((size_t)jj - 1) * P] = c.ac * A.ph.rel_vel;
+          rr = EQ == EQ_U ? u_rhs<TR>(A, ii, jj, head, c.ac)
+             : EQ == EQ_V ? v_rhs<TR>(A, ii, jj, head, c.ac) : t_rhs<TR>(A, ii, jj, head, c.ac);
+        if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X) {
+          AT(A.acoef, ii, jj) = c.ac * A.ph.rel_vel;
+          AO(A.acoef2, ii, jj) = c.ac * A.ph.rel_vel;
+        }
       }
     } else {
       const bool high = pos != 1;
@@ -688,8 +712,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
       } else {
         // the node number along the side is the line index
         bool typed = bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
-        if (typed && (EQ == EQ_U || EQ == EQ_V))
-          A.acoef[((size_t)ii - 1) + ((size_t)jj - 1) * P] = 1.e40;
+        if (typed && (EQ == EQ_U || EQ == EQ_V)) {
+          AT(A.acoef, ii, jj) = 1.e40;
+          AO(A.acoef2, ii, jj) = 1.e40;
+        }
       }
     }
     sa[k] = a0;
@@ -708,6 +734,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
     const int jj = DIR == DIR_X ? line : pos;
     const double x = sr[k];
     AT(A.out, ii, jj) = x;
+    AO(A.out2, ii, jj) = x;
     if (DIR == DIR_X && EQ != EQ_U && pos >= 2 && pos <= mi) {
       double old = A.snap ? AT(A.snap, ii, jj) : 0.0;
       double d = x - old;
@@ -736,7 +763,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
       const int pos = k + 1;
       const int ii = DIR == DIR_X ? pos : other;
       const int jj = DIR == DIR_X ? other : pos;
-      AT(A.out, ii, jj) = (EQ == EQ_P && A.in_zero) ? 0.0 : AT(src, ii, jj);
+      const double x = (EQ == EQ_P && A.in_zero) ? 0.0 : AT(src, ii, jj);
+      AT(A.out, ii, jj) = x;
+      AO(A.out2, ii, jj) = x;
     }
   }
 }
@@ -744,25 +773,53 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
 // ------------------------------------------------------------- correction
 // p += 0.6 p' on ii 2..mi, jj 2..nj; u += dyu (p'_P - p'_E)/a_u on ii 2..mi-1,
 // jj 2..nj; v += dxv (p'_P - p'_N)/a_v on ii 2..mi, jj 2..nj-1
-// (source/main.f90:931-962), in place.
+// (source/main.f90:931-962), in place.  A workgroup updates a 32 x 32 tile
+// in the row layout and writes the same values to the transposed copies
+// through LDS, so both stores are coalesced.
 struct CorrArgs {
-  int mi, nj, P;
+  int mi, nj, P, Q;
   Geo g;
-  double *u, *v, *p;
-  const double *pc, *au, *av;
+  double *u, *v, *p;          // row layout
+  double *uT, *vT, *pT;       // transposed copies (pitch Q)
+  const double *pc, *au, *av; // row layout
 };
 
+constexpr int kTile = 32;
+
 __global__ void __launch_bounds__(kBlock) k_correct(CorrArgs A) {
+  __shared__ double tu[kTile][kTile + 1], tv[kTile][kTile + 1], tp[kTile][kTile + 1];
+  constexpr bool TR = false;
   const int P = A.P;
-  const int ii = 1 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-  const int jj = 1 + (int)blockIdx.y;
-  if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) return;
-  const double pcP = AT(A.pc, ii, jj);
-  AT(A.p, ii, jj) = AT(A.p, ii, jj) + 0.6 * pcP;
-  if (ii <= A.mi - 1)
-    AT(A.u, ii, jj) = AT(A.u, ii, jj) + A.g.dyu[jj] * (pcP - AT(A.pc, ii + 1, jj)) / AT(A.au, ii, jj);
-  if (jj <= A.nj - 1)
-    AT(A.v, ii, jj) = AT(A.v, ii, jj) + A.g.dxv[ii] * (pcP - AT(A.pc, ii, jj + 1)) / AT(A.av, ii, jj);
+  const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x >> 5;  // 32 x 8
+  const int i0 = 1 + (int)blockIdx.x * kTile, j0 = 1 + (int)blockIdx.y * kTile;
+  for (int r = ty; r < kTile; r += kBlock / kTile) {
+    const int ii = i0 + tx, jj = j0 + r;
+    if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) continue;
+    const double pcP = AT(A.pc, ii, jj);
+    const double pn = AT(A.p, ii, jj) + 0.6 * pcP;
+    AT(A.p, ii, jj) = pn;
+    tp[r][tx] = pn;
+    if (ii <= A.mi - 1) {
+      const double un = AT(A.u, ii, jj) + A.g.dyu[jj] * (pcP - AT(A.pc, ii + 1, jj)) / AT(A.au, ii, jj);
+      AT(A.u, ii, jj) = un;
+      tu[r][tx] = un;
+    }
+    if (jj <= A.nj - 1) {
+      const double vn = AT(A.v, ii, jj) + A.g.dxv[ii] * (pcP - AT(A.pc, ii, jj + 1)) / AT(A.av, ii, jj);
+      AT(A.v, ii, jj) = vn;
+      tv[r][tx] = vn;
+    }
+  }
+  __syncthreads();
+  const int Q = A.Q;
+  for (int r = ty; r < kTile; r += kBlock / kTile) {
+    const int ii = i0 + r, jj = j0 + tx;  // lanes run along jj: coalesced in the transposed copy
+    if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) continue;
+    const size_t o = ((size_t)jj - 1) + ((size_t)ii - 1) * (size_t)Q;
+    A.pT[o] = tp[tx][r];
+    if (ii <= A.mi - 1) A.uT[o] = tu[tx][r];
+    if (jj <= A.nj - 1) A.vT[o] = tv[tx][r];
+  }
 }
 
 // ---------------------------------------------------------------- residual
@@ -780,6 +837,7 @@ struct ResArgs {
 
 __global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
   __shared__ double red[kBlock];
+  constexpr bool TR = false;
   const int P = A.P;
   const Geo &g = A.g;
   const double gm = A.ph.gm;
@@ -885,6 +943,7 @@ struct NuArgs {
 
 __global__ void __launch_bounds__(kBlock) k_nusselt(NuArgs A) {
   __shared__ double der[kNmax + 4];
+  constexpr bool TR = false;
   const int P = A.P;
   const Geo &g = A.g;
   if (!A.vertical) {
