@@ -7,6 +7,7 @@ loops): the u, v, p', T line sweeps, the correction, the residual and the
 norms.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver pcr|thomas]
+  python bench.py --workload batch256   # 256 mixed-size cavities, one launch per phase
 """
 from __future__ import annotations
 
@@ -85,12 +86,80 @@ def cpu_baseline(case, budget_s: float = 20.0):
             "sample": f"C oracle (OpenMP), {n} iterations of the 301x701 case"}
 
 
+def batch_bench(args):
+    """Aggregate throughput of the mixed-size batch (256 cavities of 65^2,
+    129^2, 257^2 and 129x301 cells, lid-driven and natural convection, 2 of
+    16 with a cylinder), schedule from the autotuner."""
+    import torch
+    from ixchel2d_amd import case as C
+    from ixchel2d_amd.simple import Batch, Cavity
+
+    cases = C.batch_cases(args.batch, seed=0)
+    cav = [Cavity(c, solver=args.solver) for c in cases]
+    b = Batch(cav)
+    order, graph, table = b.autotune(trial_steps=3)
+    b.run(args.warmup, cap=1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b.run(args.steps, cap=1)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    cells = sum(c.mesh.mi * c.mesh.nj for c in cases)
+    ms_per_step = dt * 1e3 / args.steps
+    # algorithmic bytes of one step: the per-kernel array counts over every
+    # cavity; the v sweeps read T only with buoyancy; sources (cylinders) add
+    # Sc, Sp to the u, v, T sweeps and to the residual
+    iter_bytes = 0
+    for c in cases:
+        npts = (c.mesh.mi + 1) * (c.mesh.nj + 1)
+        arrays = sum(ALG_ARRAYS.values())
+        if c.params.derived()["Riy"] == 0.0:
+            arrays -= 2
+        if c.sources:
+            arrays += 2 * 7
+        iter_bytes += arrays * npts * 8
+    gbs = iter_bytes / (ms_per_step * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu:
+        from oracle.pyoracle import Oracle
+        sample = cases[:16]
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        t = time.perf_counter()
+        for c in sample:
+            Oracle(c).run(10)
+        per = time.perf_counter() - t
+        scells = sum(c.mesh.mi * c.mesh.nj for c in sample)
+        cpu = {"value": scells * 10 / per / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": threads, "kind": "port",
+               "sample": "C oracle (OpenMP), 10 iterations of each of the first 16 cavities, one after another"}
+    out = {
+        "metric": "aggregate Mcell-SIMPLE-iters/s, 256 mixed-size cavities",
+        "value": round(cells * args.steps / dt / 1e6, 2),
+        "unit": "Mcell-SIMPLE-iters/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.batch} cavities: sizes 65^2/129^2/257^2/129x301 cells, half lid-driven "
+                               "Re 100..3200, half natural convection Ra 1e3..1e6, 2 of 16 with a cylinder",
+                   "cavities": args.batch, "cells": cells, "solver": args.solver, "path": "fp64",
+                   "schedule": {"order": order, "graph": graph},
+                   "autotune_ms_per_step": {f"{k[0]}{'+graph' if k[1] else ''}": round(v, 4)
+                                            for k, v in table.items()}},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "whole step",
+                     "iteration_alg_bytes": iter_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--solver", default="pcr", choices=["pcr", "thomas"])
+    ap.add_argument("--workload", default="nc301x701", choices=["nc301x701", "batch256"])
+    ap.add_argument("--batch", type=int, default=256, help="cavities of the batch workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MB (reference kernel for "
@@ -106,6 +175,9 @@ def main():
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
+    if args.workload == "batch256":
+        batch_bench(args)
+        return
     if args.calibrate_copy_mb:
         n = args.calibrate_copy_mb * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")

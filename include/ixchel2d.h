@@ -171,6 +171,35 @@ int ixh_set_profiling(ixh_state *s, int32_t on);
 int ixh_kernel_times(ixh_state *s, double *ms, int32_t *launches, int32_t cap, int32_t *nkernels);
 const char *ixh_kernel_name(int32_t id);
 
+/* ---- batches: many independent cavities (any sizes up to 1023 x 1023,
+ * their own parameters, boundary conditions and sources, one common solver)
+ * advanced together.  Each phase of a time step is one launch over the lines
+ * of every cavity; a cavity's arithmetic and phase order are those of
+ * ixh_simple_iterations' sequential step, so its fields equal a run of that
+ * cavity alone.  No upstream counterpart: upstream runs one cavity per
+ * process (main.f90:348-1272). */
+typedef struct ixh_batch ixh_batch;
+/* workgroup orderings of a batch launch */
+enum {
+  IXH_ORDER_CAVITY = 0,  /* cavity by cavity, lines in order */
+  IXH_ORDER_LARGEST = 1, /* largest cavities first */
+  IXH_ORDER_XCD = 2      /* whole cavities binned onto the 8 XCDs (default) */
+};
+/* the states stay owned by the caller and must outlive the batch */
+int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out);
+int ixh_batch_destroy(ixh_batch *b);
+/* ordering, and whether each step runs as one captured HIP graph */
+int ixh_batch_set_schedule(ixh_batch *b, int32_t order, int32_t graph);
+/* nsteps single-pass time steps of every cavity (each needs
+ * iter_ecuaci_max < 0 and iter_simple_max < 0); the records of the first
+ * min(nsteps, cap) steps of cavity c go to hist[c*cap + k] */
+int ixh_batch_iterations(ixh_batch *b, int32_t nsteps, ixh_hist *hist, int32_t cap);
+/* time trial_steps steps under every ordering with and without graphs, keep
+ * the fastest and return it; ms_per_step (NULL or 6 entries, index
+ * 2*order + graph) receives the timings.  The trial steps are real time steps
+ * of every cavity (6 * (trial_steps + 4) in all), identical under every schedule. */
+int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_t *graph, double *ms_per_step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -28,7 +28,10 @@ EXPORTS = [
     "ixh_momentum_u", "ixh_momentum_v", "ixh_pcorr_zero", "ixh_pcorr", "ixh_correct", "ixh_energy",
     "ixh_residual_u", "ixh_lag", "ixh_simple_iterations", "ixh_nusselt_y", "ixh_nusselt_x",
     "ixh_tridiagonal", "ixh_set_profiling", "ixh_kernel_times", "ixh_kernel_name",
+    "ixh_batch_create", "ixh_batch_destroy", "ixh_batch_set_schedule", "ixh_batch_iterations",
+    "ixh_batch_autotune",
 ]
+ORDER_ID = {"cavity": 0, "largest": 1, "xcd": 2}
 
 
 class Grid(C.Structure):
@@ -98,6 +101,11 @@ def load(path: str = LIB_PATH):
     L.ixh_tridiagonal.argtypes = [dp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32]
     L.ixh_set_profiling.argtypes = [vp, C.c_int32]
     L.ixh_kernel_times.argtypes = [vp, dp, ip, C.c_int32, ip]
+    L.ixh_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]
+    L.ixh_batch_destroy.argtypes = [vp]
+    L.ixh_batch_set_schedule.argtypes = [vp, C.c_int32, C.c_int32]
+    L.ixh_batch_iterations.argtypes = [vp, C.c_int32, C.POINTER(Hist), C.c_int32]
+    L.ixh_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, dp]
     for name in EXPORTS:
         fn = getattr(L, name)
         if fn.restype is C.c_int or fn.restype is None:

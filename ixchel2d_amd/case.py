@@ -318,6 +318,48 @@ def cylinder_sources(case: Case, xc: float, yc: float, radius: float,
 
 
 # --------------------------------------------------------------------------
+# batches of uneven cavities
+# --------------------------------------------------------------------------
+BATCH_SIZES = ((66, 66, 1.0), (130, 130, 1.0), (258, 258, 1.0), (130, 302, 2.33))
+
+
+def batch_cases(n: int = 256, seed: int = 0, iters: int = 1000, lid_dt: float = 1.0,
+                nc_dt: float = 0.01) -> List[Case]:
+    """Mixed-size batch on uniform meshes: sizes 65^2, 129^2, 257^2 and
+    129 x 301 pressure cells (mi, nj = cells + 1; the last is a tall cavity
+    of height 2.33), drawn uniformly from `seed`.  Even indices are lid-driven
+    cavities with Re log-uniform in [100, 3200]; odd ones natural convection
+    with Ra log-uniform in [1e3, 1e6], Pr = 0.71.  Cavities 3 and 11 of every
+    16 hold a circular cylinder (radius 0.05..0.15 of the width, random
+    centre at least 0.05 from the walls) held at T = 1."""
+    rng = np.random.default_rng(seed)
+    out: List[Case] = []
+    for c in range(n):
+        mi, nj, h = BATCH_SIZES[int(rng.integers(len(BATCH_SIZES)))]
+        if c % 2 == 0:
+            re = float(10.0 ** rng.uniform(2.0, math.log10(3200.0)))
+            case = lid_driven_case(mi, Re=re, iters=iters, dt=lid_dt)
+            if nj != mi or h != 1.0:
+                mesh = make_mesh(mi, nj, 1.0, h, "unif")
+                case = Case(mesh, case.params, case.bcs, _zeros((mi, nj + 1)), _zeros((mi + 1, nj)),
+                            _zeros((mi + 1, nj + 1)), _zeros((mi + 1, nj + 1)))
+            case.name = f"b{c:03d}_lid{mi}x{nj}_Re{re:.0f}"
+        else:
+            ra = float(10.0 ** rng.uniform(3.0, 6.0))
+            case = natural_convection_case(mi, nj, height=h, Ra=ra, iters=iters, dt=nc_dt,
+                                           seed=int(rng.integers(1 << 30)), mesh_kind="unif")
+            case.name = f"b{c:03d}_nc{mi}x{nj}_Ra{ra:.3g}"
+        if c % 16 in (3, 11):
+            r = float(rng.uniform(0.05, 0.15))
+            xc = float(rng.uniform(r + 0.05, 1.0 - r - 0.05))
+            yc = float(rng.uniform(r + 0.05, h - r - 0.05))
+            cylinder_sources(case, xc, yc, r, t_body=1.0)
+            case.name += f"_cyl{xc:.2f},{yc:.2f},{r:.2f}"
+        out.append(case)
+    return out
+
+
+# --------------------------------------------------------------------------
 # upstream file formats
 # --------------------------------------------------------------------------
 def _d23(x: float) -> str:

@@ -295,120 +295,136 @@ int check_bcs(ixh_state *s, int var) {
 
 double *part_of(ixh_state *s, int k) { return s->part + (size_t)k * s->plen; }
 
-// ---- phases
-int sweep_u(ixh_state *s, int dir) {
+// ---- phases.  prep_* build the kernel arguments of one phase and advance
+// the buffer rotation (shared by the single-cavity and the batch paths);
+// the launchers run them on one cavity.
+int prep_u(ixh_state *s, int dir, SweepArgs &A) {
   CHK(check_bcs(s, IXH_VAR_U));
-  SweepArgs A = base_args(s, dir);
+  A = base_args(s, dir);
   set_bc(s, A, IXH_VAR_U, dir);
   const int out = pick(s->u_cur, s->u_ant, s->u_hold);
   set_out(A, dir, s->ub, s->ubT, out);
   A.acoef = dir == IXH_DIR_Y ? s->auT : s->au;
   A.acoef2 = dir == IXH_DIR_Y ? s->au : s->auT;
-  Timed tm(s, dir == IXH_DIR_Y ? K_U_Y : K_U_X);
-  if (dir == IXH_DIR_Y) {
-    A.line0 = 2;
-    A.nlines = s->mi - 2;
-    CHK((launch_sweep<EQ_U, DIR_Y>(s, A, s->nj + 1)));
-  } else {
-    A.line0 = 2;
-    A.nlines = s->nj - 1;
-    CHK((launch_sweep<EQ_U, DIR_X>(s, A, s->mi)));
-  }
-  CHK(tm.done());
+  A.line0 = 2;
+  A.nlines = dir == IXH_DIR_Y ? s->mi - 2 : s->nj - 1;
   s->u_cur = out;
   return 0;
 }
 
-int sweep_v(ixh_state *s, int dir) {
+int prep_v(ixh_state *s, int dir, SweepArgs &A) {
   CHK(check_bcs(s, IXH_VAR_V));
-  SweepArgs A = base_args(s, dir);
+  A = base_args(s, dir);
   set_bc(s, A, IXH_VAR_V, dir);
   A.acoef = dir == IXH_DIR_Y ? s->avT : s->av;
   A.acoef2 = dir == IXH_DIR_Y ? s->av : s->avT;
+  A.line0 = 2;
+  int out;
   if (dir == IXH_DIR_Y) {
     s->v_snap = s->v_cur;  // fv = v (main.f90:559-564)
-    const int out = pick(s->v_cur, s->v_ant);
-    set_out(A, dir, s->vb, s->vbT, out);
-    A.line0 = 2;
+    out = pick(s->v_cur, s->v_ant);
     A.nlines = s->mi - 1;
-    Timed tm(s, K_V_Y);
-    CHK((launch_sweep<EQ_V, DIR_Y>(s, A, s->nj)));
-    CHK(tm.done());
-    s->v_cur = out;
   } else {
-    const int out = pick(s->v_cur, s->v_ant, s->v_snap);
-    set_out(A, dir, s->vb, s->vbT, out);
+    out = pick(s->v_cur, s->v_ant, s->v_snap);
     A.snap = s->v_snap >= 0 ? s->vb[s->v_snap] : s->vb[s->v_cur];
     A.part0 = part_of(s, 0);
-    A.line0 = 2;
     A.nlines = s->nj - 2;
-    Timed tm(s, K_V_X);
-    CHK((launch_sweep<EQ_V, DIR_X>(s, A, s->mi + 1)));
-    CHK(tm.done());
-    s->v_cur = out;
   }
+  set_out(A, dir, s->vb, s->vbT, out);
+  s->v_cur = out;
   return 0;
 }
 
-int sweep_p(ixh_state *s, int dir) {
-  SweepArgs A = base_args(s, dir);
+int prep_p(ixh_state *s, int dir, SweepArgs &A) {
+  A = base_args(s, dir);
   A.in_zero = s->pc_zero ? 1 : 0;
+  A.line0 = 2;
+  if (dir == IXH_DIR_Y) s->pc_snap = s->pc_zero ? -1 : s->pc_cur;  // fcorr = p' (main.f90:762-767)
+  const int out = pick(s->pc_cur, s->pc_snap);
+  set_out(A, dir, s->pcb, s->pcbT, out);
   if (dir == IXH_DIR_Y) {
-    s->pc_snap = s->pc_zero ? -1 : s->pc_cur;  // fcorr = p' (main.f90:762-767)
-    const int out = pick(s->pc_cur, s->pc_snap);
-    set_out(A, dir, s->pcb, s->pcbT, out);
-    A.line0 = 2;
     A.nlines = s->mi - 1;
-    Timed tm(s, K_P_Y);
-    CHK((launch_sweep<EQ_P, DIR_Y>(s, A, s->nj + 1)));
-    CHK(tm.done());
-    s->pc_cur = out;
-    s->pc_zero = false;
   } else {
-    const int out = pick(s->pc_cur, s->pc_snap);
-    set_out(A, dir, s->pcb, s->pcbT, out);
     A.snap = pc_of(s, s->pc_snap);
     A.bo = s->bo;
     A.part0 = part_of(s, 1);
     A.part1 = part_of(s, 2);
-    A.line0 = 2;
     A.nlines = s->nj - 1;
-    Timed tm(s, K_P_X);
-    CHK((launch_sweep<EQ_P, DIR_X>(s, A, s->mi + 1)));
-    CHK(tm.done());
-    s->pc_cur = out;
-    s->pc_zero = false;
   }
+  s->pc_cur = out;
+  s->pc_zero = false;
   return 0;
 }
 
-int sweep_t(ixh_state *s, int dir) {
+int prep_t(ixh_state *s, int dir, SweepArgs &A) {
   CHK(check_bcs(s, IXH_VAR_T));
-  SweepArgs A = base_args(s, dir);
+  A = base_args(s, dir);
   set_bc(s, A, IXH_VAR_T, dir);
+  A.line0 = 2;
+  int out;
   if (dir == IXH_DIR_Y) {
     s->t_snap = s->t_cur;  // ftemp = T (main.f90:974-979)
-    const int out = pick(s->t_cur, s->t_ant);
-    set_out(A, dir, s->tb, s->tbT, out);
-    A.line0 = 2;
+    out = pick(s->t_cur, s->t_ant);
     A.nlines = s->mi - 1;
-    Timed tm(s, K_T_Y);
-    CHK((launch_sweep<EQ_T, DIR_Y>(s, A, s->nj + 1)));
-    CHK(tm.done());
-    s->t_cur = out;
   } else {
-    const int out = pick(s->t_cur, s->t_ant, s->t_snap);
-    set_out(A, dir, s->tb, s->tbT, out);
+    out = pick(s->t_cur, s->t_ant, s->t_snap);
     A.snap = s->t_snap >= 0 ? s->tb[s->t_snap] : s->tb[s->t_cur];
     A.part0 = part_of(s, 3);
-    A.line0 = 2;
     A.nlines = s->nj - 1;
-    Timed tm(s, K_T_X);
-    CHK((launch_sweep<EQ_T, DIR_X>(s, A, s->mi + 1)));
-    CHK(tm.done());
-    s->t_cur = out;
   }
+  set_out(A, dir, s->tb, s->tbT, out);
+  s->t_cur = out;
   return 0;
+}
+
+// line length of a sweep (sets the LDS size of the kernel)
+int line_len(const ixh_state *s, int eq, int dir) {
+  if (dir == IXH_DIR_X) return eq == EQ_U ? s->mi : s->mi + 1;
+  return eq == EQ_V ? s->nj : s->nj + 1;
+}
+
+int sweep_u(ixh_state *s, int dir) {
+  SweepArgs A;
+  CHK(prep_u(s, dir, A));
+  Timed tm(s, dir == IXH_DIR_Y ? K_U_Y : K_U_X);
+  if (dir == IXH_DIR_Y)
+    CHK((launch_sweep<EQ_U, DIR_Y>(s, A, line_len(s, EQ_U, dir))));
+  else
+    CHK((launch_sweep<EQ_U, DIR_X>(s, A, line_len(s, EQ_U, dir))));
+  return tm.done();
+}
+
+int sweep_v(ixh_state *s, int dir) {
+  SweepArgs A;
+  CHK(prep_v(s, dir, A));
+  Timed tm(s, dir == IXH_DIR_Y ? K_V_Y : K_V_X);
+  if (dir == IXH_DIR_Y)
+    CHK((launch_sweep<EQ_V, DIR_Y>(s, A, line_len(s, EQ_V, dir))));
+  else
+    CHK((launch_sweep<EQ_V, DIR_X>(s, A, line_len(s, EQ_V, dir))));
+  return tm.done();
+}
+
+int sweep_p(ixh_state *s, int dir) {
+  SweepArgs A;
+  CHK(prep_p(s, dir, A));
+  Timed tm(s, dir == IXH_DIR_Y ? K_P_Y : K_P_X);
+  if (dir == IXH_DIR_Y)
+    CHK((launch_sweep<EQ_P, DIR_Y>(s, A, line_len(s, EQ_P, dir))));
+  else
+    CHK((launch_sweep<EQ_P, DIR_X>(s, A, line_len(s, EQ_P, dir))));
+  return tm.done();
+}
+
+int sweep_t(ixh_state *s, int dir) {
+  SweepArgs A;
+  CHK(prep_t(s, dir, A));
+  Timed tm(s, dir == IXH_DIR_Y ? K_T_Y : K_T_X);
+  if (dir == IXH_DIR_Y)
+    CHK((launch_sweep<EQ_T, DIR_Y>(s, A, line_len(s, EQ_T, dir))));
+  else
+    CHK((launch_sweep<EQ_T, DIR_X>(s, A, line_len(s, EQ_T, dir))));
+  return tm.done();
 }
 
 // move field f (IXH_U/V/T or their _ANT) to a buffer of its own when the
@@ -428,8 +444,15 @@ int unalias(ixh_state *s, int f) {
   return 0;
 }
 
-int do_correct(ixh_state *s) {
-  CorrArgs A;
+// arguments of the correction; false when p' is zero (nothing changes)
+int prep_correct(ixh_state *s, CorrArgs &A, bool &active) {
+  memset(&A, 0, sizeof A);
+  active = !s->pc_zero;
+  if (!active) return 0;
+  // the correction updates u, v in place; give them their own buffers when
+  // they still share one with the lagged copies
+  CHK(unalias(s, IXH_U));
+  CHK(unalias(s, IXH_V));
   A.mi = s->mi;
   A.nj = s->nj;
   A.P = s->P;
@@ -440,25 +463,29 @@ int do_correct(ixh_state *s) {
   A.pc = s->pcb[s->pc_cur];
   A.au = s->au;
   A.av = s->av;
-  if (s->pc_zero) return 0;  // p' = 0: nothing changes
-  // the correction updates u, v in place; give them their own buffers when
-  // they still share one with the lagged copies
-  CHK(unalias(s, IXH_U));
-  CHK(unalias(s, IXH_V));
   A.u = s->ub[s->u_cur];
   A.v = s->vb[s->v_cur];
   A.uT = s->ubT[s->u_cur];
   A.vT = s->vbT[s->v_cur];
-  dim3 block(kBlock), grid((s->mi + 1 + kTile - 1) / kTile, (s->nj + 1 + kTile - 1) / kTile);
-  Timed tm(s, K_CORR);
-  hipLaunchKernelGGL(k_correct, grid, block, 0, s->launch, A);
-  HIPCHK(hipGetLastError());
-  CHK(tm.done());
   return 0;
 }
 
-int do_residual(ixh_state *s) {
-  ResArgs A;
+int corr_tiles_x(const ixh_state *s) { return (s->mi + 1 + kTile - 1) / kTile; }
+int corr_tiles_y(const ixh_state *s) { return (s->nj + 1 + kTile - 1) / kTile; }
+
+int do_correct(ixh_state *s) {
+  CorrArgs A;
+  bool active;
+  CHK(prep_correct(s, A, active));
+  if (!active) return 0;
+  dim3 block(kBlock), grid(corr_tiles_x(s), corr_tiles_y(s));
+  Timed tm(s, K_CORR);
+  hipLaunchKernelGGL(k_correct, grid, block, 0, s->launch, A);
+  HIPCHK(hipGetLastError());
+  return tm.done();
+}
+
+void prep_residual(ixh_state *s, ResArgs &A) {
   memset(&A, 0, sizeof A);
   A.mi = s->mi;
   A.nj = s->nj;
@@ -474,15 +501,18 @@ int do_residual(ixh_state *s) {
   A.sp = s->has_src[0] ? s->src_p[0] : nullptr;
   A.resu = s->resu;
   A.part = part_of(s, 4);
+}
+
+int do_residual(ixh_state *s) {
+  ResArgs A;
+  prep_residual(s, A);
   Timed tm(s, K_RES);
   hipLaunchKernelGGL(k_residual_u, dim3(s->nj - 1), dim3(kBlock), 0, s->launch, A);
   HIPCHK(hipGetLastError());
-  CHK(tm.done());
-  return 0;
+  return tm.done();
 }
 
-int do_finalize(ixh_state *s, int which, double *out, int *slot) {
-  FinArgs F;
+void prep_finalize(ixh_state *s, FinArgs &F, int which, double *out, int *slot) {
   F.nj = s->nj;
   F.pv = part_of(s, 0);
   F.pp = part_of(s, 1);
@@ -492,11 +522,15 @@ int do_finalize(ixh_state *s, int which, double *out, int *slot) {
   F.out = out;
   F.slot = slot;
   F.which = which;
+}
+
+int do_finalize(ixh_state *s, int which, double *out, int *slot) {
+  FinArgs F;
+  prep_finalize(s, F, which, out, slot);
   Timed tm(s, K_FIN);
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s->launch, F);
   HIPCHK(hipGetLastError());
-  CHK(tm.done());
-  return 0;
+  return tm.done();
 }
 
 // fold the selected norms and copy them to the host
@@ -575,6 +609,19 @@ int pipe_step(ixh_state *s, bool last) {
   return 0;
 }
 
+// streams and events of the pipelined step, created on first use (states
+// that only run in batches never need them)
+int ensure_aux(ixh_state *s) {
+  if (s->aux[0]) return 0;
+  for (int k = 0; k < 2; ++k) {
+    HIPCHK(hipStreamCreate(&s->aux[k]));
+    HIPCHK(hipEventCreateWithFlags(&s->ev_j[k], hipEventDisableTiming));
+  }
+  HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
+  HIPCHK(hipEventCreateWithFlags(&s->ev_t, hipEventDisableTiming));
+  return 0;
+}
+
 int ensure_hist(ixh_state *s, int n) {
   if (n <= s->hist_cap) return 0;
   if (s->d_hist) HIPCHK(hipFree(s->d_hist));
@@ -619,13 +666,6 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   if (hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess)
     return bail(fail(IXH_ERR_HIP, "hipEventCreate failed"));
   s->launch = s->stream;
-  for (int k = 0; k < 2; ++k)
-    if (hipStreamCreate(&s->aux[k]) != hipSuccess ||
-        hipEventCreateWithFlags(&s->ev_j[k], hipEventDisableTiming) != hipSuccess)
-      return bail(fail(IXH_ERR_HIP, "auxiliary stream/event creation failed"));
-  if (hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&s->ev_t, hipEventDisableTiming) != hipSuccess)
-    return bail(fail(IXH_ERR_HIP, "hipEventCreate failed"));
   // geometry on the host (mod_malla.f90:167-217), 1-based
   const int gx = mi + 2, gy = nj + 2;
   std::vector<double> xu(gx, 0.0), yv(gy, 0.0), xp(gx, 0.0), yp(gy, 0.0);
@@ -971,6 +1011,7 @@ int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t 
       // direct launches: on this stack a captured HIP graph of the same step
       // ran slower (145.7 us vs 140.0 us per step sequential, 155.5 us vs
       // 138.5 us pipelined, 301x701 FP64)
+      CHK(ensure_aux(s));
       CHK(sweep_u(s, IXH_DIR_Y));  // prologue: u sweeps of the first step
       CHK(sweep_u(s, IXH_DIR_X));
       for (int k = 0; k < nsteps; ++k) CHK(pipe_step(s, k == nsteps - 1));
@@ -1161,6 +1202,434 @@ int ixh_kernel_times(ixh_state *s, double *ms, int32_t *launches, int32_t cap, i
     if (launches) launches[k] = s->kcount[k];
   }
   if (nk) *nk = K_NUM;
+  return 0;
+}
+
+}  // extern "C"
+
+// ====================================================================== batch
+// Many cavities advanced together.  Each phase of the single-pass time step
+// is one launch over the lines (tiles, rows) of every cavity, through a
+// workgroup -> (cavity, unit) table; per cavity the arithmetic and the phase
+// order are those of ixh_simple_iterations' sequential step, so a cavity's
+// fields match a run of that cavity alone bit for bit.
+namespace {
+
+enum BatchPhase { PH_U_Y = 0, PH_U_X, PH_V_Y, PH_V_X, PH_P_Y, PH_P_X, PH_T_Y, PH_T_X, PH_CORR, PH_RES, PH_NUM };
+const int kPhEq[8] = {EQ_U, EQ_U, EQ_V, EQ_V, EQ_P, EQ_P, EQ_T, EQ_T};
+const int kPhDir[8] = {IXH_DIR_Y, IXH_DIR_X, IXH_DIR_Y, IXH_DIR_X, IXH_DIR_Y, IXH_DIR_X, IXH_DIR_Y, IXH_DIR_X};
+constexpr int kXcd = 8;          // workgroup b of a launch runs on XCD b % 8
+constexpr int kRotLen = 12;      // ints of a cavity's rotation state
+constexpr int kMaxCached = 16;   // distinct rotation states kept per call
+
+// first unit and unit count of a phase on one cavity (lines, rows or tiles)
+void phase_units(const ixh_state *s, int ph, int *first, int *count) {
+  *first = 2;
+  switch (ph) {
+    case PH_U_Y: *count = s->mi - 2; break;
+    case PH_U_X: case PH_P_X: case PH_T_X: case PH_RES: *count = s->nj - 1; break;
+    case PH_V_X: *count = s->nj - 2; break;
+    case PH_CORR: *first = 0; *count = corr_tiles_x(s) * corr_tiles_y(s); break;
+    default: *count = s->mi - 1; break;  // v, p', T y-sweeps
+  }
+}
+
+void rot_get(const ixh_state *s, int *r) {
+  r[0] = s->u_cur; r[1] = s->u_ant; r[2] = s->v_cur; r[3] = s->v_ant; r[4] = s->t_cur; r[5] = s->t_ant;
+  r[6] = s->pc_cur; r[7] = s->v_snap; r[8] = s->t_snap; r[9] = s->pc_snap; r[10] = s->pc_zero; r[11] = s->u_hold;
+}
+
+void rot_set(ixh_state *s, const int *r) {
+  s->u_cur = r[0]; s->u_ant = r[1]; s->v_cur = r[2]; s->v_ant = r[3]; s->t_cur = r[4]; s->t_ant = r[5];
+  s->pc_cur = r[6]; s->v_snap = r[7]; s->t_snap = r[8]; s->pc_snap = r[9]; s->pc_zero = r[10] != 0;
+  s->u_hold = r[11];
+}
+
+// kernel arguments of one time step of every cavity, for one rotation state
+struct BatchStep {
+  std::vector<int> key, post;  // rotation states before and after the step
+  char *d = nullptr;           // SweepArgs[8][n], CorrArgs[n], ResArgs[n]
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+}  // namespace
+
+struct ixh_batch {
+  std::vector<ixh_state *> st;
+  int n = 0, solver = 0;
+  int ept[8] = {};
+  int order = IXH_ORDER_XCD;
+  bool graph = false;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  int2 *d_wg = nullptr;
+  size_t wg_cap = 0;
+  size_t wg_off[PH_NUM] = {};
+  int nwg[PH_NUM] = {};
+  std::vector<BatchStep> steps;
+  FinArgs *d_fin = nullptr;
+  double *d_hist = nullptr;
+  size_t hist_len = 0;
+  int *d_slot = nullptr;
+};
+
+namespace {
+
+void batch_clear_steps(ixh_batch *b) {
+  for (BatchStep &E : b->steps) {
+    if (E.exec) (void)hipGraphExecDestroy(E.exec);
+    if (E.graph) (void)hipGraphDestroy(E.graph);
+    if (E.d) (void)hipFree(E.d);
+  }
+  b->steps.clear();
+}
+
+// workgroup tables of every phase for the current ordering
+int batch_tables(ixh_batch *b) {
+  const int n = b->n;
+  std::vector<int> byc(n);
+  for (int c = 0; c < n; ++c) byc[c] = c;
+  auto cells = [&](int c) { return (long long)b->st[c]->mi * b->st[c]->nj; };
+  if (b->order != IXH_ORDER_CAVITY)
+    std::stable_sort(byc.begin(), byc.end(), [&](int x, int y) { return cells(x) > cells(y); });
+  // XCD bins: longest-processing-time-first assignment of whole cavities
+  std::vector<int> bin(n, 0);
+  if (b->order == IXH_ORDER_XCD) {
+    long long load[kXcd] = {};
+    for (int c : byc) {
+      int k = 0;
+      for (int x = 1; x < kXcd; ++x)
+        if (load[x] < load[k]) k = x;
+      bin[c] = k;
+      load[k] += cells(c);
+    }
+  }
+  std::vector<int2> all;
+  for (int ph = 0; ph < PH_NUM; ++ph) {
+    b->wg_off[ph] = all.size();
+    if (b->order != IXH_ORDER_XCD) {
+      for (int c : byc) {
+        int f, m;
+        phase_units(b->st[c], ph, &f, &m);
+        for (int k = 0; k < m; ++k) all.push_back(make_int2(c, f + k));
+      }
+    } else {
+      std::vector<int2> lists[kXcd];
+      for (int c : byc) {
+        int f, m;
+        phase_units(b->st[c], ph, &f, &m);
+        for (int k = 0; k < m; ++k) lists[bin[c]].push_back(make_int2(c, f + k));
+      }
+      size_t rounds = 0;
+      for (int x = 0; x < kXcd; ++x) rounds = std::max(rounds, lists[x].size());
+      for (size_t r = 0; r < rounds; ++r)
+        for (int x = 0; x < kXcd; ++x)
+          all.push_back(r < lists[x].size() ? lists[x][r] : make_int2(-1, -1));
+    }
+    b->nwg[ph] = (int)(all.size() - b->wg_off[ph]);
+  }
+  if (all.size() > b->wg_cap) {
+    if (b->d_wg) HIPCHK(hipFree(b->d_wg));
+    b->d_wg = nullptr;
+    HIPCHK(hipMalloc((void **)&b->d_wg, all.size() * sizeof(int2)));
+    b->wg_cap = all.size();
+  }
+  HIPCHK(hipMemcpy(b->d_wg, all.data(), all.size() * sizeof(int2), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// build the kernel arguments of one step of every cavity (advancing their
+// rotation) and upload them
+int batch_prepare(ixh_batch *b, BatchStep &E) {
+  const int n = b->n;
+  std::vector<SweepArgs> sw(8 * (size_t)n);
+  std::vector<CorrArgs> co(n);
+  std::vector<ResArgs> re(n);
+  for (int c = 0; c < n; ++c) {
+    ixh_state *s = b->st[c];
+    CHK(prep_u(s, IXH_DIR_Y, sw[PH_U_Y * (size_t)n + c]));
+    CHK(prep_u(s, IXH_DIR_X, sw[PH_U_X * (size_t)n + c]));
+    CHK(prep_v(s, IXH_DIR_Y, sw[PH_V_Y * (size_t)n + c]));
+    CHK(prep_v(s, IXH_DIR_X, sw[PH_V_X * (size_t)n + c]));
+    s->pc_zero = true;
+    CHK(prep_p(s, IXH_DIR_Y, sw[PH_P_Y * (size_t)n + c]));
+    CHK(prep_p(s, IXH_DIR_X, sw[PH_P_X * (size_t)n + c]));
+    // after the sweeps u, v never share a buffer with u_ant, v_ant, so the
+    // correction's unaliasing issues no copy here
+    bool active = false;
+    CHK(prep_correct(s, co[c], active));
+    if (!active) return fail(IXH_ERR_ARG, "batch: p' is zero after its sweeps");
+    CHK(prep_t(s, IXH_DIR_Y, sw[PH_T_Y * (size_t)n + c]));
+    CHK(prep_t(s, IXH_DIR_X, sw[PH_T_X * (size_t)n + c]));
+    prep_residual(s, re[c]);
+    s->u_ant = s->u_cur;
+    s->v_ant = s->v_cur;
+    s->t_ant = s->t_cur;
+  }
+  const size_t bs = sw.size() * sizeof(SweepArgs), bc = co.size() * sizeof(CorrArgs),
+               br = re.size() * sizeof(ResArgs);
+  HIPCHK(hipMalloc((void **)&E.d, bs + bc + br));
+  HIPCHK(hipMemcpy(E.d, sw.data(), bs, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(E.d + bs, co.data(), bc, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(E.d + bs + bc, re.data(), br, hipMemcpyHostToDevice));
+  return 0;
+}
+
+template <int EQ, int DIR, int SOLVER>
+int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
+  dim3 grid(b->nwg[ph]), block(kBlock);
+  const int2 *wg = b->d_wg + b->wg_off[ph];
+  switch (b->ept[ph]) {
+    case 1: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 1>), grid, block, 0, st, D, wg); break;
+    case 2: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 2>), grid, block, 0, st, D, wg); break;
+    case 3: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 3>), grid, block, 0, st, D, wg); break;
+    default: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 4>), grid, block, 0, st, D, wg); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <int EQ, int DIR>
+int batch_sweep(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
+  if (b->nwg[ph] == 0) return 0;
+  D += (size_t)ph * b->n;
+  if (b->solver == IXH_SOLVER_THOMAS) return batch_sweep_t<EQ, DIR, SOLVE_THOMAS>(b, ph, D, st);
+  if (b->solver == IXH_SOLVER_PCR_EXACT) return batch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT>(b, ph, D, st);
+  return batch_sweep_t<EQ, DIR, SOLVE_PCR>(b, ph, D, st);
+}
+
+// the launches of one step, in the sequential order of a single cavity
+int batch_launch(ixh_batch *b, const BatchStep &E, hipStream_t st, int cap) {
+  const int n = b->n;
+  const SweepArgs *sw = (const SweepArgs *)E.d;
+  const CorrArgs *co = (const CorrArgs *)(E.d + 8 * (size_t)n * sizeof(SweepArgs));
+  const ResArgs *re = (const ResArgs *)((const char *)co + (size_t)n * sizeof(CorrArgs));
+  CHK((batch_sweep<EQ_U, DIR_Y>(b, PH_U_Y, sw, st)));
+  CHK((batch_sweep<EQ_U, DIR_X>(b, PH_U_X, sw, st)));
+  CHK((batch_sweep<EQ_V, DIR_Y>(b, PH_V_Y, sw, st)));
+  CHK((batch_sweep<EQ_V, DIR_X>(b, PH_V_X, sw, st)));
+  CHK((batch_sweep<EQ_P, DIR_Y>(b, PH_P_Y, sw, st)));
+  CHK((batch_sweep<EQ_P, DIR_X>(b, PH_P_X, sw, st)));
+  hipLaunchKernelGGL(k_correct_batch, dim3(b->nwg[PH_CORR]), dim3(kBlock), 0, st, co,
+                     (const int2 *)(b->d_wg + b->wg_off[PH_CORR]));
+  HIPCHK(hipGetLastError());
+  CHK((batch_sweep<EQ_T, DIR_Y>(b, PH_T_Y, sw, st)));
+  CHK((batch_sweep<EQ_T, DIR_X>(b, PH_T_X, sw, st)));
+  hipLaunchKernelGGL(k_residual_u_batch, dim3(b->nwg[PH_RES]), dim3(kBlock), 0, st, re,
+                     (const int2 *)(b->d_wg + b->wg_off[PH_RES]));
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_finalize_batch, dim3(n), dim3(kBlock), 0, st, (const FinArgs *)b->d_fin,
+                     (const int *)b->d_slot, cap);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, st, b->d_slot, 1);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// one time step of every cavity: look up (or build) the arguments for the
+// current rotation state, then launch them directly or as a graph
+int batch_step(ixh_batch *b, int cap, std::vector<int> &key) {
+  const int n = b->n;
+  key.resize((size_t)n * kRotLen);
+  for (int c = 0; c < n; ++c) rot_get(b->st[c], &key[(size_t)c * kRotLen]);
+  BatchStep *E = nullptr;
+  for (BatchStep &x : b->steps)
+    if (x.key == key) E = &x;
+  if (E) {
+    for (int c = 0; c < n; ++c) rot_set(b->st[c], &E->post[(size_t)c * kRotLen]);
+  } else {
+    if ((int)b->steps.size() >= kMaxCached) {
+      HIPCHK(hipStreamSynchronize(b->stream));
+      batch_clear_steps(b);
+    }
+    b->steps.emplace_back();
+    E = &b->steps.back();
+    E->key = key;
+    CHK(batch_prepare(b, *E));
+    E->post.resize(key.size());
+    for (int c = 0; c < n; ++c) rot_get(b->st[c], &E->post[(size_t)c * kRotLen]);
+  }
+  if (!b->graph) return batch_launch(b, *E, b->stream, cap);
+  if (!E->exec) {
+    HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_launch(b, *E, b->stream, cap);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(b->stream, &g);
+    CHK(rc);
+    if (e != hipSuccess) return fail(IXH_ERR_HIP, "graph capture: %s", hipGetErrorString(e));
+    E->graph = g;
+    HIPCHK(hipGraphInstantiate(&E->exec, g, nullptr, nullptr, 0));
+  }
+  HIPCHK(hipGraphLaunch(E->exec, b->stream));
+  return 0;
+}
+
+// wait for work queued on the cavities' own streams
+int batch_join_cavities(ixh_batch *b) {
+  for (ixh_state *s : b->st) HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+// per-cavity fold arguments and the history buffer for `cap` records
+int batch_begin(ixh_batch *b, int cap) {
+  const int n = b->n;
+  const size_t per = (size_t)8 * (cap + 1);
+  if (per * n > b->hist_len) {
+    if (b->d_hist) HIPCHK(hipFree(b->d_hist));
+    b->d_hist = nullptr;
+    CHK(dmalloc(&b->d_hist, per * n));
+    b->hist_len = per * n;
+  }
+  std::vector<FinArgs> fin(n);
+  for (int c = 0; c < n; ++c) prep_finalize(b->st[c], fin[c], 31, b->d_hist + per * c, nullptr);
+  HIPCHK(hipMemcpy(b->d_fin, fin.data(), n * sizeof(FinArgs), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, b->stream, b->d_slot, 0);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// the arguments depend on each cavity's parameters, sources and boundary
+// conditions, which may change between calls: rebuild them per call
+int batch_run(ixh_batch *b, int nsteps, int cap) {
+  CHK(batch_join_cavities(b));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  batch_clear_steps(b);
+  CHK(batch_begin(b, cap));
+  std::vector<int> key;
+  for (int k = 0; k < nsteps; ++k) CHK(batch_step(b, cap, key));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
+  if (!states || !out || n <= 0) return fail(IXH_ERR_ARG, "batch: bad arguments");
+  *out = nullptr;
+  const int solver = states[0] ? states[0]->prm.solver : 0;
+  for (int c = 0; c < n; ++c) {
+    const ixh_state *s = states[c];
+    if (!s) return fail(IXH_ERR_ARG, "batch: state %d is NULL", c);
+    for (int d = 0; d < c; ++d)
+      if (states[d] == s) return fail(IXH_ERR_ARG, "batch: state %d appears twice", c);
+    if (s->prm.solver != solver) return fail(IXH_ERR_ARG, "batch: every cavity must use the same solver");
+    if (s->mi + 1 > 4 * kBlock || s->nj + 1 > 4 * kBlock)
+      return fail(IXH_ERR_UNSUPP, "batch: lines longer than %d nodes are not built (cavity %d)", 4 * kBlock, c);
+  }
+  ixh_batch *b = new ixh_batch();
+  b->st.assign(states, states + n);
+  b->n = n;
+  b->solver = solver;
+  for (int ph = 0; ph < 8; ++ph) {
+    int m = 1;
+    for (ixh_state *s : b->st) m = std::max(m, line_len(s, kPhEq[ph], kPhDir[ph]));
+    b->ept[ph] = ept_for(m);
+  }
+  auto bail = [&](int code) {
+    ixh_batch_destroy(b);
+    return code;
+  };
+  if (hipStreamCreate(&b->stream) != hipSuccess || hipEventCreate(&b->ev0) != hipSuccess ||
+      hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(fail(IXH_ERR_HIP, "batch: stream/event creation failed"));
+  if (hipMalloc((void **)&b->d_fin, n * sizeof(FinArgs)) != hipSuccess ||
+      hipMalloc((void **)&b->d_slot, 256) != hipSuccess)
+    return bail(fail(IXH_ERR_NOMEM, "batch: allocation failed"));
+  if (int rc = batch_tables(b)) return bail(rc);
+  *out = b;
+  return 0;
+}
+
+int ixh_batch_destroy(ixh_batch *b) {
+  if (!b) return 0;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  batch_clear_steps(b);
+  if (b->d_wg) (void)hipFree(b->d_wg);
+  if (b->d_fin) (void)hipFree(b->d_fin);
+  if (b->d_hist) (void)hipFree(b->d_hist);
+  if (b->d_slot) (void)hipFree(b->d_slot);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int ixh_batch_set_schedule(ixh_batch *b, int32_t order, int32_t graph) {
+  if (!b) return fail(IXH_ERR_ARG, "batch is NULL");
+  if (order != IXH_ORDER_CAVITY && order != IXH_ORDER_LARGEST && order != IXH_ORDER_XCD)
+    return fail(IXH_ERR_ARG, "batch: unknown order %d", order);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  batch_clear_steps(b);
+  b->order = order;
+  b->graph = graph != 0;
+  return batch_tables(b);
+}
+
+int ixh_batch_iterations(ixh_batch *b, int32_t nsteps, ixh_hist *hist, int32_t cap) {
+  if (!b || nsteps < 0 || cap < 0 || (cap > 0 && !hist)) return fail(IXH_ERR_ARG, "batch: bad arguments");
+  for (ixh_state *s : b->st) {
+    for (int v = 0; v < 3; ++v) CHK(check_bcs(s, v));
+    if (s->prm.iter_ecuaci_max >= 0 || s->prm.iter_simple_max >= 0)
+      return fail(IXH_ERR_UNSUPP, "batch: every cavity needs single-pass loops "
+                                  "(iter_ecuaci_max < 0 and iter_simple_max < 0)");
+    if (s->prm.solver != b->solver) return fail(IXH_ERR_ARG, "batch: a cavity changed its solver");
+  }
+  const int keep = std::min(nsteps, cap);
+  CHK(batch_run(b, nsteps, keep));
+  const int n = b->n;
+  const size_t per = (size_t)8 * (keep + 1);
+  std::vector<double> h(per * n);
+  HIPCHK(hipMemcpyAsync(h.data(), b->d_hist, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (int c = 0; c < n; ++c) {
+    const ixh_params &p = b->st[c]->prm;
+    for (int k = 0; k < keep; ++k) {
+      const double *o = &h[per * c + 8 * (size_t)k];
+      ixh_hist &r = hist[(size_t)c * cap + k];
+      r.err_v = o[0];
+      r.err_p = o[1];
+      r.maxbo = o[2];
+      r.err_t = o[3];
+      r.residuo = o[4];
+      r.passes_mom = r.passes_p = r.passes_t = 1;
+      r.simple_iter = 0;
+      r.logged = (r.maxbo < p.conv_paso && r.residuo < p.conv_resi) ? 1 : 0;
+      r.pad_ = 0;
+    }
+  }
+  return 0;
+}
+
+int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_t *graph, double *ms_per_step) {
+  if (!b || trial_steps < 1) return fail(IXH_ERR_ARG, "batch: bad arguments");
+  int best = -1;
+  double best_ms = 0.0;
+  for (int cfg = 0; cfg < 6; ++cfg) {
+    CHK(ixh_batch_set_schedule(b, cfg >> 1, cfg & 1));
+    // untimed steps build the arguments (and graphs) of the rotation states
+    // the timed steps meet
+    CHK(batch_run(b, 2, 0));
+    std::vector<int> key;
+    for (int k = 0; k < 2; ++k) CHK(batch_step(b, 0, key));
+    HIPCHK(hipEventRecord(b->ev0, b->stream));
+    for (int k = 0; k < trial_steps; ++k) CHK(batch_step(b, 0, key));
+    HIPCHK(hipEventRecord(b->ev1, b->stream));
+    HIPCHK(hipEventSynchronize(b->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    const double per = ms / trial_steps;
+    if (ms_per_step) ms_per_step[cfg] = per;
+    if (best < 0 || per < best_ms) {
+      best = cfg;
+      best_ms = per;
+    }
+  }
+  CHK(ixh_batch_set_schedule(b, best >> 1, best & 1));
+  if (order) *order = best >> 1;
+  if (graph) *graph = best & 1;
   return 0;
 }
 

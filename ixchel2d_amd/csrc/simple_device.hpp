@@ -638,15 +638,15 @@ struct Layout {
   }
 };
 
+// One line of a sweep, run by one workgroup; `line` is 1-based.
 template <int EQ, int DIR, int SOLVER, int EPT>
-__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
+__device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
   __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
   __shared__ double red[kBlock];
   constexpr bool TR = DIR == DIR_Y;
   const int P = A.P;
   const int P2 = A.P2;
   const int mi = A.mi, nj = A.nj;
-  const int line = A.line0 + xcd_line((int)blockIdx.x, A.nlines);
   const int n = Layout<EQ, DIR>::n(mi, nj);
   const int hi = Layout<EQ, DIR>::hi(mi, nj);
   const int lastl = Layout<EQ, DIR>::last_line(mi, nj);
@@ -770,6 +770,22 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
   }
 }
 
+template <int EQ, int DIR, int SOLVER, int EPT>
+__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
+  sweep_line<EQ, DIR, SOLVER, EPT>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines));
+}
+
+// A batch of cavities: workgroup b runs line wg[b].y of cavity wg[b].x, whose
+// arguments are D[wg[b].x] (same kernel body as the single-cavity sweep).
+// Entries with wg[b].x < 0 pad a table to a multiple of the XCD count.
+template <int EQ, int DIR, int SOLVER, int EPT>
+__global__ void __launch_bounds__(kBlock) k_sweep_batch(const SweepArgs *__restrict__ D,
+                                                        const int2 *__restrict__ wg) {
+  const int2 w = wg[blockIdx.x];
+  if (w.x < 0) return;
+  sweep_line<EQ, DIR, SOLVER, EPT>(D[w.x], w.y);
+}
+
 // ------------------------------------------------------------- correction
 // p += 0.6 p' on ii 2..mi, jj 2..nj; u += dyu (p'_P - p'_E)/a_u on ii 2..mi-1,
 // jj 2..nj; v += dxv (p'_P - p'_N)/a_v on ii 2..mi, jj 2..nj-1
@@ -786,12 +802,12 @@ struct CorrArgs {
 
 constexpr int kTile = 32;
 
-__global__ void __launch_bounds__(kBlock) k_correct(CorrArgs A) {
+__device__ __forceinline__ void correct_tile(const CorrArgs &A, const int bx, const int by) {
   __shared__ double tu[kTile][kTile + 1], tv[kTile][kTile + 1], tp[kTile][kTile + 1];
   constexpr bool TR = false;
   const int P = A.P;
   const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x >> 5;  // 32 x 8
-  const int i0 = 1 + (int)blockIdx.x * kTile, j0 = 1 + (int)blockIdx.y * kTile;
+  const int i0 = 1 + bx * kTile, j0 = 1 + by * kTile;
   for (int r = ty; r < kTile; r += kBlock / kTile) {
     const int ii = i0 + tx, jj = j0 + r;
     if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) continue;
@@ -822,6 +838,21 @@ __global__ void __launch_bounds__(kBlock) k_correct(CorrArgs A) {
   }
 }
 
+__global__ void __launch_bounds__(kBlock) k_correct(CorrArgs A) {
+  correct_tile(A, (int)blockIdx.x, (int)blockIdx.y);
+}
+
+// batch: workgroup b corrects tile (wg[b].y % tiles_x, wg[b].y / tiles_x) of
+// cavity wg[b].x
+__global__ void __launch_bounds__(kBlock) k_correct_batch(const CorrArgs *__restrict__ D,
+                                                          const int2 *__restrict__ wg) {
+  const int2 w = wg[blockIdx.x];
+  if (w.x < 0) return;
+  const CorrArgs &A = D[w.x];
+  const int tiles_x = (A.mi + 1 + kTile - 1) / kTile;
+  correct_tile(A, w.y % tiles_x, w.y / tiles_x);
+}
+
 // ---------------------------------------------------------------- residual
 // residuo_u (source/mod_ec_momento.f90:1107-1266): unrelaxed u residual on
 // ii 2..mi-1, jj 2..nj; one workgroup per row, per-row sum of squares.  The
@@ -835,13 +866,12 @@ struct ResArgs {
   double *resu, *part;
 };
 
-__global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
+__device__ __forceinline__ void residual_row(const ResArgs &A, const int jj) {
   __shared__ double red[kBlock];
   constexpr bool TR = false;
   const int P = A.P;
   const Geo &g = A.g;
   const double gm = A.ph.gm;
-  const int jj = 2 + xcd_line((int)blockIdx.x, A.nj - 1);
   double acc = 0.0;
   for (int ii = 2 + (int)threadIdx.x; ii <= A.mi - 1; ii += kBlock) {
     double ud = g.fexu[ii] * AT(A.u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(A.u, ii, jj);
@@ -879,6 +909,17 @@ __global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
   if (threadIdx.x == 0) A.part[jj] = s;
 }
 
+__global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
+  residual_row(A, 2 + xcd_line((int)blockIdx.x, A.nj - 1));
+}
+
+__global__ void __launch_bounds__(kBlock) k_residual_u_batch(const ResArgs *__restrict__ D,
+                                                             const int2 *__restrict__ wg) {
+  const int2 w = wg[blockIdx.x];
+  if (w.x < 0) return;
+  residual_row(D[w.x], w.y);
+}
+
 // ---------------------------------------------------------------- finalize
 // Fold per-line partials into the five norms of one SIMPLE iteration:
 // out[0] err_v (lines 2..nj-1), out[1] err_p, out[2] maxbo, out[3] err_t,
@@ -895,7 +936,7 @@ struct FinArgs {
   int which;
 };
 
-__global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) {
+__device__ __forceinline__ void finalize_body(const FinArgs &A) {
   __shared__ double red[5][kBlock / 64];
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
   for (int j = 2 + (int)threadIdx.x; j <= A.nj; j += kBlock) {
@@ -923,6 +964,24 @@ __global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) {
       if (A.which & (1 << k)) o[k] = sqrt(r[k]);
     if (A.slot) *A.slot = *A.slot + 1;
   }
+}
+
+__global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) { finalize_body(A); }
+
+// batch: one workgroup per cavity; the norms of step *slot go to
+// D[c].out + 8 * min(*slot, cap) (record `cap` is a discard row)
+__global__ void __launch_bounds__(kBlock) k_finalize_batch(const FinArgs *__restrict__ D, const int *slot,
+                                                           int cap) {
+  FinArgs A = D[blockIdx.x];
+  const int k = *slot;
+  A.out = A.out + 8 * (size_t)(k < cap ? k : cap);
+  A.slot = nullptr;
+  finalize_body(A);
+}
+
+// advance the batch step counter once every cavity has read it
+__global__ void k_add_int(int *p, int v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + v;
 }
 
 __global__ void k_set_int(int *p, int v) {

@@ -178,6 +178,35 @@ module ixchel_hip
        integer(c_int32_t), value :: cap
        integer(c_int32_t), intent(out) :: nk
      end function ixh_kernel_times
+     ! batches of independent cavities (no upstream counterpart)
+     integer(c_int) function ixh_batch_create(states, n, batch) bind(C, name='ixh_batch_create')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), intent(in) :: states(*)
+       integer(c_int32_t), value :: n
+       type(c_ptr), intent(out) :: batch
+     end function ixh_batch_create
+     integer(c_int) function ixh_batch_destroy(batch) bind(C, name='ixh_batch_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: batch
+     end function ixh_batch_destroy
+     integer(c_int) function ixh_batch_set_schedule(batch, order, graph) bind(C, name='ixh_batch_set_schedule')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: batch
+       integer(c_int32_t), value :: order, graph
+     end function ixh_batch_set_schedule
+     integer(c_int) function ixh_batch_iterations(batch, nsteps, hist, cap) bind(C, name='ixh_batch_iterations')
+       import :: c_int, c_ptr, c_int32_t, ixh_hist_t
+       type(c_ptr), value :: batch
+       integer(c_int32_t), value :: nsteps, cap
+       type(ixh_hist_t), intent(out) :: hist(*)
+     end function ixh_batch_iterations
+     integer(c_int) function ixh_batch_autotune(batch, trial_steps, order, graph, ms) bind(C, name='ixh_batch_autotune')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: batch
+       integer(c_int32_t), value :: trial_steps
+       integer(c_int32_t), intent(out) :: order, graph
+       real(c_double), intent(out) :: ms(*)
+     end function ixh_batch_autotune
   end interface
 
   ! the device state of this program's one cavity

@@ -192,6 +192,64 @@ class Cavity:
         return {self.L.ixh_kernel_name(k).decode(): (ms[k], cnt[k]) for k in range(nk.value)}
 
 
+def _records(hist, k0: int, k: int) -> Dict[str, np.ndarray]:
+    rec = {f: np.array([getattr(hist[k0 + i], f) for i in range(k)])
+           for f in ("err_v", "err_p", "maxbo", "err_t", "residuo")}
+    for f in ("passes_mom", "passes_p", "passes_t", "simple_iter", "logged"):
+        rec[f] = np.array([getattr(hist[k0 + i], f) for i in range(k)], dtype=np.int32)
+    return rec
+
+
+class Batch:
+    """Many cavities advanced together (``ixh_batch``): one launch per phase
+    covers the lines of every cavity.  The cavities stay usable on their own
+    (fields, Nusselt numbers) and must outlive the batch."""
+
+    def __init__(self, cavities, order: str = "xcd", graph: bool = False):
+        L = _lib.load()
+        self.L = L
+        self.cavities = list(cavities)
+        arr = (C.c_void_p * len(self.cavities))(*[c.h.value for c in self.cavities])
+        h = C.c_void_p()
+        _lib.check(L.ixh_batch_create(arr, len(self.cavities), C.byref(h)))
+        self.h = h
+        self.set_schedule(order, graph)
+
+    def set_schedule(self, order: str = "xcd", graph: bool = False) -> None:
+        _lib.check(self.L.ixh_batch_set_schedule(self.h, _lib.ORDER_ID[order], 1 if graph else 0))
+
+    def autotune(self, trial_steps: int = 10):
+        """Time every ordering with and without graphs, keep the fastest.
+        Returns (order, graph, {(order, graph): ms per step})."""
+        o, g = C.c_int32(), C.c_int32()
+        ms = (C.c_double * 6)()
+        _lib.check(self.L.ixh_batch_autotune(self.h, trial_steps, C.byref(o), C.byref(g), ms))
+        names = {v: k for k, v in _lib.ORDER_ID.items()}
+        table = {(names[k >> 1], bool(k & 1)): ms[k] for k in range(6)}
+        return names[o.value], bool(g.value), table
+
+    def run(self, nsteps: int, cap: Optional[int] = None):
+        """``nsteps`` single-pass time steps of every cavity; one history dict
+        per cavity (the first ``cap`` steps)."""
+        cap = max(1, nsteps if cap is None else cap)
+        n = len(self.cavities)
+        hist = (_lib.Hist * (n * cap))()
+        _lib.check(self.L.ixh_batch_iterations(self.h, nsteps, hist, cap))
+        k = min(nsteps, cap)
+        return [_records(hist, c * cap, k) for c in range(n)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.ixh_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def tridiagonal(a, b, c, r, solver: str = "pcr"):
     """Batched ``tridiagonal(a,b,c,r,n)`` on the device; the last axis is the
     system.  Returns (solution, overwritten b)."""
