@@ -4,7 +4,10 @@ Workload: the 301x701 differentially heated tall cavity (tanh beta = 1.2
 mesh, H/W = 2.33, Ra = 1e5, Pr = 0.71, dt = 0.003), all fields resident in
 HBM.  A step is one SIMPLE iteration (one time step with single-pass inner
 loops): the u, v, p', T line sweeps, the correction, the residual and the
-norms.  Prints one JSON line.
+norms.  The same workload also runs on the FP32-mixed path (sweeps and
+assembly in FP32; mass residual, p' right-hand side, reductions, p and the
+Nusselt integral in FP64), reported under "fp32_mixed" with its speed-up and
+its error against FP64.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver pcr|thomas]
   python bench.py --workload batch256   # 256 mixed-size cavities, one launch per phase
@@ -25,21 +28,31 @@ sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
 
 HBM_PEAK_GBS = 8000.0
 
-# arrays each kernel must read or write per launch (every array holds
-# (mi+1)(nj+1) FP64 values); T is read by the v sweeps only with buoyancy
-ALG_ARRAYS = {
-    "sweep_u_y": 5,   # u, u_ant, v, p -> u
-    "sweep_u_x": 6,   # u, u_ant, v, p -> u, a_u
-    "sweep_v_y": 6,   # v, v_ant, u, p, T -> v
-    "sweep_v_x": 8,   # v, v_ant, u, p, T, v at pass start -> v, a_v
-    "sweep_p_y": 5,   # u, v, a_u, a_v -> p'
-    "sweep_p_x": 7,   # u, v, a_u, a_v, p' -> p', b_o
-    "correct": 9,     # p', a_u, a_v, u, v, p -> u, v, p
-    "sweep_t_y": 5,   # T, T_ant, u, v -> T
-    "sweep_t_x": 6,   # T, T_ant, u, v, T at pass start -> T
-    "residual_u": 5,  # u, u_ant, v, p -> Res_u
-    "finalize": 0,
+# operands each kernel must read or write once per launch, as (FP64 arrays,
+# arrays of the path's real type); every array holds (mi+1)(nj+1) values.  T
+# is read by the v sweeps only with buoyancy.  In the FP64 path both columns
+# are 8-byte; in the FP32-mixed path the second column is 4-byte (u, v, T,
+# p', a_u, a_v) and p, b_o, Res_u stay FP64.
+ALG_OPERANDS = {
+    "sweep_u_y": (1, 4),   # u, u_ant, v, p -> u
+    "sweep_u_x": (1, 5),   # u, u_ant, v, p -> u, a_u
+    "sweep_v_y": (1, 5),   # v, v_ant, u, p, T -> v
+    "sweep_v_x": (1, 7),   # v, v_ant, u, p, T, v at pass start -> v, a_v
+    "sweep_p_y": (0, 5),   # u, v, a_u, a_v -> p'
+    "sweep_p_x": (1, 6),   # u, v, a_u, a_v, p' -> p', b_o
+    "correct": (2, 7),     # p', a_u, a_v, u, v, p -> u, v, p
+    "sweep_t_y": (0, 5),   # T, T_ant, u, v -> T
+    "sweep_t_x": (0, 6),   # T, T_ant, u, v, T at pass start -> T
+    "residual_u": (2, 3),  # u, u_ant, v, p -> Res_u
+    "finalize": (0, 0),
 }
+
+
+def alg_bytes(kernel: str, npts: int, precision: str, buoyancy: bool) -> int:
+    n64, nr = ALG_OPERANDS[kernel]
+    if not buoyancy and kernel in ("sweep_v_y", "sweep_v_x"):
+        nr -= 1
+    return (8 * n64 + (4 if precision == "fp32_mixed" else 8) * nr) * npts
 
 
 def build_case():
@@ -95,7 +108,7 @@ def batch_bench(args):
     from ixchel2d_amd.simple import Batch, Cavity
 
     cases = C.batch_cases(args.batch, seed=0)
-    cav = [Cavity(c, solver=args.solver) for c in cases]
+    cav = [Cavity(c, solver=args.solver, precision=args.precision) for c in cases]
     b = Batch(cav)
     order, graph, table = b.autotune(trial_steps=3)
     b.run(args.warmup, cap=1)
@@ -112,12 +125,10 @@ def batch_bench(args):
     iter_bytes = 0
     for c in cases:
         npts = (c.mesh.mi + 1) * (c.mesh.nj + 1)
-        arrays = sum(ALG_ARRAYS.values())
-        if c.params.derived()["Riy"] == 0.0:
-            arrays -= 2
+        buoy = c.params.derived()["Riy"] != 0.0
+        iter_bytes += sum(alg_bytes(k, npts, args.precision, buoy) for k in ALG_OPERANDS)
         if c.sources:
-            arrays += 2 * 7
-        iter_bytes += arrays * npts * 8
+            iter_bytes += 2 * 7 * 8 * npts
     gbs = iter_bytes / (ms_per_step * 1e-3) / 1e9
     cpu = None
     if not args.no_cpu:
@@ -137,10 +148,10 @@ def batch_bench(args):
         "unit": "Mcell-SIMPLE-iters/s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "vs_baseline": None, "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
         "config": {"workload": f"{args.batch} cavities: sizes 65^2/129^2/257^2/129x301 cells, half lid-driven "
                                "Re 100..3200, half natural convection Ra 1e3..1e6, 2 of 16 with a cylinder",
-                   "cavities": args.batch, "cells": cells, "solver": args.solver, "path": "fp64",
+                   "cavities": args.batch, "cells": cells, "solver": args.solver, "path": args.precision,
                    "schedule": {"order": order, "graph": graph},
                    "autotune_ms_per_step": {f"{k[0]}{'+graph' if k[1] else ''}": round(v, 4)
                                             for k, v in table.items()}},
@@ -152,6 +163,69 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def time_path(case, solver, precision, steps, warmup):
+    """Whole-step throughput of one cavity, then per-kernel durations from
+    HIP events around every launch on the library's stream."""
+    import torch
+    from ixchel2d_amd.simple import Cavity
+    m = case.mesh
+    cells = m.mi * m.nj
+    npts = (m.mi + 1) * (m.nj + 1)
+    gpu = Cavity(case, solver=solver, precision=precision)
+    gpu.run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    gpu.run(steps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    ms_per_step = dt * 1e3 / steps
+    gpu.set_profiling(True)
+    gpu.run(min(steps, 500))
+    kt = gpu.kernel_times()
+    gpu.set_profiling(False)
+    buoy = case.params.derived()["Riy"] != 0.0
+    per_kernel = {}
+    for name, (ms, n) in kt.items():
+        if n == 0:
+            continue
+        avg_us = ms * 1e3 / n
+        b = alg_bytes(name, npts, precision, buoy)
+        per_kernel[name] = {"avg_us": round(avg_us, 3), "launches": n, "alg_bytes": b,
+                            "GB/s": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches"])
+    dk = per_kernel[dom]
+    iter_bytes = sum(alg_bytes(k, npts, precision, buoy) for k in ALG_OPERANDS)
+    it_gbs = iter_bytes / (ms_per_step * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": dk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dk["GB/s"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"],
+                "iteration_alg_bytes": iter_bytes, "iteration_GB/s": round(it_gbs, 1),
+                "iteration_frac": round(it_gbs / HBM_PEAK_GBS, 4)}
+    return {"value": round(cells * steps / dt / 1e6, 2), "ms_per_step": round(ms_per_step, 5),
+            "roofline": roofline, "kernels": per_kernel}
+
+
+def accuracy(case, solver, steps):
+    """FP32-mixed against FP64 from the same start after `steps` iterations:
+    relative error of the hot-wall mean Nusselt number and max-norm errors
+    of u, v, T (relative to the FP64 field's max)."""
+    from ixchel2d_amd.simple import Cavity
+    import numpy as np
+    res = {}
+    for prec in ("fp64", "fp32_mixed"):
+        g = Cavity(case, solver=solver, precision=prec)
+        h = g.run(steps)
+        res[prec] = ({f: g.get(f) for f in "uvt"}, g.nusselt_x()[0], float(h["residuo"][-1]))
+        g.close()
+    f64, nu64, r64 = res["fp64"]
+    f32, nu32, r32 = res["fp32_mixed"]
+    out = {"iterations": steps, "nu_hot_fp64": nu64, "nu_hot_fp32_mixed": nu32,
+           "nu_rel_err": abs(nu32 - nu64) / abs(nu64), "residuo_fp64": r64, "residuo_fp32_mixed": r32}
+    for f in "uvt":
+        out[f"linf_rel_err_{f}"] = float(np.max(np.abs(f32[f] - f64[f])) / np.max(np.abs(f64[f])))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +235,11 @@ def main():
     ap.add_argument("--workload", default="nc301x701", choices=["nc301x701", "batch256"])
     ap.add_argument("--batch", type=int, default=256, help="cavities of the batch workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed"],
+                    help="path of the batch workload")
+    ap.add_argument("--fp64-only", action="store_true", help="skip the FP32-mixed line of the 301x701 workload")
+    ap.add_argument("--accuracy-steps", type=int, default=5000,
+                    help="iterations of the FP32-mixed vs FP64 accuracy comparison")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MB (reference kernel for "
                          "the FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -171,7 +250,6 @@ def main():
         sys.exit(2)
 
     import torch
-    from ixchel2d_amd.simple import Cavity
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
@@ -189,56 +267,21 @@ def main():
     case = build_case()
     m = case.mesh
     cells = m.mi * m.nj
-    npts = (m.mi + 1) * (m.nj + 1)
-    gpu = Cavity(case, solver=args.solver)
-    gpu.run(args.warmup)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    gpu.run(args.steps)
-    torch.cuda.synchronize()
-    dt = time.perf_counter() - t0
-    ms_per_step = dt * 1e3 / args.steps
-    value = cells * args.steps / dt / 1e6
-
-    # per-kernel durations: HIP events around every launch on the library's
-    # stream, over a second run of the same number of steps
-    prof_steps = min(args.steps, 500)
-    gpu.set_profiling(True)
-    gpu.run(prof_steps)
-    kt = gpu.kernel_times()
-    gpu.set_profiling(False)
-    ri_y = case.params.derived()["Riy"]
-    arrays = dict(ALG_ARRAYS)
-    if ri_y == 0.0:
-        arrays["sweep_v_y"] -= 1
-        arrays["sweep_v_x"] -= 1
-    per_kernel = {}
-    for name, (ms, n) in kt.items():
-        if n == 0:
-            continue
-        avg_us = ms * 1e3 / n
-        b = arrays.get(name, 0) * npts * 8
-        per_kernel[name] = {"avg_us": round(avg_us, 3), "launches": n,
-                            "alg_bytes": b, "GB/s": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches"])
-    dk = per_kernel[dom]
-    achieved = dk["GB/s"]
-    iter_bytes = sum(arrays[k] for k in arrays) * npts * 8
-    roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
-                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"],
-                "iteration_alg_bytes": iter_bytes,
-                "iteration_GB/s": round(iter_bytes / (ms_per_step * 1e-3) / 1e9, 1),
-                "iteration_frac": round(iter_bytes / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    r64 = time_path(case, args.solver, "fp64", args.steps, args.warmup)
+    r32 = None
+    if args.solver == "pcr" and not args.fp64_only:
+        r32 = time_path(build_case(), "pcr", "fp32_mixed", args.steps, args.warmup)
+        r32["speedup_vs_fp64"] = round(r32["value"] / r64["value"], 3)
+        r32["accuracy"] = accuracy(build_case(), "pcr", args.accuracy_steps)
     cpu = None if args.no_cpu else cpu_baseline(build_case())
     out = {
         "metric": "Mcell-SIMPLE-iters/s @301x701",
-        "value": round(value, 2),
+        "value": r64["value"],
         "unit": "Mcell-SIMPLE-iters/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 5),
+        "ms_per_step": r64["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -247,9 +290,10 @@ def main():
         "config": {"workload": "natural-convection tall cavity 301x701 (tanh beta=1.2, H/W=2.33, Ra=1e5, "
                                "Pr=0.71, dt=0.003), FP64 plain path, one SIMPLE iteration per step",
                    "mesh": [m.mi, m.nj], "cells": cells, "solver": args.solver, "path": "fp64"},
-        "roofline": roofline,
+        "roofline": r64["roofline"],
         "cpu_baseline": cpu,
-        "kernels": per_kernel,
+        "kernels": r64["kernels"],
+        "fp32_mixed": r32,
     }
     print(json.dumps(out))
 

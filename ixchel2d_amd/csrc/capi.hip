@@ -68,12 +68,19 @@ struct ixh_state {
   hipEvent_t ev_fork = nullptr, ev_t = nullptr, ev_j[2] = {nullptr, nullptr};
   int u_hold = -1;  // u buffer still read by an in-flight residual
   ixh_params prm{};
+  // FP32-mixed: u, v, T, p' (all copies) and a_u, a_v are stored as float in
+  // the same buffers (same element pitch); p, b_o, Res_u stay FP64
+  bool f32 = false;
   // geometry (one allocation, 1-based arrays)
   double *d_geo = nullptr;
   Geo geo{};
   std::vector<double> h_dx[6];  // host copies of dxp, dyp, dxu, dyu, dxv, dyv (1-based)
+  std::vector<double> h_fx[4], h_fy[4];  // fexp, fexu, xu, xp / feyp, feyv, yv, yp
   double *d_fast = nullptr;     // FastGeo tables
   FastGeo fast{};
+  float *d_f32 = nullptr;       // FP32 copies of the geometry and FastGeo tables
+  GeoF geof{};
+  FastGeoF fastf{};
   // solved fields with rotation buffers
   double *ub[kNbuf]{}, *vb[kNbuf]{}, *tb[kNbuf]{}, *pcb[kNbuf]{};
   // transposed twins (y contiguous), rotated together with the row buffers
@@ -143,8 +150,10 @@ int check_params(const ixh_params *p) {
     return fail(IXH_ERR_ARG, "diffusivities must be positive");
   if (p->solver != IXH_SOLVER_PCR && p->solver != IXH_SOLVER_THOMAS && p->solver != IXH_SOLVER_PCR_EXACT)
     return fail(IXH_ERR_ARG, "unknown solver %d", p->solver);
-  if (p->precision != IXH_FP64)
-    return fail(IXH_ERR_UNSUPP, "precision %d is not built in this library", p->precision);
+  if (p->precision != IXH_FP64 && p->precision != IXH_FP32_MIXED)
+    return fail(IXH_ERR_ARG, "unknown precision %d", p->precision);
+  if (p->precision == IXH_FP32_MIXED && p->solver != IXH_SOLVER_PCR)
+    return fail(IXH_ERR_UNSUPP, "the FP32-mixed path uses the PCR solver only");
   return 0;
 }
 
@@ -182,16 +191,16 @@ struct Timed {
   }
 };
 
-template <int EQ, int DIR, int SOLVER>
+template <int EQ, int DIR, int SOLVER, typename R>
 int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
   dim3 grid(A.nlines), block(kBlock);
   switch (ept_for(n)) {
-    case 1: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 1>), grid, block, 0, s->launch, A); break;
-    case 2: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 2>), grid, block, 0, s->launch, A); break;
-    case 3: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 3>), grid, block, 0, s->launch, A); break;
-    case 4: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 4>), grid, block, 0, s->launch, A); break;
-    case 6: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 6>), grid, block, 0, s->launch, A); break;
-    default: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 8>), grid, block, 0, s->launch, A); break;
+    case 1: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 1, R>), grid, block, 0, s->launch, A); break;
+    case 2: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 2, R>), grid, block, 0, s->launch, A); break;
+    case 3: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 3, R>), grid, block, 0, s->launch, A); break;
+    case 4: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 4, R>), grid, block, 0, s->launch, A); break;
+    case 6: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 6, R>), grid, block, 0, s->launch, A); break;
+    default: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 8, R>), grid, block, 0, s->launch, A); break;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -199,9 +208,10 @@ int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
 
 template <int EQ, int DIR>
 int launch_sweep(ixh_state *s, const SweepArgs &A, int n) {
-  if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_t<EQ, DIR, SOLVE_THOMAS>(s, A, n);
-  if (s->prm.solver == IXH_SOLVER_PCR_EXACT) return launch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT>(s, A, n);
-  return launch_sweep_t<EQ, DIR, SOLVE_PCR>(s, A, n);
+  if (s->f32) return launch_sweep_t<EQ, DIR, SOLVE_PCR, float>(s, A, n);
+  if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(s, A, n);
+  if (s->prm.solver == IXH_SOLVER_PCR_EXACT) return launch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT, double>(s, A, n);
+  return launch_sweep_t<EQ, DIR, SOLVE_PCR, double>(s, A, n);
 }
 
 // FastGeo tables for the current diffusivities, dt and relaxation factors
@@ -242,6 +252,40 @@ int build_fast(ixh_state *s) {
   s->fast.inv_dt = 1.0 / s->prm.dt;
   s->fast.inv_rel_vel = 1.0 / s->prm.rel_vel;
   s->fast.inv_rel_ener = 1.0 / s->prm.rel_ener;
+  // FP32 copies: the 1-D geometry arrays (Geo order) followed by the tables
+  const size_t ngeo = 7 * (size_t)gx + 7 * (size_t)gy;
+  std::vector<float> f(ngeo + h.size());
+  {
+    const std::vector<double> *xs[] = {&s->h_dx[0], &s->h_dx[2], &s->h_dx[4], &s->h_fx[0], &s->h_fx[1],
+                                       &s->h_fx[2], &s->h_fx[3]};
+    const std::vector<double> *ys[] = {&s->h_dx[1], &s->h_dx[3], &s->h_dx[5], &s->h_fy[0], &s->h_fy[1],
+                                       &s->h_fy[2], &s->h_fy[3]};
+    size_t off = 0;
+    for (auto *v : xs) { for (int i = 0; i < gx; ++i) f[off + i] = (float)(*v)[i]; off += gx; }
+    for (auto *v : ys) { for (int j = 0; j < gy; ++j) f[off + j] = (float)(*v)[j]; off += gy; }
+    for (size_t k = 0; k < h.size(); ++k) f[ngeo + k] = (float)h[k];
+  }
+  if (!s->d_f32) {
+    hipError_t e = hipMalloc((void **)&s->d_f32, f.size() * sizeof(float) + 256);
+    if (e != hipSuccess) return fail(IXH_ERR_NOMEM, "hipMalloc(fp32 tables): %s", hipGetErrorString(e));
+  }
+  HIPCHK(hipMemcpy(s->d_f32, f.data(), f.size() * sizeof(float), hipMemcpyHostToDevice));
+  const float *d = s->d_f32;
+  s->geof.dxp = d; s->geof.dxu = d + gx; s->geof.dxv = d + 2 * gx; s->geof.fexp = d + 3 * gx;
+  s->geof.fexu = d + 4 * gx; s->geof.xu = d + 5 * gx; s->geof.xp = d + 6 * gx;
+  d += 7 * (size_t)gx;
+  s->geof.dyp = d; s->geof.dyu = d + gy; s->geof.dyv = d + 2 * gy; s->geof.feyp = d + 3 * gy;
+  s->geof.feyv = d + 4 * gy; s->geof.yv = d + 5 * gy; s->geof.yp = d + 6 * gy;
+  d += 7 * (size_t)gy;
+  for (int e = 0; e < 3; ++e) {
+    const float *b = d + e * per;
+    s->fastf.cw[e] = b; s->fastf.pw[e] = b + gx; s->fastf.ce[e] = b + 2 * gx; s->fastf.pe[e] = b + 3 * gx;
+    b += 4 * (size_t)gx;
+    s->fastf.cs[e] = b; s->fastf.ps[e] = b + gy; s->fastf.cn[e] = b + 2 * gy; s->fastf.pn[e] = b + 3 * gy;
+  }
+  s->fastf.inv_dt = (float)s->fast.inv_dt;
+  s->fastf.inv_rel_vel = (float)s->fast.inv_rel_vel;
+  s->fastf.inv_rel_ener = (float)s->fast.inv_rel_ener;
   return 0;
 }
 
@@ -257,6 +301,8 @@ SweepArgs base_args(ixh_state *s, int dir) {
   A.g = s->geo;
   A.ph = phys_of(s);
   A.fg = s->fast;
+  A.gf = s->geof;
+  A.ff = s->fastf;
   A.u = (tr ? s->ubT : s->ub)[s->u_cur];
   A.v = (tr ? s->vbT : s->vb)[s->v_cur];
   A.t = (tr ? s->tbT : s->tb)[s->t_cur];
@@ -480,7 +526,10 @@ int do_correct(ixh_state *s) {
   if (!active) return 0;
   dim3 block(kBlock), grid(corr_tiles_x(s), corr_tiles_y(s));
   Timed tm(s, K_CORR);
-  hipLaunchKernelGGL(k_correct, grid, block, 0, s->launch, A);
+  if (s->f32)
+    hipLaunchKernelGGL(k_correct<float>, grid, block, 0, s->launch, A);
+  else
+    hipLaunchKernelGGL(k_correct<double>, grid, block, 0, s->launch, A);
   HIPCHK(hipGetLastError());
   return tm.done();
 }
@@ -507,7 +556,10 @@ int do_residual(ixh_state *s) {
   ResArgs A;
   prep_residual(s, A);
   Timed tm(s, K_RES);
-  hipLaunchKernelGGL(k_residual_u, dim3(s->nj - 1), dim3(kBlock), 0, s->launch, A);
+  if (s->f32)
+    hipLaunchKernelGGL(k_residual_u<float>, dim3(s->nj - 1), dim3(kBlock), 0, s->launch, A);
+  else
+    hipLaunchKernelGGL(k_residual_u<double>, dim3(s->nj - 1), dim3(kBlock), 0, s->launch, A);
   HIPCHK(hipGetLastError());
   return tm.done();
 }
@@ -637,7 +689,7 @@ int ensure_hist(ixh_state *s, int n) {
 extern "C" {
 
 const char *ixh_last_error(void) { return g_err.c_str(); }
-const char *ixh_version(void) { return "ixchel2d-amd 0.1 (gfx950, fp64)"; }
+const char *ixh_version(void) { return "ixchel2d-amd 0.2 (gfx950, fp64 and fp32-mixed)"; }
 const char *ixh_kernel_name(int32_t id) { return (id >= 0 && id < K_NUM) ? kKernelNames[id] : ""; }
 
 int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) {
@@ -656,6 +708,7 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   s->Q = ((nj + 1) + 15) / 16 * 16;  // so do the columns of the transposed copies
   s->fsize = std::max((size_t)s->P * (nj + 1), (size_t)s->Q * (mi + 1));
   s->prm = *params;
+  s->f32 = params->precision == IXH_FP32_MIXED;
   int rc = 0;
   auto bail = [&](int code) {
     ixh_destroy(s);
@@ -686,6 +739,8 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   for (int i = 1; i <= mi - 1; ++i) fexu[i] = (xu[i + 1] - xp[i + 1]) / (xu[i + 1] - xu[i]);
   for (int j = 1; j <= nj - 1; ++j) feyv[j] = (yv[j + 1] - yp[j + 1]) / (yv[j + 1] - yv[j]);
   s->h_dx[0] = dxp; s->h_dx[1] = dyp; s->h_dx[2] = dxu; s->h_dx[3] = dyu; s->h_dx[4] = dxv; s->h_dx[5] = dyv;
+  s->h_fx[0] = fexp; s->h_fx[1] = fexu; s->h_fx[2] = xu; s->h_fx[3] = xp;
+  s->h_fy[0] = feyp; s->h_fy[1] = feyv; s->h_fy[2] = yv; s->h_fy[3] = yp;
   const std::vector<double> *xs[] = {&dxp, &dxu, &dxv, &fexp, &fexu, &xu, &xp};
   const std::vector<double> *ys[] = {&dyp, &dyu, &dyv, &feyp, &feyv, &yv, &yp};
   const size_t ngeo = 7 * (size_t)gx + 7 * (size_t)gy;
@@ -716,11 +771,15 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
       (rc = dmalloc(&s->auT, s->fsize)) || (rc = dmalloc(&s->avT, s->fsize)))
     return bail(rc);
   {
-    std::vector<double> big(s->fsize, 1.e40);  // a_u = a_v = 1e40 (main.f90:309-310)
-    if (hipMemcpy(s->au, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->av, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->auT, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->avT, big.data(), s->fsize * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    // a_u = a_v = 1e40 (main.f90:309-310); beyond the FP32 range: infinite
+    std::vector<double> big(s->fsize, 1.e40);
+    std::vector<float> bigf(s->fsize, HUGE_VALF);
+    const void *src = s->f32 ? (const void *)bigf.data() : (const void *)big.data();
+    const size_t bytes = s->fsize * (s->f32 ? sizeof(float) : sizeof(double));
+    if (hipMemcpy(s->au, src, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->av, src, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->auT, src, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->avT, src, bytes, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(IXH_ERR_HIP, "a_u/a_v initialisation failed"));
   }
   s->plen = (mi > nj ? mi : nj) + 4;
@@ -766,6 +825,7 @@ int ixh_destroy(ixh_state *s) {
   }
   if (s->d_slot) (void)hipFree(s->d_slot);
   if (s->d_fast) (void)hipFree(s->d_fast);
+  if (s->d_f32) (void)hipFree(s->d_f32);
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   for (int k = 0; k < 2; ++k) {
@@ -782,6 +842,8 @@ int ixh_destroy(ixh_state *s) {
 int ixh_set_params(ixh_state *s, const ixh_params *params) {
   if (!s) return fail(IXH_ERR_ARG, "state is NULL");
   CHK(check_params(params));
+  if (params->precision != s->prm.precision)
+    return fail(IXH_ERR_ARG, "the precision of a state is fixed when it is created");
   s->prm = *params;
   return build_fast(s);
 }
@@ -841,14 +903,27 @@ static double *field_ptr_T(ixh_state *s, int f) {
   return nullptr;  // b_o and Res_u live in the row layout only
 }
 
-// host array (extent m x n, leading dimension ld) -> transposed device copy
-static int upload_T(ixh_state *s, double *dst, const double *src, int ld, int m, int n) {
-  std::vector<double> t((size_t)s->Q * m, 0.0);
+}  // extern "C"
+
+// fields held in the state's real type (float in the FP32-mixed path)
+static bool real_field(int f) { return f != IXH_P && f != IXH_BO && f != IXH_RESU; }
+
+// host array (extent m x n, leading dimension ld) -> device copy in the row
+// layout (pitch P) or the transposed one (pitch Q), as T
+template <typename T>
+static int upload_as(ixh_state *s, double *dst, const double *src, int ld, int m, int n, bool tr) {
+  const size_t pitch = tr ? s->Q : s->P;
+  std::vector<T> t(pitch * (tr ? m : n), T(0));
   for (int j = 0; j < n; ++j)
-    for (int i = 0; i < m; ++i) t[(size_t)j + (size_t)i * s->Q] = src[(size_t)i + (size_t)j * ld];
-  HIPCHK(hipMemcpy(dst, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+    for (int i = 0; i < m; ++i) {
+      const size_t o = tr ? (size_t)j + (size_t)i * pitch : (size_t)i + (size_t)j * pitch;
+      t[o] = (T)src[(size_t)i + (size_t)j * ld];
+    }
+  HIPCHK(hipMemcpy(dst, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice));
   return 0;
 }
+
+extern "C" {
 
 int ixh_upload_field(ixh_state *s, int32_t f, const double *src, int32_t ld) {
   if (!s || !src || f < 0 || f >= IXH_NFIELDS) return fail(IXH_ERR_ARG, "bad upload arguments");
@@ -858,10 +933,18 @@ int ixh_upload_field(ixh_state *s, int32_t f, const double *src, int32_t ld) {
   // a field that shares its buffer with its lagged copy gets its own first
   CHK(unalias(s, f));
   if (f == IXH_PCORR) s->pc_zero = false;
-  HIPCHK(hipMemcpy2DAsync(field_ptr(s, f), (size_t)s->P * 8, src, (size_t)ld * 8, (size_t)m * 8, n,
-                          hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
-  if (double *dT = field_ptr_T(s, f)) CHK(upload_T(s, dT, src, ld, m, n));
+  const bool fl = s->f32 && real_field(f);
+  if (fl)
+    CHK(upload_as<float>(s, field_ptr(s, f), src, ld, m, n, false));
+  else
+    CHK(upload_as<double>(s, field_ptr(s, f), src, ld, m, n, false));
+  if (double *dT = field_ptr_T(s, f)) {
+    if (fl)
+      CHK(upload_as<float>(s, dT, src, ld, m, n, true));
+    else
+      CHK(upload_as<double>(s, dT, src, ld, m, n, true));
+  }
   return 0;
 }
 
@@ -873,6 +956,15 @@ int ixh_download_field(ixh_state *s, int32_t f, double *dst, int32_t ld) {
   if (f == IXH_PCORR && s->pc_zero) {
     HIPCHK(hipStreamSynchronize(s->stream));
     for (int j = 0; j < n; ++j) memset(dst + (size_t)j * ld, 0, (size_t)m * 8);
+    return 0;
+  }
+  if (s->f32 && real_field(f)) {
+    std::vector<float> t((size_t)s->P * n);
+    HIPCHK(hipMemcpyAsync(t.data(), field_ptr(s, f), t.size() * sizeof(float), hipMemcpyDeviceToHost,
+                          s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) dst[(size_t)i + (size_t)j * ld] = t[(size_t)i + (size_t)j * s->P];
     return 0;
   }
   HIPCHK(hipMemcpy2DAsync(dst, (size_t)ld * 8, field_ptr(s, f), (size_t)s->P * 8, (size_t)m * 8, n,
@@ -917,8 +1009,8 @@ int ixh_set_sources(ixh_state *s, int32_t var, const double *sc, const double *s
   field_extent(s, fid, &m, &n);
   HIPCHK(hipMemcpy2D(s->src_c[var], (size_t)s->P * 8, sc, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy2D(s->src_p[var], (size_t)s->P * 8, sp, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
-  CHK(upload_T(s, s->src_cT[var], sc, m, m, n));
-  CHK(upload_T(s, s->src_pT[var], sp, m, m, n));
+  CHK(upload_as<double>(s, s->src_cT[var], sc, m, m, n, true));
+  CHK(upload_as<double>(s, s->src_pT[var], sp, m, m, n, true));
   s->has_src[var] = true;
   return 0;
 }
@@ -1117,7 +1209,10 @@ int ixh_nusselt_y(ixh_state *s, int32_t i0, int32_t i1, double *nu0, double *nu1
   NuArgs A;
   A.mi = s->mi; A.nj = s->nj; A.P = s->P; A.i0 = i0; A.i1 = i1;
   A.g = s->geo; A.t = s->tb[s->t_cur]; A.out = s->d_out; A.vertical = 0;
-  hipLaunchKernelGGL(k_nusselt, dim3(1), dim3(kBlock), 0, s->stream, A);
+  if (s->f32)
+    hipLaunchKernelGGL(k_nusselt<float>, dim3(1), dim3(kBlock), 0, s->stream, A);
+  else
+    hipLaunchKernelGGL(k_nusselt<double>, dim3(1), dim3(kBlock), 0, s->stream, A);
   HIPCHK(hipGetLastError());
   double o[2];
   HIPCHK(hipMemcpyAsync(o, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1132,7 +1227,10 @@ int ixh_nusselt_x(ixh_state *s, double *nu_w, double *nu_e) {
   NuArgs A;
   A.mi = s->mi; A.nj = s->nj; A.P = s->P; A.i0 = 0; A.i1 = 0;
   A.g = s->geo; A.t = s->tb[s->t_cur]; A.out = s->d_out; A.vertical = 1;
-  hipLaunchKernelGGL(k_nusselt, dim3(1), dim3(kBlock), 0, s->stream, A);
+  if (s->f32)
+    hipLaunchKernelGGL(k_nusselt<float>, dim3(1), dim3(kBlock), 0, s->stream, A);
+  else
+    hipLaunchKernelGGL(k_nusselt<double>, dim3(1), dim3(kBlock), 0, s->stream, A);
   HIPCHK(hipGetLastError());
   double o[2];
   HIPCHK(hipMemcpyAsync(o, s->d_out, 2 * sizeof(double), hipMemcpyDeviceToHost, s->stream));
@@ -1258,6 +1356,7 @@ struct BatchStep {
 struct ixh_batch {
   std::vector<ixh_state *> st;
   int n = 0, solver = 0;
+  bool f32 = false;
   int ept[8] = {};
   int order = IXH_ORDER_XCD;
   bool graph = false;
@@ -1376,15 +1475,15 @@ int batch_prepare(ixh_batch *b, BatchStep &E) {
   return 0;
 }
 
-template <int EQ, int DIR, int SOLVER>
+template <int EQ, int DIR, int SOLVER, typename R>
 int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
   dim3 grid(b->nwg[ph]), block(kBlock);
   const int2 *wg = b->d_wg + b->wg_off[ph];
   switch (b->ept[ph]) {
-    case 1: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 1>), grid, block, 0, st, D, wg); break;
-    case 2: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 2>), grid, block, 0, st, D, wg); break;
-    case 3: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 3>), grid, block, 0, st, D, wg); break;
-    default: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 4>), grid, block, 0, st, D, wg); break;
+    case 1: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 1, R>), grid, block, 0, st, D, wg); break;
+    case 2: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 2, R>), grid, block, 0, st, D, wg); break;
+    case 3: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 3, R>), grid, block, 0, st, D, wg); break;
+    default: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 4, R>), grid, block, 0, st, D, wg); break;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -1394,9 +1493,10 @@ template <int EQ, int DIR>
 int batch_sweep(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
   if (b->nwg[ph] == 0) return 0;
   D += (size_t)ph * b->n;
-  if (b->solver == IXH_SOLVER_THOMAS) return batch_sweep_t<EQ, DIR, SOLVE_THOMAS>(b, ph, D, st);
-  if (b->solver == IXH_SOLVER_PCR_EXACT) return batch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT>(b, ph, D, st);
-  return batch_sweep_t<EQ, DIR, SOLVE_PCR>(b, ph, D, st);
+  if (b->f32) return batch_sweep_t<EQ, DIR, SOLVE_PCR, float>(b, ph, D, st);
+  if (b->solver == IXH_SOLVER_THOMAS) return batch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(b, ph, D, st);
+  if (b->solver == IXH_SOLVER_PCR_EXACT) return batch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT, double>(b, ph, D, st);
+  return batch_sweep_t<EQ, DIR, SOLVE_PCR, double>(b, ph, D, st);
 }
 
 // the launches of one step, in the sequential order of a single cavity
@@ -1411,13 +1511,19 @@ int batch_launch(ixh_batch *b, const BatchStep &E, hipStream_t st, int cap) {
   CHK((batch_sweep<EQ_V, DIR_X>(b, PH_V_X, sw, st)));
   CHK((batch_sweep<EQ_P, DIR_Y>(b, PH_P_Y, sw, st)));
   CHK((batch_sweep<EQ_P, DIR_X>(b, PH_P_X, sw, st)));
-  hipLaunchKernelGGL(k_correct_batch, dim3(b->nwg[PH_CORR]), dim3(kBlock), 0, st, co,
-                     (const int2 *)(b->d_wg + b->wg_off[PH_CORR]));
+  const int2 *wgc = b->d_wg + b->wg_off[PH_CORR];
+  if (b->f32)
+    hipLaunchKernelGGL(k_correct_batch<float>, dim3(b->nwg[PH_CORR]), dim3(kBlock), 0, st, co, wgc);
+  else
+    hipLaunchKernelGGL(k_correct_batch<double>, dim3(b->nwg[PH_CORR]), dim3(kBlock), 0, st, co, wgc);
   HIPCHK(hipGetLastError());
   CHK((batch_sweep<EQ_T, DIR_Y>(b, PH_T_Y, sw, st)));
   CHK((batch_sweep<EQ_T, DIR_X>(b, PH_T_X, sw, st)));
-  hipLaunchKernelGGL(k_residual_u_batch, dim3(b->nwg[PH_RES]), dim3(kBlock), 0, st, re,
-                     (const int2 *)(b->d_wg + b->wg_off[PH_RES]));
+  const int2 *wgr = b->d_wg + b->wg_off[PH_RES];
+  if (b->f32)
+    hipLaunchKernelGGL(k_residual_u_batch<float>, dim3(b->nwg[PH_RES]), dim3(kBlock), 0, st, re, wgr);
+  else
+    hipLaunchKernelGGL(k_residual_u_batch<double>, dim3(b->nwg[PH_RES]), dim3(kBlock), 0, st, re, wgr);
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_finalize_batch, dim3(n), dim3(kBlock), 0, st, (const FinArgs *)b->d_fin,
                      (const int *)b->d_slot, cap);
@@ -1515,6 +1621,7 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
     for (int d = 0; d < c; ++d)
       if (states[d] == s) return fail(IXH_ERR_ARG, "batch: state %d appears twice", c);
     if (s->prm.solver != solver) return fail(IXH_ERR_ARG, "batch: every cavity must use the same solver");
+    if (s->f32 != states[0]->f32) return fail(IXH_ERR_ARG, "batch: every cavity must use the same precision");
     if (s->mi + 1 > 4 * kBlock || s->nj + 1 > 4 * kBlock)
       return fail(IXH_ERR_UNSUPP, "batch: lines longer than %d nodes are not built (cavity %d)", 4 * kBlock, c);
   }
@@ -1522,6 +1629,7 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
   b->st.assign(states, states + n);
   b->n = n;
   b->solver = solver;
+  b->f32 = states[0]->f32;
   for (int ph = 0; ph < 8; ++ph) {
     int m = 1;
     for (ixh_state *s : b->st) m = std::max(m, line_len(s, kPhEq[ph], kPhDir[ph]));

@@ -32,12 +32,15 @@ enum Dir { DIR_Y = 0, DIR_X = 1 };
 enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1, SOLVE_PCR_EXACT = 2 };
 
 // 1-based geometry arrays (index 0 unused), built as in
-// source/mod_malla.f90:167-217
-struct Geo {
-  const double *dxp, *dyp, *dxu, *dyu, *dxv, *dyv;
-  const double *fexp, *feyp, *fexu, *feyv;
-  const double *xu, *yv, *xp, *yp;
+// source/mod_malla.f90:167-217; the FP32-mixed sweeps read FP32 copies
+template <typename R>
+struct GeoT {
+  const R *dxp, *dyp, *dxu, *dyu, *dxv, *dyv;
+  const R *fexp, *feyp, *fexu, *feyv;
+  const R *xu, *yv, *xp, *yp;
 };
+using Geo = GeoT<double>;
+using GeoF = GeoT<float>;
 
 struct Phys {
   double dt, rel_pres, rel_vel, rel_ener, gm, gt, ri_x, ri_y;
@@ -55,17 +58,27 @@ struct Side {
 // factor d/G (constant G), west/east faces indexed by ii, south/north by jj.
 // Upstream computes G*L/d and w*d/G with divisions in every cell; these
 // tables leave only multiplications (results move by a few ulp).
-struct FastGeo {
-  const double *cw[3], *pw[3], *ce[3], *pe[3], *cs[3], *ps[3], *cn[3], *pn[3];
-  double inv_dt, inv_rel_vel, inv_rel_ener;
+template <typename R>
+struct FastGeoT {
+  const R *cw[3], *pw[3], *ce[3], *pe[3], *cs[3], *ps[3], *cn[3], *pn[3];
+  R inv_dt, inv_rel_vel, inv_rel_ener;
 };
+using FastGeo = FastGeoT<double>;
+using FastGeoF = FastGeoT<float>;
 
+// Kernel arguments of one sweep.  The solved fields (u, v, T, p' and their
+// lagged and snapshot copies) and a_u, a_v are held in the sweep's real type
+// R: the pointers below are typed double and the kernels reinterpret them as
+// R (float in the FP32-mixed path).  p, b_o, the sources and the partial sums
+// are always FP64.
 struct SweepArgs {
   int mi, nj, P;          // mesh extents and the common row pitch of all fields
   int line0, nlines;      // first line (1-based) and number of lines
   Geo g;
   Phys ph;
   FastGeo fg;
+  GeoF gf;                // FP32 copies of g and fg (FP32-mixed path)
+  FastGeoF ff;
   const double *u, *v, *t, *p, *uant, *vant, *tant, *pc, *au, *av;
   const double *sc, *sp;  // sources of this equation (nullptr = zero)
   Side bc[4];             // a, b, c, d of this equation
@@ -78,6 +91,40 @@ struct SweepArgs {
   double *part0, *part1;  // per-line partial sums (indexed by line number)
   int in_zero;            // p' input identically zero (first pass)
 };
+
+// a field pointer seen as R
+template <typename R>
+__device__ __forceinline__ const R *fld(const double *p) {
+  return reinterpret_cast<const R *>(p);
+}
+template <typename R>
+__device__ __forceinline__ R *fldw(double *p) {
+  return reinterpret_cast<R *>(p);
+}
+template <typename R>
+__device__ __forceinline__ const GeoT<R> &geo_of(const SweepArgs &A);
+template <>
+__device__ __forceinline__ const GeoT<double> &geo_of<double>(const SweepArgs &A) { return A.g; }
+template <>
+__device__ __forceinline__ const GeoT<float> &geo_of<float>(const SweepArgs &A) { return A.gf; }
+template <typename R>
+__device__ __forceinline__ const FastGeoT<R> &fast_of(const SweepArgs &A);
+template <>
+__device__ __forceinline__ const FastGeoT<double> &fast_of<double>(const SweepArgs &A) { return A.fg; }
+template <>
+__device__ __forceinline__ const FastGeoT<float> &fast_of<float>(const SweepArgs &A) { return A.ff; }
+
+// quotient of the line solvers: IEEE division in FP64; in FP32 the
+// hardware reciprocal (1 ulp) times the numerator
+__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __forceinline__ double rabs(double x) { return fabs(x); }
+__device__ __forceinline__ float rabs(float x) { return fabsf(x); }
+
+// an FP32-mixed momentum or energy row whose source is an immersed-body
+// penalty (|Sp| above this) is replaced by phi = Sc / -Sp: the penalty
+// itself (1e51) does not fit FP32
+constexpr double kPenaltyRow = 1.0e30;
 
 // Field element (i,j), 1-based.  Every field is kept twice: row layout
 // (x contiguous, pitch P >= mi+1) for the x-sweeps and transposed layout
@@ -204,25 +251,30 @@ __device__ __forceinline__ double v_rhs(const SweepArgs &A, int ii, int jj, doub
 }
 
 // ---- pressure correction, ensambla_corr_pres_x/_y (mod_ec_momento.f90:145-355)
-struct PCoef {
-  double ai, ad, bs, bn, ac, b;
+// Coefficients in the sweep's real type R; the mass residual b_o always in
+// FP64 from the stored velocities.
+template <typename R>
+struct PCoefT {
+  R ai, ad, bs, bn, ac;
+  double b;
 };
-template <bool TR>
-__device__ __forceinline__ PCoef p_coef(const SweepArgs &A, int ii, int jj) {
+template <bool TR, typename R>
+__device__ __forceinline__ PCoefT<R> p_coef(const SweepArgs &A, int ii, int jj) {
   const int P = A.P;
-  const Geo &g = A.g;
-  PCoef c;
-  double ud = AT(A.u, ii, jj), ui = AT(A.u, ii - 1, jj);
-  double vn = AT(A.v, ii, jj), vs = AT(A.v, ii, jj - 1);
-  double ai = AT(A.au, ii - 1, jj), ad = AT(A.au, ii, jj);
-  double as = AT(A.av, ii, jj - 1), an = AT(A.av, ii, jj);
-  double dyp = g.dyp[jj], dxp = g.dxp[ii];
-  c.ai = -dyp * dyp / ai;
-  c.ad = -dyp * dyp / ad;
-  c.bs = -dxp * dxp / as;
-  c.bn = -dxp * dxp / an;
-  c.ac = (-c.ai - c.ad - c.bs - c.bn) / A.ph.rel_pres;
-  c.b = (ui - ud) * dyp + (vs - vn) * dxp;
+  const GeoT<R> &g = geo_of<R>(A);
+  const R *u = fld<R>(A.u), *v = fld<R>(A.v), *au = fld<R>(A.au), *av = fld<R>(A.av);
+  PCoefT<R> c;
+  R ud = AT(u, ii, jj), ui = AT(u, ii - 1, jj);
+  R vn = AT(v, ii, jj), vs = AT(v, ii, jj - 1);
+  R ai = AT(au, ii - 1, jj), ad = AT(au, ii, jj);
+  R as = AT(av, ii, jj - 1), an = AT(av, ii, jj);
+  R dyp = g.dyp[jj], dxp = g.dxp[ii];
+  c.ai = qdiv(-dyp * dyp, ai);
+  c.ad = qdiv(-dyp * dyp, ad);
+  c.bs = qdiv(-dxp * dxp, as);
+  c.bn = qdiv(-dxp * dxp, an);
+  c.ac = qdiv(-c.ai - c.ad - c.bs - c.bn, R(A.ph.rel_pres));
+  c.b = ((double)ui - (double)ud) * A.g.dyp[jj] + ((double)vs - (double)vn) * A.g.dxp[ii];
   return c;
 }
 
@@ -268,83 +320,107 @@ __device__ __forceinline__ double t_rhs(const SweepArgs &A, int ii, int jj, doub
 // ---- division-free assembly (PCR path): same discretisation, geometry
 // quotients from FastGeo, products instead of quotients by dt and the
 // relaxation factor; the interface diffusivities of a constant G are G.
-__device__ __forceinline__ double nbf(double cond, double len, double pec, double w, bool east) {
-  double conv = w * len;
+// R is the arithmetic of the assembly (double, or float in the FP32-mixed
+// path).
+template <typename R>
+__device__ __forceinline__ R nbf(R cond, R len, R pec, R w, bool east) {
+  R conv = w * len;
   if (east) conv = -conv;
-  const double t = 1.0 - 0.1 * fabs(w * pec);
-  const double t2 = t * t;
-  return -(cond * len * dmax0(t2 * t2 * t) + dmax0(conv));
+  const R t = R(1.0) - R(0.1) * rabs(w * pec);
+  const R t2 = t * t;
+  const R pw = t2 * t2 * t;
+  return -(cond * len * (pw < R(0) ? R(0) : pw) + (conv < R(0) ? R(0) : conv));
 }
 
-template <int EQ, bool TR>
-__device__ __forceinline__ Coef fast_coef(const SweepArgs &A, int ii, int jj, double &area) {
+template <typename R>
+struct CoefT {
+  R ai, ad, bs, bn, ac;
+};
+
+// `pin` (FP32 only) receives the value of a penalised row, NaN otherwise
+template <int EQ, bool TR, typename R>
+__device__ __forceinline__ CoefT<R> fast_coef(const SweepArgs &A, int ii, int jj, R &area, double &pin) {
   const int P = A.P;
-  const Geo &g = A.g;
-  const FastGeo &F = A.fg;
+  const GeoT<R> &g = geo_of<R>(A);
+  const FastGeoT<R> &F = fast_of<R>(A);
+  const R *u = fld<R>(A.u), *v = fld<R>(A.v);
   constexpr int e = EQ == EQ_U ? 0 : EQ == EQ_V ? 1 : 2;
-  double ui, ud, vs, vn, lx, ly;
+  R ui, ud, vs, vn, lx, ly;
   if (EQ == EQ_U) {
-    ud = g.fexu[ii] * AT(A.u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(A.u, ii, jj);
-    ui = g.fexu[ii - 1] * AT(A.u, ii, jj) + (1.0 - g.fexu[ii - 1]) * AT(A.u, ii - 1, jj);
-    vn = g.fexp[ii] * AT(A.v, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj);
-    vs = g.fexp[ii] * AT(A.v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj - 1);
+    ud = g.fexu[ii] * AT(u, ii + 1, jj) + (R(1.0) - g.fexu[ii]) * AT(u, ii, jj);
+    ui = g.fexu[ii - 1] * AT(u, ii, jj) + (R(1.0) - g.fexu[ii - 1]) * AT(u, ii - 1, jj);
+    vn = g.fexp[ii] * AT(v, ii + 1, jj) + (R(1.0) - g.fexp[ii]) * AT(v, ii, jj);
+    vs = g.fexp[ii] * AT(v, ii + 1, jj - 1) + (R(1.0) - g.fexp[ii]) * AT(v, ii, jj - 1);
     lx = g.dxu[ii];
     ly = g.dyu[jj];
   } else if (EQ == EQ_V) {
-    ud = g.feyp[jj] * AT(A.u, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.u, ii, jj);
-    ui = g.feyp[jj] * AT(A.u, ii - 1, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.u, ii - 1, jj);
-    vn = g.feyv[jj] * AT(A.v, ii, jj + 1) + (1.0 - g.feyv[jj]) * AT(A.v, ii, jj);
-    vs = g.feyv[jj - 1] * AT(A.v, ii, jj) + (1.0 - g.feyv[jj - 1]) * AT(A.v, ii, jj - 1);
+    ud = g.feyp[jj] * AT(u, ii, jj + 1) + (R(1.0) - g.feyp[jj]) * AT(u, ii, jj);
+    ui = g.feyp[jj] * AT(u, ii - 1, jj + 1) + (R(1.0) - g.feyp[jj]) * AT(u, ii - 1, jj);
+    vn = g.feyv[jj] * AT(v, ii, jj + 1) + (R(1.0) - g.feyv[jj]) * AT(v, ii, jj);
+    vs = g.feyv[jj - 1] * AT(v, ii, jj) + (R(1.0) - g.feyv[jj - 1]) * AT(v, ii, jj - 1);
     lx = g.dxv[ii];
     ly = g.dyv[jj];
   } else {
-    ud = AT(A.u, ii, jj);
-    ui = AT(A.u, ii - 1, jj);
-    vn = AT(A.v, ii, jj);
-    vs = AT(A.v, ii, jj - 1);
+    ud = AT(u, ii, jj);
+    ui = AT(u, ii - 1, jj);
+    vn = AT(v, ii, jj);
+    vs = AT(v, ii, jj - 1);
     lx = g.dxp[ii];
     ly = g.dyp[jj];
   }
-  Coef c;
-  c.ai = nbf(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
-  c.ad = nbf(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
-  c.bs = nbf(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
-  c.bn = nbf(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
+  CoefT<R> c;
+  c.ai = nbf<R>(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
+  c.ad = nbf<R>(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
+  c.bs = nbf<R>(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
+  c.bn = nbf<R>(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
   area = lx * ly;
-  double acc = -c.ai - c.ad - c.bs - c.bn;
-  if (A.sp) acc = acc - area * AT(A.sp, ii, jj);
+  R acc = -c.ai - c.ad - c.bs - c.bn;
+  pin = __builtin_nan("");
+  if (A.sp) {
+    const double spv = AT(A.sp, ii, jj);
+    if constexpr (sizeof(R) == 8) {
+      acc = acc - area * spv;
+    } else {
+      if (spv < -kPenaltyRow) pin = AT(A.sc, ii, jj) / -spv;
+      else acc = acc - area * R(spv);
+    }
+  }
   c.ac = (acc + area * F.inv_dt) * (EQ == EQ_T ? F.inv_rel_ener : F.inv_rel_vel);
   return c;
 }
 
-template <int EQ, bool TR>
-__device__ __forceinline__ double fast_rhs(const SweepArgs &A, int ii, int jj, double head, double ac,
-                                           double area) {
+template <int EQ, bool TR, typename R>
+__device__ __forceinline__ R fast_rhs(const SweepArgs &A, int ii, int jj, R head, R ac, R area) {
   const int P = A.P;
-  const Geo &g = A.g;
-  double r = head;
+  const GeoT<R> &g = geo_of<R>(A);
+  const R inv_dt = fast_of<R>(A).inv_dt;
+  R r = head;
   if (EQ == EQ_U) {
+    const R *u = fld<R>(A.u);
     if (A.ph.ri_x != 0.0) {
-      double tint = g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.t, ii, jj);
-      r = r - area * A.ph.ri_x * tint;
+      const R *t = fld<R>(A.t);
+      R tint = g.fexp[ii] * AT(t, ii + 1, jj) + (R(1.0) - g.fexp[ii]) * AT(t, ii, jj);
+      r = r - area * R(A.ph.ri_x) * tint;
     }
-    if (A.sc) r = r + area * AT(A.sc, ii, jj);
-    r = r + area * AT(A.uant, ii, jj) * A.fg.inv_dt;
-    r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * g.dyu[jj];
-    r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.u, ii, jj);
+    if (A.sc) r = r + area * R(AT(A.sc, ii, jj));
+    r = r + area * AT(fld<R>(A.uant), ii, jj) * inv_dt;
+    r = r + R((AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * A.g.dyu[jj]);
+    r = r + ac * R(1.0 - A.ph.rel_vel) * AT(u, ii, jj);
   } else if (EQ == EQ_V) {
+    const R *v = fld<R>(A.v);
     if (A.ph.ri_y != 0.0) {
-      double tint = g.feyp[jj] * AT(A.t, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.t, ii, jj);
-      r = r - area * A.ph.ri_y * tint;
+      const R *t = fld<R>(A.t);
+      R tint = g.feyp[jj] * AT(t, ii, jj + 1) + (R(1.0) - g.feyp[jj]) * AT(t, ii, jj);
+      r = r - area * R(A.ph.ri_y) * tint;
     }
-    if (A.sc) r = r + area * AT(A.sc, ii, jj);
-    r = r + area * AT(A.vant, ii, jj) * A.fg.inv_dt;
-    r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * g.dxv[ii];
-    r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.v, ii, jj);
+    if (A.sc) r = r + area * R(AT(A.sc, ii, jj));
+    r = r + area * AT(fld<R>(A.vant), ii, jj) * inv_dt;
+    r = r + R((AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * A.g.dxv[ii]);
+    r = r + ac * R(1.0 - A.ph.rel_vel) * AT(v, ii, jj);
   } else {
-    if (A.sc) r = r + area * AT(A.sc, ii, jj);
-    r = r + area * AT(A.tant, ii, jj) * A.fg.inv_dt;
-    r = r + ac * (1.0 - A.ph.rel_ener) * AT(A.t, ii, jj);
+    if (A.sc) r = r + area * R(AT(A.sc, ii, jj));
+    r = r + area * AT(fld<R>(A.tant), ii, jj) * inv_dt;
+    r = r + ac * R(1.0 - A.ph.rel_ener) * AT(fld<R>(A.t), ii, jj);
   }
   return r;
 }
@@ -352,10 +428,11 @@ __device__ __forceinline__ double fast_rhs(const SweepArgs &A, int ii, int jj, d
 // ---------------------------------------------------------------- solvers
 // Thomas without pivoting in upstream's order (mod_solucionador.f90:25-47),
 // one lane; a[0] and c[n-1] are not read.
-__device__ __forceinline__ void thomas_lds(double *a, double *b, double *c, double *r, int n) {
+template <typename T>
+__device__ __forceinline__ void thomas_lds(T *a, T *b, T *c, T *r, int n) {
   if (threadIdx.x == 0) {
     for (int k = 1; k < n; ++k) {
-      double m = a[k] / b[k - 1];
+      T m = a[k] / b[k - 1];
       r[k] = r[k] - m * r[k - 1];
       b[k] = b[k] - m * c[k - 1];
     }
@@ -368,29 +445,29 @@ __device__ __forceinline__ void thomas_lds(double *a, double *b, double *c, doub
 // Parallel cyclic reduction (Hockney) over the whole line in LDS; EPT rows
 // per thread.  Row k couples to k-s and k+s after step s; a[0] and c[n-1]
 // are set to zero first.  On exit r[k] holds x[k].
-template <int EPT>
-__device__ __forceinline__ void pcr_lds(double *a, double *b, double *c, double *r, int n) {
+template <int EPT, typename T>
+__device__ __forceinline__ void pcr_lds(T *a, T *b, T *c, T *r, int n) {
   if (threadIdx.x == 0) {
-    a[0] = 0.0;
-    c[n - 1] = 0.0;
+    a[0] = T(0);
+    c[n - 1] = T(0);
   }
   __syncthreads();
   for (int s = 1; s < n; s <<= 1) {
-    double na[EPT], nb[EPT], nc[EPT], nr[EPT];
+    T na[EPT], nb[EPT], nc[EPT], nr[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       int k = threadIdx.x + e * kBlock;
       if (k < n) {
-        double ak = a[k], bk = b[k], ck = c[k], rk = r[k];
-        double ta = 0.0, tc = 0.0;
+        T ak = a[k], bk = b[k], ck = c[k], rk = r[k];
+        T ta = T(0), tc = T(0);
         if (k - s >= 0) {
-          double al = -ak / b[k - s];
+          T al = -qdiv(ak, b[k - s]);
           ta = al * a[k - s];
           bk = bk + al * c[k - s];
           rk = rk + al * r[k - s];
         }
         if (k + s < n) {
-          double ga = -ck / b[k + s];
+          T ga = -qdiv(ck, b[k + s]);
           tc = ga * c[k + s];
           bk = bk + ga * a[k + s];
           rk = rk + ga * r[k + s];
@@ -417,7 +494,7 @@ __device__ __forceinline__ void pcr_lds(double *a, double *b, double *c, double 
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     int k = threadIdx.x + e * kBlock;
-    if (k < n) r[k] = r[k] / b[k];
+    if (k < n) r[k] = qdiv(r[k], b[k]);
   }
   __syncthreads();
 }
@@ -433,34 +510,34 @@ __device__ __forceinline__ void pcr_lds(double *a, double *b, double *c, double 
 //  3. each lane back-substitutes its interior rows from x[s-1] = z[2l-1] and
 //     x[e+1] = z[2l+2].
 // MMAX bounds M at compile time (registers); M itself is uniform per wave.
-template <int MMAX>
-__device__ __forceinline__ void wpt_solve(double *a, double *b, double *c, double *r, int n) {
+template <int MMAX, typename T>
+__device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
   if (threadIdx.x < 64) {
     const int l = threadIdx.x;
     const int M = (n + 63) >> 6 < 2 ? 2 : (n + 63) >> 6;
     const int s0 = l * M;
-    double A[MMAX], B[MMAX], C[MMAX], R[MMAX], L[MMAX], Cp[MMAX];
+    T A[MMAX], B[MMAX], C[MMAX], R[MMAX], L[MMAX], Cp[MMAX];
 #pragma unroll
     for (int i = 0; i < MMAX; ++i) {
       if (i < M) {
         const int k = s0 + i;
         if (k < n) {
-          A[i] = k == 0 ? 0.0 : a[k];
+          A[i] = k == 0 ? T(0) : a[k];
           B[i] = b[k];
-          C[i] = k == n - 1 ? 0.0 : c[k];
+          C[i] = k == n - 1 ? T(0) : c[k];
           R[i] = r[k];
         } else {
-          A[i] = 0.0; B[i] = 1.0; C[i] = 0.0; R[i] = 0.0;
+          A[i] = T(0); B[i] = T(1); C[i] = T(0); R[i] = T(0);
         }
       }
     }
-    const bool hasL = A[0] != 0.0;
+    const bool hasL = A[0] != T(0);
     // downward: L_i = spike on x[s-1]
     L[0] = A[0];
 #pragma unroll
     for (int i = 1; i < MMAX; ++i) {
       if (i < M) {
-        const double f = A[i] / B[i - 1];
+        const T f = qdiv(A[i], B[i - 1]);
         B[i] = B[i] - f * C[i - 1];
         R[i] = R[i] - f * R[i - 1];
         L[i] = -f * L[i - 1];
@@ -472,75 +549,75 @@ __device__ __forceinline__ void wpt_solve(double *a, double *b, double *c, doubl
       if (i == M - 1) {
         Cp[i] = C[i];
       } else if (i < M - 1) {
-        const double g = C[i] / B[i + 1];
+        const T g = qdiv(C[i], B[i + 1]);
         R[i] = R[i] - g * R[i + 1];
         L[i] = L[i] - g * L[i + 1];
         Cp[i] = -g * Cp[i + 1];
       }
     }
-    double Ls = L[0], Bs = B[0], Cs = Cp[0], Rs = R[0];
-    double Le = 0.0, Be = 0.0, Ce = 0.0, Re = 0.0;
+    T Ls = L[0], Bs = B[0], Cs = Cp[0], Rs = R[0];
+    T Le = T(0), Be = T(0), Ce = T(0), Re = T(0);
 #pragma unroll
     for (int i = 0; i < MMAX; ++i)
       if (i == M - 1) {
         Le = L[i]; Be = B[i]; Ce = Cp[i]; Re = R[i];
       }
-    const bool hasC = Ce != 0.0;
+    const bool hasC = Ce != T(0);
     // reduced rows: p = 0 -> z[2l] (couples 2l-1, 2l, 2l+1), p = 1 -> z[2l+1]
-    double ra[2], rb[2], rc[2], rr[2];
+    T ra[2], rb[2], rc[2], rr[2];
     if (hasC) {
       ra[0] = Ce * Ls - Cs * Le; rb[0] = Ce * Bs; rc[0] = -(Cs * Be); rr[0] = Ce * Rs - Cs * Re;
     } else {
-      ra[0] = Ls; rb[0] = Bs; rc[0] = 0.0; rr[0] = Rs;
+      ra[0] = Ls; rb[0] = Bs; rc[0] = T(0); rr[0] = Rs;
     }
     if (hasL) {
       ra[1] = -(Le * Bs); rb[1] = Ls * Be; rc[1] = Ls * Ce - Le * Cs; rr[1] = Ls * Re - Le * Rs;
     } else {
-      ra[1] = 0.0; rb[1] = Be; rc[1] = Ce; rr[1] = Re;
+      ra[1] = T(0); rb[1] = Be; rc[1] = Ce; rr[1] = Re;
     }
     // PCR over the 128 reduced rows k = 2l + p
     for (int st = 1; st < 128; st <<= 1) {
-      const double ib0 = 1.0 / rb[0], ib1 = 1.0 / rb[1];
-      double na[2], nb[2], nc[2], nr[2];
+      const T ib0 = qdiv(T(1), rb[0]), ib1 = qdiv(T(1), rb[1]);
+      T na[2], nb[2], nc[2], nr[2];
       if (st == 1) {
         // row 2l: k-1 = 2l-1 (lane l-1, p=1), k+1 = 2l+1 (own p=1)
         // row 2l+1: k-1 = 2l (own p=0), k+1 = 2l+2 (lane l+1, p=0)
-        const double pa = __shfl_up(ra[1], 1, 64), pc = __shfl_up(rc[1], 1, 64);
-        const double pr = __shfl_up(rr[1], 1, 64), pib = __shfl_up(ib1, 1, 64);
-        const double qa = __shfl_down(ra[0], 1, 64), qc = __shfl_down(rc[0], 1, 64);
-        const double qr = __shfl_down(rr[0], 1, 64), qib = __shfl_down(ib0, 1, 64);
+        const T pa = __shfl_up(ra[1], 1, 64), pc = __shfl_up(rc[1], 1, 64);
+        const T pr = __shfl_up(rr[1], 1, 64), pib = __shfl_up(ib1, 1, 64);
+        const T qa = __shfl_down(ra[0], 1, 64), qc = __shfl_down(rc[0], 1, 64);
+        const T qr = __shfl_down(rr[0], 1, 64), qib = __shfl_down(ib0, 1, 64);
         {
-          const double al = l > 0 ? -ra[0] * pib : 0.0;
-          const double ga = -rc[0] * ib1;
-          na[0] = l > 0 ? al * pa : 0.0;
+          const T al = l > 0 ? -ra[0] * pib : T(0);
+          const T ga = -rc[0] * ib1;
+          na[0] = l > 0 ? al * pa : T(0);
           nc[0] = ga * rc[1];
-          nb[0] = rb[0] + (l > 0 ? al * pc : 0.0) + ga * ra[1];
-          nr[0] = rr[0] + (l > 0 ? al * pr : 0.0) + ga * rr[1];
+          nb[0] = rb[0] + (l > 0 ? al * pc : T(0)) + ga * ra[1];
+          nr[0] = rr[0] + (l > 0 ? al * pr : T(0)) + ga * rr[1];
         }
         {
-          const double al = -ra[1] * ib0;
-          const double ga = l < 63 ? -rc[1] * qib : 0.0;
+          const T al = -ra[1] * ib0;
+          const T ga = l < 63 ? -rc[1] * qib : T(0);
           na[1] = al * ra[0];
-          nc[1] = l < 63 ? ga * qc : 0.0;
-          nb[1] = rb[1] + al * rc[0] + (l < 63 ? ga * qa : 0.0);
-          nr[1] = rr[1] + al * rr[0] + (l < 63 ? ga * qr : 0.0);
+          nc[1] = l < 63 ? ga * qc : T(0);
+          nb[1] = rb[1] + al * rc[0] + (l < 63 ? ga * qa : T(0));
+          nr[1] = rr[1] + al * rr[0] + (l < 63 ? ga * qr : T(0));
         }
       } else {
         const int d = st >> 1;
 #pragma unroll
         for (int p = 0; p < 2; ++p) {
-          const double ib = p ? ib1 : ib0;
-          const double pa = __shfl_up(ra[p], d, 64), pc = __shfl_up(rc[p], d, 64);
-          const double pr = __shfl_up(rr[p], d, 64), pib = __shfl_up(ib, d, 64);
-          const double qa = __shfl_down(ra[p], d, 64), qc = __shfl_down(rc[p], d, 64);
-          const double qr = __shfl_down(rr[p], d, 64), qib = __shfl_down(ib, d, 64);
+          const T ib = p ? ib1 : ib0;
+          const T pa = __shfl_up(ra[p], d, 64), pc = __shfl_up(rc[p], d, 64);
+          const T pr = __shfl_up(rr[p], d, 64), pib = __shfl_up(ib, d, 64);
+          const T qa = __shfl_down(ra[p], d, 64), qc = __shfl_down(rc[p], d, 64);
+          const T qr = __shfl_down(rr[p], d, 64), qib = __shfl_down(ib, d, 64);
           const bool lo = l - d >= 0, hi = l + d <= 63;
-          const double al = lo ? -ra[p] * pib : 0.0;
-          const double ga = hi ? -rc[p] * qib : 0.0;
-          na[p] = lo ? al * pa : 0.0;
-          nc[p] = hi ? ga * qc : 0.0;
-          nb[p] = rb[p] + (lo ? al * pc : 0.0) + (hi ? ga * qa : 0.0);
-          nr[p] = rr[p] + (lo ? al * pr : 0.0) + (hi ? ga * qr : 0.0);
+          const T al = lo ? -ra[p] * pib : T(0);
+          const T ga = hi ? -rc[p] * qib : T(0);
+          na[p] = lo ? al * pa : T(0);
+          nc[p] = hi ? ga * qc : T(0);
+          nb[p] = rb[p] + (lo ? al * pc : T(0)) + (hi ? ga * qa : T(0));
+          nr[p] = rr[p] + (lo ? al * pr : T(0)) + (hi ? ga * qr : T(0));
         }
       }
 #pragma unroll
@@ -548,17 +625,17 @@ __device__ __forceinline__ void wpt_solve(double *a, double *b, double *c, doubl
         ra[p] = na[p]; rb[p] = nb[p]; rc[p] = nc[p]; rr[p] = nr[p];
       }
     }
-    const double zs = rr[0] / rb[0], ze = rr[1] / rb[1];
-    const double xl = __shfl_up(ze, 1, 64);    // x[s-1]
-    const double xr = __shfl_down(zs, 1, 64);  // x[e+1]
+    const T zs = qdiv(rr[0], rb[0]), ze = qdiv(rr[1], rb[1]);
+    const T xl = __shfl_up(ze, 1, 64);    // x[s-1]
+    const T xr = __shfl_down(zs, 1, 64);  // x[e+1]
 #pragma unroll
     for (int i = 0; i < MMAX; ++i) {
       if (i < M) {
         const int k = s0 + i;
-        double x;
+        T x;
         if (i == 0) x = zs;
         else if (i == M - 1) x = ze;
-        else x = (R[i] - (l > 0 ? L[i] * xl : 0.0) - (l < 63 ? Cp[i] * xr : 0.0)) / B[i];
+        else x = qdiv(R[i] - (l > 0 ? L[i] * xl : T(0)) - (l < 63 ? Cp[i] * xr : T(0)), B[i]);
         if (k < n) r[k] = x;
       }
     }
@@ -566,8 +643,8 @@ __device__ __forceinline__ void wpt_solve(double *a, double *b, double *c, doubl
   __syncthreads();
 }
 
-template <int SOLVER, int EPT>
-__device__ __forceinline__ void solve_line(double *a, double *b, double *c, double *r, int n) {
+template <int SOLVER, int EPT, typename T>
+__device__ __forceinline__ void solve_line(T *a, T *b, T *c, T *r, int n) {
   if (SOLVER == SOLVE_THOMAS)
     thomas_lds(a, b, c, r, n);
   else if (EPT <= 4)
@@ -638,10 +715,14 @@ struct Layout {
   }
 };
 
-// One line of a sweep, run by one workgroup; `line` is 1-based.
-template <int EQ, int DIR, int SOLVER, int EPT>
+// One line of a sweep, run by one workgroup; `line` is 1-based.  R is the
+// real type of the stored fields, the assembly and the line solve (double,
+// or float in the FP32-mixed path, which always uses the PCR solver); rows
+// are formed in FP64 registers, p' right-hand sides and norms in FP64.
+template <int EQ, int DIR, int SOLVER, int EPT, typename R>
 __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
-  __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
+  static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR, "the FP32 sweeps use the PCR solver");
+  __shared__ R sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
   __shared__ double red[kBlock];
   constexpr bool TR = DIR == DIR_Y;
   const int P = A.P;
@@ -654,6 +735,9 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
   // y-lines on b (low) and d
   const Side &lo_side = DIR == DIR_X ? A.bc[0] : A.bc[1];
   const Side &hi_side = DIR == DIR_X ? A.bc[2] : A.bc[3];
+  // a_u / a_v of a boundary face (1e40, main.f90:309-310; infinite in FP32)
+  const R big = sizeof(R) == 8 ? R(1.e40) : R(__builtin_huge_valf());
+  R *acoef = fldw<R>(A.acoef), *acoef2 = fldw<R>(A.acoef2);
 
   for (int k = threadIdx.x; k < n; k += kBlock) {
     const int pos = k + 1;                 // 1-based position along the line
@@ -662,28 +746,54 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
     double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
     if (pos >= 2 && pos <= hi) {
       if (EQ == EQ_P) {
-        PCoef c = p_coef<TR>(A, ii, jj);
-        double pcP = A.in_zero ? 0.0 : AT(A.pc, ii, jj);
+        const PCoefT<R> c = p_coef<TR, R>(A, ii, jj);
+        const R *pc = fld<R>(A.pc);
+        double pcP = A.in_zero ? 0.0 : (double)AT(pc, ii, jj);
         double head;
         if (DIR == DIR_X) {
           a0 = c.ai; a2 = c.ad;
-          double pS = A.in_zero ? 0.0 : AT(A.pc, ii, jj - 1);
-          double pN = A.in_zero ? 0.0 : AT(A.pc, ii, jj + 1);
-          head = -c.bs * pS - c.bn * pN;
+          double pS = A.in_zero ? 0.0 : (double)AT(pc, ii, jj - 1);
+          double pN = A.in_zero ? 0.0 : (double)AT(pc, ii, jj + 1);
+          head = -(double)c.bs * pS - (double)c.bn * pN;
           AT(A.bo, ii, jj) = c.b;  // x-sweep: row layout
         } else {
           a0 = c.bs; a2 = c.bn;
-          double pW = A.in_zero ? 0.0 : AT(A.pc, ii - 1, jj);
-          double pE = A.in_zero ? 0.0 : AT(A.pc, ii + 1, jj);
-          head = -c.ai * pW - c.ad * pE;
+          double pW = A.in_zero ? 0.0 : (double)AT(pc, ii - 1, jj);
+          double pE = A.in_zero ? 0.0 : (double)AT(pc, ii + 1, jj);
+          head = -(double)c.ai * pW - (double)c.ad * pE;
         }
         a1 = c.ac;
-        rr = head + c.b + (1.0 - A.ph.rel_pres) * c.ac * pcP;
+        rr = head + c.b + (1.0 - A.ph.rel_pres) * (double)c.ac * pcP;
+        if constexpr (sizeof(R) == 4) {
+          // every neighbour behind an infinite a_u, a_v (inside a body): p' = 0
+          if (c.ac == R(0)) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = 0.0; }
+        }
+      } else if constexpr (SOLVER == SOLVE_PCR) {
+        R area;
+        double pin;
+        const CoefT<R> c = fast_coef<EQ, TR, R>(A, ii, jj, area, pin);
+        const R *phi = fld<R>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t);
+        R head;
+        if (DIR == DIR_X) {
+          a0 = c.ai; a2 = c.ad;
+          head = -c.bs * AT(phi, ii, jj - 1) - c.bn * AT(phi, ii, jj + 1);
+        } else {
+          a0 = c.bs; a2 = c.bn;
+          head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
+        }
+        a1 = c.ac;
+        rr = fast_rhs<EQ, TR, R>(A, ii, jj, head, c.ac, area);
+        R aw = c.ac * R(A.ph.rel_vel);
+        if constexpr (sizeof(R) == 4) {
+          if (pin == pin) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = pin; aw = big; }
+        }
+        if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X) {
+          AT(acoef, ii, jj) = aw;
+          AO(acoef2, ii, jj) = aw;
+        }
       } else {
-        constexpr bool fast = SOLVER == SOLVE_PCR;
-        double area = 0.0;
-        Coef c = fast ? fast_coef<EQ, TR>(A, ii, jj, area)
-               : EQ == EQ_U ? u_coef<TR>(A, ii, jj) : EQ == EQ_V ? v_coef<TR>(A, ii, jj) : t_coef<TR>(A, ii, jj);
+        const Coef c = EQ == EQ_U ? u_coef<TR>(A, ii, jj) : EQ == EQ_V ? v_coef<TR>(A, ii, jj)
+                                                                         : t_coef<TR>(A, ii, jj);
         const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
         double head;
         if (DIR == DIR_X) {
@@ -694,11 +804,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
           head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
         }
         a1 = c.ac;
-        if (fast)
-          rr = fast_rhs<EQ, TR>(A, ii, jj, head, c.ac, area);
-        else
-          rr = EQ == EQ_U ? u_rhs<TR>(A, ii, jj, head, c.ac)
-             : EQ == EQ_V ? v_rhs<TR>(A, ii, jj, head, c.ac) : t_rhs<TR>(A, ii, jj, head, c.ac);
+        rr = EQ == EQ_U ? u_rhs<TR>(A, ii, jj, head, c.ac)
+           : EQ == EQ_V ? v_rhs<TR>(A, ii, jj, head, c.ac) : t_rhs<TR>(A, ii, jj, head, c.ac);
         if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X) {
           AT(A.acoef, ii, jj) = c.ac * A.ph.rel_vel;
           AO(A.acoef2, ii, jj) = c.ac * A.ph.rel_vel;
@@ -713,31 +820,32 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
         // the node number along the side is the line index
         bool typed = bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
         if (typed && (EQ == EQ_U || EQ == EQ_V)) {
-          AT(A.acoef, ii, jj) = 1.e40;
-          AO(A.acoef2, ii, jj) = 1.e40;
+          AT(acoef, ii, jj) = big;
+          AO(acoef2, ii, jj) = big;
         }
       }
     }
-    sa[k] = a0;
-    sb[k] = a1;
-    sc_[k] = a2;
-    sr[k] = rr;
+    sa[k] = R(a0);
+    sb[k] = R(a1);
+    sc_[k] = R(a2);
+    sr[k] = R(rr);
   }
   __syncthreads();
   solve_line<SOLVER, EPT>(sa, sb, sc_, sr, n);
 
   // write the line, accumulate the x-sweep norms
+  R *out = fldw<R>(A.out), *out2 = fldw<R>(A.out2);
   double acc0 = 0.0, acc1 = 0.0;
   for (int k = threadIdx.x; k < n; k += kBlock) {
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
-    const double x = sr[k];
-    AT(A.out, ii, jj) = x;
-    AO(A.out2, ii, jj) = x;
+    const R x = sr[k];
+    AT(out, ii, jj) = x;
+    AO(out2, ii, jj) = x;
     if (DIR == DIR_X && EQ != EQ_U && pos >= 2 && pos <= mi) {
-      double old = A.snap ? AT(A.snap, ii, jj) : 0.0;
-      double d = x - old;
+      double old = A.snap ? (double)AT(fld<R>(A.snap), ii, jj) : 0.0;
+      double d = (double)x - old;
       acc0 = acc0 + d * d;
       if (EQ == EQ_P) {
         double b = AT(A.bo, ii, jj);
@@ -755,7 +863,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
   }
 
   // carry the lines this sweep does not solve into the output buffer
-  const double *src = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc;
+  const R *src = fld<R>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc);
   for (int side = 0; side < 2; ++side) {
     if (line != (side == 0 ? A.line0 : lastl)) continue;
     const int other = side == 0 ? A.line0 - 1 : lastl + 1;
@@ -763,27 +871,27 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
       const int pos = k + 1;
       const int ii = DIR == DIR_X ? pos : other;
       const int jj = DIR == DIR_X ? other : pos;
-      const double x = (EQ == EQ_P && A.in_zero) ? 0.0 : AT(src, ii, jj);
-      AT(A.out, ii, jj) = x;
-      AO(A.out2, ii, jj) = x;
+      const R x = (EQ == EQ_P && A.in_zero) ? R(0) : AT(src, ii, jj);
+      AT(out, ii, jj) = x;
+      AO(out2, ii, jj) = x;
     }
   }
 }
 
-template <int EQ, int DIR, int SOLVER, int EPT>
+template <int EQ, int DIR, int SOLVER, int EPT, typename R>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
-  sweep_line<EQ, DIR, SOLVER, EPT>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines));
+  sweep_line<EQ, DIR, SOLVER, EPT, R>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines));
 }
 
 // A batch of cavities: workgroup b runs line wg[b].y of cavity wg[b].x, whose
 // arguments are D[wg[b].x] (same kernel body as the single-cavity sweep).
 // Entries with wg[b].x < 0 pad a table to a multiple of the XCD count.
-template <int EQ, int DIR, int SOLVER, int EPT>
+template <int EQ, int DIR, int SOLVER, int EPT, typename R>
 __global__ void __launch_bounds__(kBlock) k_sweep_batch(const SweepArgs *__restrict__ D,
                                                         const int2 *__restrict__ wg) {
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
-  sweep_line<EQ, DIR, SOLVER, EPT>(D[w.x], w.y);
+  sweep_line<EQ, DIR, SOLVER, EPT, R>(D[w.x], w.y);
 }
 
 // ------------------------------------------------------------- correction
@@ -802,55 +910,62 @@ struct CorrArgs {
 
 constexpr int kTile = 32;
 
+template <typename R>
 __device__ __forceinline__ void correct_tile(const CorrArgs &A, const int bx, const int by) {
-  __shared__ double tu[kTile][kTile + 1], tv[kTile][kTile + 1], tp[kTile][kTile + 1];
+  __shared__ R tu[kTile][kTile + 1], tv[kTile][kTile + 1];
+  __shared__ double tp[kTile][kTile + 1];
   constexpr bool TR = false;
   const int P = A.P;
+  const R *pc = fld<R>(A.pc), *au = fld<R>(A.au), *av = fld<R>(A.av);
+  R *u = fldw<R>(A.u), *v = fldw<R>(A.v);
   const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x >> 5;  // 32 x 8
   const int i0 = 1 + bx * kTile, j0 = 1 + by * kTile;
   for (int r = ty; r < kTile; r += kBlock / kTile) {
     const int ii = i0 + tx, jj = j0 + r;
     if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) continue;
-    const double pcP = AT(A.pc, ii, jj);
+    const double pcP = AT(pc, ii, jj);
     const double pn = AT(A.p, ii, jj) + 0.6 * pcP;
     AT(A.p, ii, jj) = pn;
     tp[r][tx] = pn;
     if (ii <= A.mi - 1) {
-      const double un = AT(A.u, ii, jj) + A.g.dyu[jj] * (pcP - AT(A.pc, ii + 1, jj)) / AT(A.au, ii, jj);
-      AT(A.u, ii, jj) = un;
+      const R un = R(AT(u, ii, jj) + A.g.dyu[jj] * (pcP - (double)AT(pc, ii + 1, jj)) / (double)AT(au, ii, jj));
+      AT(u, ii, jj) = un;
       tu[r][tx] = un;
     }
     if (jj <= A.nj - 1) {
-      const double vn = AT(A.v, ii, jj) + A.g.dxv[ii] * (pcP - AT(A.pc, ii, jj + 1)) / AT(A.av, ii, jj);
-      AT(A.v, ii, jj) = vn;
+      const R vn = R(AT(v, ii, jj) + A.g.dxv[ii] * (pcP - (double)AT(pc, ii, jj + 1)) / (double)AT(av, ii, jj));
+      AT(v, ii, jj) = vn;
       tv[r][tx] = vn;
     }
   }
   __syncthreads();
   const int Q = A.Q;
+  R *uT = fldw<R>(A.uT), *vT = fldw<R>(A.vT);
   for (int r = ty; r < kTile; r += kBlock / kTile) {
     const int ii = i0 + r, jj = j0 + tx;  // lanes run along jj: coalesced in the transposed copy
     if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) continue;
     const size_t o = ((size_t)jj - 1) + ((size_t)ii - 1) * (size_t)Q;
     A.pT[o] = tp[tx][r];
-    if (ii <= A.mi - 1) A.uT[o] = tu[tx][r];
-    if (jj <= A.nj - 1) A.vT[o] = tv[tx][r];
+    if (ii <= A.mi - 1) uT[o] = tu[tx][r];
+    if (jj <= A.nj - 1) vT[o] = tv[tx][r];
   }
 }
 
+template <typename R>
 __global__ void __launch_bounds__(kBlock) k_correct(CorrArgs A) {
-  correct_tile(A, (int)blockIdx.x, (int)blockIdx.y);
+  correct_tile<R>(A, (int)blockIdx.x, (int)blockIdx.y);
 }
 
 // batch: workgroup b corrects tile (wg[b].y % tiles_x, wg[b].y / tiles_x) of
 // cavity wg[b].x
+template <typename R>
 __global__ void __launch_bounds__(kBlock) k_correct_batch(const CorrArgs *__restrict__ D,
                                                           const int2 *__restrict__ wg) {
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
   const CorrArgs &A = D[w.x];
   const int tiles_x = (A.mi + 1 + kTile - 1) / kTile;
-  correct_tile(A, w.y % tiles_x, w.y / tiles_x);
+  correct_tile<R>(A, w.y % tiles_x, w.y / tiles_x);
 }
 
 // ---------------------------------------------------------------- residual
@@ -866,18 +981,21 @@ struct ResArgs {
   double *resu, *part;
 };
 
+template <typename R>
 __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj) {
   __shared__ double red[kBlock];
   constexpr bool TR = false;
   const int P = A.P;
+  // the stored fields in their real type, read as FP64
+  const R *u = fld<R>(A.u), *v = fld<R>(A.v), *t = fld<R>(A.t), *uant = fld<R>(A.uant);
   const Geo &g = A.g;
   const double gm = A.ph.gm;
   double acc = 0.0;
   for (int ii = 2 + (int)threadIdx.x; ii <= A.mi - 1; ii += kBlock) {
-    double ud = g.fexu[ii] * AT(A.u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(A.u, ii, jj);
-    double ui = g.fexu[ii - 1] * AT(A.u, ii, jj) + (1.0 - g.fexu[ii - 1]) * AT(A.u, ii - 1, jj);
-    double vn = g.fexp[ii] * AT(A.v, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj);
-    double vs = g.fexp[ii] * AT(A.v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * AT(A.v, ii, jj - 1);
+    double ud = g.fexu[ii] * (double)AT(u, ii + 1, jj) + (1.0 - g.fexu[ii]) * (double)AT(u, ii, jj);
+    double ui = g.fexu[ii - 1] * (double)AT(u, ii, jj) + (1.0 - g.fexu[ii - 1]) * (double)AT(u, ii - 1, jj);
+    double vn = g.fexp[ii] * (double)AT(v, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj);
+    double vs = g.fexp[ii] * (double)AT(v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj - 1);
     double gd = harm(gm, gm, g.feyp[jj]);
     double gi = harm(gm, gm, g.feyp[jj]);
     double gn = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
@@ -893,14 +1011,14 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj) {
     double ac = ai + ad + bs + bn;
     if (A.sp) ac = ac - dx * dy * AT(A.sp, ii, jj);
     ac = ac + dx * dy / A.ph.dt;
-    double r = ai * AT(A.u, ii - 1, jj) - ac * AT(A.u, ii, jj) + ad * AT(A.u, ii + 1, jj) +
-               bs * AT(A.u, ii, jj - 1) + bn * AT(A.u, ii, jj + 1);
+    double r = ai * (double)AT(u, ii - 1, jj) - ac * (double)AT(u, ii, jj) + ad * (double)AT(u, ii + 1, jj) +
+               bs * (double)AT(u, ii, jj - 1) + bn * (double)AT(u, ii, jj + 1);
     if (A.ph.ri_x != 0.0) {
-      double tint = g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.t, ii, jj);
+      double tint = g.fexp[ii] * (double)AT(t, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(t, ii, jj);
       r = r - dx * dy * A.ph.ri_x * tint;
     }
     if (A.sc) r = r + dx * dy * AT(A.sc, ii, jj);
-    r = r + dx * dy * AT(A.uant, ii, jj) / A.ph.dt;
+    r = r + dx * dy * (double)AT(uant, ii, jj) / A.ph.dt;
     r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * dy;
     AT(A.resu, ii, jj) = r;
     acc = acc + r * r;
@@ -909,15 +1027,17 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj) {
   if (threadIdx.x == 0) A.part[jj] = s;
 }
 
+template <typename R>
 __global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
-  residual_row(A, 2 + xcd_line((int)blockIdx.x, A.nj - 1));
+  residual_row<R>(A, 2 + xcd_line((int)blockIdx.x, A.nj - 1));
 }
 
+template <typename R>
 __global__ void __launch_bounds__(kBlock) k_residual_u_batch(const ResArgs *__restrict__ D,
                                                              const int2 *__restrict__ wg) {
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
-  residual_row(D[w.x], w.y);
+  residual_row<R>(D[w.x], w.y);
 }
 
 // ---------------------------------------------------------------- finalize
@@ -1000,19 +1120,21 @@ struct NuArgs {
   int vertical;
 };
 
+template <typename R>
 __global__ void __launch_bounds__(kBlock) k_nusselt(NuArgs A) {
   __shared__ double der[kNmax + 4];
   constexpr bool TR = false;
   const int P = A.P;
   const Geo &g = A.g;
+  const R *t = fld<R>(A.t);
   if (!A.vertical) {
     for (int w = 0; w < 2; ++w) {
       const int jj = w == 0 ? 1 : A.nj - 1;
       for (int k = threadIdx.x; k < A.mi + 4; k += kBlock) der[k] = 0.0;
       __syncthreads();
       for (int ii = A.i0 + (int)threadIdx.x; ii <= A.i1; ii += kBlock) {
-        double dy1 = AT(A.t, ii, jj + 1) - AT(A.t, ii, jj);
-        double dy2 = AT(A.t, ii, jj + 2) - AT(A.t, ii, jj + 1);
+        double dy1 = (double)AT(t, ii, jj + 1) - (double)AT(t, ii, jj);
+        double dy2 = (double)AT(t, ii, jj + 2) - (double)AT(t, ii, jj + 1);
         double dx1 = g.yp[jj + 1] - g.yp[jj];
         double dx2 = g.yp[jj + 2] - g.yp[jj + 1];
         double a = (dy2 / dx2 - dy1 / dx1) / (dx1 + dx2);
@@ -1043,8 +1165,8 @@ __global__ void __launch_bounds__(kBlock) k_nusselt(NuArgs A) {
       const double xw = w == 0 ? g.xu[1] : g.xu[A.mi];
       for (int jj = 2 + (int)threadIdx.x; jj <= A.nj; jj += kBlock) {
         double x1 = g.xp[i1], x2 = g.xp[i1 + 1], x3 = g.xp[i1 + 2];
-        double d1 = (AT(A.t, i1 + 1, jj) - AT(A.t, i1, jj)) / (x2 - x1);
-        double d2 = (AT(A.t, i1 + 2, jj) - AT(A.t, i1 + 1, jj)) / (x3 - x2);
+        double d1 = ((double)AT(t, i1 + 1, jj) - (double)AT(t, i1, jj)) / (x2 - x1);
+        double d2 = ((double)AT(t, i1 + 2, jj) - (double)AT(t, i1 + 1, jj)) / (x3 - x2);
         double a = (d2 - d1) / (x3 - x1);
         der[jj] = (d1 + a * (2.0 * xw - x1 - x2)) * g.dyp[jj];
       }
