@@ -200,6 +200,23 @@ int ixh_batch_iterations(ixh_batch *b, int32_t nsteps, ixh_hist *hist, int32_t c
  * of every cavity (6 * (trial_steps + 4) in all), identical under every schedule. */
 int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_t *graph, double *ms_per_step);
 
+/* ---- discrete adjoint of the energy transport.  No upstream counterpart.
+ * ixh_adjoint_record runs nsteps single-pass time steps from the current
+ * state (the state advances as under ixh_simple_iterations) and keeps a
+ * checkpoint of the fields every `every` steps.  ixh_adjoint_energy then runs
+ * the adjoint of the recorded energy steps backward, recomputing each
+ * segment's velocities from its checkpoint: given tbar = dJ/dT of the final
+ * temperature ((mi+1) x (nj+1), leading dimension ld) it returns dJ/d(value)
+ * of every T boundary node of sides a, b, c, d (NULL = not wanted; lengths as
+ * in ixh_set_bc) and dJ/dT of the initial temperature (with T_ant taken equal
+ * to it).  The adjoint holds the velocities of each step fixed: it is the
+ * exact gradient when the flow does not depend on T (ri_x = ri_y = 0), and
+ * the frozen-flow part of it under buoyancy.  The state ends as after the
+ * recorded run.  FP64 path only. */
+int ixh_adjoint_record(ixh_state *s, int32_t nsteps, int32_t every);
+int ixh_adjoint_energy(ixh_state *s, const double *tbar, int32_t ld, double *grad_a, double *grad_b,
+                       double *grad_c, double *grad_d, double *grad_t0, int32_t ld0);
+
 #ifdef __cplusplus
 }
 #endif

@@ -284,6 +284,20 @@ def natural_convection_case(mi: int = 301, nj: int = 701, height: float = 2.33,
                 t0, _zeros((mi + 1, nj + 1)), name=f"nc{mi}x{nj}")
 
 
+def heated_lid_case(n: int = 34, Re: float = 100.0, iters: int = 100, dt: float = 0.05,
+                    seed: int = 0) -> Case:
+    """Lid-driven square cavity carrying a passive temperature (no
+    buoyancy): hot wall x=0 (T=1), cold wall x=W (T=0), adiabatic lid and
+    bottom, T0 = 0.5 + U(-1e-3, 1e-3).  The flow does not depend on T."""
+    case = lid_driven_case(n, Re=Re, iters=iters, dt=dt)
+    case.bcs["t"] = uniform_bc("t", ("diri", 1.0), ("neum", 0.0), ("diri", 0.0), ("neum", 0.0))
+    case.params.tempe_ini = "archivo"
+    rng = np.random.default_rng(seed)
+    case.t0 = np.asfortranarray(0.5 + rng.uniform(-1e-3, 1e-3, size=(n + 1, n + 1)))
+    case.name = f"hlid{n}"
+    return case
+
+
 # --------------------------------------------------------------------------
 # immersed bodies (large-source penalty, source/mod_front_inmersa.f90:54-179)
 # --------------------------------------------------------------------------

@@ -102,6 +102,13 @@ struct ixh_state {
   double *d_hist = nullptr;  // 8 doubles per recorded iteration
   int hist_cap = 0;
   int *d_slot = nullptr;
+  // energy adjoint: checkpoints (kCkArrays fields each) of the recorded run,
+  // the u, v tape of one segment and the adjoint work arrays
+  int adj_steps = 0, adj_every = 0;
+  std::vector<double *> adj_ck;
+  std::vector<double *> adj_tape;
+  double *adj_w[4]{};  // lambda, lambda', mu, adjoint of T_ant
+  double *adj_g[4]{};  // adjoints of the boundary values of T, sides a..d
   // profiling
   bool profiling = false;
   double kms[K_NUM]{};
@@ -826,6 +833,12 @@ int ixh_destroy(ixh_state *s) {
   if (s->d_slot) (void)hipFree(s->d_slot);
   if (s->d_fast) (void)hipFree(s->d_fast);
   if (s->d_f32) (void)hipFree(s->d_f32);
+  for (double *p : s->adj_ck) (void)hipFree(p);
+  for (double *p : s->adj_tape) (void)hipFree(p);
+  for (int k = 0; k < 4; ++k) {
+    if (s->adj_w[k]) (void)hipFree(s->adj_w[k]);
+    if (s->adj_g[k]) (void)hipFree(s->adj_g[k]);
+  }
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   for (int k = 0; k < 2; ++k) {
@@ -1738,6 +1751,190 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
   CHK(ixh_batch_set_schedule(b, best >> 1, best & 1));
   if (order) *order = best >> 1;
   if (graph) *graph = best & 1;
+  return 0;
+}
+
+}  // extern "C"
+
+// ============================================================ energy adjoint
+// Record a run with checkpoints, then run the discrete adjoint of its energy
+// steps backward, recomputing the velocities of each segment of `every`
+// steps from its checkpoint.
+namespace {
+
+constexpr int kCkArrays = 14;  // u, v, T, p and the lagged u, v, T, both layouts
+
+void ck_fields(ixh_state *s, double *f[kCkArrays]) {
+  double *a[kCkArrays] = {s->ub[s->u_cur], s->ubT[s->u_cur], s->vb[s->v_cur], s->vbT[s->v_cur],
+                          s->tb[s->t_cur], s->tbT[s->t_cur], s->p,          s->pT,
+                          s->ub[s->u_ant], s->ubT[s->u_ant], s->vb[s->v_ant], s->vbT[s->v_ant],
+                          s->tb[s->t_ant], s->tbT[s->t_ant]};
+  for (int k = 0; k < kCkArrays; ++k) f[k] = a[k];
+}
+
+int ck_save(ixh_state *s, double *dst) {
+  double *f[kCkArrays];
+  ck_fields(s, f);
+  for (int k = 0; k < kCkArrays; ++k)
+    HIPCHK(hipMemcpyAsync(dst + (size_t)k * s->fsize, f[k], s->fsize * sizeof(double), hipMemcpyDeviceToDevice,
+                          s->stream));
+  return 0;
+}
+
+// restore a checkpoint as the state at a step boundary (p' zero, no snapshots)
+int ck_load(ixh_state *s, const double *src) {
+  s->u_ant = pick(s->u_cur, -1);
+  s->v_ant = pick(s->v_cur, -1);
+  s->t_ant = pick(s->t_cur, -1);
+  s->pc_zero = true;
+  s->v_snap = s->t_snap = s->pc_snap = -1;
+  s->u_hold = -1;
+  double *f[kCkArrays];
+  ck_fields(s, f);
+  for (int k = 0; k < kCkArrays; ++k)
+    HIPCHK(hipMemcpyAsync(f[k], src + (size_t)k * s->fsize, s->fsize * sizeof(double), hipMemcpyDeviceToDevice,
+                          s->stream));
+  return 0;
+}
+
+int adj_ept(int n) {
+  const int e = ept_for(n);
+  return e;
+}
+
+template <int DIR>
+int adj_sweep_launch(ixh_state *s, const AdjArgs &G, int nlines, int n) {
+  dim3 grid(nlines), block(kBlock);
+  switch (adj_ept(n)) {
+    case 1: hipLaunchKernelGGL((k_adj_sweep<DIR, 1>), grid, block, 0, s->stream, G); break;
+    case 2: hipLaunchKernelGGL((k_adj_sweep<DIR, 2>), grid, block, 0, s->stream, G); break;
+    case 3: hipLaunchKernelGGL((k_adj_sweep<DIR, 3>), grid, block, 0, s->stream, G); break;
+    case 4: hipLaunchKernelGGL((k_adj_sweep<DIR, 4>), grid, block, 0, s->stream, G); break;
+    case 6: hipLaunchKernelGGL((k_adj_sweep<DIR, 6>), grid, block, 0, s->stream, G); break;
+    default: hipLaunchKernelGGL((k_adj_sweep<DIR, 8>), grid, block, 0, s->stream, G); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// adjoint of one energy step with the velocities u, v (row layout):
+// lambda (dJ/dT^{k+1}) -> lambda (dJ/dT^k), boundary adjoints accumulated
+int adj_step(ixh_state *s, const double *u, const double *v) {
+  AdjArgs G;
+  memset(&G, 0, sizeof G);
+  G.S = base_args(s, IXH_DIR_X);  // row layout throughout
+  set_bc(s, G.S, IXH_VAR_T, IXH_DIR_X);
+  G.S.u = u;
+  G.S.v = v;
+  double *lam = s->adj_w[0], *lamp = s->adj_w[1], *mu = s->adj_w[2], *tant = s->adj_w[3];
+  const int npts = (s->mi + 1) * (s->nj + 1);
+  const dim3 pg((npts + kBlock - 1) / kBlock), block(kBlock);
+  // x-sweep (applied last in the step)
+  G.lam = lam; G.mu = mu; G.g_lo = s->adj_g[IXH_SIDE_A]; G.g_hi = s->adj_g[IXH_SIDE_C];
+  CHK(adj_sweep_launch<DIR_X>(s, G, s->nj - 1, s->mi + 1));
+  G.mu_in = mu; G.out = lamp; G.tant = tant;
+  hipLaunchKernelGGL(k_adj_gather<DIR_X>, pg, block, 0, s->stream, G);
+  HIPCHK(hipGetLastError());
+  // y-sweep
+  G.lam = lamp; G.mu = mu; G.g_lo = s->adj_g[IXH_SIDE_B]; G.g_hi = s->adj_g[IXH_SIDE_D];
+  CHK(adj_sweep_launch<DIR_Y>(s, G, s->mi - 1, s->nj + 1));
+  G.mu_in = mu; G.out = lam; G.tant = tant;
+  hipLaunchKernelGGL(k_adj_gather<DIR_Y>, pg, block, 0, s->stream, G);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+void adj_free_ck(ixh_state *s) {
+  for (double *p : s->adj_ck) (void)hipFree(p);
+  s->adj_ck.clear();
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixh_adjoint_record(ixh_state *s, int32_t nsteps, int32_t every) {
+  if (!s || nsteps < 1 || every < 1) return fail(IXH_ERR_ARG, "adjoint: bad arguments");
+  if (s->f32) return fail(IXH_ERR_UNSUPP, "adjoint: the FP64 path only");
+  if (s->prm.iter_ecuaci_max >= 0 || s->prm.iter_simple_max >= 0)
+    return fail(IXH_ERR_UNSUPP, "adjoint: single-pass loops only (iter_ecuaci_max < 0, iter_simple_max < 0)");
+  for (int v = 0; v < 3; ++v) CHK(check_bcs(s, v));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  const int nck = (nsteps + every - 1) / every + 1;  // segment starts and the final state
+  if ((int)s->adj_ck.size() != nck) {
+    adj_free_ck(s);
+    for (int k = 0; k < nck; ++k) {
+      double *p = nullptr;
+      CHK(dmalloc(&p, (size_t)kCkArrays * s->fsize));
+      s->adj_ck.push_back(p);
+    }
+  }
+  CHK(ensure_hist(s, nsteps));
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->d_slot, 0);
+  HIPCHK(hipGetLastError());
+  for (int k = 0; k < nsteps; ++k) {
+    if (k % every == 0) CHK(ck_save(s, s->adj_ck[k / every]));
+    CHK(fast_step(s));
+  }
+  CHK(ck_save(s, s->adj_ck[nck - 1]));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  s->adj_steps = nsteps;
+  s->adj_every = every;
+  return 0;
+}
+
+int ixh_adjoint_energy(ixh_state *s, const double *tbar, int32_t ld, double *grad_a, double *grad_b,
+                       double *grad_c, double *grad_d, double *grad_t0, int32_t ld0) {
+  if (!s || !tbar || ld < s->mi + 1 || (grad_t0 && ld0 < s->mi + 1))
+    return fail(IXH_ERR_ARG, "adjoint: bad arguments");
+  if (s->adj_steps < 1) return fail(IXH_ERR_ARG, "adjoint: no recorded run (ixh_adjoint_record)");
+  const int K = s->adj_steps, C = s->adj_every;
+  for (int k = 0; k < 4; ++k) {
+    if (!s->adj_w[k]) CHK(dmalloc(&s->adj_w[k], s->fsize));
+    if (!s->adj_g[k]) CHK(dmalloc(&s->adj_g[k], (size_t)std::max(s->mi, s->nj) + 4));
+    HIPCHK(hipMemsetAsync(s->adj_g[k], 0, ((size_t)std::max(s->mi, s->nj) + 4) * sizeof(double), s->stream));
+  }
+  while ((int)s->adj_tape.size() < 2 * C) {
+    double *p = nullptr;
+    CHK(dmalloc(&p, s->fsize));
+    s->adj_tape.push_back(p);
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  CHK(upload_as<double>(s, s->adj_w[0], tbar, ld, s->mi + 1, s->nj + 1, false));
+  const int nseg = (K + C - 1) / C;
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int k0 = seg * C, k1 = std::min(K, k0 + C);
+    CHK(ck_load(s, s->adj_ck[seg]));
+    hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->d_slot, k0);
+    HIPCHK(hipGetLastError());
+    for (int k = k0; k < k1; ++k) {
+      CHK(fast_step(s));
+      HIPCHK(hipMemcpyAsync(s->adj_tape[2 * (k - k0)], s->ub[s->u_cur], s->fsize * sizeof(double),
+                            hipMemcpyDeviceToDevice, s->stream));
+      HIPCHK(hipMemcpyAsync(s->adj_tape[2 * (k - k0) + 1], s->vb[s->v_cur], s->fsize * sizeof(double),
+                            hipMemcpyDeviceToDevice, s->stream));
+    }
+    for (int k = k1 - 1; k >= k0; --k) CHK(adj_step(s, s->adj_tape[2 * (k - k0)], s->adj_tape[2 * (k - k0) + 1]));
+  }
+  CHK(ck_load(s, s->adj_ck[nseg]));  // back to the end of the recorded run
+  // results
+  std::vector<double> g((size_t)std::max(s->mi, s->nj) + 4);
+  double *outs[4] = {grad_a, grad_b, grad_c, grad_d};
+  for (int sd = 0; sd < 4; ++sd) {
+    if (!outs[sd]) continue;
+    HIPCHK(hipMemcpyAsync(g.data(), s->adj_g[sd], g.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int n = side_len(s, IXH_VAR_T, sd);
+    for (int k = 1; k <= n; ++k) outs[sd][k - 1] = g[k];
+  }
+  if (grad_t0) {
+    std::vector<double> t((size_t)s->P * (s->nj + 1));
+    HIPCHK(hipMemcpyAsync(t.data(), s->adj_w[0], t.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int j = 0; j <= s->nj; ++j)
+      for (int i = 0; i <= s->mi; ++i) grad_t0[(size_t)i + (size_t)j * ld0] = t[(size_t)i + (size_t)j * s->P];
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
   return 0;
 }
 

@@ -1108,6 +1108,150 @@ __global__ void k_set_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// ----------------------------------------------------------- energy adjoint
+// Discrete adjoint of one energy step T^{k+1} = X(Y(T^k)), where Y and X are
+// the y- and x-sweeps of ensambla_energia_y/_x + boundary rows + line solves
+// (main.f90:974-1121) with the velocities of step k.  For fixed velocities the
+// step is affine in T^k, T_ant = T^k and the boundary values, so its adjoint
+// is exact: per sweep, a transposed line solve A^T mu = lambda on every
+// solved line (k_adj_sweep) and the transposed right-hand-side stencil
+// (k_adj_gather).  Everything is in the row layout (TR = false); y-lines read
+// with stride P.
+struct AdjArgs {
+  SweepArgs S;             // geometry, physics, boundary conditions of T, u and v of step k
+  const double *lam;       // adjoint of the sweep's output (row layout)
+  double *mu;              // transposed-solve result (row layout)
+  const double *mu_in;     // (gather) the sweep's mu
+  double *out;             // (gather) adjoint of the sweep's input
+  double *tant;            // (gather) adjoint of T_ant: written by X, added by Y
+  double *g_lo, *g_hi;     // boundary-value adjoints of the low / high side (per node)
+};
+
+template <int DIR, int EPT>
+__global__ void __launch_bounds__(kBlock) k_adj_sweep(AdjArgs G) {
+  __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
+  constexpr bool TR = false;
+  const SweepArgs &A = G.S;
+  const int P = A.P;
+  const int mi = A.mi, nj = A.nj;
+  const int n = Layout<EQ_T, DIR>::n(mi, nj);
+  const int hi = Layout<EQ_T, DIR>::hi(mi, nj);
+  const int line = 2 + (int)blockIdx.x;  // lines 2 .. last
+  const Side &lo_side = DIR == DIR_X ? A.bc[0] : A.bc[1];
+  const Side &hi_side = DIR == DIR_X ? A.bc[2] : A.bc[3];
+  // forward rows
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int pos = k + 1;
+    const int ii = DIR == DIR_X ? pos : line;
+    const int jj = DIR == DIR_X ? line : pos;
+    double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
+    if (pos >= 2 && pos <= hi) {
+      double area, pin;
+      const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii, jj, area, pin);
+      a0 = DIR == DIR_X ? c.ai : c.bs;
+      a2 = DIR == DIR_X ? c.ad : c.bn;
+      a1 = c.ac;
+    } else {
+      const bool high = pos != 1;
+      (void)bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
+    }
+    sa[k] = a0;
+    sb[k] = a1;
+    sc_[k] = a2;
+  }
+  __syncthreads();
+  // transpose: row k of A^T couples k-1 through c[k-1] and k+1 through a[k+1]
+  double ta[EPT], tc[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    if (k < n) {
+      ta[e] = k > 0 ? sc_[k - 1] : 0.0;
+      tc[e] = k < n - 1 ? sa[k + 1] : 0.0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    if (k < n) {
+      const int pos = k + 1;
+      const int ii = DIR == DIR_X ? pos : line;
+      const int jj = DIR == DIR_X ? line : pos;
+      sa[k] = ta[e];
+      sc_[k] = tc[e];
+      sr[k] = AT(G.lam, ii, jj);
+    }
+  }
+  __syncthreads();
+  solve_line<SOLVE_PCR, EPT>(sa, sb, sc_, sr, n);
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int pos = k + 1;
+    const int ii = DIR == DIR_X ? pos : line;
+    const int jj = DIR == DIR_X ? line : pos;
+    AT(G.mu, ii, jj) = sr[k];
+  }
+  // boundary rows take their value from the side: its adjoint is mu there
+  if (threadIdx.x == 0) {
+    G.g_lo[line] = G.g_lo[line] + sr[0];
+    G.g_hi[line] = G.g_hi[line] + sr[n - 1];
+  }
+}
+
+// adjoint of the sweep's right-hand side at node (i, j): out = d/d(input);
+// X also writes tant, Y adds its own T_ant part and tant into out
+template <int DIR>
+__global__ void __launch_bounds__(kBlock) k_adj_gather(AdjArgs G) {
+  constexpr bool TR = false;
+  const SweepArgs &A = G.S;
+  const int P = A.P;
+  const int mi = A.mi, nj = A.nj;
+  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (idx >= (mi + 1) * (nj + 1)) return;
+  const int ii = 1 + idx % (mi + 1), jj = 1 + idx / (mi + 1);
+  const double keep = 1.0 - A.ph.rel_ener;
+  const double *mu = G.mu_in;
+  double area, pin;
+  double o = 0.0, ta = 0.0;
+  // lines the sweep does not solve are copied from its input
+  const bool passthrough = DIR == DIR_X ? (jj == 1 || jj == nj + 1) : (ii == 1 || ii == mi + 1);
+  if (passthrough) {
+    o = AT(G.lam, ii, jj);
+  } else if (ii >= 2 && ii <= mi && jj >= 2 && jj <= nj) {
+    // the row of (ii, jj) reads its own node through (1 - alpha) a_P phi
+    const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii, jj, area, pin);
+    o = keep * c.ac * AT(mu, ii, jj);
+    ta = area * A.fg.inv_dt * AT(mu, ii, jj);
+  }
+  // neighbouring rows read this node in their head term
+  if (DIR == DIR_X) {
+    if (ii >= 2 && ii <= mi) {
+      if (jj + 1 >= 2 && jj + 1 <= nj) {
+        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii, jj + 1, area, pin);
+        o = o - c.bs * AT(mu, ii, jj + 1);
+      }
+      if (jj - 1 >= 2 && jj - 1 <= nj) {
+        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii, jj - 1, area, pin);
+        o = o - c.bn * AT(mu, ii, jj - 1);
+      }
+    }
+    AT(G.out, ii, jj) = o;
+    AT(G.tant, ii, jj) = ta;
+  } else {
+    if (jj >= 2 && jj <= nj) {
+      if (ii + 1 >= 2 && ii + 1 <= mi) {
+        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii + 1, jj, area, pin);
+        o = o - c.ai * AT(mu, ii + 1, jj);
+      }
+      if (ii - 1 >= 2 && ii - 1 <= mi) {
+        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii - 1, jj, area, pin);
+        o = o - c.ad * AT(mu, ii - 1, jj);
+      }
+    }
+    AT(G.out, ii, jj) = o + ta + AT(G.tant, ii, jj);
+  }
+}
+
 // ----------------------------------------------------------------- Nusselt
 // nusselt_promedio_y (source/mod_postproceso.f90:100-215) and the
 // vertical-wall mean Nusselt numbers; a single workgroup, sums in upstream

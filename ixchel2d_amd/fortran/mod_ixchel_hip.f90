@@ -207,6 +207,20 @@ module ixchel_hip
        integer(c_int32_t), intent(out) :: order, graph
        real(c_double), intent(out) :: ms(*)
      end function ixh_batch_autotune
+     ! discrete adjoint of the energy transport (no upstream counterpart)
+     integer(c_int) function ixh_adjoint_record(state, nsteps, every) bind(C, name='ixh_adjoint_record')
+       import :: c_int, c_ptr, c_int32_t
+       type(c_ptr), value :: state
+       integer(c_int32_t), value :: nsteps, every
+     end function ixh_adjoint_record
+     integer(c_int) function ixh_adjoint_energy(state, tbar, ld, grad_a, grad_b, grad_c, grad_d, grad_t0, ld0) &
+          bind(C, name='ixh_adjoint_energy')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: state
+       real(c_double), intent(in) :: tbar(*)
+       integer(c_int32_t), value :: ld, ld0
+       type(c_ptr), value :: grad_a, grad_b, grad_c, grad_d, grad_t0
+     end function ixh_adjoint_energy
   end interface
 
   ! the device state of this program's one cavity

@@ -98,6 +98,13 @@ class Cavity:
     def fields(self) -> Dict[str, np.ndarray]:
         return {n: self.get(n) for n in _lib.FIELD_ID}
 
+    def set_bc(self, var: str, side: str, types, values) -> None:
+        """Per-node boundary types (IXH_BC_*) and values of one side."""
+        t = np.ascontiguousarray(types, dtype=np.int32)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _lib.check(self.L.ixh_set_bc(self.h, _lib.VAR_ID[var], _lib.SIDE_ID[side],
+                                     t.ctypes.data_as(C.POINTER(C.c_int32)), _ptr(v)))
+
     def set_sources(self, var: str, sc, sp) -> None:
         if sc is None and sp is None:
             _lib.check(self.L.ixh_set_sources(self.h, _lib.VAR_ID[var], None, None))
@@ -179,6 +186,25 @@ class Cavity:
         a, b = C.c_double(), C.c_double()
         _lib.check(self.L.ixh_nusselt_x(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+    # ------------------------------------------------------------- adjoint
+    def adjoint_record(self, nsteps: int, every: int = 20) -> None:
+        """Run ``nsteps`` single-pass steps keeping a checkpoint every
+        ``every`` steps for ``adjoint_energy``."""
+        _lib.check(self.L.ixh_adjoint_record(self.h, nsteps, every))
+
+    def adjoint_energy(self, tbar):
+        """Backward pass of the recorded energy steps for dJ/dT(final) =
+        ``tbar``: returns ({side: dJ/d(boundary values)}, dJ/dT(initial))."""
+        tb = _f64(tbar)
+        if tb.shape != self.shape("t"):
+            raise ValueError(f"tbar: shape {tb.shape} != {self.shape('t')}")
+        mi, nj = self.mi, self.nj
+        g = {"a": np.zeros(nj + 1), "b": np.zeros(mi + 1), "c": np.zeros(nj + 1), "d": np.zeros(mi + 1)}
+        t0 = np.zeros(self.shape("t"), order="F")
+        _lib.check(self.L.ixh_adjoint_energy(self.h, _ptr(tb), tb.shape[0], _ptr(g["a"]), _ptr(g["b"]),
+                                             _ptr(g["c"]), _ptr(g["d"]), _ptr(t0), t0.shape[0]))
+        return g, t0
 
     # -------------------------------------------------------------- timing
     def set_profiling(self, on: bool) -> None:

@@ -120,6 +120,13 @@ class Oracle:
                           ("t_ant", t)):
             self.set(name, arr)
 
+    def set_bc(self, var, side, types, values):
+        """Per-node boundary types and values of one side."""
+        L = lib()
+        typ = np.ascontiguousarray(types, dtype=np.int32)
+        val = np.ascontiguousarray(values, dtype=np.float64)
+        L.ox_set_bc(self.h, VAR_ID[var], SIDE_ID[side], typ.ctypes.data_as(C.POINTER(C.c_int32)), _dp(val))
+
     def __del__(self):
         try:
             if getattr(self, "h", None):

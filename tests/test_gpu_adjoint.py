@@ -1,0 +1,119 @@
+"""GPU checks of the discrete energy adjoint (ixh_adjoint_record /
+ixh_adjoint_energy) on a flow that does not depend on T (lid-driven cavity
+carrying a passive temperature), where the adjoint is the exact gradient of
+the K-step map and J = Nu(T^K) is affine in the wall values:
+  - against central finite differences on the GPU (h = 1e-6): 1e-6 relative;
+  - against the FP64 reference gradient from the oracle (exact differences
+    of the affine map on the CPU): 1e-8 relative;
+  - dJ/dT(initial) against the change of J under a random perturbation of
+    the initial temperature (dot-product test): 1e-8 relative;
+  - checkpoint spacing (K not a multiple of it) does not change the result,
+    and the state ends as after the recorded run."""
+import numpy as np
+import pytest
+
+from ixchel2d_amd import adjoint as A
+from ixchel2d_amd import case as C
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+K = 60
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    torch = pytest.importorskip("torch")
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    torch.cuda.set_device(0)
+
+
+@pytest.fixture(scope="module")
+def problem():
+    from ixchel2d_amd.simple import Cavity
+    case = C.heated_lid_case(34, Re=100.0, dt=0.05)
+    warm = Cavity(case, solver="pcr")
+    warm.run(100)
+    start = {f: warm.get(f) for f in ("u", "v", "t", "p")}
+    return A.HotWallProblem(case, ncontrols=8, steps=K, every=20, start=start)
+
+
+def _controls(n, seed=1):
+    rng = np.random.default_rng(seed)
+    return 1.0 + rng.uniform(-0.1, 0.1, n)
+
+
+def test_adjoint_against_central_differences(problem):
+    c = _controls(8)
+    problem.forward(c)
+    g = problem.gradient()
+    h = 1e-6
+    fd = np.zeros_like(c)
+    for i in range(len(c)):
+        e = np.zeros_like(c)
+        e[i] = h
+        fd[i] = (problem.forward(c + e) - problem.forward(c - e)) / (2 * h)
+    assert np.max(np.abs(g - fd)) <= 1e-6 * np.max(np.abs(fd)), (g, fd)
+
+
+def test_adjoint_against_oracle_reference_gradient(problem):
+    c = _controls(8, seed=2)
+    problem.forward(c)
+    g = problem.gradient()
+    case = problem.case
+
+    def J(cc):
+        o = Oracle(case)
+        typ = problem.types_a
+        o.set_bc("t", "a", typ, problem.wall_values(cc))
+        for f, arr in problem.start.items():
+            o.set(f, arr)
+            if f in ("u", "v", "t"):
+                o.set(f + "_ant", arr)
+        o.run(K)
+        return o.nusselt_x()[0]
+
+    j0 = J(c)
+    ref = np.array([J(c + np.eye(len(c))[i]) - j0 for i in range(len(c))])
+    assert np.max(np.abs(g - ref)) <= 1e-8 * np.max(np.abs(ref)), (g, ref)
+
+
+def test_initial_temperature_adjoint_dot_product(problem):
+    c = _controls(8, seed=3)
+    problem.forward(c)
+    _, gt0 = problem.cav.adjoint_energy(problem.W)
+    j0 = problem.cav.nusselt_x()[0]
+    rng = np.random.default_rng(4)
+    d = rng.standard_normal(problem.start["t"].shape) * 1e-3
+    saved = problem.start["t"]
+    try:
+        problem.start = dict(problem.start, t=saved + d)
+        j1 = problem.forward(c)
+    finally:
+        problem.start = dict(problem.start, t=saved)
+    assert abs((j1 - j0) - float(np.sum(gt0 * d))) <= 1e-8 * abs(j1 - j0)
+
+
+def test_checkpoint_spacing_and_final_state(problem):
+    c = _controls(8, seed=5)
+    problem.forward(c)
+    t_end = problem.cav.get("t")
+    g20 = problem.gradient()
+    assert np.array_equal(problem.cav.get("t"), t_end)
+    problem.every = 7  # 60 = 8 * 7 + 4
+    try:
+        problem.forward(c)
+        g7 = problem.gradient()
+    finally:
+        problem.every = 20
+    np.testing.assert_allclose(g7, g20, rtol=1e-12, atol=1e-15)
+
+
+def test_autograd_function(problem):
+    torch = pytest.importorskip("torch")
+    c = torch.tensor(_controls(8, seed=6), dtype=torch.float64, requires_grad=True)
+    nu = A.HotWallNusselt.apply(c, problem)
+    nu.backward()
+    ref = problem.gradient()
+    np.testing.assert_allclose(c.grad.numpy(), ref, rtol=1e-14, atol=0)
