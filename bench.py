@@ -11,6 +11,7 @@ its error against FP64.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver pcr|thomas]
   python bench.py --workload batch256   # 256 mixed-size cavities, one launch per phase
+  python bench.py --workload adjoint129 # hot-wall Nu gradient through the energy adjoint
 """
 from __future__ import annotations
 
@@ -163,6 +164,113 @@ def batch_bench(args):
     print(json.dumps(out))
 
 
+def adjoint_bench(args):
+    """Gradient of the mean hot-wall Nusselt number with respect to 64
+    piecewise-linear hot-wall temperature controls (base 1 +- U(0.1)) after
+    K = 400 SIMPLE iterations from a converged start, 129^2-cell natural
+    convection (Ra = 1e5, Pr = 0.71, uniform mesh), checkpoints every 20
+    iterations: gradient evaluations per second, device memory, and the
+    gradient against central differences (h = 1e-6) of the coupled run and
+    of the same run without buoyancy (where the adjoint is exact)."""
+    import numpy as np
+    import torch
+    from ixchel2d_amd import adjoint as A
+    from ixchel2d_amd import case as C
+    from ixchel2d_amd.simple import Cavity
+
+    case = C.natural_convection_case(130, 130, height=1.0, Ra=1e5, iters=400, dt=args.adj_dt, seed=0,
+                                     mesh_kind="unif")
+    warm = Cavity(case, solver="pcr")
+    hw = warm.run(args.adj_warm, cap=1)
+    hw2 = warm.run(1)
+    start = {f: warm.get(f) for f in ("u", "v", "t", "p")}
+    warm.close()
+    torch.cuda.synchronize()
+    free0 = torch.cuda.mem_get_info()[0]
+    prob = A.HotWallProblem(case, ncontrols=64, steps=400, every=20, start=start)
+    rng = np.random.default_rng(0)
+    c = 1.0 + rng.uniform(-0.1, 0.1, 64)
+    prob.forward(c)
+    prob.gradient()
+    torch.cuda.synchronize()
+    reps = args.adj_reps
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        nu = prob.forward(c)
+        g = prob.gradient()
+    torch.cuda.synchronize()
+    per = (time.perf_counter() - t0) / reps
+    used = free0 - torch.cuda.mem_get_info()[0]
+
+    def fd(prob, h=1e-6):
+        out = np.zeros(64)
+        for i in range(64):
+            e = np.zeros(64)
+            e[i] = h
+            out[i] = (prob.forward(c + e) - prob.forward(c - e)) / (2 * h)
+        return out
+
+    t1 = time.perf_counter()
+    fd_coupled = fd(prob)
+    fd_time = time.perf_counter() - t1
+    prob.forward(c)
+    g = prob.gradient()
+    rel_coupled = float(np.max(np.abs(g - fd_coupled)) / np.max(np.abs(fd_coupled)))
+    prob.cav.set_params(ri_y=0.0)
+    prob.forward(c)
+    g0 = prob.gradient()
+    fd_passive = fd(prob)
+    rel_passive = float(np.max(np.abs(g0 - fd_passive)) / np.max(np.abs(fd_passive)))
+    cpu = None
+    if not args.no_cpu:
+        from oracle.pyoracle import Oracle
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+        def J(cc):
+            o = Oracle(case)
+            o.set_bc("t", "a", prob.types_a, prob.wall_values(cc))
+            for f, arr in start.items():
+                o.set(f, arr)
+                if f in ("u", "v", "t"):
+                    o.set(f + "_ant", arr)
+            o.run(400)
+            return o.nusselt_x()[0]
+        nsample = 4
+        t2 = time.perf_counter()
+        ref = []
+        for i in range(nsample):
+            e = np.zeros(64)
+            e[i] = 1e-6
+            ref.append((J(c + e) - J(c - e)) / 2e-6)
+        tc = time.perf_counter() - t2
+        cpu = {"value": round(1.0 / (tc / nsample * 64), 5), "unit": "gradient evaluations/s", "cores": threads,
+               "kind": "port",
+               "sample": f"C oracle (OpenMP) central differences of the coupled run for {nsample} of the 64 "
+                         "controls (2 x 400 iterations each), scaled to 64 controls",
+               "fd_reference_vs_gpu_fd_rel": float(np.max(np.abs(np.array(ref) - fd_coupled[:nsample]))
+                                                   / np.max(np.abs(fd_coupled[:nsample])))}
+    out = {
+        "metric": "gradient evaluations/s, d(hot-wall Nu)/d(64 wall-temperature controls), K=400",
+        "value": round(1.0 / per, 3), "unit": "gradient evaluations/s", "n_gpus": 1, "steps": reps,
+        "warmup": 1, "ms_per_step": round(per * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "natural convection 129^2 cells uniform, Ra=1e5, Pr=0.71, K=400 iterations "
+                               "from a converged start, checkpoints every 20",
+                   "dt": args.adj_dt, "warm_iterations": args.adj_warm,
+                   "start_residuo": float(hw2["residuo"][-1]), "start_maxbo": float(hw2["maxbo"][-1])},
+        "nu": nu,
+        "device_memory_MB": round(used / 2**20, 1),
+        "gradient_check": {
+            "coupled_rel_err_vs_central_fd": rel_coupled,
+            "without_buoyancy_rel_err_vs_central_fd": rel_passive,
+            "fd_seconds_64_controls": round(fd_time, 2),
+            "note": "the adjoint holds each step's velocities fixed: exact without buoyancy, its frozen-flow "
+                    "part with buoyancy"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def time_path(case, solver, precision, steps, warmup):
     """Whole-step throughput of one cavity, then per-kernel durations from
     HIP events around every launch on the library's stream."""
@@ -232,7 +340,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--solver", default="pcr", choices=["pcr", "thomas"])
-    ap.add_argument("--workload", default="nc301x701", choices=["nc301x701", "batch256"])
+    ap.add_argument("--workload", default="nc301x701", choices=["nc301x701", "batch256", "adjoint129"])
+    ap.add_argument("--adj-dt", type=float, default=0.01, help="time step of the adjoint workload")
+    ap.add_argument("--adj-warm", type=int, default=20000, help="iterations to the converged start")
+    ap.add_argument("--adj-reps", type=int, default=5, help="timed gradient evaluations")
     ap.add_argument("--batch", type=int, default=256, help="cavities of the batch workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed"],
@@ -255,6 +366,9 @@ def main():
     torch.cuda.set_device(0)
     if args.workload == "batch256":
         batch_bench(args)
+        return
+    if args.workload == "adjoint129":
+        adjoint_bench(args)
         return
     if args.calibrate_copy_mb:
         n = args.calibrate_copy_mb * (1 << 20) // 4
