@@ -145,10 +145,11 @@ int ixh_lag(ixh_state *s);
 
 /* nsteps time steps of main.f90:391-1270 with upstream's inner-loop and
  * SIMPLE exit rules.  When every loop is single-pass (iter_ecuaci_max < 0 and
- * iter_simple_max < 0) the steps run as a pipelined launch sequence on three streams with no host
- * synchronisation; otherwise the errors are read back after every pass.  One
- * record per SIMPLE iteration goes to hist (up to cap); *nrec receives the
- * count (may be NULL). */
+ * iter_simple_max < 0) the steps run as 7 launches per step on the state's
+ * stream (independent phases of consecutive steps share a launch) with no
+ * host synchronisation; otherwise the errors are read back after every pass.
+ * One record per SIMPLE iteration goes to hist (up to cap); *nrec receives
+ * the count (may be NULL). */
 int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t cap, int32_t *nrec);
 
 /* nusselt_promedio_y on the current T (source/mod_postproceso.f90:100-215):

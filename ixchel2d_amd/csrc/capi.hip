@@ -1,6 +1,5 @@
 // Host side of the C-ABI (include/ixchel2d.h): device state, buffer
-// rotation, phase launches, the SIMPLE control loop and the pipelined
-// multi-stream time step.
+// rotation, phase launches, the SIMPLE control loop and the fused time step.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -61,12 +60,8 @@ struct ixh_state {
   size_t fsize = 0;  // doubles per field buffer (either layout)
   hipStream_t stream = nullptr;
   bool own_stream = false;
-  // kernels go to `launch` (normally `stream`); the pipelined time step
-  // forks two more branches onto aux[0], aux[1] and joins them back
-  hipStream_t launch = nullptr;
-  hipStream_t aux[2] = {nullptr, nullptr};
-  hipEvent_t ev_fork = nullptr, ev_t = nullptr, ev_j[2] = {nullptr, nullptr};
-  int u_hold = -1;  // u buffer still read by an in-flight residual
+  hipStream_t launch = nullptr;  // the stream kernels go to (the state's stream)
+  int u_hold = -1;  // u buffer still read by a residual running beside the next u sweeps
   ixh_params prm{};
   // FP32-mixed: u, v, T, p' (all copies) and a_u, a_v are stored as float in
   // the same buffers (same element pitch); p, b_o, Res_u stay FP64
@@ -621,63 +616,101 @@ int fast_step(ixh_state *s) {
   return 0;
 }
 
-// Pipelined time step k: v and p' sweeps and the correction, then three
-// branches that each need only the corrected fields of step k:
-//   stream: T sweeps of step k
-//   aux[1]: residual of step k, then (after the T sweeps) the norm fold
-//   aux[0]: u sweeps of step k+1 (u_ant of step k+1 is u of step k)
-// joined before the v sweeps of step k+1.  The u equation must not read T
-// (Ri = 0), so the branches share no written buffer; the u buffer the
-// residual still reads as u_ant is held out of the rotation.  The u sweeps
-// of the first step run before the first call (prologue).
-int pipe_step(ixh_state *s, bool last) {
-  CHK(sweep_v(s, IXH_DIR_Y));
+// ---- fused step: 7 launches on the state's stream per time step
+// k_multi with sweeps (EQ0, DIR0) and (EQ1, DIR1) at a common EPT
+template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
+int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept) {
+  dim3 grid(nwg), block(kBlock);
+  switch (ept) {
+    case 1: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 1, R>), grid, block, 0, s->stream, M); break;
+    case 2: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R>), grid, block, 0, s->stream, M); break;
+    case 3: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 3, R>), grid, block, 0, s->stream, M); break;
+    case 4: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R>), grid, block, 0, s->stream, M); break;
+    case 6: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 6, R>), grid, block, 0, s->stream, M); break;
+    default: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 8, R>), grid, block, 0, s->stream, M); break;
+  }
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+template <int EQ0, int DIR0, int EQ1, int DIR1>
+int launch_multi(ixh_state *s, const MultiArgs &M, int nwg, int ept) {
+  if (s->f32) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, float>(s, M, nwg, ept);
+  if (s->prm.solver == IXH_SOLVER_THOMAS)
+    return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_THOMAS, double>(s, M, nwg, ept);
+  if (s->prm.solver == IXH_SOLVER_PCR_EXACT)
+    return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR_EXACT, double>(s, M, nwg, ept);
+  return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, double>(s, M, nwg, ept);
+}
+
+int round8(int x) { return (x + 7) & ~7; }
+
+// lay out the parts of a fused launch; returns the workgroup count
+int multi_layout(MultiArgs &M, int n0, int n1, int nres, int nfin) {
+  M.cnt[0] = n0; M.cnt[1] = n1; M.cnt[2] = nres; M.cnt[3] = nfin;
+  M.off[0] = 0;
+  M.off[1] = round8(n0);
+  M.off[2] = M.off[1] + round8(n1);
+  M.off[3] = M.off[2] + round8(nres);
+  return M.off[3] + nfin;
+}
+
+// One time step k with independent phases fused (the u equation must not
+// read T):
+//   v_y(k) [+ fold of step k-1]; v_x(k); p'_y(k); p'_x(k); correction(k);
+//   T_y(k) + residual(k) + u_y(k+1);  T_x(k) + u_x(k+1)
+// The u sweeps of the first step run before the first call, the fold of the
+// last step after the last call.  u_ant of step k+1 is u of step k; the u
+// buffer the residual of step k reads as its u_ant is held out of the
+// rotation while the u sweeps of step k+1 run beside it.
+int fused_step(ixh_state *s, bool first, bool last) {
+  {
+    MultiArgs M;
+    memset(&M, 0, sizeof M);
+    CHK(prep_v(s, IXH_DIR_Y, M.s[0]));
+    if (!first) prep_finalize(s, M.f, 31, s->d_hist, s->d_slot);
+    const int nwg = multi_layout(M, M.s[0].nlines, 0, 0, first ? 0 : 1);
+    CHK((launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, M, nwg, ept_for(line_len(s, EQ_V, IXH_DIR_Y)))));
+  }
   CHK(sweep_v(s, IXH_DIR_X));
   s->pc_zero = true;
   CHK(sweep_p(s, IXH_DIR_Y));
   CHK(sweep_p(s, IXH_DIR_X));
   CHK(do_correct(s));
-  HIPCHK(hipEventRecord(s->ev_fork, s->stream));
-  HIPCHK(hipStreamWaitEvent(s->aux[0], s->ev_fork, 0));
-  HIPCHK(hipStreamWaitEvent(s->aux[1], s->ev_fork, 0));
-  s->launch = s->aux[1];
-  CHK(do_residual(s));
-  s->launch = s->stream;
-  CHK(sweep_t(s, IXH_DIR_Y));
-  CHK(sweep_t(s, IXH_DIR_X));
-  HIPCHK(hipEventRecord(s->ev_t, s->stream));
-  HIPCHK(hipStreamWaitEvent(s->aux[1], s->ev_t, 0));
-  s->launch = s->aux[1];
-  CHK(do_finalize(s, 31, s->d_hist, s->d_slot));
-  s->launch = s->aux[0];
+  // the T sweeps and the residual of step k read u of step k: build their
+  // arguments before the u sweeps of step k+1 advance the rotation
+  MultiArgs M6, M7;
+  memset(&M6, 0, sizeof M6);
+  memset(&M7, 0, sizeof M7);
+  CHK(prep_t(s, IXH_DIR_Y, M6.s[0]));
+  CHK(prep_t(s, IXH_DIR_X, M7.s[0]));
+  prep_residual(s, M6.r);
   const int held = s->u_ant;
   s->u_ant = s->u_cur;
+  int ept6 = ept_for(line_len(s, EQ_T, IXH_DIR_Y)), ept7 = ept_for(line_len(s, EQ_T, IXH_DIR_X));
+  int nu6 = 0, nu7 = 0;
   if (!last) {
     s->u_hold = held;
-    CHK(sweep_u(s, IXH_DIR_Y));
-    CHK(sweep_u(s, IXH_DIR_X));
+    CHK(prep_u(s, IXH_DIR_Y, M6.s[1]));
+    CHK(prep_u(s, IXH_DIR_X, M7.s[1]));
     s->u_hold = -1;
+    nu6 = M6.s[1].nlines;
+    nu7 = M7.s[1].nlines;
+    ept6 = std::max(ept6, ept_for(line_len(s, EQ_U, IXH_DIR_Y)));
+    ept7 = std::max(ept7, ept_for(line_len(s, EQ_U, IXH_DIR_X)));
   }
-  s->launch = s->stream;
-  HIPCHK(hipEventRecord(s->ev_j[0], s->aux[0]));
-  HIPCHK(hipEventRecord(s->ev_j[1], s->aux[1]));
-  HIPCHK(hipStreamWaitEvent(s->stream, s->ev_j[0], 0));
-  HIPCHK(hipStreamWaitEvent(s->stream, s->ev_j[1], 0));
+  const int nwg6 = multi_layout(M6, M6.s[0].nlines, nu6, s->nj - 1, 0);
+  const int nwg7 = multi_layout(M7, M7.s[0].nlines, nu7, 0, 0);
+  if (last) {
+    CHK((launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, M6, nwg6, ept6)));
+    CHK((launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, M7, nwg7, ept7)));
+  } else {
+    CHK((launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, M6, nwg6, ept6)));
+    CHK((launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, M7, nwg7, ept7)));
+  }
   s->v_ant = s->v_cur;
   s->t_ant = s->t_cur;
-  return 0;
-}
-
-// streams and events of the pipelined step, created on first use (states
-// that only run in batches never need them)
-int ensure_aux(ixh_state *s) {
-  if (s->aux[0]) return 0;
-  for (int k = 0; k < 2; ++k) {
-    HIPCHK(hipStreamCreate(&s->aux[k]));
-    HIPCHK(hipEventCreateWithFlags(&s->ev_j[k], hipEventDisableTiming));
-  }
-  HIPCHK(hipEventCreateWithFlags(&s->ev_fork, hipEventDisableTiming));
-  HIPCHK(hipEventCreateWithFlags(&s->ev_t, hipEventDisableTiming));
+  if (last) CHK(do_finalize(s, 31, s->d_hist, s->d_slot));
   return 0;
 }
 
@@ -841,12 +874,6 @@ int ixh_destroy(ixh_state *s) {
   }
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
-  for (int k = 0; k < 2; ++k) {
-    if (s->aux[k]) (void)hipStreamDestroy(s->aux[k]);
-    if (s->ev_j[k]) (void)hipEventDestroy(s->ev_j[k]);
-  }
-  if (s->ev_fork) (void)hipEventDestroy(s->ev_fork);
-  if (s->ev_t) (void)hipEventDestroy(s->ev_t);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return 0;
@@ -1113,13 +1140,13 @@ int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t 
       // or a u equation that reads T)
       for (int k = 0; k < nsteps; ++k) CHK(fast_step(s));
     } else if (nsteps > 0) {
-      // direct launches: on this stack a captured HIP graph of the same step
-      // ran slower (145.7 us vs 140.0 us per step sequential, 155.5 us vs
-      // 138.5 us pipelined, 301x701 FP64)
-      CHK(ensure_aux(s));
+      // fused launches on one stream: work on separate streams did not
+      // overlap on this stack (the cross-stream waits cost more than the
+      // overlap won), and a captured HIP graph of the step ran slower than
+      // direct launches
       CHK(sweep_u(s, IXH_DIR_Y));  // prologue: u sweeps of the first step
       CHK(sweep_u(s, IXH_DIR_X));
-      for (int k = 0; k < nsteps; ++k) CHK(pipe_step(s, k == nsteps - 1));
+      for (int k = 0; k < nsteps; ++k) CHK(fused_step(s, k == 0, k == nsteps - 1));
     }
     const int keep = nsteps < cap ? nsteps : cap;
     std::vector<double> h((size_t)8 * (keep > 0 ? keep : 1));

@@ -683,7 +683,9 @@ __device__ __forceinline__ int xcd_line(int w, int n) {
 // the node is untyped
 __device__ __forceinline__ bool bc_row(const Side &sd, int node, bool high, double &a0, double &a1,
                                        double &a2, double &rr) {
-  int t = sd.type[node];
+  // both loads issued together (one memory round trip)
+  const int t = sd.type[node];
+  const double v = sd.val[node];
   if (t == 0) return false;
   if (t == 1) {
     a0 = 0.0; a1 = 1.0; a2 = 0.0;
@@ -692,7 +694,7 @@ __device__ __forceinline__ bool bc_row(const Side &sd, int node, bool high, doub
   } else {
     a0 = -1.0; a1 = 1.0; a2 = 0.0;
   }
-  rr = sd.val[node];
+  rr = v;
   return true;
 }
 
@@ -719,17 +721,29 @@ struct Layout {
 // real type of the stored fields, the assembly and the line solve (double,
 // or float in the FP32-mixed path, which always uses the PCR solver); rows
 // are formed in FP64 registers, p' right-hand sides and norms in FP64.
+// Every global load of the line (stencil operands, boundary values, the
+// pass-start field for the norms, the lines carried through) is issued in
+// the assembly loop, before any global store, so a workgroup waits for
+// memory once; rows past the line or at its ends assemble at a clamped
+// interior position and are then replaced.
+// LDS of one line workgroup, in doubles: the four line arrays and the
+// reduction scratch
+template <typename R, int EPT>
+constexpr int line_lds() { return (4 * EPT * kBlock * (int)sizeof(R)) / 8 + kBlock; }
+
 template <int EQ, int DIR, int SOLVER, int EPT, typename R>
-__device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
+__device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, double *lds) {
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR, "the FP32 sweeps use the PCR solver");
-  __shared__ R sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
-  __shared__ double red[kBlock];
+  R *sa = reinterpret_cast<R *>(lds), *sb = sa + EPT * kBlock, *sc_ = sb + EPT * kBlock, *sr = sc_ + EPT * kBlock;
+  double *red = lds + (4 * EPT * kBlock * (int)sizeof(R)) / 8;
   constexpr bool TR = DIR == DIR_Y;
+  constexpr bool MOM = EQ == EQ_U || EQ == EQ_V;
+  constexpr bool NORM = DIR == DIR_X && EQ != EQ_U;
   const int P = A.P;
   const int P2 = A.P2;
   const int mi = A.mi, nj = A.nj;
   const int n = Layout<EQ, DIR>::n(mi, nj);
-  const int hi = Layout<EQ, DIR>::hi(mi, nj);
+  const int hi = Layout<EQ, DIR>::hi(mi, nj);  // = n - 1
   const int lastl = Layout<EQ, DIR>::last_line(mi, nj);
   // sides holding the two ends of this line: x-lines end on a (low) and c,
   // y-lines on b (low) and d
@@ -738,122 +752,156 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
   // a_u / a_v of a boundary face (1e40, main.f90:309-310; infinite in FP32)
   const R big = sizeof(R) == 8 ? R(1.e40) : R(__builtin_huge_valf());
   R *acoef = fldw<R>(A.acoef), *acoef2 = fldw<R>(A.acoef2);
+  R *out = fldw<R>(A.out), *out2 = fldw<R>(A.out2);
+  // the boundary rows at both ends (p': Dirichlet 0, main.f90:789-800, 845-856)
+  double lb0 = 0.0, lb1 = 1.0, lb2 = 0.0, lbr = 0.0, hb0 = 0.0, hb1 = 1.0, hb2 = 0.0, hbr = 0.0;
+  bool lo_typed = false, hi_typed = false;
+  if (EQ != EQ_P) {
+    lo_typed = bc_row(lo_side, line, false, lb0, lb1, lb2, lbr);
+    hi_typed = bc_row(hi_side, line, true, hb0, hb1, hb2, hbr);
+  }
+  // the first and last solved lines also carry the lines next to them
+  const int other = line == A.line0 ? A.line0 - 1 : (line == lastl ? lastl + 1 : 0);
+  const R *src = fld<R>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc);
 
-  for (int k = threadIdx.x; k < n; k += kBlock) {
-    const int pos = k + 1;                 // 1-based position along the line
-    const int ii = DIR == DIR_X ? pos : line;
-    const int jj = DIR == DIR_X ? line : pos;
-    double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
-    if (pos >= 2 && pos <= hi) {
-      if (EQ == EQ_P) {
-        const PCoefT<R> c = p_coef<TR, R>(A, ii, jj);
-        const R *pc = fld<R>(A.pc);
-        double pcP = A.in_zero ? 0.0 : (double)AT(pc, ii, jj);
-        double head;
-        if (DIR == DIR_X) {
-          a0 = c.ai; a2 = c.ad;
-          double pS = A.in_zero ? 0.0 : (double)AT(pc, ii, jj - 1);
-          double pN = A.in_zero ? 0.0 : (double)AT(pc, ii, jj + 1);
-          head = -(double)c.bs * pS - (double)c.bn * pN;
-          AT(A.bo, ii, jj) = c.b;  // x-sweep: row layout
-        } else {
-          a0 = c.bs; a2 = c.bn;
-          double pW = A.in_zero ? 0.0 : (double)AT(pc, ii - 1, jj);
-          double pE = A.in_zero ? 0.0 : (double)AT(pc, ii + 1, jj);
-          head = -(double)c.ai * pW - (double)c.ad * pE;
-        }
-        a1 = c.ac;
-        rr = head + c.b + (1.0 - A.ph.rel_pres) * (double)c.ac * pcP;
-        if constexpr (sizeof(R) == 4) {
-          // every neighbour behind an infinite a_u, a_v (inside a body): p' = 0
-          if (c.ac == R(0)) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = 0.0; }
-        }
-      } else if constexpr (SOLVER == SOLVE_PCR) {
-        R area;
-        double pin;
-        const CoefT<R> c = fast_coef<EQ, TR, R>(A, ii, jj, area, pin);
-        const R *phi = fld<R>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t);
-        R head;
-        if (DIR == DIR_X) {
-          a0 = c.ai; a2 = c.ad;
-          head = -c.bs * AT(phi, ii, jj - 1) - c.bn * AT(phi, ii, jj + 1);
-        } else {
-          a0 = c.bs; a2 = c.bn;
-          head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
-        }
-        a1 = c.ac;
-        rr = fast_rhs<EQ, TR, R>(A, ii, jj, head, c.ac, area);
-        R aw = c.ac * R(A.ph.rel_vel);
-        if constexpr (sizeof(R) == 4) {
-          if (pin == pin) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = pin; aw = big; }
-        }
-        if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X) {
-          AT(acoef, ii, jj) = aw;
-          AO(acoef2, ii, jj) = aw;
-        }
+  R keep_aw[EPT], carry[EPT];
+  double keep_old[EPT], keep_b[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    const int pos = k + 1;                                    // 1-based position along the line
+    const int pc = pos < 2 ? 2 : (pos > hi ? hi : pos);       // assembled position
+    const int pr = pos > n ? n : pos;                         // position for the plain loads
+    const int ii = DIR == DIR_X ? pc : line;
+    const int jj = DIR == DIR_X ? line : pc;
+    const int iir = DIR == DIR_X ? pr : line;
+    const int jjr = DIR == DIR_X ? line : pr;
+    double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0, bval = 0.0;
+    R aw = R(0);
+    if (EQ == EQ_P) {
+      const PCoefT<R> c = p_coef<TR, R>(A, ii, jj);
+      const R *pcf = fld<R>(A.pc);
+      double pcP = A.in_zero ? 0.0 : (double)AT(pcf, ii, jj);
+      double head;
+      if (DIR == DIR_X) {
+        a0 = c.ai; a2 = c.ad;
+        double pS = A.in_zero ? 0.0 : (double)AT(pcf, ii, jj - 1);
+        double pN = A.in_zero ? 0.0 : (double)AT(pcf, ii, jj + 1);
+        head = -(double)c.bs * pS - (double)c.bn * pN;
+        bval = c.b;
       } else {
-        const Coef c = EQ == EQ_U ? u_coef<TR>(A, ii, jj) : EQ == EQ_V ? v_coef<TR>(A, ii, jj)
-                                                                         : t_coef<TR>(A, ii, jj);
-        const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
-        double head;
-        if (DIR == DIR_X) {
-          a0 = c.ai; a2 = c.ad;
-          head = -c.bs * AT(phi, ii, jj - 1) - c.bn * AT(phi, ii, jj + 1);
-        } else {
-          a0 = c.bs; a2 = c.bn;
-          head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
-        }
-        a1 = c.ac;
-        rr = EQ == EQ_U ? u_rhs<TR>(A, ii, jj, head, c.ac)
-           : EQ == EQ_V ? v_rhs<TR>(A, ii, jj, head, c.ac) : t_rhs<TR>(A, ii, jj, head, c.ac);
-        if ((EQ == EQ_U || EQ == EQ_V) && DIR == DIR_X) {
-          AT(A.acoef, ii, jj) = c.ac * A.ph.rel_vel;
-          AO(A.acoef2, ii, jj) = c.ac * A.ph.rel_vel;
-        }
+        a0 = c.bs; a2 = c.bn;
+        double pW = A.in_zero ? 0.0 : (double)AT(pcf, ii - 1, jj);
+        double pE = A.in_zero ? 0.0 : (double)AT(pcf, ii + 1, jj);
+        head = -(double)c.ai * pW - (double)c.ad * pE;
+      }
+      a1 = c.ac;
+      rr = head + c.b + (1.0 - A.ph.rel_pres) * (double)c.ac * pcP;
+      if constexpr (sizeof(R) == 4) {
+        // every neighbour behind an infinite a_u, a_v (inside a body): p' = 0
+        if (c.ac == R(0)) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = 0.0; }
+      }
+    } else if constexpr (SOLVER == SOLVE_PCR) {
+      R area;
+      double pin;
+      const CoefT<R> c = fast_coef<EQ, TR, R>(A, ii, jj, area, pin);
+      const R *phi = fld<R>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t);
+      R head;
+      if (DIR == DIR_X) {
+        a0 = c.ai; a2 = c.ad;
+        head = -c.bs * AT(phi, ii, jj - 1) - c.bn * AT(phi, ii, jj + 1);
+      } else {
+        a0 = c.bs; a2 = c.bn;
+        head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
+      }
+      a1 = c.ac;
+      rr = fast_rhs<EQ, TR, R>(A, ii, jj, head, c.ac, area);
+      aw = c.ac * R(A.ph.rel_vel);
+      if constexpr (sizeof(R) == 4) {
+        if (pin == pin) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = pin; aw = big; }
       }
     } else {
-      const bool high = pos != 1;
-      if (EQ == EQ_P) {
-        // p' ghost rows: Dirichlet 0 (main.f90:789-800, 845-856)
-        a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = 0.0;
+      const Coef c = EQ == EQ_U ? u_coef<TR>(A, ii, jj) : EQ == EQ_V ? v_coef<TR>(A, ii, jj)
+                                                                       : t_coef<TR>(A, ii, jj);
+      const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
+      double head;
+      if (DIR == DIR_X) {
+        a0 = c.ai; a2 = c.ad;
+        head = -c.bs * AT(phi, ii, jj - 1) - c.bn * AT(phi, ii, jj + 1);
       } else {
-        // the node number along the side is the line index
-        bool typed = bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
-        if (typed && (EQ == EQ_U || EQ == EQ_V)) {
-          AT(acoef, ii, jj) = big;
-          AO(acoef2, ii, jj) = big;
-        }
+        a0 = c.bs; a2 = c.bn;
+        head = -c.ai * AT(phi, ii - 1, jj) - c.ad * AT(phi, ii + 1, jj);
       }
+      a1 = c.ac;
+      rr = EQ == EQ_U ? u_rhs<TR>(A, ii, jj, head, c.ac)
+         : EQ == EQ_V ? v_rhs<TR>(A, ii, jj, head, c.ac) : t_rhs<TR>(A, ii, jj, head, c.ac);
+      aw = R(c.ac * A.ph.rel_vel);
     }
-    sa[k] = R(a0);
-    sb[k] = R(a1);
-    sc_[k] = R(a2);
-    sr[k] = R(rr);
+    if (pos == 1) { a0 = lb0; a1 = lb1; a2 = lb2; rr = lbr; }
+    if (pos == n) { a0 = hb0; a1 = hb1; a2 = hb2; rr = hbr; }
+    keep_aw[e] = aw;
+    keep_b[e] = bval;
+    if (NORM) keep_old[e] = A.snap ? (double)AT(fld<R>(A.snap), iir, jjr) : 0.0;
+    if (other) {
+      const int io = DIR == DIR_X ? pr : other;
+      const int jo = DIR == DIR_X ? other : pr;
+      carry[e] = (EQ == EQ_P && A.in_zero) ? R(0) : AT(src, io, jo);
+    }
+    if (k < n) {
+      sa[k] = R(a0);
+      sb[k] = R(a1);
+      sc_[k] = R(a2);
+      sr[k] = R(rr);
+    }
+  }
+  // global stores of the assembly: a_u / a_v (x-sweeps of u, v; typed
+  // boundary faces in both sweeps), b_o (p' x-sweep, row layout)
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    const int pos = k + 1;
+    if (k >= n) continue;
+    const int ii = DIR == DIR_X ? pos : line;
+    const int jj = DIR == DIR_X ? line : pos;
+    if (pos >= 2 && pos <= hi) {
+      if (MOM && DIR == DIR_X) {
+        AT(acoef, ii, jj) = keep_aw[e];
+        AO(acoef2, ii, jj) = keep_aw[e];
+      }
+      if (EQ == EQ_P && DIR == DIR_X) AT(A.bo, ii, jj) = keep_b[e];
+    } else if (MOM && (pos == 1 ? lo_typed : hi_typed)) {
+      AT(acoef, ii, jj) = big;
+      AO(acoef2, ii, jj) = big;
+    }
   }
   __syncthreads();
   solve_line<SOLVER, EPT>(sa, sb, sc_, sr, n);
 
-  // write the line, accumulate the x-sweep norms
-  R *out = fldw<R>(A.out), *out2 = fldw<R>(A.out2);
+  // write the line and the carried lines, accumulate the x-sweep norms
   double acc0 = 0.0, acc1 = 0.0;
-  for (int k = threadIdx.x; k < n; k += kBlock) {
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    if (k >= n) continue;
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
     const R x = sr[k];
     AT(out, ii, jj) = x;
     AO(out2, ii, jj) = x;
-    if (DIR == DIR_X && EQ != EQ_U && pos >= 2 && pos <= mi) {
-      double old = A.snap ? (double)AT(fld<R>(A.snap), ii, jj) : 0.0;
-      double d = (double)x - old;
+    if (NORM && pos >= 2 && pos <= mi) {
+      double d = (double)x - keep_old[e];
       acc0 = acc0 + d * d;
-      if (EQ == EQ_P) {
-        double b = AT(A.bo, ii, jj);
-        acc1 = acc1 + b * b;
-      }
+      if (EQ == EQ_P) acc1 = acc1 + keep_b[e] * keep_b[e];
+    }
+    if (other) {
+      const int io = DIR == DIR_X ? pos : other;
+      const int jo = DIR == DIR_X ? other : pos;
+      AT(out, io, jo) = carry[e];
+      AO(out2, io, jo) = carry[e];
     }
   }
-  if (DIR == DIR_X && EQ != EQ_U) {
+  if (NORM) {
     double s0 = block_sum(acc0, red);
     if (EQ == EQ_P) {
       double s1 = block_sum(acc1, red);
@@ -861,26 +909,12 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line) {
     }
     if (threadIdx.x == 0) A.part0[line] = s0;
   }
-
-  // carry the lines this sweep does not solve into the output buffer
-  const R *src = fld<R>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc);
-  for (int side = 0; side < 2; ++side) {
-    if (line != (side == 0 ? A.line0 : lastl)) continue;
-    const int other = side == 0 ? A.line0 - 1 : lastl + 1;
-    for (int k = threadIdx.x; k < n; k += kBlock) {
-      const int pos = k + 1;
-      const int ii = DIR == DIR_X ? pos : other;
-      const int jj = DIR == DIR_X ? other : pos;
-      const R x = (EQ == EQ_P && A.in_zero) ? R(0) : AT(src, ii, jj);
-      AT(out, ii, jj) = x;
-      AO(out2, ii, jj) = x;
-    }
-  }
 }
 
 template <int EQ, int DIR, int SOLVER, int EPT, typename R>
 __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
-  sweep_line<EQ, DIR, SOLVER, EPT, R>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines));
+  __shared__ double lds[line_lds<R, EPT>()];
+  sweep_line<EQ, DIR, SOLVER, EPT, R>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines), lds);
 }
 
 // A batch of cavities: workgroup b runs line wg[b].y of cavity wg[b].x, whose
@@ -889,9 +923,10 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
 template <int EQ, int DIR, int SOLVER, int EPT, typename R>
 __global__ void __launch_bounds__(kBlock) k_sweep_batch(const SweepArgs *__restrict__ D,
                                                         const int2 *__restrict__ wg) {
+  __shared__ double lds[line_lds<R, EPT>()];
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
-  sweep_line<EQ, DIR, SOLVER, EPT, R>(D[w.x], w.y);
+  sweep_line<EQ, DIR, SOLVER, EPT, R>(D[w.x], w.y, lds);
 }
 
 // ------------------------------------------------------------- correction
@@ -915,27 +950,42 @@ __device__ __forceinline__ void correct_tile(const CorrArgs &A, const int bx, co
   __shared__ R tu[kTile][kTile + 1], tv[kTile][kTile + 1];
   __shared__ double tp[kTile][kTile + 1];
   constexpr bool TR = false;
+  constexpr int ROWS = kTile / (kBlock / kTile);  // rows per thread
   const int P = A.P;
+  const int mi = A.mi, nj = A.nj;
   const R *pc = fld<R>(A.pc), *au = fld<R>(A.au), *av = fld<R>(A.av);
   R *u = fldw<R>(A.u), *v = fldw<R>(A.v);
   const int tx = threadIdx.x & (kTile - 1), ty = threadIdx.x >> 5;  // 32 x 8
   const int i0 = 1 + bx * kTile, j0 = 1 + by * kTile;
-  for (int r = ty; r < kTile; r += kBlock / kTile) {
+  // all loads first (positions clamped into the updated ranges), then stores
+  double pn[ROWS];
+  R un[ROWS], vn[ROWS];
+#pragma unroll
+  for (int m = 0; m < ROWS; ++m) {
+    const int r = ty + m * (kBlock / kTile);
+    const int ic = min(max(i0 + tx, 2), mi), jc = min(max(j0 + r, 2), nj);
+    const int iu = min(ic, mi - 1), jv = min(jc, nj - 1);
+    const double pcP = AT(pc, ic, jc);
+    pn[m] = AT(A.p, ic, jc) + 0.6 * pcP;
+    const double pcU = AT(pc, iu, jc);
+    un[m] = R(AT(u, iu, jc) + A.g.dyu[jc] * (pcU - (double)AT(pc, iu + 1, jc)) / (double)AT(au, iu, jc));
+    const double pcV = AT(pc, ic, jv);
+    vn[m] = R(AT(v, ic, jv) + A.g.dxv[ic] * (pcV - (double)AT(pc, ic, jv + 1)) / (double)AT(av, ic, jv));
+  }
+#pragma unroll
+  for (int m = 0; m < ROWS; ++m) {
+    const int r = ty + m * (kBlock / kTile);
     const int ii = i0 + tx, jj = j0 + r;
-    if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) continue;
-    const double pcP = AT(pc, ii, jj);
-    const double pn = AT(A.p, ii, jj) + 0.6 * pcP;
-    AT(A.p, ii, jj) = pn;
-    tp[r][tx] = pn;
-    if (ii <= A.mi - 1) {
-      const R un = R(AT(u, ii, jj) + A.g.dyu[jj] * (pcP - (double)AT(pc, ii + 1, jj)) / (double)AT(au, ii, jj));
-      AT(u, ii, jj) = un;
-      tu[r][tx] = un;
+    if (ii < 2 || ii > mi || jj < 2 || jj > nj) continue;
+    AT(A.p, ii, jj) = pn[m];
+    tp[r][tx] = pn[m];
+    if (ii <= mi - 1) {
+      AT(u, ii, jj) = un[m];
+      tu[r][tx] = un[m];
     }
-    if (jj <= A.nj - 1) {
-      const R vn = R(AT(v, ii, jj) + A.g.dxv[ii] * (pcP - (double)AT(pc, ii, jj + 1)) / (double)AT(av, ii, jj));
-      AT(v, ii, jj) = vn;
-      tv[r][tx] = vn;
+    if (jj <= nj - 1) {
+      AT(v, ii, jj) = vn[m];
+      tv[r][tx] = vn[m];
     }
   }
   __syncthreads();
@@ -943,11 +993,11 @@ __device__ __forceinline__ void correct_tile(const CorrArgs &A, const int bx, co
   R *uT = fldw<R>(A.uT), *vT = fldw<R>(A.vT);
   for (int r = ty; r < kTile; r += kBlock / kTile) {
     const int ii = i0 + r, jj = j0 + tx;  // lanes run along jj: coalesced in the transposed copy
-    if (ii < 2 || ii > A.mi || jj < 2 || jj > A.nj) continue;
+    if (ii < 2 || ii > mi || jj < 2 || jj > nj) continue;
     const size_t o = ((size_t)jj - 1) + ((size_t)ii - 1) * (size_t)Q;
     A.pT[o] = tp[tx][r];
-    if (ii <= A.mi - 1) uT[o] = tu[tx][r];
-    if (jj <= A.nj - 1) vT[o] = tv[tx][r];
+    if (ii <= mi - 1) uT[o] = tu[tx][r];
+    if (jj <= nj - 1) vT[o] = tv[tx][r];
   }
 }
 
@@ -982,46 +1032,57 @@ struct ResArgs {
 };
 
 template <typename R>
-__device__ __forceinline__ void residual_row(const ResArgs &A, const int jj) {
-  __shared__ double red[kBlock];
+__device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, double *red) {
   constexpr bool TR = false;
   const int P = A.P;
   // the stored fields in their real type, read as FP64
   const R *u = fld<R>(A.u), *v = fld<R>(A.v), *t = fld<R>(A.t), *uant = fld<R>(A.uant);
   const Geo &g = A.g;
   const double gm = A.ph.gm;
+  // two rows per thread per pass: all loads before the stores
   double acc = 0.0;
-  for (int ii = 2 + (int)threadIdx.x; ii <= A.mi - 1; ii += kBlock) {
-    double ud = g.fexu[ii] * (double)AT(u, ii + 1, jj) + (1.0 - g.fexu[ii]) * (double)AT(u, ii, jj);
-    double ui = g.fexu[ii - 1] * (double)AT(u, ii, jj) + (1.0 - g.fexu[ii - 1]) * (double)AT(u, ii - 1, jj);
-    double vn = g.fexp[ii] * (double)AT(v, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj);
-    double vs = g.fexp[ii] * (double)AT(v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj - 1);
-    double gd = harm(gm, gm, g.feyp[jj]);
-    double gi = harm(gm, gm, g.feyp[jj]);
-    double gn = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
-    gd = harm(gm, gm, g.feyp[jj - 1]);
-    gi = (gm * gm) / (gm * (1.0 - g.feyp[jj - 1]) + gm * g.feyp[jj - 1]);
-    double gs = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
-    double di = g.dxp[ii], dd = g.dxp[ii + 1], ds = g.dyv[jj - 1], dn = g.dyv[jj];
-    double dx = g.dxu[ii], dy = g.dyu[jj];
-    double ai = -nbc(gm, dy, di, ui, false);
-    double ad = -nbc(gm, dy, dd, ud, true);
-    double bs = -nbc(gs, dx, ds, vs, false);
-    double bn = -nbc(gn, dx, dn, vn, true);
-    double ac = ai + ad + bs + bn;
-    if (A.sp) ac = ac - dx * dy * AT(A.sp, ii, jj);
-    ac = ac + dx * dy / A.ph.dt;
-    double r = ai * (double)AT(u, ii - 1, jj) - ac * (double)AT(u, ii, jj) + ad * (double)AT(u, ii + 1, jj) +
-               bs * (double)AT(u, ii, jj - 1) + bn * (double)AT(u, ii, jj + 1);
-    if (A.ph.ri_x != 0.0) {
-      double tint = g.fexp[ii] * (double)AT(t, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(t, ii, jj);
-      r = r - dx * dy * A.ph.ri_x * tint;
+  for (int base = 2; base <= A.mi - 1; base += 2 * kBlock) {
+    double rv[2];
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int ii = min(base + (int)threadIdx.x + m * kBlock, A.mi - 1);
+      double ud = g.fexu[ii] * (double)AT(u, ii + 1, jj) + (1.0 - g.fexu[ii]) * (double)AT(u, ii, jj);
+      double ui = g.fexu[ii - 1] * (double)AT(u, ii, jj) + (1.0 - g.fexu[ii - 1]) * (double)AT(u, ii - 1, jj);
+      double vn = g.fexp[ii] * (double)AT(v, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj);
+      double vs = g.fexp[ii] * (double)AT(v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj - 1);
+      double gd = harm(gm, gm, g.feyp[jj]);
+      double gi = harm(gm, gm, g.feyp[jj]);
+      double gn = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
+      gd = harm(gm, gm, g.feyp[jj - 1]);
+      gi = (gm * gm) / (gm * (1.0 - g.feyp[jj - 1]) + gm * g.feyp[jj - 1]);
+      double gs = gi * gd / (gd * (1.0 - g.fexp[ii]) + gi * g.fexp[ii]);
+      double di = g.dxp[ii], dd = g.dxp[ii + 1], ds = g.dyv[jj - 1], dn = g.dyv[jj];
+      double dx = g.dxu[ii], dy = g.dyu[jj];
+      double ai = -nbc(gm, dy, di, ui, false);
+      double ad = -nbc(gm, dy, dd, ud, true);
+      double bs = -nbc(gs, dx, ds, vs, false);
+      double bn = -nbc(gn, dx, dn, vn, true);
+      double ac = ai + ad + bs + bn;
+      if (A.sp) ac = ac - dx * dy * AT(A.sp, ii, jj);
+      ac = ac + dx * dy / A.ph.dt;
+      double r = ai * (double)AT(u, ii - 1, jj) - ac * (double)AT(u, ii, jj) + ad * (double)AT(u, ii + 1, jj) +
+                 bs * (double)AT(u, ii, jj - 1) + bn * (double)AT(u, ii, jj + 1);
+      if (A.ph.ri_x != 0.0) {
+        double tint = g.fexp[ii] * (double)AT(t, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(t, ii, jj);
+        r = r - dx * dy * A.ph.ri_x * tint;
+      }
+      if (A.sc) r = r + dx * dy * AT(A.sc, ii, jj);
+      r = r + dx * dy * (double)AT(uant, ii, jj) / A.ph.dt;
+      r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * dy;
+      rv[m] = r;
     }
-    if (A.sc) r = r + dx * dy * AT(A.sc, ii, jj);
-    r = r + dx * dy * (double)AT(uant, ii, jj) / A.ph.dt;
-    r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * dy;
-    AT(A.resu, ii, jj) = r;
-    acc = acc + r * r;
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+      const int ii = base + (int)threadIdx.x + m * kBlock;
+      if (ii > A.mi - 1) continue;
+      AT(A.resu, ii, jj) = rv[m];
+      acc = acc + rv[m] * rv[m];
+    }
   }
   double s = block_sum(acc, red);
   if (threadIdx.x == 0) A.part[jj] = s;
@@ -1029,15 +1090,17 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj) {
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
-  residual_row<R>(A, 2 + xcd_line((int)blockIdx.x, A.nj - 1));
+  __shared__ double red[kBlock];
+  residual_row<R>(A, 2 + xcd_line((int)blockIdx.x, A.nj - 1), red);
 }
 
 template <typename R>
 __global__ void __launch_bounds__(kBlock) k_residual_u_batch(const ResArgs *__restrict__ D,
                                                              const int2 *__restrict__ wg) {
+  __shared__ double red[kBlock];
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
-  residual_row<R>(D[w.x], w.y);
+  residual_row<R>(D[w.x], w.y, red);
 }
 
 // ---------------------------------------------------------------- finalize
@@ -1056,8 +1119,9 @@ struct FinArgs {
   int which;
 };
 
-__device__ __forceinline__ void finalize_body(const FinArgs &A) {
-  __shared__ double red[5][kBlock / 64];
+__device__ __forceinline__ void finalize_body(const FinArgs &A, double *lds) {
+  double(*red)[kBlock / 64] = reinterpret_cast<double(*)[kBlock / 64]>(lds);
+  const int slot = A.slot ? *A.slot : 0;  // loaded with the partials
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
   for (int j = 2 + (int)threadIdx.x; j <= A.nj; j += kBlock) {
     if ((A.which & 1) && j <= A.nj - 1) a0 = a0 + A.pv[j];
@@ -1077,31 +1141,69 @@ __device__ __forceinline__ void finalize_body(const FinArgs &A) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double *o = A.out + (A.slot ? 8 * (size_t)(*A.slot) : 0);
+    double *o = A.out + 8 * (size_t)slot;
     double r[5];
     for (int k = 0; k < 5; ++k) r[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
     for (int k = 0; k < 5; ++k)
       if (A.which & (1 << k)) o[k] = sqrt(r[k]);
-    if (A.slot) *A.slot = *A.slot + 1;
+    if (A.slot) *A.slot = slot + 1;
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) { finalize_body(A); }
+__global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) {
+  __shared__ double lds[5 * (kBlock / 64)];
+  finalize_body(A, lds);
+}
 
 // batch: one workgroup per cavity; the norms of step *slot go to
 // D[c].out + 8 * min(*slot, cap) (record `cap` is a discard row)
 __global__ void __launch_bounds__(kBlock) k_finalize_batch(const FinArgs *__restrict__ D, const int *slot,
                                                            int cap) {
+  __shared__ double lds[5 * (kBlock / 64)];
   FinArgs A = D[blockIdx.x];
   const int k = *slot;
   A.out = A.out + 8 * (size_t)(k < cap ? k : cap);
   A.slot = nullptr;
-  finalize_body(A);
+  finalize_body(A, lds);
 }
 
 // advance the batch step counter once every cavity has read it
 __global__ void k_add_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + v;
+}
+
+// ---------------------------------------------------- fused time-step launch
+// Independent phases of consecutive steps in one launch (one stream, no
+// cross-stream waits): workgroups [off[p], off[p] + cnt[p]) run part p --
+// sweep 0, sweep 1 (EQ1 >= 0), the u residual, the norm fold.  Each part
+// starts at a multiple of 8 so that its lines keep their XCD mapping; padding
+// workgroups return.  The parts share one LDS allocation.
+struct MultiArgs {
+  SweepArgs s[2];
+  ResArgs r;
+  FinArgs f;
+  int off[4], cnt[4];
+};
+
+template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, int EPT, typename R>
+__global__ void __launch_bounds__(kBlock) k_multi(MultiArgs M) {
+  __shared__ double lds[line_lds<R, EPT>()];
+  const int b = (int)blockIdx.x;
+  if (b < M.off[1]) {
+    const int w = b - M.off[0];
+    if (w < M.cnt[0]) sweep_line<EQ0, DIR0, SOLVER, EPT, R>(M.s[0], M.s[0].line0 + xcd_line(w, M.s[0].nlines), lds);
+  } else if (b < M.off[2]) {
+    if constexpr (EQ1 >= 0) {
+      const int w = b - M.off[1];
+      if (w < M.cnt[1])
+        sweep_line<EQ1, DIR1, SOLVER, EPT, R>(M.s[1], M.s[1].line0 + xcd_line(w, M.s[1].nlines), lds);
+    }
+  } else if (b < M.off[3]) {
+    const int w = b - M.off[2];
+    if (w < M.cnt[2]) residual_row<R>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
+  } else {
+    if (b == M.off[3] && M.cnt[3]) finalize_body(M.f, lds);
+  }
 }
 
 __global__ void k_set_int(int *p, int v) {
