@@ -217,6 +217,13 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
 int ixh_adjoint_record(ixh_state *s, int32_t nsteps, int32_t every);
 int ixh_adjoint_energy(ixh_state *s, const double *tbar, int32_t ld, double *grad_a, double *grad_b,
                        double *grad_c, double *grad_d, double *grad_t0, int32_t ld0);
+/* The same outputs through the adjoint of the whole step (u, v sweeps with
+ * their coefficient dependence on the velocities, p' sweeps, correction,
+ * energy): the exact gradient of the recorded K-step map under buoyancy.
+ * Needs ri_x = 0 (upstream's u equation never reads T).  dJ/dT(initial) is
+ * taken with u, v, p of the start held fixed and T_ant equal to T. */
+int ixh_adjoint_coupled(ixh_state *s, const double *tbar, int32_t ld, double *grad_a, double *grad_b,
+                        double *grad_c, double *grad_d, double *grad_t0, int32_t ld0);
 
 #ifdef __cplusplus
 }

@@ -30,6 +30,7 @@ EXPORTS = [
     "ixh_tridiagonal", "ixh_set_profiling", "ixh_kernel_times", "ixh_kernel_name",
     "ixh_batch_create", "ixh_batch_destroy", "ixh_batch_set_schedule", "ixh_batch_iterations",
     "ixh_batch_autotune", "ixh_adjoint_record", "ixh_adjoint_energy",
+    "ixh_adjoint_coupled",
 ]
 ORDER_ID = {"cavity": 0, "largest": 1, "xcd": 2}
 
@@ -108,6 +109,7 @@ def load(path: str = LIB_PATH):
     L.ixh_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, dp]
     L.ixh_adjoint_record.argtypes = [vp, C.c_int32, C.c_int32]
     L.ixh_adjoint_energy.argtypes = [vp, dp, C.c_int32, dp, dp, dp, dp, dp, C.c_int32]
+    L.ixh_adjoint_coupled.argtypes = [vp, dp, C.c_int32, dp, dp, dp, dp, dp, C.c_int32]
     for name in EXPORTS:
         fn = getattr(L, name)
         if fn.restype is C.c_int or fn.restype is None:

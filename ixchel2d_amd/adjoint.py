@@ -10,9 +10,11 @@ backward pass runs ixh_adjoint_energy with dJ/dT = the Nusselt integral's
 weights and maps the boundary-node adjoints back to the controls.
 ``HotWallNusselt`` exposes this as a torch.autograd.Function.
 
-The adjoint holds each step's velocities fixed: it is the exact gradient of
-the discrete K-step map when the flow does not depend on T (no buoyancy),
-and its frozen-flow part under buoyancy.
+By default the backward pass is the adjoint of the whole step
+(ixh_adjoint_coupled): the exact gradient of the discrete K-step map, buoyancy
+included.  With ``coupled=False`` it runs through the energy steps only with
+the velocities held fixed (ixh_adjoint_energy): exact when the flow does not
+depend on T, the frozen-flow part of the gradient otherwise.
 """
 from __future__ import annotations
 
@@ -77,11 +79,12 @@ class HotWallProblem:
     from a fixed start, with hot-wall temperatures B c."""
 
     def __init__(self, case: Case, ncontrols: int = 64, steps: int = 400, every: int = 20,
-                 solver: str = "pcr", start: Optional[Dict[str, np.ndarray]] = None):
+                 solver: str = "pcr", start: Optional[Dict[str, np.ndarray]] = None, coupled: bool = True):
         if case.params.iter_ecuaci_max >= 0 or case.params.iter_simple_max >= 0:
             raise ValueError("the adjoint needs single-pass loops")
         self.case = case
         self.steps, self.every = steps, every
+        self.coupled = coupled
         self.cav = Cavity(case, solver=solver)
         self.B = control_matrix(case.mesh, ncontrols)
         self.W = nusselt_x_weights(case.mesh)
@@ -106,7 +109,7 @@ class HotWallProblem:
 
     def gradient(self, scale: float = 1.0) -> np.ndarray:
         """dJ/dc of the last forward evaluation, times ``scale``."""
-        g, _ = self.cav.adjoint_energy(scale * self.W)
+        g, _ = self.cav.adjoint_energy(scale * self.W, coupled=self.coupled)
         return self.B.T @ g["a"]
 
 

@@ -1,0 +1,446 @@
+// Device code of the discrete adjoint of one single-pass SIMPLE step (FP64,
+// division-free assembly, row layout throughout).
+//
+// Forward step k (main.f90:391-1270 with one pass per loop), every field of
+// it kept on a tape:
+//   u1 = Y_u(u0; v0)   u2 = X_u(u1; v0) -> a_u   v1 = Y_v(v0; u2, T0)
+//   v2 = X_v(v1; u2, T0) -> a_v   p'1 = Y_p(0; u2, v2, a_u, a_v)
+//   p'2 = X_p(p'1; ...)   p += 0.6 p'2, u3, v3 = corrected u2, v2
+//   T1 = Y_T(T0; u3, v3)   T2 = X_T(T1; u3, v3)
+// Each sweep solves A(w) x = r(phi, w) line by line.  With R = A x - r and
+// mu = A^-T xbar on every solved line (k_adj2_solve), the adjoint of every
+// input w of the sweep is  wbar += -mu dR/dw : the right-hand-side stencil
+// (neighbours, (1-alpha) a_P phi_P, phi_ant, the pressure difference, the
+// buoyancy term, b_o) and the coefficients, whose power-law form depends on
+// the interpolated face velocities.  k_adj2_rows forms, per row, the adjoint
+// of the four neighbour coefficients (through a_P = -sum/alpha) times their
+// derivative with respect to their face velocity; k_adj2_gather collects
+// everything per node, so each node is written by one thread (deterministic,
+// no atomics).
+#pragma once
+#include "simple_device.hpp"
+
+namespace ixh {
+
+// d nbf / d w (nbf: power law + upwind, division-free form)
+__device__ __forceinline__ double nbf_dw(double cond, double len, double pec, double w, bool east) {
+  double conv = w * len;
+  if (east) conv = -conv;
+  const double wp = w * pec;
+  const double t = 1.0 - 0.1 * fabs(wp);
+  const double t2 = t * t;
+  const double pw = t2 * t2 * t;
+  double d = 0.0;
+  if (pw > 0.0) {
+    const double sg = wp > 0.0 ? 1.0 : (wp < 0.0 ? -1.0 : 0.0);
+    d = d + cond * len * 5.0 * (t2 * t2) * (-0.1 * pec * sg);
+  }
+  if (conv > 0.0) d = d + (east ? -len : len);
+  return -d;
+}
+
+// coefficients of row (ii, jj) of equation EQ (U, V or T), as fast_coef
+// computes them, and the derivatives of ai, ad, bs, bn with respect to their
+// face velocities ui, ud, vs, vn
+struct CoefD {
+  double ai, ad, bs, bn, ac, area;
+  double dai, dad, dbs, dbn;
+};
+
+template <int EQ>
+__device__ __forceinline__ CoefD coef_d(const SweepArgs &A, int ii, int jj) {
+  constexpr bool TR = false;
+  const int P = A.P;
+  const Geo &g = A.g;
+  const FastGeo &F = A.fg;
+  constexpr int e = EQ == EQ_U ? 0 : EQ == EQ_V ? 1 : 2;
+  const double *u = A.u, *v = A.v;
+  double ui, ud, vs, vn, lx, ly;
+  if (EQ == EQ_U) {
+    ud = g.fexu[ii] * AT(u, ii + 1, jj) + (1.0 - g.fexu[ii]) * AT(u, ii, jj);
+    ui = g.fexu[ii - 1] * AT(u, ii, jj) + (1.0 - g.fexu[ii - 1]) * AT(u, ii - 1, jj);
+    vn = g.fexp[ii] * AT(v, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(v, ii, jj);
+    vs = g.fexp[ii] * AT(v, ii + 1, jj - 1) + (1.0 - g.fexp[ii]) * AT(v, ii, jj - 1);
+    lx = g.dxu[ii];
+    ly = g.dyu[jj];
+  } else if (EQ == EQ_V) {
+    ud = g.feyp[jj] * AT(u, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(u, ii, jj);
+    ui = g.feyp[jj] * AT(u, ii - 1, jj + 1) + (1.0 - g.feyp[jj]) * AT(u, ii - 1, jj);
+    vn = g.feyv[jj] * AT(v, ii, jj + 1) + (1.0 - g.feyv[jj]) * AT(v, ii, jj);
+    vs = g.feyv[jj - 1] * AT(v, ii, jj) + (1.0 - g.feyv[jj - 1]) * AT(v, ii, jj - 1);
+    lx = g.dxv[ii];
+    ly = g.dyv[jj];
+  } else {
+    ud = AT(u, ii, jj);
+    ui = AT(u, ii - 1, jj);
+    vn = AT(v, ii, jj);
+    vs = AT(v, ii, jj - 1);
+    lx = g.dxp[ii];
+    ly = g.dyp[jj];
+  }
+  CoefD c;
+  c.ai = nbf<double>(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
+  c.ad = nbf<double>(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
+  c.bs = nbf<double>(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
+  c.bn = nbf<double>(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
+  c.dai = nbf_dw(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
+  c.dad = nbf_dw(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
+  c.dbs = nbf_dw(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
+  c.dbn = nbf_dw(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
+  c.area = lx * ly;
+  double acc = -c.ai - c.ad - c.bs - c.bn;
+  if (A.sp) acc = acc - c.area * AT(A.sp, ii, jj);
+  c.ac = (acc + c.area * F.inv_dt) * (EQ == EQ_T ? F.inv_rel_ener : F.inv_rel_vel);
+  return c;
+}
+
+// rows a sweep of EQ assembles (both directions): ii in [2, ihi], jj in [2, jhi]
+template <int EQ>
+__device__ __forceinline__ bool adj_row(int ii, int jj, int mi, int nj) {
+  const int ihi = EQ == EQ_U ? mi - 1 : mi;
+  const int jhi = EQ == EQ_V ? nj - 1 : nj;
+  return ii >= 2 && ii <= ihi && jj >= 2 && jj <= jhi;
+}
+
+struct Adj2Args {
+  SweepArgs S;          // geometry, physics, BCs and sources of EQ; S.u, S.v = the
+                        // coefficient velocities, S.t = T (buoyancy), S.au, S.av (p')
+  const double *x;      // the sweep's output
+  const double *phi;    // the sweep's input (nullptr: zero, the first p' sweep)
+  const double *ybar;   // adjoint of the output
+  const double *acb;    // adjoint of a_u / a_v written by the x-sweep (nullptr: none)
+  double *mu;           // transposed solve
+  double *fa[4];        // per-row adjoints: face velocities (U, V, T) or coefficients (p')
+  double *phib;         // adjoint of the input: written (=)
+  double *velub, *velvb;  // adjoints of the coefficient velocities (+=); the input's own
+                          // array when it is one of them
+  double *antb;         // adjoint of phi_ant (+=)
+  double *pb;           // adjoint of p (+=)
+  double *tb;           // adjoint of T through the buoyancy term (+=)
+  double *aub, *avb;    // (p') adjoints of a_u, a_v (+=)
+  double *g_lo, *g_hi;  // (T) adjoints of the boundary values (+=, per node)
+};
+
+template <int EQ, int DIR>
+__device__ __forceinline__ int adj_n(int mi, int nj) { return Layout<EQ, DIR>::n(mi, nj); }
+
+// mu = A^-T ybar on every solved line (lines 2 .. last of the sweep)
+template <int EQ, int DIR, int EPT>
+__global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
+  __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
+  constexpr bool TR = false;
+  const SweepArgs &A = G.S;
+  const int P = A.P;
+  const int mi = A.mi, nj = A.nj;
+  const int n = Layout<EQ, DIR>::n(mi, nj);
+  const int hi = Layout<EQ, DIR>::hi(mi, nj);
+  const int line = 2 + (int)blockIdx.x;
+  const Side &lo_side = DIR == DIR_X ? A.bc[0] : A.bc[1];
+  const Side &hi_side = DIR == DIR_X ? A.bc[2] : A.bc[3];
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int pos = k + 1;
+    const int ii = DIR == DIR_X ? pos : line;
+    const int jj = DIR == DIR_X ? line : pos;
+    double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
+    if (pos >= 2 && pos <= hi) {
+      if (EQ == EQ_P) {
+        const PCoefT<double> c = p_coef<TR, double>(A, ii, jj);
+        a0 = DIR == DIR_X ? c.ai : c.bs;
+        a2 = DIR == DIR_X ? c.ad : c.bn;
+        a1 = c.ac;
+      } else {
+        const CoefD c = coef_d<EQ>(A, ii, jj);
+        a0 = DIR == DIR_X ? c.ai : c.bs;
+        a2 = DIR == DIR_X ? c.ad : c.bn;
+        a1 = c.ac;
+      }
+    } else if (EQ != EQ_P) {
+      const bool high = pos != 1;
+      (void)bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
+    }
+    sa[k] = a0;
+    sb[k] = a1;
+    sc_[k] = a2;
+  }
+  __syncthreads();
+  double ta[EPT], tc[EPT];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    if (k < n) {
+      ta[e] = k > 0 ? sc_[k - 1] : 0.0;
+      tc[e] = k < n - 1 ? sa[k + 1] : 0.0;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    if (k < n) {
+      const int pos = k + 1;
+      const int ii = DIR == DIR_X ? pos : line;
+      const int jj = DIR == DIR_X ? line : pos;
+      sa[k] = ta[e];
+      sc_[k] = tc[e];
+      sr[k] = AT(G.ybar, ii, jj);
+    }
+  }
+  __syncthreads();
+  solve_line<SOLVE_PCR, EPT>(sa, sb, sc_, sr, n);
+  for (int k = threadIdx.x; k < n; k += kBlock) {
+    const int pos = k + 1;
+    const int ii = DIR == DIR_X ? pos : line;
+    const int jj = DIR == DIR_X ? line : pos;
+    AT(G.mu, ii, jj) = sr[k];
+  }
+  if (EQ == EQ_T && threadIdx.x == 0) {
+    G.g_lo[line] = G.g_lo[line] + sr[0];
+    G.g_hi[line] = G.g_hi[line] + sr[n - 1];
+  }
+}
+
+// per assembled row: the adjoints of the four neighbour coefficients, times
+// their face-velocity derivative (U, V, T) or as they are (p')
+template <int EQ, int DIR>
+__global__ void __launch_bounds__(kBlock) k_adj2_rows(Adj2Args G) {
+  constexpr bool TR = false;
+  const SweepArgs &A = G.S;
+  const int P = A.P;
+  const int mi = A.mi, nj = A.nj;
+  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (idx >= (mi + 1) * (nj + 1)) return;
+  const int ii = 1 + idx % (mi + 1), jj = 1 + idx / (mi + 1);
+  if (!adj_row<EQ == EQ_P ? EQ_T : EQ>(ii, jj, mi, nj)) return;
+  const double mu = AT(G.mu, ii, jj);
+  const double *x = G.x, *phi = G.phi;
+  const double xP = AT(x, ii, jj);
+  const double fP = phi ? AT(phi, ii, jj) : 0.0;
+  // matrix neighbours (along the sweep) read x, right-hand-side neighbours phi
+  double xW, xE, xS, xN, fW, fE, fS, fN;
+  if (DIR == DIR_X) {
+    xW = AT(x, ii - 1, jj); xE = AT(x, ii + 1, jj);
+    fS = phi ? AT(phi, ii, jj - 1) : 0.0; fN = phi ? AT(phi, ii, jj + 1) : 0.0;
+    xS = xN = fW = fE = 0.0;
+  } else {
+    xS = AT(x, ii, jj - 1); xN = AT(x, ii, jj + 1);
+    fW = phi ? AT(phi, ii - 1, jj) : 0.0; fE = phi ? AT(phi, ii + 1, jj) : 0.0;
+    xW = xE = fS = fN = 0.0;
+  }
+  const double alpha = EQ == EQ_P ? A.ph.rel_pres : EQ == EQ_T ? A.ph.rel_ener : A.ph.rel_vel;
+  double jac = -mu * (xP - (1.0 - alpha) * fP);  // dJ/d a_P
+  if (G.acb) jac = jac + alpha * AT(G.acb, ii, jj);
+  // a_P = (-sum a_nb ...) / alpha
+  const double dsum = EQ == EQ_P ? -jac / alpha
+                                 : -jac * (EQ == EQ_T ? A.fg.inv_rel_ener : A.fg.inv_rel_vel);
+  double jai, jad, jbs, jbn;
+  if (DIR == DIR_X) {
+    jai = -mu * xW; jad = -mu * xE; jbs = -mu * fS; jbn = -mu * fN;
+  } else {
+    jai = -mu * fW; jad = -mu * fE; jbs = -mu * xS; jbn = -mu * xN;
+  }
+  jai = jai + dsum; jad = jad + dsum; jbs = jbs + dsum; jbn = jbn + dsum;
+  if (EQ == EQ_P) {
+    AT(G.fa[0], ii, jj) = jai;
+    AT(G.fa[1], ii, jj) = jad;
+    AT(G.fa[2], ii, jj) = jbs;
+    AT(G.fa[3], ii, jj) = jbn;
+  } else {
+    const CoefD c = coef_d<EQ == EQ_P ? EQ_T : EQ>(A, ii, jj);
+    AT(G.fa[0], ii, jj) = jai * c.dai;
+    AT(G.fa[1], ii, jj) = jad * c.dad;
+    AT(G.fa[2], ii, jj) = jbs * c.dbs;
+    AT(G.fa[3], ii, jj) = jbn * c.dbn;
+  }
+}
+
+// per node: adjoint of the sweep's input and of everything else the rows read
+template <int EQ, int DIR>
+__global__ void __launch_bounds__(kBlock) k_adj2_gather(Adj2Args G) {
+  constexpr bool TR = false;
+  constexpr int RQ = EQ == EQ_P ? EQ_T : EQ;  // row set of p' equals that of T
+  const SweepArgs &A = G.S;
+  const int P = A.P;
+  const int mi = A.mi, nj = A.nj;
+  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (idx >= (mi + 1) * (nj + 1)) return;
+  const int I = 1 + idx % (mi + 1), J = 1 + idx / (mi + 1);
+  const Geo &g = A.g;
+  const double *mu = G.mu;
+  auto row = [&](int i, int j) { return adj_row<RQ>(i, j, mi, nj); };
+  const double alpha = EQ == EQ_P ? A.ph.rel_pres : EQ == EQ_T ? A.ph.rel_ener : A.ph.rel_vel;
+  // extent of the swept variable
+  const int im = EQ == EQ_U ? mi : mi + 1;
+  const int jm = EQ == EQ_V ? nj : nj + 1;
+  const bool in_phi = I <= im && J <= jm;
+
+  // ---- input of the sweep
+  double pb_in = 0.0;
+  if (in_phi && G.phi) {
+    const int line = DIR == DIR_X ? J : I;
+    const int last = Layout<EQ, DIR>::last_line(mi, nj);
+    if (line == 1 || line == last + 1) pb_in = AT(G.ybar, I, J);  // carried line
+    if (row(I, J)) {
+      double ac;
+      if (EQ == EQ_P) ac = p_coef<TR, double>(A, I, J).ac;
+      else ac = coef_d<RQ>(A, I, J).ac;
+      pb_in = pb_in + (1.0 - alpha) * ac * AT(mu, I, J);
+    }
+    if (DIR == DIR_X) {
+      if (row(I, J + 1)) {
+        const double bs = EQ == EQ_P ? p_coef<TR, double>(A, I, J + 1).bs : coef_d<RQ>(A, I, J + 1).bs;
+        pb_in = pb_in - bs * AT(mu, I, J + 1);
+      }
+      if (row(I, J - 1)) {
+        const double bn = EQ == EQ_P ? p_coef<TR, double>(A, I, J - 1).bn : coef_d<RQ>(A, I, J - 1).bn;
+        pb_in = pb_in - bn * AT(mu, I, J - 1);
+      }
+    } else {
+      if (row(I + 1, J)) {
+        const double ai = EQ == EQ_P ? p_coef<TR, double>(A, I + 1, J).ai : coef_d<RQ>(A, I + 1, J).ai;
+        pb_in = pb_in - ai * AT(mu, I + 1, J);
+      }
+      if (row(I - 1, J)) {
+        const double ad = EQ == EQ_P ? p_coef<TR, double>(A, I - 1, J).ad : coef_d<RQ>(A, I - 1, J).ad;
+        pb_in = pb_in - ad * AT(mu, I - 1, J);
+      }
+    }
+  }
+
+  // ---- face velocities of the coefficients (U, V, T)
+  double gu = 0.0, gv = 0.0;  // adjoints of u(I, J), v(I, J)
+  double *const *fa = G.fa;
+  if (EQ == EQ_U) {
+    if (row(I, J)) gu = gu + (1.0 - g.fexu[I]) * AT(fa[1], I, J) + g.fexu[I - 1] * AT(fa[0], I, J);
+    if (row(I - 1, J)) gu = gu + g.fexu[I - 1] * AT(fa[1], I - 1, J);
+    if (row(I + 1, J)) gu = gu + (1.0 - g.fexu[I]) * AT(fa[0], I + 1, J);
+    if (row(I, J)) gv = gv + (1.0 - g.fexp[I]) * AT(fa[3], I, J);
+    if (row(I - 1, J)) gv = gv + g.fexp[I - 1] * AT(fa[3], I - 1, J);
+    if (row(I, J + 1)) gv = gv + (1.0 - g.fexp[I]) * AT(fa[2], I, J + 1);
+    if (row(I - 1, J + 1)) gv = gv + g.fexp[I - 1] * AT(fa[2], I - 1, J + 1);
+  } else if (EQ == EQ_V) {
+    if (row(I, J)) gu = gu + (1.0 - g.feyp[J]) * AT(fa[1], I, J);
+    if (row(I, J - 1)) gu = gu + g.feyp[J - 1] * AT(fa[1], I, J - 1);
+    if (row(I + 1, J)) gu = gu + (1.0 - g.feyp[J]) * AT(fa[0], I + 1, J);
+    if (row(I + 1, J - 1)) gu = gu + g.feyp[J - 1] * AT(fa[0], I + 1, J - 1);
+    if (row(I, J)) gv = gv + (1.0 - g.feyv[J]) * AT(fa[3], I, J) + g.feyv[J - 1] * AT(fa[2], I, J);
+    if (row(I, J - 1)) gv = gv + g.feyv[J - 1] * AT(fa[3], I, J - 1);
+    if (row(I, J + 1)) gv = gv + (1.0 - g.feyv[J]) * AT(fa[2], I, J + 1);
+  } else if (EQ == EQ_T) {
+    if (row(I, J)) gu = gu + AT(fa[1], I, J);
+    if (row(I + 1, J)) gu = gu + AT(fa[0], I + 1, J);
+    if (row(I, J)) gv = gv + AT(fa[3], I, J);
+    if (row(I, J + 1)) gv = gv + AT(fa[2], I, J + 1);
+  } else {
+    // p': b_o = (u_w - u_e) dyp + (v_s - v_n) dxp and the coefficients
+    // -dyp^2 / a_u, -dxp^2 / a_v
+    if (row(I + 1, J)) gu = gu + AT(mu, I + 1, J) * g.dyp[J];
+    if (row(I, J)) gu = gu - AT(mu, I, J) * g.dyp[J];
+    if (row(I, J + 1)) gv = gv + AT(mu, I, J + 1) * g.dxp[I];
+    if (row(I, J)) gv = gv - AT(mu, I, J) * g.dxp[I];
+    if (I <= mi && J <= nj + 1) {
+      double s = 0.0;
+      if (row(I, J)) s = s + AT(fa[1], I, J);
+      if (row(I + 1, J)) s = s + AT(fa[0], I + 1, J);
+      if (s != 0.0) {
+        const double au = AT(A.au, I, J);
+        AT(G.aub, I, J) = AT(G.aub, I, J) + s * g.dyp[J] * g.dyp[J] / (au * au);
+      }
+    }
+    if (I <= mi + 1 && J <= nj) {
+      double s = 0.0;
+      if (row(I, J)) s = s + AT(fa[3], I, J);
+      if (row(I, J + 1)) s = s + AT(fa[2], I, J + 1);
+      if (s != 0.0) {
+        const double av = AT(A.av, I, J);
+        AT(G.avb, I, J) = AT(G.avb, I, J) + s * g.dxp[I] * g.dxp[I] / (av * av);
+      }
+    }
+  }
+
+  // ---- phi_ant, pressure and buoyancy terms of the momentum and energy rows
+  double ant = 0.0, pp = 0.0, tt = 0.0;
+  if (EQ != EQ_P && row(I, J)) {
+    const CoefD c = coef_d<RQ>(A, I, J);
+    ant = c.area * A.fg.inv_dt * AT(mu, I, J);
+  }
+  if (EQ == EQ_U) {
+    if (row(I, J)) pp = pp + AT(mu, I, J) * g.dyu[J];
+    if (row(I - 1, J)) pp = pp - AT(mu, I - 1, J) * g.dyu[J];
+  } else if (EQ == EQ_V) {
+    if (row(I, J)) pp = pp + AT(mu, I, J) * g.dxv[I];
+    if (row(I, J - 1)) pp = pp - AT(mu, I, J - 1) * g.dxv[I];
+    if (A.ph.ri_y != 0.0) {
+      if (row(I, J)) tt = tt - AT(mu, I, J) * g.dxv[I] * g.dyv[J] * A.ph.ri_y * (1.0 - g.feyp[J]);
+      if (row(I, J - 1)) tt = tt - AT(mu, I, J - 1) * g.dxv[I] * g.dyv[J - 1] * A.ph.ri_y * g.feyp[J - 1];
+    }
+  }
+
+  // ---- stores (one thread per node)
+  if (in_phi && G.phib) {
+    double v = pb_in;
+    if (EQ == EQ_U) v = v + gu;
+    if (EQ == EQ_V) v = v + gv;
+    AT(G.phib, I, J) = v;
+  }
+  if (EQ != EQ_U && G.velub && I <= mi && J <= nj + 1) AT(G.velub, I, J) = AT(G.velub, I, J) + gu;
+  if (EQ != EQ_V && G.velvb && I <= mi + 1 && J <= nj) AT(G.velvb, I, J) = AT(G.velvb, I, J) + gv;
+  if (EQ != EQ_P && G.antb && in_phi) AT(G.antb, I, J) = AT(G.antb, I, J) + ant;
+  if ((EQ == EQ_U || EQ == EQ_V) && G.pb) AT(G.pb, I, J) = AT(G.pb, I, J) + pp;
+  if (EQ == EQ_V && G.tb && tt != 0.0) AT(G.tb, I, J) = AT(G.tb, I, J) + tt;
+}
+
+// adjoint of the correction (main.f90:931-962):  p += 0.6 p',
+// u3 = u2 + dyu (p'_P - p'_E) / a_u,  v3 = v2 + dxv (p'_P - p'_N) / a_v
+struct AdjCorrArgs {
+  int mi, nj, P;
+  Geo g;
+  const double *pc, *au, *av;       // p'2 and the a_u, a_v the correction divided by
+  const double *ub3, *vb3, *pbn;    // adjoints of u3, v3 and of the new p
+  double *ub2, *vb2, *pcb;          // written (=)
+  double *aub, *avb;                // (+=)
+};
+
+__global__ void __launch_bounds__(kBlock) k_adj2_correct(AdjCorrArgs C) {
+  constexpr bool TR = false;
+  const int P = C.P;
+  const int mi = C.mi, nj = C.nj;
+  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (idx >= (mi + 1) * (nj + 1)) return;
+  const int I = 1 + idx % (mi + 1), J = 1 + idx / (mi + 1);
+  const Geo &g = C.g;
+  auto ucor = [&](int i, int j) { return i >= 2 && i <= mi - 1 && j >= 2 && j <= nj; };
+  auto vcor = [&](int i, int j) { return i >= 2 && i <= mi && j >= 2 && j <= nj - 1; };
+  if (I <= mi) {
+    const double ub = AT(C.ub3, I, J);
+    AT(C.ub2, I, J) = ub;
+    if (ucor(I, J)) {
+      const double au = AT(C.au, I, J);
+      const double d = AT(C.pc, I, J) - AT(C.pc, I + 1, J);
+      AT(C.aub, I, J) = AT(C.aub, I, J) - ub * g.dyu[J] * d / (au * au);
+    }
+  }
+  if (J <= nj) {
+    const double vb = AT(C.vb3, I, J);
+    AT(C.vb2, I, J) = vb;
+    if (vcor(I, J)) {
+      const double av = AT(C.av, I, J);
+      const double d = AT(C.pc, I, J) - AT(C.pc, I, J + 1);
+      AT(C.avb, I, J) = AT(C.avb, I, J) - vb * g.dxv[I] * d / (av * av);
+    }
+  }
+  double s = 0.0;
+  if (I >= 2 && I <= mi && J >= 2 && J <= nj) s = 0.6 * AT(C.pbn, I, J);
+  if (ucor(I, J)) s = s + AT(C.ub3, I, J) * g.dyu[J] / AT(C.au, I, J);
+  if (ucor(I - 1, J)) s = s - AT(C.ub3, I - 1, J) * g.dyu[J] / AT(C.au, I - 1, J);
+  if (vcor(I, J)) s = s + AT(C.vb3, I, J) * g.dxv[I] / AT(C.av, I, J);
+  if (vcor(I, J - 1)) s = s - AT(C.vb3, I, J - 1) * g.dxv[I] / AT(C.av, I, J - 1);
+  AT(C.pcb, I, J) = s;
+}
+
+// y += x over the whole buffer (n doubles)
+__global__ void k_axpy1(double *y, const double *x, int n) {
+  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
+  if (i < n) y[i] = y[i] + x[i];
+}
+
+}  // namespace ixh

@@ -12,6 +12,7 @@
 
 #include "../../include/ixchel2d.h"
 #include "simple_device.hpp"
+#include "adjoint_device.hpp"
 
 using namespace ixh;
 
@@ -104,6 +105,8 @@ struct ixh_state {
   std::vector<double *> adj_tape;
   double *adj_w[4]{};  // lambda, lambda', mu, adjoint of T_ant
   double *adj_g[4]{};  // adjoints of the boundary values of T, sides a..d
+  std::vector<double *> adj2_tape;  // coupled adjoint: 16 fields per step of a segment
+  std::vector<double *> adj2_w;     // coupled adjoint: work arrays
   // profiling
   bool profiling = false;
   double kms[K_NUM]{};
@@ -868,6 +871,8 @@ int ixh_destroy(ixh_state *s) {
   if (s->d_f32) (void)hipFree(s->d_f32);
   for (double *p : s->adj_ck) (void)hipFree(p);
   for (double *p : s->adj_tape) (void)hipFree(p);
+  for (double *p : s->adj2_tape) (void)hipFree(p);
+  for (double *p : s->adj2_w) (void)hipFree(p);
   for (int k = 0; k < 4; ++k) {
     if (s->adj_w[k]) (void)hipFree(s->adj_w[k]);
     if (s->adj_g[k]) (void)hipFree(s->adj_g[k]);
@@ -1957,6 +1962,231 @@ int ixh_adjoint_energy(ixh_state *s, const double *tbar, int32_t ld, double *gra
   if (grad_t0) {
     std::vector<double> t((size_t)s->P * (s->nj + 1));
     HIPCHK(hipMemcpyAsync(t.data(), s->adj_w[0], t.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    for (int j = 0; j <= s->nj; ++j)
+      for (int i = 0; i <= s->mi; ++i) grad_t0[(size_t)i + (size_t)j * ld0] = t[(size_t)i + (size_t)j * s->P];
+  }
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return 0;
+}
+
+}  // extern "C"
+
+// ====================================================== coupled adjoint
+// Adjoint of the whole single-pass step (momentum, pressure correction,
+// correction, energy) over a recorded run: each segment is recomputed from
+// its checkpoint phase by phase with every intermediate field on a tape, then
+// the steps are differentiated backward (adjoint_device.hpp).
+namespace {
+
+enum Tape { TP_U0, TP_V0, TP_T0, TP_P0, TP_U1, TP_U2, TP_AU, TP_V1, TP_V2, TP_AV, TP_PC1, TP_PC2, TP_U3, TP_V3,
+            TP_T1, TP_T2, TP_N };
+enum Work { W_UB3, W_VB3, W_TB2, W_PB, W_TB1, W_TB0, W_UB2, W_VB2, W_PCB2, W_PCB1, W_AUB, W_AVB, W_VB1, W_VB0,
+            W_UB1, W_UB0, W_UANTB, W_VANTB, W_TANTB, W_MU, W_FA0, W_FA1, W_FA2, W_FA3, W_N };
+
+int dcopy(ixh_state *s, double *dst, const double *src) {
+  HIPCHK(hipMemcpyAsync(dst, src, s->fsize * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
+  return 0;
+}
+
+// one time step in the order of fast_step, every produced field copied to
+// the tape (the residual and the norms do not change fields and are skipped)
+int tape_step(ixh_state *s, double *const *tp) {
+  CHK(dcopy(s, tp[TP_U0], s->ub[s->u_cur]));
+  CHK(dcopy(s, tp[TP_V0], s->vb[s->v_cur]));
+  CHK(dcopy(s, tp[TP_T0], s->tb[s->t_cur]));
+  CHK(dcopy(s, tp[TP_P0], s->p));
+  CHK(sweep_u(s, IXH_DIR_Y));
+  CHK(dcopy(s, tp[TP_U1], s->ub[s->u_cur]));
+  CHK(sweep_u(s, IXH_DIR_X));
+  CHK(dcopy(s, tp[TP_U2], s->ub[s->u_cur]));
+  CHK(dcopy(s, tp[TP_AU], s->au));
+  CHK(sweep_v(s, IXH_DIR_Y));
+  CHK(dcopy(s, tp[TP_V1], s->vb[s->v_cur]));
+  CHK(sweep_v(s, IXH_DIR_X));
+  CHK(dcopy(s, tp[TP_V2], s->vb[s->v_cur]));
+  CHK(dcopy(s, tp[TP_AV], s->av));
+  s->pc_zero = true;
+  CHK(sweep_p(s, IXH_DIR_Y));
+  CHK(dcopy(s, tp[TP_PC1], s->pcb[s->pc_cur]));
+  CHK(sweep_p(s, IXH_DIR_X));
+  CHK(dcopy(s, tp[TP_PC2], s->pcb[s->pc_cur]));
+  CHK(do_correct(s));
+  CHK(dcopy(s, tp[TP_U3], s->ub[s->u_cur]));
+  CHK(dcopy(s, tp[TP_V3], s->vb[s->v_cur]));
+  CHK(sweep_t(s, IXH_DIR_Y));
+  CHK(dcopy(s, tp[TP_T1], s->tb[s->t_cur]));
+  CHK(sweep_t(s, IXH_DIR_X));
+  CHK(dcopy(s, tp[TP_T2], s->tb[s->t_cur]));
+  s->u_ant = s->u_cur;
+  s->v_ant = s->v_cur;
+  s->t_ant = s->t_cur;
+  return 0;
+}
+
+template <int EQ, int DIR>
+int adj2_sweep(ixh_state *s, Adj2Args &G, int nlines, int n) {
+  dim3 block(kBlock), grid(nlines);
+  switch (ept_for(n)) {
+    case 1: hipLaunchKernelGGL((k_adj2_solve<EQ, DIR, 1>), grid, block, 0, s->stream, G); break;
+    case 2: hipLaunchKernelGGL((k_adj2_solve<EQ, DIR, 2>), grid, block, 0, s->stream, G); break;
+    case 3: hipLaunchKernelGGL((k_adj2_solve<EQ, DIR, 3>), grid, block, 0, s->stream, G); break;
+    case 4: hipLaunchKernelGGL((k_adj2_solve<EQ, DIR, 4>), grid, block, 0, s->stream, G); break;
+    case 6: hipLaunchKernelGGL((k_adj2_solve<EQ, DIR, 6>), grid, block, 0, s->stream, G); break;
+    default: hipLaunchKernelGGL((k_adj2_solve<EQ, DIR, 8>), grid, block, 0, s->stream, G); break;
+  }
+  HIPCHK(hipGetLastError());
+  const int npts = (s->mi + 1) * (s->nj + 1);
+  const dim3 pg((npts + kBlock - 1) / kBlock);
+  hipLaunchKernelGGL((k_adj2_rows<EQ, DIR>), pg, block, 0, s->stream, G);
+  HIPCHK(hipGetLastError());
+  hipLaunchKernelGGL((k_adj2_gather<EQ, DIR>), pg, block, 0, s->stream, G);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+// arguments of the adjoint of a sweep of variable `var` (row layout)
+Adj2Args adj2_args(ixh_state *s, int var, double *const *W) {
+  Adj2Args G;
+  memset(&G, 0, sizeof G);
+  G.S = base_args(s, IXH_DIR_X);
+  if (var >= 0) set_bc(s, G.S, var, IXH_DIR_X);
+  G.mu = W[W_MU];
+  for (int k = 0; k < 4; ++k) G.fa[k] = W[W_FA0 + k];
+  return G;
+}
+
+// adjoint of one step: (ubar, vbar, Tbar, pbar) of its end, held in
+// W_UB3, W_VB3, W_TB2, W_PB, become those of its start in W_UB0, W_VB0,
+// W_TB0, W_PB; boundary adjoints of T accumulate in adj_g
+int adj2_step(ixh_state *s, double *const *tp, double *const *W) {
+  const size_t fb = s->fsize * sizeof(double);
+  for (int k : {W_AUB, W_AVB, W_UANTB, W_VANTB, W_TANTB})
+    HIPCHK(hipMemsetAsync(W[k], 0, fb, s->stream));
+  const int mi = s->mi, nj = s->nj;
+  // T x-sweep, then T y-sweep
+  {
+    Adj2Args G = adj2_args(s, IXH_VAR_T, W);
+    G.S.u = tp[TP_U3]; G.S.v = tp[TP_V3];
+    G.x = tp[TP_T2]; G.phi = tp[TP_T1]; G.ybar = W[W_TB2];
+    G.phib = W[W_TB1]; G.velub = W[W_UB3]; G.velvb = W[W_VB3]; G.antb = W[W_TANTB];
+    G.g_lo = s->adj_g[IXH_SIDE_A]; G.g_hi = s->adj_g[IXH_SIDE_C];
+    CHK((adj2_sweep<EQ_T, DIR_X>(s, G, nj - 1, mi + 1)));
+    G.x = tp[TP_T1]; G.phi = tp[TP_T0]; G.ybar = W[W_TB1]; G.phib = W[W_TB0];
+    G.g_lo = s->adj_g[IXH_SIDE_B]; G.g_hi = s->adj_g[IXH_SIDE_D];
+    CHK((adj2_sweep<EQ_T, DIR_Y>(s, G, mi - 1, nj + 1)));
+  }
+  // correction
+  {
+    AdjCorrArgs C;
+    memset(&C, 0, sizeof C);
+    C.mi = mi; C.nj = nj; C.P = s->P; C.g = s->geo;
+    C.pc = tp[TP_PC2]; C.au = tp[TP_AU]; C.av = tp[TP_AV];
+    C.ub3 = W[W_UB3]; C.vb3 = W[W_VB3]; C.pbn = W[W_PB];
+    C.ub2 = W[W_UB2]; C.vb2 = W[W_VB2]; C.pcb = W[W_PCB2]; C.aub = W[W_AUB]; C.avb = W[W_AVB];
+    const int npts = (mi + 1) * (nj + 1);
+    hipLaunchKernelGGL(k_adj2_correct, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream, C);
+    HIPCHK(hipGetLastError());
+  }
+  // p' x-sweep, then p' y-sweep (input zero)
+  {
+    Adj2Args G = adj2_args(s, -1, W);
+    G.S.u = tp[TP_U2]; G.S.v = tp[TP_V2]; G.S.au = tp[TP_AU]; G.S.av = tp[TP_AV];
+    G.x = tp[TP_PC2]; G.phi = tp[TP_PC1]; G.ybar = W[W_PCB2]; G.phib = W[W_PCB1];
+    G.velub = W[W_UB2]; G.velvb = W[W_VB2]; G.aub = W[W_AUB]; G.avb = W[W_AVB];
+    CHK((adj2_sweep<EQ_P, DIR_X>(s, G, nj - 1, mi + 1)));
+    G.x = tp[TP_PC1]; G.phi = nullptr; G.ybar = W[W_PCB1]; G.phib = nullptr;
+    CHK((adj2_sweep<EQ_P, DIR_Y>(s, G, mi - 1, nj + 1)));
+  }
+  // v x-sweep, then v y-sweep
+  {
+    Adj2Args G = adj2_args(s, IXH_VAR_V, W);
+    G.S.u = tp[TP_U2]; G.S.v = tp[TP_V1]; G.S.t = tp[TP_T0];
+    G.x = tp[TP_V2]; G.phi = tp[TP_V1]; G.ybar = W[W_VB2]; G.acb = W[W_AVB];
+    G.phib = W[W_VB1]; G.velub = W[W_UB2]; G.antb = W[W_VANTB]; G.pb = W[W_PB]; G.tb = W[W_TB0];
+    CHK((adj2_sweep<EQ_V, DIR_X>(s, G, nj - 2, mi + 1)));
+    G.S.v = tp[TP_V0];
+    G.x = tp[TP_V1]; G.phi = tp[TP_V0]; G.ybar = W[W_VB1]; G.acb = nullptr; G.phib = W[W_VB0];
+    CHK((adj2_sweep<EQ_V, DIR_Y>(s, G, mi - 1, nj)));
+  }
+  // u x-sweep, then u y-sweep
+  {
+    Adj2Args G = adj2_args(s, IXH_VAR_U, W);
+    G.S.u = tp[TP_U1]; G.S.v = tp[TP_V0];
+    G.x = tp[TP_U2]; G.phi = tp[TP_U1]; G.ybar = W[W_UB2]; G.acb = W[W_AUB];
+    G.phib = W[W_UB1]; G.velvb = W[W_VB0]; G.antb = W[W_UANTB]; G.pb = W[W_PB];
+    CHK((adj2_sweep<EQ_U, DIR_X>(s, G, nj - 1, mi)));
+    G.S.u = tp[TP_U0];
+    G.x = tp[TP_U1]; G.phi = tp[TP_U0]; G.ybar = W[W_UB1]; G.acb = nullptr; G.phib = W[W_UB0];
+    CHK((adj2_sweep<EQ_U, DIR_Y>(s, G, mi - 2, nj + 1)));
+  }
+  // the lagged copies are the fields of the step's start
+  const int nf = (int)s->fsize;
+  const dim3 fg((nf + kBlock - 1) / kBlock), block(kBlock);
+  hipLaunchKernelGGL(k_axpy1, fg, block, 0, s->stream, W[W_UB0], (const double *)W[W_UANTB], nf);
+  hipLaunchKernelGGL(k_axpy1, fg, block, 0, s->stream, W[W_VB0], (const double *)W[W_VANTB], nf);
+  hipLaunchKernelGGL(k_axpy1, fg, block, 0, s->stream, W[W_TB0], (const double *)W[W_TANTB], nf);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixh_adjoint_coupled(ixh_state *s, const double *tbar, int32_t ld, double *grad_a, double *grad_b,
+                        double *grad_c, double *grad_d, double *grad_t0, int32_t ld0) {
+  if (!s || !tbar || ld < s->mi + 1 || (grad_t0 && ld0 < s->mi + 1))
+    return fail(IXH_ERR_ARG, "adjoint: bad arguments");
+  if (s->adj_steps < 1) return fail(IXH_ERR_ARG, "adjoint: no recorded run (ixh_adjoint_record)");
+  if (s->prm.ri_x != 0.0) return fail(IXH_ERR_UNSUPP, "coupled adjoint: the u equation must not read T (ri_x = 0)");
+  const int K = s->adj_steps, C = s->adj_every;
+  const size_t glen = (size_t)std::max(s->mi, s->nj) + 4;
+  for (int k = 0; k < 4; ++k) {
+    if (!s->adj_g[k]) CHK(dmalloc(&s->adj_g[k], glen));
+    HIPCHK(hipMemsetAsync(s->adj_g[k], 0, glen * sizeof(double), s->stream));
+  }
+  while ((int)s->adj2_tape.size() < TP_N * C) {
+    double *p = nullptr;
+    CHK(dmalloc(&p, s->fsize));
+    s->adj2_tape.push_back(p);
+  }
+  while ((int)s->adj2_w.size() < W_N) {
+    double *p = nullptr;
+    CHK(dmalloc(&p, s->fsize));
+    s->adj2_w.push_back(p);
+  }
+  std::vector<double *> W(s->adj2_w.begin(), s->adj2_w.begin() + W_N);
+  const size_t fb = s->fsize * sizeof(double);
+  for (int k : {W_UB3, W_VB3, W_PB}) HIPCHK(hipMemsetAsync(W[k], 0, fb, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  CHK(upload_as<double>(s, W[W_TB2], tbar, ld, s->mi + 1, s->nj + 1, false));
+  const int nseg = (K + C - 1) / C;
+  for (int seg = nseg - 1; seg >= 0; --seg) {
+    const int k0 = seg * C, k1 = std::min(K, k0 + C);
+    CHK(ck_load(s, s->adj_ck[seg]));
+    for (int k = k0; k < k1; ++k) CHK(tape_step(s, &s->adj2_tape[(size_t)TP_N * (k - k0)]));
+    for (int k = k1 - 1; k >= k0; --k) {
+      CHK(adj2_step(s, &s->adj2_tape[(size_t)TP_N * (k - k0)], W.data()));
+      // the start of step k is the end of step k-1
+      std::swap(W[W_UB3], W[W_UB0]);
+      std::swap(W[W_VB3], W[W_VB0]);
+      std::swap(W[W_TB2], W[W_TB0]);
+    }
+  }
+  CHK(ck_load(s, s->adj_ck[nseg]));  // back to the end of the recorded run
+  std::vector<double> g(glen);
+  double *outs[4] = {grad_a, grad_b, grad_c, grad_d};
+  for (int sd = 0; sd < 4; ++sd) {
+    if (!outs[sd]) continue;
+    HIPCHK(hipMemcpyAsync(g.data(), s->adj_g[sd], glen * sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    const int n = side_len(s, IXH_VAR_T, sd);
+    for (int k = 1; k <= n; ++k) outs[sd][k - 1] = g[k];
+  }
+  if (grad_t0) {
+    std::vector<double> t((size_t)s->P * (s->nj + 1));
+    HIPCHK(hipMemcpyAsync(t.data(), W[W_TB2], t.size() * sizeof(double), hipMemcpyDeviceToHost, s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     for (int j = 0; j <= s->nj; ++j)
       for (int i = 0; i <= s->mi; ++i) grad_t0[(size_t)i + (size_t)j * ld0] = t[(size_t)i + (size_t)j * s->P];

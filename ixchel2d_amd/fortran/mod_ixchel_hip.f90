@@ -221,6 +221,14 @@ module ixchel_hip
        integer(c_int32_t), value :: ld, ld0
        type(c_ptr), value :: grad_a, grad_b, grad_c, grad_d, grad_t0
      end function ixh_adjoint_energy
+     integer(c_int) function ixh_adjoint_coupled(state, tbar, ld, grad_a, grad_b, grad_c, grad_d, grad_t0, ld0) &
+          bind(C, name='ixh_adjoint_coupled')
+       import :: c_int, c_ptr, c_int32_t, c_double
+       type(c_ptr), value :: state
+       real(c_double), intent(in) :: tbar(*)
+       integer(c_int32_t), value :: ld, ld0
+       type(c_ptr), value :: grad_a, grad_b, grad_c, grad_d, grad_t0
+     end function ixh_adjoint_coupled
   end interface
 
   ! the device state of this program's one cavity

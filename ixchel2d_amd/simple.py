@@ -193,17 +193,21 @@ class Cavity:
         ``every`` steps for ``adjoint_energy``."""
         _lib.check(self.L.ixh_adjoint_record(self.h, nsteps, every))
 
-    def adjoint_energy(self, tbar):
-        """Backward pass of the recorded energy steps for dJ/dT(final) =
-        ``tbar``: returns ({side: dJ/d(boundary values)}, dJ/dT(initial))."""
+    def adjoint_energy(self, tbar, coupled: bool = False):
+        """Backward pass of the recorded steps for dJ/dT(final) = ``tbar``:
+        returns ({side: dJ/d(boundary values)}, dJ/dT(initial)).  With
+        ``coupled`` the adjoint runs through the whole step (exact under
+        buoyancy); otherwise through the energy steps with the velocities
+        held fixed."""
         tb = _f64(tbar)
         if tb.shape != self.shape("t"):
             raise ValueError(f"tbar: shape {tb.shape} != {self.shape('t')}")
         mi, nj = self.mi, self.nj
         g = {"a": np.zeros(nj + 1), "b": np.zeros(mi + 1), "c": np.zeros(nj + 1), "d": np.zeros(mi + 1)}
         t0 = np.zeros(self.shape("t"), order="F")
-        _lib.check(self.L.ixh_adjoint_energy(self.h, _ptr(tb), tb.shape[0], _ptr(g["a"]), _ptr(g["b"]),
-                                             _ptr(g["c"]), _ptr(g["d"]), _ptr(t0), t0.shape[0]))
+        fn = self.L.ixh_adjoint_coupled if coupled else self.L.ixh_adjoint_energy
+        _lib.check(fn(self.h, _ptr(tb), tb.shape[0], _ptr(g["a"]), _ptr(g["b"]), _ptr(g["c"]), _ptr(g["d"]),
+                      _ptr(t0), t0.shape[0]))
         return g, t0
 
     # -------------------------------------------------------------- timing

@@ -117,3 +117,65 @@ def test_autograd_function(problem):
     nu.backward()
     ref = problem.gradient()
     np.testing.assert_allclose(c.grad.numpy(), ref, rtol=1e-14, atol=0)
+
+
+# ---- adjoint of the whole step (ixh_adjoint_coupled)
+
+def test_coupled_equals_energy_adjoint_without_buoyancy(problem):
+    """Without buoyancy the velocity adjoints never return to T: the coupled
+    gradient equals the energy-only one."""
+    c = _controls(8, seed=7)
+    problem.forward(c)
+    problem.coupled = False
+    g_e = problem.gradient()
+    problem.coupled = True
+    g_c = problem.gradient()
+    np.testing.assert_allclose(g_c, g_e, rtol=1e-11, atol=1e-14 * np.max(np.abs(g_e)))
+
+
+@pytest.fixture(scope="module")
+def buoyant():
+    from ixchel2d_amd.simple import Cavity
+    case = C.natural_convection_case(24, 30, height=1.3, Ra=3e4, iters=40, dt=0.01, seed=3, beta=1.1)
+    warm = Cavity(case, solver="pcr")
+    warm.run(150)
+    start = {f: warm.get(f) for f in ("u", "v", "t", "p")}
+    return A.HotWallProblem(case, ncontrols=6, steps=40, every=15, start=start, coupled=True)
+
+
+def test_coupled_adjoint_against_central_differences_with_buoyancy(buoyant):
+    c = _controls(6, seed=8)
+    buoyant.forward(c)
+    g = buoyant.gradient()
+    buoyant.coupled = False
+    g_frozen = buoyant.gradient()
+    buoyant.coupled = True
+    h = 1e-6
+    fd = np.zeros_like(c)
+    for i in range(len(c)):
+        e = np.zeros_like(c)
+        e[i] = h
+        fd[i] = (buoyant.forward(c + e) - buoyant.forward(c - e)) / (2 * h)
+    err = np.max(np.abs(g - fd)) / np.max(np.abs(fd))
+    err_frozen = np.max(np.abs(g_frozen - fd)) / np.max(np.abs(fd))
+    assert err <= 1e-5, (err, err_frozen, g, fd)
+    assert err < err_frozen
+
+
+def test_coupled_initial_temperature_with_buoyancy(buoyant):
+    c = _controls(6, seed=9)
+    buoyant.forward(c)
+    _, gt0 = buoyant.cav.adjoint_energy(buoyant.W, coupled=True)
+    rng = np.random.default_rng(10)
+    d = rng.standard_normal(buoyant.start["t"].shape)
+    eps = 1e-5
+    saved = buoyant.start["t"]
+    try:
+        buoyant.start = dict(buoyant.start, t=saved + eps * d)
+        jp = buoyant.forward(c)
+        buoyant.start = dict(buoyant.start, t=saved - eps * d)
+        jm = buoyant.forward(c)
+    finally:
+        buoyant.start = dict(buoyant.start, t=saved)
+    fd = (jp - jm) / (2 * eps)
+    assert abs(fd - float(np.sum(gt0 * d))) <= 1e-5 * abs(fd)
