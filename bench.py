@@ -283,9 +283,20 @@ def adjoint_bench(args):
     print(json.dumps(out))
 
 
+LAUNCH_PARTS = {
+    "multi_v_y+fold": ("sweep_v_y", "finalize"),
+    "step_sweep_v_x": ("sweep_v_x",),
+    "step_sweep_p_y": ("sweep_p_y",),
+    "step_sweep_p_x": ("sweep_p_x",),
+    "step_correct": ("correct",),
+    "multi_t_y+residual+u_y": ("sweep_t_y", "residual_u", "sweep_u_y"),
+    "multi_t_x+u_x": ("sweep_t_x", "sweep_u_x"),
+}
+
+
 def time_path(case, solver, precision, steps, warmup):
-    """Whole-step throughput of one cavity, then per-kernel durations from
-    HIP events around every launch on the library's stream."""
+    """Whole-step throughput of one cavity and the average kernel duration of
+    each of the step's 7 launches over the same timed steps."""
     import torch
     from ixchel2d_amd.simple import Cavity
     m = case.mesh
@@ -299,30 +310,45 @@ def time_path(case, solver, precision, steps, warmup):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     ms_per_step = dt * 1e3 / steps
+    # the same number of steps again with every launch recording its
+    # kernel's start and end (HIP events on the library's stream, read back
+    # from a ring).  The events slow the host-side launches, so this run
+    # gives the kernel durations and the timed run above the throughput.
     gpu.set_profiling(True)
-    gpu.run(min(steps, 500))
+    t1 = time.perf_counter()
+    gpu.run(steps)
+    torch.cuda.synchronize()
+    ms_events = (time.perf_counter() - t1) * 1e3 / steps
     kt = gpu.kernel_times()
     gpu.set_profiling(False)
     buoy = case.params.derived()["Riy"] != 0.0
-    per_kernel = {}
+    launches = {}
     for name, (ms, n) in kt.items():
-        if n == 0:
+        if name not in LAUNCH_PARTS or n == 0:
             continue
-        avg_us = ms * 1e3 / n
-        b = alg_bytes(name, npts, precision, buoy)
-        per_kernel[name] = {"avg_us": round(avg_us, 3), "launches": n, "alg_bytes": b,
-                            "GB/s": round(b / (avg_us * 1e-6) / 1e9, 1) if b else None}
-    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_us"] * per_kernel[k]["launches"])
-    dk = per_kernel[dom]
+        us = ms * 1e3 / n
+        b = sum(alg_bytes(k, npts, precision, buoy) for k in LAUNCH_PARTS[name])
+        launches[name] = {"avg_us": round(us, 3), "launches": n, "alg_bytes": b,
+                          "GB/s": round(b / (us * 1e-6) / 1e9, 1)}
+    dom = max(launches, key=lambda k: launches[k]["avg_us"])
+    dk = launches[dom]
     iter_bytes = sum(alg_bytes(k, npts, precision, buoy) for k in ALG_OPERANDS)
     it_gbs = iter_bytes / (ms_per_step * 1e-3) / 1e9
+    # HBM bytes per launch from the committed counter profile of this
+    # workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, scaled on a 512 MiB copy)
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_nc301x701_fused.json")
+    if os.path.exists(tpath) and (m.mi, m.nj) == (301, 701):
+        with open(tpath) as fh:
+            traffic = json.load(fh).get(precision, {}).get(dom, {}).get("hbm_bytes")
     roofline = {"bound": "hbm", "achieved": dk["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(dk["GB/s"] / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dom,
+                "frac": round(dk["GB/s"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dom,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"],
                 "iteration_alg_bytes": iter_bytes, "iteration_GB/s": round(it_gbs, 1),
-                "iteration_frac": round(it_gbs / HBM_PEAK_GBS, 4)}
+                "iteration_frac": round(it_gbs / HBM_PEAK_GBS, 4),
+                "sum_launch_us": round(sum(v["avg_us"] for v in launches.values()), 2)}
     return {"value": round(cells * steps / dt / 1e6, 2), "ms_per_step": round(ms_per_step, 5),
-            "roofline": roofline, "kernels": per_kernel}
+            "ms_per_step_with_events": round(ms_events, 5), "roofline": roofline, "launches": launches}
 
 
 def accuracy(case, solver, steps):
@@ -418,7 +444,7 @@ def main():
                    "mesh": [m.mi, m.nj], "cells": cells, "solver": args.solver, "path": "fp64"},
         "roofline": r64["roofline"],
         "cpu_baseline": cpu,
-        "kernels": r64["kernels"],
+        "launches": r64["launches"],
         "fp32_mixed": r32,
     }
     print(json.dumps(out))

@@ -165,9 +165,13 @@ int ixh_nusselt_x(ixh_state *s, double *nu_west, double *nu_east);
 int ixh_tridiagonal(const double *a, double *b, const double *c, double *r, int32_t n,
                     int32_t nsys, int32_t solver);
 
-/* per-kernel timing of the last ixh_simple_iterations call made with
- * profiling on (ixh_set_profiling): for each kernel id the total milliseconds
- * and the launch count, measured with HIP events on the state's stream */
+/* per-kernel timing of the ixh_simple_iterations calls made with profiling
+ * on (ixh_set_profiling resets it): for each kernel id the total milliseconds
+ * and the launch count, measured with HIP events on the state's stream.  Ids
+ * 0-10 are the single kernels of the sequential step (events around each
+ * launch); ids 11-17 the 7 launches of the fused step, whose events carry
+ * the kernels' own start and end (hipExtLaunchKernelGGL) and are read back
+ * from a ring without stalling the launches. */
 int ixh_set_profiling(ixh_state *s, int32_t on);
 int ixh_kernel_times(ixh_state *s, double *ms, int32_t *launches, int32_t cap, int32_t *nkernels);
 const char *ixh_kernel_name(int32_t id);

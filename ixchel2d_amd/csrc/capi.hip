@@ -1,6 +1,7 @@
 // Host side of the C-ABI (include/ixchel2d.h): device state, buffer
 // rotation, phase launches, the SIMPLE control loop and the fused time step.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdarg>
@@ -45,12 +46,27 @@ int fail(int code, const char *fmt, ...) {
     if (rc_ != 0) return rc_; \
   } while (0)
 
+// a kernel launch on the state's stream; with timing events set, the
+// kernel's own start and end are recorded into them
+#define IXH_LAUNCH(s, kern, grid, block, ...)                                                      \
+  do {                                                                                             \
+    if ((s)->t_start)                                                                              \
+      hipExtLaunchKernelGGL(kern, grid, block, 0, (s)->launch, (s)->t_start, (s)->t_stop, 0,     \
+                            __VA_ARGS__);                                                          \
+    else                                                                                           \
+      hipLaunchKernelGGL(kern, grid, block, 0, (s)->launch, __VA_ARGS__);                          \
+  } while (0)
+
 enum KernelId {
-  K_U_Y = 0, K_U_X, K_V_Y, K_V_X, K_P_Y, K_P_X, K_CORR, K_T_Y, K_T_X, K_RES, K_FIN, K_NUM
+  K_U_Y = 0, K_U_X, K_V_Y, K_V_X, K_P_Y, K_P_X, K_CORR, K_T_Y, K_T_X, K_RES, K_FIN,
+  K_L1, K_L2, K_L3, K_L4, K_L5, K_L6, K_L7,  // the 7 launches of the fused step
+  K_NUM
 };
 const char *kKernelNames[K_NUM] = {"sweep_u_y", "sweep_u_x", "sweep_v_y", "sweep_v_x", "sweep_p_y",
                                    "sweep_p_x", "correct",   "sweep_t_y", "sweep_t_x", "residual_u",
-                                   "finalize"};
+                                   "finalize",  "multi_v_y+fold", "step_sweep_v_x", "step_sweep_p_y",
+                                   "step_sweep_p_x", "step_correct", "multi_t_y+residual+u_y",
+                                   "multi_t_x+u_x"};
 
 constexpr int kNbuf = 4;  // rotation buffers per solved field
 
@@ -111,6 +127,10 @@ struct ixh_state {
   bool profiling = false;
   double kms[K_NUM]{};
   int kcount[K_NUM]{};
+  // fused-step launch timing: kernel start/end events of every launch
+  // (hipExtLaunchKernelGGL), a ring of kRing steps harvested as it wraps
+  hipEvent_t t_start = nullptr, t_stop = nullptr;  // events of the launch being issued
+  std::vector<hipEvent_t> lev;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 };
 
@@ -200,12 +220,12 @@ template <int EQ, int DIR, int SOLVER, typename R>
 int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
   dim3 grid(A.nlines), block(kBlock);
   switch (ept_for(n)) {
-    case 1: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 1, R>), grid, block, 0, s->launch, A); break;
-    case 2: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 2, R>), grid, block, 0, s->launch, A); break;
-    case 3: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 3, R>), grid, block, 0, s->launch, A); break;
-    case 4: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 4, R>), grid, block, 0, s->launch, A); break;
-    case 6: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 6, R>), grid, block, 0, s->launch, A); break;
-    default: hipLaunchKernelGGL((k_sweep<EQ, DIR, SOLVER, 8, R>), grid, block, 0, s->launch, A); break;
+    case 1: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 1, R>), grid, block, A); break;
+    case 2: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 2, R>), grid, block, A); break;
+    case 3: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 3, R>), grid, block, A); break;
+    case 4: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 4, R>), grid, block, A); break;
+    case 6: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 6, R>), grid, block, A); break;
+    default: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 8, R>), grid, block, A); break;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -625,12 +645,12 @@ template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
 int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept) {
   dim3 grid(nwg), block(kBlock);
   switch (ept) {
-    case 1: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 1, R>), grid, block, 0, s->stream, M); break;
-    case 2: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R>), grid, block, 0, s->stream, M); break;
-    case 3: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 3, R>), grid, block, 0, s->stream, M); break;
-    case 4: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R>), grid, block, 0, s->stream, M); break;
-    case 6: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 6, R>), grid, block, 0, s->stream, M); break;
-    default: hipLaunchKernelGGL((k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 8, R>), grid, block, 0, s->stream, M); break;
+    case 1: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 1, R>), grid, block, M); break;
+    case 2: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R>), grid, block, M); break;
+    case 3: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 3, R>), grid, block, M); break;
+    case 4: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R>), grid, block, M); break;
+    case 6: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 6, R>), grid, block, M); break;
+    default: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 8, R>), grid, block, M); break;
   }
   HIPCHK(hipGetLastError());
   return 0;
@@ -658,61 +678,128 @@ int multi_layout(MultiArgs &M, int n0, int n1, int nres, int nfin) {
   return M.off[3] + nfin;
 }
 
-// One time step k with independent phases fused (the u equation must not
-// read T):
-//   v_y(k) [+ fold of step k-1]; v_x(k); p'_y(k); p'_x(k); correction(k);
-//   T_y(k) + residual(k) + u_y(k+1);  T_x(k) + u_x(k+1)
-// The u sweeps of the first step run before the first call, the fold of the
-// last step after the last call.  u_ant of step k+1 is u of step k; the u
-// buffer the residual of step k reads as its u_ant is held out of the
-// rotation while the u sweeps of step k+1 run beside it.
-int fused_step(ixh_state *s, bool first, bool last) {
-  {
-    MultiArgs M;
-    memset(&M, 0, sizeof M);
-    CHK(prep_v(s, IXH_DIR_Y, M.s[0]));
-    if (!first) prep_finalize(s, M.f, 31, s->d_hist, s->d_slot);
-    const int nwg = multi_layout(M, M.s[0].nlines, 0, 0, first ? 0 : 1);
-    CHK((launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, M, nwg, ept_for(line_len(s, EQ_V, IXH_DIR_Y)))));
-  }
-  CHK(sweep_v(s, IXH_DIR_X));
+// The 7 launches of one fused time step k (the u equation must not read T):
+//   1: v_y(k) [+ fold of step k-1]   2: v_x(k)   3: p'_y(k)   4: p'_x(k)
+//   5: correction(k)   6: T_y(k) + residual(k) + u_y(k+1)   7: T_x(k) + u_x(k+1)
+// The u sweeps of the first step run before the first step, the fold of the
+// last step after it.  u_ant of step k+1 is u of step k; the u buffer the
+// residual of step k reads as its u_ant is held out of the rotation while
+// the u sweeps of step k+1 run beside it.
+struct StepLaunches {
+  MultiArgs m1, m6, m7;
+  SweepArgs vx, py, px;
+  CorrArgs corr;
+  int nwg1, nwg6, nwg7, ept1, ept6, ept7;
+  bool last;
+};
+
+// arguments of the 7 launches of one step (advances the rotation as the
+// step does); `fold` null: no fold in launch 1 (first step)
+int build_step(ixh_state *s, bool first, bool last, StepLaunches &L) {
+  memset(&L, 0, sizeof L);
+  L.last = last;
+  CHK(prep_v(s, IXH_DIR_Y, L.m1.s[0]));
+  if (!first) prep_finalize(s, L.m1.f, 31, s->d_hist, s->d_slot);
+  L.nwg1 = multi_layout(L.m1, L.m1.s[0].nlines, 0, 0, first ? 0 : 1);
+  L.ept1 = ept_for(line_len(s, EQ_V, IXH_DIR_Y));
+  CHK(prep_v(s, IXH_DIR_X, L.vx));
   s->pc_zero = true;
-  CHK(sweep_p(s, IXH_DIR_Y));
-  CHK(sweep_p(s, IXH_DIR_X));
-  CHK(do_correct(s));
+  CHK(prep_p(s, IXH_DIR_Y, L.py));
+  CHK(prep_p(s, IXH_DIR_X, L.px));
+  bool active = false;
+  CHK(prep_correct(s, L.corr, active));
+  if (!active) return fail(IXH_ERR_ARG, "fused step: p' is zero after its sweeps");
   // the T sweeps and the residual of step k read u of step k: build their
   // arguments before the u sweeps of step k+1 advance the rotation
-  MultiArgs M6, M7;
-  memset(&M6, 0, sizeof M6);
-  memset(&M7, 0, sizeof M7);
-  CHK(prep_t(s, IXH_DIR_Y, M6.s[0]));
-  CHK(prep_t(s, IXH_DIR_X, M7.s[0]));
-  prep_residual(s, M6.r);
+  CHK(prep_t(s, IXH_DIR_Y, L.m6.s[0]));
+  CHK(prep_t(s, IXH_DIR_X, L.m7.s[0]));
+  prep_residual(s, L.m6.r);
   const int held = s->u_ant;
   s->u_ant = s->u_cur;
-  int ept6 = ept_for(line_len(s, EQ_T, IXH_DIR_Y)), ept7 = ept_for(line_len(s, EQ_T, IXH_DIR_X));
+  L.ept6 = ept_for(line_len(s, EQ_T, IXH_DIR_Y));
+  L.ept7 = ept_for(line_len(s, EQ_T, IXH_DIR_X));
   int nu6 = 0, nu7 = 0;
   if (!last) {
     s->u_hold = held;
-    CHK(prep_u(s, IXH_DIR_Y, M6.s[1]));
-    CHK(prep_u(s, IXH_DIR_X, M7.s[1]));
+    CHK(prep_u(s, IXH_DIR_Y, L.m6.s[1]));
+    CHK(prep_u(s, IXH_DIR_X, L.m7.s[1]));
     s->u_hold = -1;
-    nu6 = M6.s[1].nlines;
-    nu7 = M7.s[1].nlines;
-    ept6 = std::max(ept6, ept_for(line_len(s, EQ_U, IXH_DIR_Y)));
-    ept7 = std::max(ept7, ept_for(line_len(s, EQ_U, IXH_DIR_X)));
+    nu6 = L.m6.s[1].nlines;
+    nu7 = L.m7.s[1].nlines;
+    L.ept6 = std::max(L.ept6, ept_for(line_len(s, EQ_U, IXH_DIR_Y)));
+    L.ept7 = std::max(L.ept7, ept_for(line_len(s, EQ_U, IXH_DIR_X)));
   }
-  const int nwg6 = multi_layout(M6, M6.s[0].nlines, nu6, s->nj - 1, 0);
-  const int nwg7 = multi_layout(M7, M7.s[0].nlines, nu7, 0, 0);
-  if (last) {
-    CHK((launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, M6, nwg6, ept6)));
-    CHK((launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, M7, nwg7, ept7)));
-  } else {
-    CHK((launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, M6, nwg6, ept6)));
-    CHK((launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, M7, nwg7, ept7)));
-  }
+  L.nwg6 = multi_layout(L.m6, L.m6.s[0].nlines, nu6, s->nj - 1, 0);
+  L.nwg7 = multi_layout(L.m7, L.m7.s[0].nlines, nu7, 0, 0);
   s->v_ant = s->v_cur;
   s->t_ant = s->t_cur;
+  return 0;
+}
+
+// launch part k (1..7) of a built step
+int launch_part(ixh_state *s, const StepLaunches &L, int k) {
+  switch (k) {
+    case 1: return launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, L.m1, L.nwg1, L.ept1);
+    case 2: return launch_sweep<EQ_V, DIR_X>(s, L.vx, line_len(s, EQ_V, IXH_DIR_X));
+    case 3: return launch_sweep<EQ_P, DIR_Y>(s, L.py, line_len(s, EQ_P, IXH_DIR_Y));
+    case 4: return launch_sweep<EQ_P, DIR_X>(s, L.px, line_len(s, EQ_P, IXH_DIR_X));
+    case 5: {
+      dim3 block(kBlock), grid(corr_tiles_x(s), corr_tiles_y(s));
+      if (s->f32)
+        IXH_LAUNCH(s, k_correct<float>, grid, block, L.corr);
+      else
+        IXH_LAUNCH(s, k_correct<double>, grid, block, L.corr);
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
+    case 6:
+      if (L.last) return launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, L.m6, L.nwg6, L.ept6);
+      return launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, L.m6, L.nwg6, L.ept6);
+    case 7:
+      if (L.last) return launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, L.m7, L.nwg7, L.ept7);
+      return launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, L.m7, L.nwg7, L.ept7);
+  }
+  return fail(IXH_ERR_ARG, "launch part %d", k);
+}
+
+constexpr int kRing = 32;  // steps of launch events in flight
+
+// add the durations of the 7 launches of ring slot `slot`
+int harvest_slot(ixh_state *s, int slot) {
+  hipEvent_t *e = &s->lev[(size_t)slot * 14];
+  HIPCHK(hipEventSynchronize(e[13]));
+  for (int k = 0; k < 7; ++k) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e[2 * k], e[2 * k + 1]));
+    s->kms[K_L1 + k] += ms;
+    s->kcount[K_L1 + k] += 1;
+  }
+  return 0;
+}
+
+// step k of a run; with profiling on, every launch records its kernel's
+// start and end into the ring slot of step k
+int fused_step(ixh_state *s, bool first, bool last, int k) {
+  StepLaunches L;
+  CHK(build_step(s, first, last, L));
+  const bool timed = s->profiling;
+  const int slot = k % kRing;
+  if (timed) {
+    if (s->lev.empty()) {
+      s->lev.resize((size_t)kRing * 14);
+      for (hipEvent_t &e : s->lev) HIPCHK(hipEventCreate(&e));
+    }
+    if (k >= kRing) CHK(harvest_slot(s, slot));
+  }
+  for (int p = 1; p <= 7; ++p) {
+    if (timed) {
+      s->t_start = s->lev[(size_t)slot * 14 + 2 * (p - 1)];
+      s->t_stop = s->lev[(size_t)slot * 14 + 2 * (p - 1) + 1];
+    }
+    const int rc = launch_part(s, L, p);
+    s->t_start = s->t_stop = nullptr;
+    CHK(rc);
+  }
   if (last) CHK(do_finalize(s, 31, s->d_hist, s->d_slot));
   return 0;
 }
@@ -879,6 +966,7 @@ int ixh_destroy(ixh_state *s) {
   }
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
+  for (hipEvent_t e : s->lev) (void)hipEventDestroy(e);
   if (s->own_stream && s->stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return 0;
@@ -1140,9 +1228,9 @@ int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t 
     CHK(ensure_hist(s, nsteps > 0 ? nsteps : 1));
     hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->d_slot, 0);
     HIPCHK(hipGetLastError());
-    if (s->profiling || p.ri_x != 0.0) {
-      // upstream order, one kernel after the other (per-kernel event timing,
-      // or a u equation that reads T)
+    if (p.ri_x != 0.0) {
+      // a u equation that reads T: upstream order, one kernel after the
+      // other (per-kernel event timing with profiling on)
       for (int k = 0; k < nsteps; ++k) CHK(fast_step(s));
     } else if (nsteps > 0) {
       // fused launches on one stream: work on separate streams did not
@@ -1151,7 +1239,9 @@ int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t 
       // direct launches
       CHK(sweep_u(s, IXH_DIR_Y));  // prologue: u sweeps of the first step
       CHK(sweep_u(s, IXH_DIR_X));
-      for (int k = 0; k < nsteps; ++k) CHK(fused_step(s, k == 0, k == nsteps - 1));
+      for (int k = 0; k < nsteps; ++k) CHK(fused_step(s, k == 0, k == nsteps - 1, k));
+      if (s->profiling)
+        for (int k = std::max(0, nsteps - kRing); k < nsteps; ++k) CHK(harvest_slot(s, k % kRing));
     }
     const int keep = nsteps < cap ? nsteps : cap;
     std::vector<double> h((size_t)8 * (keep > 0 ? keep : 1));
