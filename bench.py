@@ -11,7 +11,8 @@ its error against FP64.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver pcr|thomas]
   python bench.py --workload batch256   # 256 mixed-size cavities, one launch per phase
-  python bench.py --workload adjoint129 # hot-wall Nu gradient through the energy adjoint
+  python bench.py --workload adjoint129 # hot-wall Nu gradient through the adjoint
+  python bench.py --workload fp32_convergence  # FP64 vs FP32-mixed toward the steady state
 """
 from __future__ import annotations
 
@@ -294,6 +295,37 @@ LAUNCH_PARTS = {
 }
 
 
+def convergence_study(args):
+    """FP64 and FP32-mixed runs of the 301x701 case side by side toward the
+    steady state: residual, maxbo, hot-wall Nu and its relative difference,
+    and the max-norm differences of T and u, every --conv-chunk iterations."""
+    import numpy as np
+    import torch
+    from ixchel2d_amd.simple import Cavity
+    case = build_case()
+    g64 = Cavity(case, solver="pcr", precision="fp64")
+    g32 = Cavity(case, solver="pcr", precision="fp32_mixed")
+    rows = []
+    total = 0
+    for _ in range(args.conv_chunks):
+        g64.run(args.conv_chunk - 1, cap=1)
+        g32.run(args.conv_chunk - 1, cap=1)
+        h64, h32 = g64.run(1), g32.run(1)
+        total += args.conv_chunk
+        n64, n32 = g64.nusselt_x()[0], g32.nusselt_x()[0]
+        t64, t32, u64, u32 = g64.get("t"), g32.get("t"), g64.get("u"), g32.get("u")
+        row = {"iterations": total, "residuo_fp64": float(h64["residuo"][-1]),
+               "residuo_fp32_mixed": float(h32["residuo"][-1]), "maxbo_fp64": float(h64["maxbo"][-1]),
+               "maxbo_fp32_mixed": float(h32["maxbo"][-1]), "nu_fp64": n64, "nu_fp32_mixed": n32,
+               "nu_rel_err": abs(n32 - n64) / abs(n64), "t_linf": float(np.max(np.abs(t32 - t64))),
+               "u_linf_rel": float(np.max(np.abs(u32 - u64)) / np.max(np.abs(u64)))}
+        rows.append(row)
+        print(json.dumps(row), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    print(json.dumps({"workload": "natural-convection tall cavity 301x701, FP64 and FP32-mixed toward the steady "
+                                  "state", "chunk": args.conv_chunk, "rows": rows}))
+
+
 def time_path(case, solver, precision, steps, warmup):
     """Whole-step throughput of one cavity and the average kernel duration of
     each of the step's 7 launches over the same timed steps."""
@@ -378,7 +410,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--solver", default="pcr", choices=["pcr", "thomas"])
-    ap.add_argument("--workload", default="nc301x701", choices=["nc301x701", "batch256", "adjoint129"])
+    ap.add_argument("--workload", default="nc301x701",
+                    choices=["nc301x701", "batch256", "adjoint129", "fp32_convergence"])
+    ap.add_argument("--conv-chunk", type=int, default=30000, help="iterations per row of the convergence study")
+    ap.add_argument("--conv-chunks", type=int, default=12, help="rows of the convergence study")
     ap.add_argument("--adj-dt", type=float, default=0.01, help="time step of the adjoint workload")
     ap.add_argument("--adj-warm", type=int, default=20000, help="iterations to the converged start")
     ap.add_argument("--adj-reps", type=int, default=5, help="timed gradient evaluations")
@@ -407,6 +442,9 @@ def main():
         return
     if args.workload == "adjoint129":
         adjoint_bench(args)
+        return
+    if args.workload == "fp32_convergence":
+        convergence_study(args)
         return
     if args.calibrate_copy_mb:
         n = args.calibrate_copy_mb * (1 << 20) // 4
