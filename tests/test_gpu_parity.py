@@ -189,3 +189,29 @@ def test_full_size_5000_iterations_against_upstream():
         assert float(np.sqrt(np.sum(a * a))) == pytest.approx(float(gold[f + "_l2"]), rel=REL)
     hist = np.c_[h["maxbo"], h["residuo"]]
     np.testing.assert_allclose(hist, gold["history"], rtol=1e-8, atol=1e-15)
+
+
+def test_fused_step_launch_timing():
+    """Profiling on: each of the 7 fused launches records its kernel's start
+    and end; every launch kind is counted once per step and the durations
+    add up to about the step time."""
+    import time
+    import torch
+    case = K.nc34()
+    g = _cavity(case, "pcr")
+    g.run(10)
+    g.set_profiling(True)
+    torch.cuda.synchronize()
+    g.run(100)
+    kt = g.kernel_times()
+    g.set_profiling(False)
+    names = ["multi_v_y+fold", "step_sweep_v_x", "step_sweep_p_y", "step_sweep_p_x", "step_correct",
+             "multi_t_y+residual+u_y", "multi_t_x+u_x"]
+    for n in names:
+        ms, cnt = kt[n]
+        assert cnt == 100 and ms > 0.0, (n, kt[n])
+    # the profiled run gives the same fields as an unprofiled one
+    h = _cavity(case, "pcr")
+    h.run(110)
+    for f in "uvtp":
+        assert np.array_equal(g.get(f), h.get(f)), f
