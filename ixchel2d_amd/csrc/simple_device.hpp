@@ -114,10 +114,23 @@ __device__ __forceinline__ const FastGeoT<double> &fast_of<double>(const SweepAr
 template <>
 __device__ __forceinline__ const FastGeoT<float> &fast_of<float>(const SweepArgs &A) { return A.ff; }
 
-// quotient of the line solvers: IEEE division in FP64; in FP32 the
-// hardware reciprocal (1 ulp) times the numerator
-__device__ __forceinline__ double qdiv(double a, double b) { return a / b; }
+// quotient of the PCR line solvers: in FP64 the hardware reciprocal
+// refined by two Newton steps, times the
+// numerator with one residual correction (within about an ulp of IEEE
+// division, in a shorter dependent chain); in FP32 the hardware reciprocal
+// (1 ulp) times the numerator.  The Thomas path keeps IEEE division.
+__device__ __forceinline__ double qdiv(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  r = fma(fma(-b, r, 1.0), r, r);
+  const double q = a * r;
+  return fma(fma(-b, q, a), r, q);
+}
 __device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// quotient of the p' assembly, shared by every solver: IEEE division in FP64
+// (the Thomas path matches upstream bit for bit), the reciprocal in FP32
+__device__ __forceinline__ double adiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float adiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 __device__ __forceinline__ double rabs(double x) { return fabs(x); }
 __device__ __forceinline__ float rabs(float x) { return fabsf(x); }
 
@@ -269,11 +282,11 @@ __device__ __forceinline__ PCoefT<R> p_coef(const SweepArgs &A, int ii, int jj) 
   R ai = AT(au, ii - 1, jj), ad = AT(au, ii, jj);
   R as = AT(av, ii, jj - 1), an = AT(av, ii, jj);
   R dyp = g.dyp[jj], dxp = g.dxp[ii];
-  c.ai = qdiv(-dyp * dyp, ai);
-  c.ad = qdiv(-dyp * dyp, ad);
-  c.bs = qdiv(-dxp * dxp, as);
-  c.bn = qdiv(-dxp * dxp, an);
-  c.ac = qdiv(-c.ai - c.ad - c.bs - c.bn, R(A.ph.rel_pres));
+  c.ai = adiv(-dyp * dyp, ai);
+  c.ad = adiv(-dyp * dyp, ad);
+  c.bs = adiv(-dxp * dxp, as);
+  c.bn = adiv(-dxp * dxp, an);
+  c.ac = adiv(-c.ai - c.ad - c.bs - c.bn, R(A.ph.rel_pres));
   c.b = ((double)ui - (double)ud) * A.g.dyp[jj] + ((double)vs - (double)vn) * A.g.dxp[ii];
   return c;
 }
