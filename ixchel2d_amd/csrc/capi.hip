@@ -1493,6 +1493,10 @@ struct ixh_batch {
   int n = 0, solver = 0;
   bool f32 = false;
   int ept[8] = {};
+  // one 64-thread workgroup per line (PCR, every line at most 320 nodes):
+  // four times as many lines resident per CU as with 256 threads
+  bool wave_lines = false;
+  int ept64[8] = {};
   int order = IXH_ORDER_XCD;
   bool graph = false;
   hipStream_t stream = nullptr;
@@ -1614,6 +1618,20 @@ template <int EQ, int DIR, int SOLVER, typename R>
 int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
   dim3 grid(b->nwg[ph]), block(kBlock);
   const int2 *wg = b->d_wg + b->wg_off[ph];
+  if constexpr (SOLVER == SOLVE_PCR) {
+    if (b->wave_lines) {
+      const dim3 wave(64);
+      switch (b->ept64[ph]) {
+        case 1: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 1, R, 64>), grid, wave, 0, st, D, wg); break;
+        case 2: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 2, R, 64>), grid, wave, 0, st, D, wg); break;
+        case 3: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 3, R, 64>), grid, wave, 0, st, D, wg); break;
+        case 4: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 4, R, 64>), grid, wave, 0, st, D, wg); break;
+        default: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 5, R, 64>), grid, wave, 0, st, D, wg); break;
+      }
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
+  }
   switch (b->ept[ph]) {
     case 1: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 1, R>), grid, block, 0, st, D, wg); break;
     case 2: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 2, R>), grid, block, 0, st, D, wg); break;
@@ -1655,10 +1673,16 @@ int batch_launch(ixh_batch *b, const BatchStep &E, hipStream_t st, int cap) {
   CHK((batch_sweep<EQ_T, DIR_Y>(b, PH_T_Y, sw, st)));
   CHK((batch_sweep<EQ_T, DIR_X>(b, PH_T_X, sw, st)));
   const int2 *wgr = b->d_wg + b->wg_off[PH_RES];
-  if (b->f32)
+  if (b->wave_lines) {
+    if (b->f32)
+      hipLaunchKernelGGL((k_residual_u_batch<float, 64>), dim3(b->nwg[PH_RES]), dim3(64), 0, st, re, wgr);
+    else
+      hipLaunchKernelGGL((k_residual_u_batch<double, 64>), dim3(b->nwg[PH_RES]), dim3(64), 0, st, re, wgr);
+  } else if (b->f32) {
     hipLaunchKernelGGL(k_residual_u_batch<float>, dim3(b->nwg[PH_RES]), dim3(kBlock), 0, st, re, wgr);
-  else
+  } else {
     hipLaunchKernelGGL(k_residual_u_batch<double>, dim3(b->nwg[PH_RES]), dim3(kBlock), 0, st, re, wgr);
+  }
   HIPCHK(hipGetLastError());
   hipLaunchKernelGGL(k_finalize_batch, dim3(n), dim3(kBlock), 0, st, (const FinArgs *)b->d_fin,
                      (const int *)b->d_slot, cap);
@@ -1765,11 +1789,15 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
   b->n = n;
   b->solver = solver;
   b->f32 = states[0]->f32;
+  int longest = 1;
   for (int ph = 0; ph < 8; ++ph) {
     int m = 1;
     for (ixh_state *s : b->st) m = std::max(m, line_len(s, kPhEq[ph], kPhDir[ph]));
     b->ept[ph] = ept_for(m);
+    b->ept64[ph] = (m + 63) / 64;
+    longest = std::max(longest, m);
   }
+  b->wave_lines = solver == IXH_SOLVER_PCR && longest <= 5 * 64;
   auto bail = [&](int code) {
     ixh_batch_destroy(b);
     return code;

@@ -656,10 +656,14 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
   __syncthreads();
 }
 
-template <int SOLVER, int EPT, typename T>
+// BS: threads of the line's workgroup (256, or 64 for one wave per line,
+// where EPT = ceil(n / 64) is also the wave solver's chunk length)
+template <int SOLVER, int EPT, typename T, int BS = kBlock>
 __device__ __forceinline__ void solve_line(T *a, T *b, T *c, T *r, int n) {
   if (SOLVER == SOLVE_THOMAS)
     thomas_lds(a, b, c, r, n);
+  else if (BS == 64)
+    wpt_solve<(EPT < 2 ? 2 : EPT)>(a, b, c, r, n);
   else if (EPT <= 4)
     wpt_solve<4 * EPT>(a, b, c, r, n);  // SOLVE_PCR and SOLVE_PCR_EXACT
   else
@@ -739,16 +743,17 @@ struct Layout {
 // the assembly loop, before any global store, so a workgroup waits for
 // memory once; rows past the line or at its ends assemble at a clamped
 // interior position and are then replaced.
-// LDS of one line workgroup, in doubles: the four line arrays and the
-// reduction scratch
-template <typename R, int EPT>
-constexpr int line_lds() { return (4 * EPT * kBlock * (int)sizeof(R)) / 8 + kBlock; }
+// LDS of one line workgroup of BS threads, in doubles: the four line arrays
+// and the reduction scratch
+template <typename R, int EPT, int BS = kBlock>
+constexpr int line_lds() { return (4 * EPT * BS * (int)sizeof(R)) / 8 + kBlock; }
 
-template <int EQ, int DIR, int SOLVER, int EPT, typename R>
+template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
 __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, double *lds) {
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR, "the FP32 sweeps use the PCR solver");
-  R *sa = reinterpret_cast<R *>(lds), *sb = sa + EPT * kBlock, *sc_ = sb + EPT * kBlock, *sr = sc_ + EPT * kBlock;
-  double *red = lds + (4 * EPT * kBlock * (int)sizeof(R)) / 8;
+  static_assert(BS == kBlock || BS == 64, "line workgroups of 256 threads or one wave");
+  R *sa = reinterpret_cast<R *>(lds), *sb = sa + EPT * BS, *sc_ = sb + EPT * BS, *sr = sc_ + EPT * BS;
+  double *red = lds + (4 * EPT * BS * (int)sizeof(R)) / 8;
   constexpr bool TR = DIR == DIR_Y;
   constexpr bool MOM = EQ == EQ_U || EQ == EQ_V;
   constexpr bool NORM = DIR == DIR_X && EQ != EQ_U;
@@ -781,7 +786,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
   double keep_old[EPT], keep_b[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int k = threadIdx.x + e * kBlock;
+    if (e * BS >= n) break;  // uniform: no row of this pass lies on the line
+    const int k = threadIdx.x + e * BS;
     const int pos = k + 1;                                    // 1-based position along the line
     const int pc = pos < 2 ? 2 : (pos > hi ? hi : pos);       // assembled position
     const int pr = pos > n ? n : pos;                         // position for the plain loads
@@ -871,7 +877,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
   // boundary faces in both sweeps), b_o (p' x-sweep, row layout)
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int k = threadIdx.x + e * kBlock;
+    const int k = threadIdx.x + e * BS;
     const int pos = k + 1;
     if (k >= n) continue;
     const int ii = DIR == DIR_X ? pos : line;
@@ -888,13 +894,16 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     }
   }
   __syncthreads();
-  solve_line<SOLVER, EPT>(sa, sb, sc_, sr, n);
+  solve_line<SOLVER, EPT, R, BS>(sa, sb, sc_, sr, n);
 
-  // write the line and the carried lines, accumulate the x-sweep norms
-  double acc0 = 0.0, acc1 = 0.0;
+  // write the line and the carried lines, accumulate the x-sweep norms.
+  // With one wave per line, row k is summed into the partial of the thread
+  // of a 256-thread workgroup that would own it (wave (k % 256) / 64), so the
+  // norms equal the 256-thread ones bit for bit.
+  double acc0[4] = {0.0, 0.0, 0.0, 0.0}, acc1[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int k = threadIdx.x + e * kBlock;
+    const int k = threadIdx.x + e * BS;
     if (k >= n) continue;
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
@@ -903,9 +912,10 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     AT(out, ii, jj) = x;
     AO(out2, ii, jj) = x;
     if (NORM && pos >= 2 && pos <= mi) {
+      const int w = BS == kBlock ? 0 : (k & 255) >> 6;
       double d = (double)x - keep_old[e];
-      acc0 = acc0 + d * d;
-      if (EQ == EQ_P) acc1 = acc1 + keep_b[e] * keep_b[e];
+      acc0[w] = acc0[w] + d * d;
+      if (EQ == EQ_P) acc1[w] = acc1[w] + keep_b[e] * keep_b[e];
     }
     if (other) {
       const int io = DIR == DIR_X ? pos : other;
@@ -915,11 +925,15 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     }
   }
   if (NORM) {
-    double s0 = block_sum(acc0, red);
-    if (EQ == EQ_P) {
-      double s1 = block_sum(acc1, red);
-      if (threadIdx.x == 0) A.part1[line] = s1;
+    double s0, s1 = 0.0;
+    if (BS == kBlock) {
+      s0 = block_sum(acc0[0], red);
+      if (EQ == EQ_P) s1 = block_sum(acc1[0], red);
+    } else {
+      s0 = (wave_sum(acc0[0]) + wave_sum(acc0[1])) + (wave_sum(acc0[2]) + wave_sum(acc0[3]));
+      if (EQ == EQ_P) s1 = (wave_sum(acc1[0]) + wave_sum(acc1[1])) + (wave_sum(acc1[2]) + wave_sum(acc1[3]));
     }
+    if (EQ == EQ_P && threadIdx.x == 0) A.part1[line] = s1;
     if (threadIdx.x == 0) A.part0[line] = s0;
   }
 }
@@ -933,13 +947,13 @@ __global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
 // A batch of cavities: workgroup b runs line wg[b].y of cavity wg[b].x, whose
 // arguments are D[wg[b].x] (same kernel body as the single-cavity sweep).
 // Entries with wg[b].x < 0 pad a table to a multiple of the XCD count.
-template <int EQ, int DIR, int SOLVER, int EPT, typename R>
-__global__ void __launch_bounds__(kBlock) k_sweep_batch(const SweepArgs *__restrict__ D,
-                                                        const int2 *__restrict__ wg) {
-  __shared__ double lds[line_lds<R, EPT>()];
+template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_sweep_batch(const SweepArgs *__restrict__ D,
+                                                    const int2 *__restrict__ wg) {
+  __shared__ double lds[line_lds<R, EPT, BS>()];
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
-  sweep_line<EQ, DIR, SOLVER, EPT, R>(D[w.x], w.y, lds);
+  sweep_line<EQ, DIR, SOLVER, EPT, R, BS>(D[w.x], w.y, lds);
 }
 
 // ------------------------------------------------------------- correction
@@ -1044,7 +1058,10 @@ struct ResArgs {
   double *resu, *part;
 };
 
-template <typename R>
+// BS: threads per row (256, or 64 for one wave per row; the wave then sums
+// row positions into the partials of the 256-thread owners so the norm is
+// the same bit for bit)
+template <typename R, int BS = kBlock>
 __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, double *red) {
   constexpr bool TR = false;
   const int P = A.P;
@@ -1053,12 +1070,12 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, dou
   const Geo &g = A.g;
   const double gm = A.ph.gm;
   // two rows per thread per pass: all loads before the stores
-  double acc = 0.0;
-  for (int base = 2; base <= A.mi - 1; base += 2 * kBlock) {
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int base = 2; base <= A.mi - 1; base += 2 * BS) {
     double rv[2];
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const int ii = min(base + (int)threadIdx.x + m * kBlock, A.mi - 1);
+      const int ii = min(base + (int)threadIdx.x + m * BS, A.mi - 1);
       double ud = g.fexu[ii] * (double)AT(u, ii + 1, jj) + (1.0 - g.fexu[ii]) * (double)AT(u, ii, jj);
       double ui = g.fexu[ii - 1] * (double)AT(u, ii, jj) + (1.0 - g.fexu[ii - 1]) * (double)AT(u, ii - 1, jj);
       double vn = g.fexp[ii] * (double)AT(v, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj);
@@ -1091,13 +1108,18 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, dou
     }
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
-      const int ii = base + (int)threadIdx.x + m * kBlock;
+      const int ii = base + (int)threadIdx.x + m * BS;
       if (ii > A.mi - 1) continue;
       AT(A.resu, ii, jj) = rv[m];
-      acc = acc + rv[m] * rv[m];
+      const int w = BS == kBlock ? 0 : ((ii - 2) & 255) >> 6;
+      acc[w] = acc[w] + rv[m] * rv[m];
     }
   }
-  double s = block_sum(acc, red);
+  double s;
+  if (BS == kBlock)
+    s = block_sum(acc[0], red);
+  else
+    s = (wave_sum(acc[0]) + wave_sum(acc[1])) + (wave_sum(acc[2]) + wave_sum(acc[3]));
   if (threadIdx.x == 0) A.part[jj] = s;
 }
 
@@ -1107,13 +1129,13 @@ __global__ void __launch_bounds__(kBlock) k_residual_u(ResArgs A) {
   residual_row<R>(A, 2 + xcd_line((int)blockIdx.x, A.nj - 1), red);
 }
 
-template <typename R>
-__global__ void __launch_bounds__(kBlock) k_residual_u_batch(const ResArgs *__restrict__ D,
-                                                             const int2 *__restrict__ wg) {
+template <typename R, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_residual_u_batch(const ResArgs *__restrict__ D,
+                                                         const int2 *__restrict__ wg) {
   __shared__ double red[kBlock];
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
-  residual_row<R>(D[w.x], w.y, red);
+  residual_row<R, BS>(D[w.x], w.y, red);
 }
 
 // ---------------------------------------------------------------- finalize
