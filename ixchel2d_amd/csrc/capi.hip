@@ -216,6 +216,13 @@ struct Timed {
   }
 };
 
+// A fused launch of two PCR sweeps whose lines have at most this many nodes
+// runs one 64-thread wave per line: at 302-node x-lines the T_x + u_x launch
+// takes 17.6 us that way against 22.8 us with 256 threads per line. A single
+// sweep of about 700 such lines is faster with 256 threads per line
+// (13.4 us against 16.6 us for v_x), so single sweeps keep that shape.
+constexpr int kWaveLine = 5 * 64;
+
 template <int EQ, int DIR, int SOLVER, typename R>
 int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
   dim3 grid(A.nlines), block(kBlock);
@@ -642,8 +649,23 @@ int fast_step(ixh_state *s) {
 // ---- fused step: 7 launches on the state's stream per time step
 // k_multi with sweeps (EQ0, DIR0) and (EQ1, DIR1) at a common EPT
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
-int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept) {
+int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn) {
   dim3 grid(nwg), block(kBlock);
+  // sweeps only, every line short: one wave per line
+  if constexpr (SOLVER == SOLVE_PCR) {
+    if (maxn <= kWaveLine && M.cnt[2] == 0 && M.cnt[3] == 0) {
+      const dim3 wave(64);
+      switch ((maxn + 63) / 64) {
+        case 1: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 1, R, 64>), grid, wave, M); break;
+        case 2: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R, 64>), grid, wave, M); break;
+        case 3: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 3, R, 64>), grid, wave, M); break;
+        case 4: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R, 64>), grid, wave, M); break;
+        default: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 5, R, 64>), grid, wave, M); break;
+      }
+      HIPCHK(hipGetLastError());
+      return 0;
+    }
+  }
   switch (ept) {
     case 1: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 1, R>), grid, block, M); break;
     case 2: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R>), grid, block, M); break;
@@ -657,13 +679,13 @@ int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept) {
 }
 
 template <int EQ0, int DIR0, int EQ1, int DIR1>
-int launch_multi(ixh_state *s, const MultiArgs &M, int nwg, int ept) {
-  if (s->f32) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, float>(s, M, nwg, ept);
+int launch_multi(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn) {
+  if (s->f32) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, float>(s, M, nwg, ept, maxn);
   if (s->prm.solver == IXH_SOLVER_THOMAS)
-    return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_THOMAS, double>(s, M, nwg, ept);
+    return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_THOMAS, double>(s, M, nwg, ept, maxn);
   if (s->prm.solver == IXH_SOLVER_PCR_EXACT)
-    return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR_EXACT, double>(s, M, nwg, ept);
-  return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, double>(s, M, nwg, ept);
+    return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR_EXACT, double>(s, M, nwg, ept, maxn);
+  return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, double>(s, M, nwg, ept, maxn);
 }
 
 int round8(int x) { return (x + 7) & ~7; }
@@ -739,7 +761,7 @@ int build_step(ixh_state *s, bool first, bool last, StepLaunches &L) {
 // launch part k (1..7) of a built step
 int launch_part(ixh_state *s, const StepLaunches &L, int k) {
   switch (k) {
-    case 1: return launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, L.m1, L.nwg1, L.ept1);
+    case 1: return launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, L.m1, L.nwg1, L.ept1, line_len(s, EQ_V, IXH_DIR_Y));
     case 2: return launch_sweep<EQ_V, DIR_X>(s, L.vx, line_len(s, EQ_V, IXH_DIR_X));
     case 3: return launch_sweep<EQ_P, DIR_Y>(s, L.py, line_len(s, EQ_P, IXH_DIR_Y));
     case 4: return launch_sweep<EQ_P, DIR_X>(s, L.px, line_len(s, EQ_P, IXH_DIR_X));
@@ -753,11 +775,11 @@ int launch_part(ixh_state *s, const StepLaunches &L, int k) {
       return 0;
     }
     case 6:
-      if (L.last) return launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, L.m6, L.nwg6, L.ept6);
-      return launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, L.m6, L.nwg6, L.ept6);
+      if (L.last) return launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, L.m6, L.nwg6, L.ept6, s->nj + 1);
+      return launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, L.m6, L.nwg6, L.ept6, s->nj + 1);
     case 7:
-      if (L.last) return launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, L.m7, L.nwg7, L.ept7);
-      return launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, L.m7, L.nwg7, L.ept7);
+      if (L.last) return launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, L.m7, L.nwg7, L.ept7, s->mi + 1);
+      return launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, L.m7, L.nwg7, L.ept7, s->mi + 1);
   }
   return fail(IXH_ERR_ARG, "launch part %d", k);
 }

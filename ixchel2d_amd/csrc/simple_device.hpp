@@ -938,10 +938,10 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
   }
 }
 
-template <int EQ, int DIR, int SOLVER, int EPT, typename R>
-__global__ void __launch_bounds__(kBlock) k_sweep(SweepArgs A) {
-  __shared__ double lds[line_lds<R, EPT>()];
-  sweep_line<EQ, DIR, SOLVER, EPT, R>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines), lds);
+template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_sweep(SweepArgs A) {
+  __shared__ double lds[line_lds<R, EPT, BS>()];
+  sweep_line<EQ, DIR, SOLVER, EPT, R, BS>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines), lds);
 }
 
 // A batch of cavities: workgroup b runs line wg[b].y of cavity wg[b].x, whose
@@ -1220,24 +1220,28 @@ struct MultiArgs {
   int off[4], cnt[4];
 };
 
-template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, int EPT, typename R>
-__global__ void __launch_bounds__(kBlock) k_multi(MultiArgs M) {
-  __shared__ double lds[line_lds<R, EPT>()];
+// BS = 64: one wave per line, for launches of sweeps only (no residual, no fold)
+template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, int EPT, typename R, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_multi(MultiArgs M) {
+  __shared__ double lds[line_lds<R, EPT, BS>()];
   const int b = (int)blockIdx.x;
   if (b < M.off[1]) {
     const int w = b - M.off[0];
-    if (w < M.cnt[0]) sweep_line<EQ0, DIR0, SOLVER, EPT, R>(M.s[0], M.s[0].line0 + xcd_line(w, M.s[0].nlines), lds);
+    if (w < M.cnt[0])
+      sweep_line<EQ0, DIR0, SOLVER, EPT, R, BS>(M.s[0], M.s[0].line0 + xcd_line(w, M.s[0].nlines), lds);
   } else if (b < M.off[2]) {
     if constexpr (EQ1 >= 0) {
       const int w = b - M.off[1];
       if (w < M.cnt[1])
-        sweep_line<EQ1, DIR1, SOLVER, EPT, R>(M.s[1], M.s[1].line0 + xcd_line(w, M.s[1].nlines), lds);
+        sweep_line<EQ1, DIR1, SOLVER, EPT, R, BS>(M.s[1], M.s[1].line0 + xcd_line(w, M.s[1].nlines), lds);
     }
-  } else if (b < M.off[3]) {
-    const int w = b - M.off[2];
-    if (w < M.cnt[2]) residual_row<R>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
-  } else {
-    if (b == M.off[3] && M.cnt[3]) finalize_body(M.f, lds);
+  } else if constexpr (BS == kBlock) {
+    if (b < M.off[3]) {
+      const int w = b - M.off[2];
+      if (w < M.cnt[2]) residual_row<R>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
+    } else {
+      if (b == M.off[3] && M.cnt[3]) finalize_body(M.f, lds);
+    }
   }
 }
 
