@@ -4,15 +4,18 @@ Workload: the 301x701 differentially heated tall cavity (tanh beta = 1.2
 mesh, H/W = 2.33, Ra = 1e5, Pr = 0.71, dt = 0.003), all fields resident in
 HBM.  A step is one SIMPLE iteration (one time step with single-pass inner
 loops): the u, v, p', T line sweeps, the correction, the residual and the
-norms.  The same workload also runs on the FP32-mixed path (sweeps and
-assembly in FP32; mass residual, p' right-hand side, reductions, p and the
-Nusselt integral in FP64), reported under "fp32_mixed" with its speed-up and
-its error against FP64.  Prints one JSON line.
+norms.  The same workload also runs on the two FP32 paths, each reported
+with its speed-up and its error against FP64:
+  - "fp32_mixed": u, v, T, p' stored and swept in FP32; mass residual, p'
+    right-hand side, reductions, p and the Nusselt integral in FP64;
+  - "fp32_refined": every field in FP64; the sweeps assemble in FP32 and
+    solve for FP32 increments against each line's FP64 residual.
+Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver pcr|thomas]
   python bench.py --workload batch256   # 256 mixed-size cavities, one launch per phase
   python bench.py --workload adjoint129 # hot-wall Nu gradient through the adjoint
-  python bench.py --workload fp32_convergence  # FP64 vs FP32-mixed toward the steady state
+  python bench.py --workload fp32_convergence  # FP64 vs both FP32 paths toward the steady state
 """
 from __future__ import annotations
 
@@ -32,9 +35,9 @@ HBM_PEAK_GBS = 8000.0
 
 # operands each kernel must read or write once per launch, as (FP64 arrays,
 # arrays of the path's real type); every array holds (mi+1)(nj+1) values.  T
-# is read by the v sweeps only with buoyancy.  In the FP64 path both columns
-# are 8-byte; in the FP32-mixed path the second column is 4-byte (u, v, T,
-# p', a_u, a_v) and p, b_o, Res_u stay FP64.
+# is read by the v sweeps only with buoyancy.  In the FP64 and FP32-refined
+# paths both columns are 8-byte; in the FP32-mixed path the second column is
+# 4-byte (u, v, T, p', a_u, a_v) and p, b_o, Res_u stay FP64.
 ALG_OPERANDS = {
     "sweep_u_y": (1, 4),   # u, u_ant, v, p -> u
     "sweep_u_x": (1, 5),   # u, u_ant, v, p -> u, a_u
@@ -296,34 +299,40 @@ LAUNCH_PARTS = {
 
 
 def convergence_study(args):
-    """FP64 and FP32-mixed runs of the 301x701 case side by side toward the
-    steady state: residual, maxbo, hot-wall Nu and its relative difference,
-    and the max-norm differences of T and u, every --conv-chunk iterations."""
+    """FP64 and both FP32 paths on the 301x701 case side by side toward the
+    steady state: residual, maxbo, hot-wall Nu and its relative difference
+    from FP64, and the max-norm differences of T and u, every --conv-chunk
+    iterations."""
     import numpy as np
     import torch
     from ixchel2d_amd.simple import Cavity
     case = build_case()
-    g64 = Cavity(case, solver="pcr", precision="fp64")
-    g32 = Cavity(case, solver="pcr", precision="fp32_mixed")
+    paths = ("fp64", "fp32_mixed", "fp32_refined")
+    cav = {p: Cavity(case, solver="pcr", precision=p) for p in paths}
     rows = []
     total = 0
     for _ in range(args.conv_chunks):
-        g64.run(args.conv_chunk - 1, cap=1)
-        g32.run(args.conv_chunk - 1, cap=1)
-        h64, h32 = g64.run(1), g32.run(1)
+        hist = {}
+        for p in paths:
+            cav[p].run(args.conv_chunk - 1, cap=1)
+            hist[p] = cav[p].run(1)
         total += args.conv_chunk
-        n64, n32 = g64.nusselt_x()[0], g32.nusselt_x()[0]
-        t64, t32, u64, u32 = g64.get("t"), g32.get("t"), g64.get("u"), g32.get("u")
-        row = {"iterations": total, "residuo_fp64": float(h64["residuo"][-1]),
-               "residuo_fp32_mixed": float(h32["residuo"][-1]), "maxbo_fp64": float(h64["maxbo"][-1]),
-               "maxbo_fp32_mixed": float(h32["maxbo"][-1]), "nu_fp64": n64, "nu_fp32_mixed": n32,
-               "nu_rel_err": abs(n32 - n64) / abs(n64), "t_linf": float(np.max(np.abs(t32 - t64))),
-               "u_linf_rel": float(np.max(np.abs(u32 - u64)) / np.max(np.abs(u64)))}
+        nu = {p: cav[p].nusselt_x()[0] for p in paths}
+        t64, u64 = cav["fp64"].get("t"), cav["fp64"].get("u")
+        row = {"iterations": total}
+        for p in paths:
+            row[f"residuo_{p}"] = float(hist[p]["residuo"][-1])
+            row[f"maxbo_{p}"] = float(hist[p]["maxbo"][-1])
+            row[f"nu_{p}"] = nu[p]
+        for p in paths[1:]:
+            row[f"nu_rel_err_{p}"] = abs(nu[p] - nu["fp64"]) / abs(nu["fp64"])
+            row[f"t_linf_{p}"] = float(np.max(np.abs(cav[p].get("t") - t64)))
+            row[f"u_linf_rel_{p}"] = float(np.max(np.abs(cav[p].get("u") - u64)) / np.max(np.abs(u64)))
         rows.append(row)
         print(json.dumps(row), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    print(json.dumps({"workload": "natural-convection tall cavity 301x701, FP64 and FP32-mixed toward the steady "
-                                  "state", "chunk": args.conv_chunk, "rows": rows}))
+    print(json.dumps({"workload": "natural-convection tall cavity 301x701, FP64, FP32-mixed and FP32-refined "
+                                  "toward the steady state", "chunk": args.conv_chunk, "rows": rows}))
 
 
 def time_path(case, solver, precision, steps, warmup):
@@ -383,22 +392,22 @@ def time_path(case, solver, precision, steps, warmup):
             "ms_per_step_with_events": round(ms_events, 5), "roofline": roofline, "launches": launches}
 
 
-def accuracy(case, solver, steps):
-    """FP32-mixed against FP64 from the same start after `steps` iterations:
-    relative error of the hot-wall mean Nusselt number and max-norm errors
-    of u, v, T (relative to the FP64 field's max)."""
+def accuracy(case, solver, steps, precision):
+    """One FP32 path against FP64 from the same start after `steps`
+    iterations: relative error of the hot-wall mean Nusselt number and
+    max-norm errors of u, v, T (relative to the FP64 field's max)."""
     from ixchel2d_amd.simple import Cavity
     import numpy as np
     res = {}
-    for prec in ("fp64", "fp32_mixed"):
+    for prec in ("fp64", precision):
         g = Cavity(case, solver=solver, precision=prec)
         h = g.run(steps)
         res[prec] = ({f: g.get(f) for f in "uvt"}, g.nusselt_x()[0], float(h["residuo"][-1]))
         g.close()
     f64, nu64, r64 = res["fp64"]
-    f32, nu32, r32 = res["fp32_mixed"]
-    out = {"iterations": steps, "nu_hot_fp64": nu64, "nu_hot_fp32_mixed": nu32,
-           "nu_rel_err": abs(nu32 - nu64) / abs(nu64), "residuo_fp64": r64, "residuo_fp32_mixed": r32}
+    f32, nu32, r32 = res[precision]
+    out = {"iterations": steps, "nu_hot_fp64": nu64, f"nu_hot_{precision}": nu32,
+           "nu_rel_err": abs(nu32 - nu64) / abs(nu64), "residuo_fp64": r64, f"residuo_{precision}": r32}
     for f in "uvt":
         out[f"linf_rel_err_{f}"] = float(np.max(np.abs(f32[f] - f64[f])) / np.max(np.abs(f64[f])))
     return out
@@ -419,11 +428,11 @@ def main():
     ap.add_argument("--adj-reps", type=int, default=5, help="timed gradient evaluations")
     ap.add_argument("--batch", type=int, default=256, help="cavities of the batch workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed"],
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed", "fp32_refined"],
                     help="path of the batch workload")
-    ap.add_argument("--fp64-only", action="store_true", help="skip the FP32-mixed line of the 301x701 workload")
+    ap.add_argument("--fp64-only", action="store_true", help="skip the FP32 lines of the 301x701 workload")
     ap.add_argument("--accuracy-steps", type=int, default=5000,
-                    help="iterations of the FP32-mixed vs FP64 accuracy comparison")
+                    help="iterations of the FP32 vs FP64 accuracy comparisons")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MB (reference kernel for "
                          "the FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -458,11 +467,13 @@ def main():
     m = case.mesh
     cells = m.mi * m.nj
     r64 = time_path(case, args.solver, "fp64", args.steps, args.warmup)
-    r32 = None
+    reduced = {"fp32_mixed": None, "fp32_refined": None}
     if args.solver == "pcr" and not args.fp64_only:
-        r32 = time_path(build_case(), "pcr", "fp32_mixed", args.steps, args.warmup)
-        r32["speedup_vs_fp64"] = round(r32["value"] / r64["value"], 3)
-        r32["accuracy"] = accuracy(build_case(), "pcr", args.accuracy_steps)
+        for prec in reduced:
+            r = time_path(build_case(), "pcr", prec, args.steps, args.warmup)
+            r["speedup_vs_fp64"] = round(r["value"] / r64["value"], 3)
+            r["accuracy"] = accuracy(build_case(), "pcr", args.accuracy_steps, prec)
+            reduced[prec] = r
     cpu = None if args.no_cpu else cpu_baseline(build_case())
     out = {
         "metric": "Mcell-SIMPLE-iters/s @301x701",
@@ -483,7 +494,8 @@ def main():
         "roofline": r64["roofline"],
         "cpu_baseline": cpu,
         "launches": r64["launches"],
-        "fp32_mixed": r32,
+        "fp32_mixed": reduced["fp32_mixed"],
+        "fp32_refined": reduced["fp32_refined"],
     }
     print(json.dumps(out))
 

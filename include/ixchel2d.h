@@ -57,8 +57,15 @@ enum {
   IXH_SOLVER_THOMAS = 1,   /* upstream's Thomas and assembly order: bit-faithful */
   IXH_SOLVER_PCR_EXACT = 2 /* PCR with upstream-order assembly */
 };
-/* arithmetic of the sweeps */
-enum { IXH_FP64 = 0, IXH_FP32_MIXED = 1 };
+/* arithmetic of the sweeps.
+ * FP64: every field and sweep in FP64 (the drop-in).
+ * FP32_MIXED: u, v, T, p' stored and swept in FP32; p, mass residual, norms
+ *   and Nusselt integrals in FP64.  Stalls about 6e-8 relative above the
+ *   FP64 steady state.
+ * FP32_REFINED: every field stored in FP64; each sweep assembles its rows in
+ *   FP32 and solves for the FP32 increment of the line against the line's
+ *   FP64 residual, so the iteration converges to the FP64 steady state. */
+enum { IXH_FP64 = 0, IXH_FP32_MIXED = 1, IXH_FP32_REFINED = 2 };
 
 typedef struct ixh_state ixh_state;
 
@@ -77,7 +84,7 @@ typedef struct {
   double conv_u, conv_t, conv_p, conv_resi, conv_paso;
   int32_t iter_ecuaci_max, iter_simple_max;
   int32_t solver;                  /* IXH_SOLVER_* */
-  int32_t precision;               /* IXH_FP64 | IXH_FP32_MIXED */
+  int32_t precision;               /* IXH_FP64 | IXH_FP32_MIXED | IXH_FP32_REFINED */
 } ixh_params;
 
 /* one record per SIMPLE iteration: sqrt of the squared-difference sums of

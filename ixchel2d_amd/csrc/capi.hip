@@ -83,6 +83,9 @@ struct ixh_state {
   // FP32-mixed: u, v, T, p' (all copies) and a_u, a_v are stored as float in
   // the same buffers (same element pitch); p, b_o, Res_u stay FP64
   bool f32 = false;
+  // FP32-refined: every field stays FP64; the sweeps assemble in FP32 and
+  // solve for FP32 increments against an FP64 residual
+  bool delta = false;
   // geometry (one allocation, 1-based arrays)
   double *d_geo = nullptr;
   Geo geo{};
@@ -175,10 +178,12 @@ int check_params(const ixh_params *p) {
     return fail(IXH_ERR_ARG, "diffusivities must be positive");
   if (p->solver != IXH_SOLVER_PCR && p->solver != IXH_SOLVER_THOMAS && p->solver != IXH_SOLVER_PCR_EXACT)
     return fail(IXH_ERR_ARG, "unknown solver %d", p->solver);
-  if (p->precision != IXH_FP64 && p->precision != IXH_FP32_MIXED)
+  if (p->precision != IXH_FP64 && p->precision != IXH_FP32_MIXED && p->precision != IXH_FP32_REFINED)
     return fail(IXH_ERR_ARG, "unknown precision %d", p->precision);
   if (p->precision == IXH_FP32_MIXED && p->solver != IXH_SOLVER_PCR)
     return fail(IXH_ERR_UNSUPP, "the FP32-mixed path uses the PCR solver only");
+  if (p->precision == IXH_FP32_REFINED && p->solver != IXH_SOLVER_PCR)
+    return fail(IXH_ERR_UNSUPP, "the FP32-refined path uses the PCR solver only");
   return 0;
 }
 
@@ -241,6 +246,7 @@ int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
 template <int EQ, int DIR>
 int launch_sweep(ixh_state *s, const SweepArgs &A, int n) {
   if (s->f32) return launch_sweep_t<EQ, DIR, SOLVE_PCR, float>(s, A, n);
+  if (s->delta) return launch_sweep_t<EQ, DIR, SOLVE_DELTA, float>(s, A, n);
   if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(s, A, n);
   if (s->prm.solver == IXH_SOLVER_PCR_EXACT) return launch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT, double>(s, A, n);
   return launch_sweep_t<EQ, DIR, SOLVE_PCR, double>(s, A, n);
@@ -652,7 +658,7 @@ template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
 int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn) {
   dim3 grid(nwg), block(kBlock);
   // sweeps only, every line short: one wave per line
-  if constexpr (SOLVER == SOLVE_PCR) {
+  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
     if (maxn <= kWaveLine && M.cnt[2] == 0 && M.cnt[3] == 0) {
       const dim3 wave(64);
       switch ((maxn + 63) / 64) {
@@ -681,6 +687,7 @@ int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn)
 template <int EQ0, int DIR0, int EQ1, int DIR1>
 int launch_multi(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn) {
   if (s->f32) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, float>(s, M, nwg, ept, maxn);
+  if (s->delta) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_DELTA, float>(s, M, nwg, ept, maxn);
   if (s->prm.solver == IXH_SOLVER_THOMAS)
     return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_THOMAS, double>(s, M, nwg, ept, maxn);
   if (s->prm.solver == IXH_SOLVER_PCR_EXACT)
@@ -861,6 +868,7 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   s->fsize = std::max((size_t)s->P * (nj + 1), (size_t)s->Q * (mi + 1));
   s->prm = *params;
   s->f32 = params->precision == IXH_FP32_MIXED;
+  s->delta = params->precision == IXH_FP32_REFINED;
   int rc = 0;
   auto bail = [&](int code) {
     ixh_destroy(s);
@@ -1513,7 +1521,7 @@ struct BatchStep {
 struct ixh_batch {
   std::vector<ixh_state *> st;
   int n = 0, solver = 0;
-  bool f32 = false;
+  bool f32 = false, delta = false;
   int ept[8] = {};
   // one 64-thread workgroup per line (PCR, every line at most 320 nodes):
   // four times as many lines resident per CU as with 256 threads
@@ -1640,7 +1648,7 @@ template <int EQ, int DIR, int SOLVER, typename R>
 int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
   dim3 grid(b->nwg[ph]), block(kBlock);
   const int2 *wg = b->d_wg + b->wg_off[ph];
-  if constexpr (SOLVER == SOLVE_PCR) {
+  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
     if (b->wave_lines) {
       const dim3 wave(64);
       switch (b->ept64[ph]) {
@@ -1669,6 +1677,7 @@ int batch_sweep(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
   if (b->nwg[ph] == 0) return 0;
   D += (size_t)ph * b->n;
   if (b->f32) return batch_sweep_t<EQ, DIR, SOLVE_PCR, float>(b, ph, D, st);
+  if (b->delta) return batch_sweep_t<EQ, DIR, SOLVE_DELTA, float>(b, ph, D, st);
   if (b->solver == IXH_SOLVER_THOMAS) return batch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(b, ph, D, st);
   if (b->solver == IXH_SOLVER_PCR_EXACT) return batch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT, double>(b, ph, D, st);
   return batch_sweep_t<EQ, DIR, SOLVE_PCR, double>(b, ph, D, st);
@@ -1802,7 +1811,8 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
     for (int d = 0; d < c; ++d)
       if (states[d] == s) return fail(IXH_ERR_ARG, "batch: state %d appears twice", c);
     if (s->prm.solver != solver) return fail(IXH_ERR_ARG, "batch: every cavity must use the same solver");
-    if (s->f32 != states[0]->f32) return fail(IXH_ERR_ARG, "batch: every cavity must use the same precision");
+    if (s->prm.precision != states[0]->prm.precision)
+      return fail(IXH_ERR_ARG, "batch: every cavity must use the same precision");
     if (s->mi + 1 > 4 * kBlock || s->nj + 1 > 4 * kBlock)
       return fail(IXH_ERR_UNSUPP, "batch: lines longer than %d nodes are not built (cavity %d)", 4 * kBlock, c);
   }
@@ -1811,6 +1821,7 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
   b->n = n;
   b->solver = solver;
   b->f32 = states[0]->f32;
+  b->delta = states[0]->delta;
   int longest = 1;
   for (int ph = 0; ph < 8; ++ph) {
     int m = 1;
@@ -2027,7 +2038,7 @@ extern "C" {
 
 int ixh_adjoint_record(ixh_state *s, int32_t nsteps, int32_t every) {
   if (!s || nsteps < 1 || every < 1) return fail(IXH_ERR_ARG, "adjoint: bad arguments");
-  if (s->f32) return fail(IXH_ERR_UNSUPP, "adjoint: the FP64 path only");
+  if (s->f32 || s->delta) return fail(IXH_ERR_UNSUPP, "adjoint: the FP64 path only");
   if (s->prm.iter_ecuaci_max >= 0 || s->prm.iter_simple_max >= 0)
     return fail(IXH_ERR_UNSUPP, "adjoint: single-pass loops only (iter_ecuaci_max < 0, iter_simple_max < 0)");
   for (int v = 0; v < 3; ++v) CHK(check_bcs(s, v));

@@ -28,8 +28,20 @@ enum Eq { EQ_U = 0, EQ_V = 1, EQ_P = 2, EQ_T = 3 };
 enum Dir { DIR_Y = 0, DIR_X = 1 };
 // PCR: LDS cyclic reduction with the division-free assembly below (default);
 // THOMAS: upstream's Thomas order with upstream-order assembly (bit-faithful);
-// PCR_EXACT: PCR with upstream-order assembly (isolates the solver rounding)
-enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1, SOLVE_PCR_EXACT = 2 };
+// PCR_EXACT: PCR with upstream-order assembly (isolates the solver rounding);
+// DELTA: the FP32-refined sweeps -- FP64 fields, the line's FP64 residual
+// against FP32 coefficients, and the FP32 PCR solve of the increment
+enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1, SOLVE_PCR_EXACT = 2, SOLVE_DELTA = 3 };
+
+// storage type of the solved fields for a sweep of real type R
+template <int SOLVER, typename R>
+struct StoreOf {
+  using T = R;
+};
+template <typename R>
+struct StoreOf<SOLVE_DELTA, R> {
+  using T = double;
+};
 
 // 1-based geometry arrays (index 0 unused), built as in
 // source/mod_malla.f90:167-217; the FP32-mixed sweeps read FP32 copies
@@ -271,16 +283,17 @@ struct PCoefT {
   R ai, ad, bs, bn, ac;
   double b;
 };
-template <bool TR, typename R>
+// S: stored type of u, v, a_u, a_v (double in the FP32-refined sweeps)
+template <bool TR, typename R, typename S = R>
 __device__ __forceinline__ PCoefT<R> p_coef(const SweepArgs &A, int ii, int jj) {
   const int P = A.P;
   const GeoT<R> &g = geo_of<R>(A);
-  const R *u = fld<R>(A.u), *v = fld<R>(A.v), *au = fld<R>(A.au), *av = fld<R>(A.av);
+  const S *u = fld<S>(A.u), *v = fld<S>(A.v), *au = fld<S>(A.au), *av = fld<S>(A.av);
   PCoefT<R> c;
-  R ud = AT(u, ii, jj), ui = AT(u, ii - 1, jj);
-  R vn = AT(v, ii, jj), vs = AT(v, ii, jj - 1);
-  R ai = AT(au, ii - 1, jj), ad = AT(au, ii, jj);
-  R as = AT(av, ii, jj - 1), an = AT(av, ii, jj);
+  const S ud = AT(u, ii, jj), ui = AT(u, ii - 1, jj);
+  const S vn = AT(v, ii, jj), vs = AT(v, ii, jj - 1);
+  const R ai = R(AT(au, ii - 1, jj)), ad = R(AT(au, ii, jj));
+  const R as = R(AT(av, ii, jj - 1)), an = R(AT(av, ii, jj));
   R dyp = g.dyp[jj], dxp = g.dxp[ii];
   c.ai = adiv(-dyp * dyp, ai);
   c.ad = adiv(-dyp * dyp, ad);
@@ -348,36 +361,40 @@ __device__ __forceinline__ R nbf(R cond, R len, R pec, R w, bool east) {
 template <typename R>
 struct CoefT {
   R ai, ad, bs, bn, ac;
+  R ap, td;  // a_P before under-relaxation without the time term, and area / dt
 };
 
-// `pin` (FP32 only) receives the value of a penalised row, NaN otherwise
-template <int EQ, bool TR, typename R>
+// `pin` (FP32 only) receives the value of a penalised row, NaN otherwise.
+// S is the stored type of the velocity fields (double in the FP32-refined
+// sweeps, whose coefficients are still formed in R).
+template <int EQ, bool TR, typename R, typename S = R>
 __device__ __forceinline__ CoefT<R> fast_coef(const SweepArgs &A, int ii, int jj, R &area, double &pin) {
   const int P = A.P;
   const GeoT<R> &g = geo_of<R>(A);
   const FastGeoT<R> &F = fast_of<R>(A);
-  const R *u = fld<R>(A.u), *v = fld<R>(A.v);
+  const S *u = fld<S>(A.u), *v = fld<S>(A.v);
+  auto ld = [&](const S *f, int i, int j) { return R(AT(f, i, j)); };
   constexpr int e = EQ == EQ_U ? 0 : EQ == EQ_V ? 1 : 2;
   R ui, ud, vs, vn, lx, ly;
   if (EQ == EQ_U) {
-    ud = g.fexu[ii] * AT(u, ii + 1, jj) + (R(1.0) - g.fexu[ii]) * AT(u, ii, jj);
-    ui = g.fexu[ii - 1] * AT(u, ii, jj) + (R(1.0) - g.fexu[ii - 1]) * AT(u, ii - 1, jj);
-    vn = g.fexp[ii] * AT(v, ii + 1, jj) + (R(1.0) - g.fexp[ii]) * AT(v, ii, jj);
-    vs = g.fexp[ii] * AT(v, ii + 1, jj - 1) + (R(1.0) - g.fexp[ii]) * AT(v, ii, jj - 1);
+    ud = g.fexu[ii] * ld(u, ii + 1, jj) + (R(1.0) - g.fexu[ii]) * ld(u, ii, jj);
+    ui = g.fexu[ii - 1] * ld(u, ii, jj) + (R(1.0) - g.fexu[ii - 1]) * ld(u, ii - 1, jj);
+    vn = g.fexp[ii] * ld(v, ii + 1, jj) + (R(1.0) - g.fexp[ii]) * ld(v, ii, jj);
+    vs = g.fexp[ii] * ld(v, ii + 1, jj - 1) + (R(1.0) - g.fexp[ii]) * ld(v, ii, jj - 1);
     lx = g.dxu[ii];
     ly = g.dyu[jj];
   } else if (EQ == EQ_V) {
-    ud = g.feyp[jj] * AT(u, ii, jj + 1) + (R(1.0) - g.feyp[jj]) * AT(u, ii, jj);
-    ui = g.feyp[jj] * AT(u, ii - 1, jj + 1) + (R(1.0) - g.feyp[jj]) * AT(u, ii - 1, jj);
-    vn = g.feyv[jj] * AT(v, ii, jj + 1) + (R(1.0) - g.feyv[jj]) * AT(v, ii, jj);
-    vs = g.feyv[jj - 1] * AT(v, ii, jj) + (R(1.0) - g.feyv[jj - 1]) * AT(v, ii, jj - 1);
+    ud = g.feyp[jj] * ld(u, ii, jj + 1) + (R(1.0) - g.feyp[jj]) * ld(u, ii, jj);
+    ui = g.feyp[jj] * ld(u, ii - 1, jj + 1) + (R(1.0) - g.feyp[jj]) * ld(u, ii - 1, jj);
+    vn = g.feyv[jj] * ld(v, ii, jj + 1) + (R(1.0) - g.feyv[jj]) * ld(v, ii, jj);
+    vs = g.feyv[jj - 1] * ld(v, ii, jj) + (R(1.0) - g.feyv[jj - 1]) * ld(v, ii, jj - 1);
     lx = g.dxv[ii];
     ly = g.dyv[jj];
   } else {
-    ud = AT(u, ii, jj);
-    ui = AT(u, ii - 1, jj);
-    vn = AT(v, ii, jj);
-    vs = AT(v, ii, jj - 1);
+    ud = ld(u, ii, jj);
+    ui = ld(u, ii - 1, jj);
+    vn = ld(v, ii, jj);
+    vs = ld(v, ii, jj - 1);
     lx = g.dxp[ii];
     ly = g.dyp[jj];
   }
@@ -398,7 +415,9 @@ __device__ __forceinline__ CoefT<R> fast_coef(const SweepArgs &A, int ii, int jj
       else acc = acc - area * R(spv);
     }
   }
-  c.ac = (acc + area * F.inv_dt) * (EQ == EQ_T ? F.inv_rel_ener : F.inv_rel_vel);
+  c.ap = acc;
+  c.td = area * F.inv_dt;
+  c.ac = (acc + c.td) * (EQ == EQ_T ? F.inv_rel_ener : F.inv_rel_vel);
   return c;
 }
 
@@ -435,6 +454,42 @@ __device__ __forceinline__ R fast_rhs(const SweepArgs &A, int ii, int jj, R head
     r = r + area * AT(fld<R>(A.tant), ii, jj) * inv_dt;
     r = r + ac * R(1.0 - A.ph.rel_ener) * AT(fld<R>(A.t), ii, jj);
   }
+  return r;
+}
+
+// FP32-refined sweeps: the unrelaxed FP64 residual of row (ii,jj) of the
+// current FP64 fields against the FP32 coefficients c,
+//   r = -ai phi_W - ad phi_E - bs phi_S - bn phi_N - ap phi_P
+//       + td (phi_ant - phi_P) + sources,
+// which is the right-hand side of the row's increment equation.  The time
+// term vanishes exactly where phi = phi_ant, so the iteration's fixed point is
+// the steady state of the FP64 fields for these coefficients; the increments
+// only need FP32 accuracy.
+template <int EQ, bool TR, typename R>
+__device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, const CoefT<R> &c,
+                                            R area) {
+  const int P = A.P;
+  const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
+  const double *ant = EQ == EQ_U ? A.uant : EQ == EQ_V ? A.vant : A.tant;
+  const double pP = AT(phi, ii, jj);
+  double r = -(double)c.ai * AT(phi, ii - 1, jj) - (double)c.ad * AT(phi, ii + 1, jj);
+  r = r - (double)c.bs * AT(phi, ii, jj - 1) - (double)c.bn * AT(phi, ii, jj + 1);
+  r = r - (double)c.ap * pP + (double)c.td * (AT(ant, ii, jj) - pP);
+  const double ar = (double)area;
+  if (EQ == EQ_U) {
+    if (A.ph.ri_x != 0.0) {
+      const double tint = A.g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - A.g.fexp[ii]) * AT(A.t, ii, jj);
+      r = r - ar * A.ph.ri_x * tint;
+    }
+    r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * A.g.dyu[jj];
+  } else if (EQ == EQ_V) {
+    if (A.ph.ri_y != 0.0) {
+      const double tint = A.g.feyp[jj] * AT(A.t, ii, jj + 1) + (1.0 - A.g.feyp[jj]) * AT(A.t, ii, jj);
+      r = r - ar * A.ph.ri_y * tint;
+    }
+    r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * A.g.dxv[ii];
+  }
+  if (A.sc) r = r + ar * AT(A.sc, ii, jj);
   return r;
 }
 
@@ -750,8 +805,12 @@ constexpr int line_lds() { return (4 * EPT * BS * (int)sizeof(R)) / 8 + kBlock; 
 
 template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
 __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, double *lds) {
-  static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR, "the FP32 sweeps use the PCR solver");
+  static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
+                "the FP32 sweeps use the PCR solver");
+  static_assert(SOLVER != SOLVE_DELTA || sizeof(R) == 4, "the increment sweeps are FP32");
   static_assert(BS == kBlock || BS == 64, "line workgroups of 256 threads or one wave");
+  using S = typename StoreOf<SOLVER, R>::T;      // stored type of the fields
+  constexpr bool DELTA = SOLVER == SOLVE_DELTA && EQ != EQ_P;
   R *sa = reinterpret_cast<R *>(lds), *sb = sa + EPT * BS, *sc_ = sb + EPT * BS, *sr = sc_ + EPT * BS;
   double *red = lds + (4 * EPT * BS * (int)sizeof(R)) / 8;
   constexpr bool TR = DIR == DIR_Y;
@@ -768,9 +827,11 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
   const Side &lo_side = DIR == DIR_X ? A.bc[0] : A.bc[1];
   const Side &hi_side = DIR == DIR_X ? A.bc[2] : A.bc[3];
   // a_u / a_v of a boundary face (1e40, main.f90:309-310; infinite in FP32)
+  // and of a penalised FP32 row (infinite)
   const R big = sizeof(R) == 8 ? R(1.e40) : R(__builtin_huge_valf());
-  R *acoef = fldw<R>(A.acoef), *acoef2 = fldw<R>(A.acoef2);
-  R *out = fldw<R>(A.out), *out2 = fldw<R>(A.out2);
+  const S bigs = sizeof(S) == 8 ? S(1.e40) : S(__builtin_huge_valf());
+  S *acoef = fldw<S>(A.acoef), *acoef2 = fldw<S>(A.acoef2);
+  S *out = fldw<S>(A.out), *out2 = fldw<S>(A.out2);
   // the boundary rows at both ends (p': Dirichlet 0, main.f90:789-800, 845-856)
   double lb0 = 0.0, lb1 = 1.0, lb2 = 0.0, lbr = 0.0, hb0 = 0.0, hb1 = 1.0, hb2 = 0.0, hbr = 0.0;
   bool lo_typed = false, hi_typed = false;
@@ -780,10 +841,16 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
   }
   // the first and last solved lines also carry the lines next to them
   const int other = line == A.line0 ? A.line0 - 1 : (line == lastl ? lastl + 1 : 0);
-  const R *src = fld<R>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc);
+  const S *src = fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc);
+  // the field at position q of this line (increment sweeps)
+  auto along = [&](int q) -> double {
+    const double *f = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
+    return DIR == DIR_X ? AT(f, q, line) : AT(f, line, q);
+  };
 
-  R keep_aw[EPT], carry[EPT];
-  double keep_old[EPT], keep_b[EPT];
+  R keep_aw[EPT];
+  S carry[EPT];
+  double keep_old[EPT], keep_b[EPT], keep_phi[EPT];
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
     if (e * BS >= n) break;  // uniform: no row of this pass lies on the line
@@ -798,8 +865,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0, bval = 0.0;
     R aw = R(0);
     if (EQ == EQ_P) {
-      const PCoefT<R> c = p_coef<TR, R>(A, ii, jj);
-      const R *pcf = fld<R>(A.pc);
+      const PCoefT<R> c = p_coef<TR, R, S>(A, ii, jj);
+      const S *pcf = fld<S>(A.pc);
       double pcP = A.in_zero ? 0.0 : (double)AT(pcf, ii, jj);
       double head;
       if (DIR == DIR_X) {
@@ -820,6 +887,16 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
         // every neighbour behind an infinite a_u, a_v (inside a body): p' = 0
         if (c.ac == R(0)) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = 0.0; }
       }
+    } else if constexpr (SOLVER == SOLVE_DELTA) {
+      R area;
+      double pin;
+      const CoefT<R> c = fast_coef<EQ, TR, R, double>(A, ii, jj, area, pin);
+      a0 = DIR == DIR_X ? c.ai : c.bs;
+      a2 = DIR == DIR_X ? c.ad : c.bn;
+      a1 = c.ac;
+      rr = delta_res<EQ, TR, R>(A, ii, jj, c, area);
+      aw = c.ac * R(A.ph.rel_vel);
+      if (pin == pin) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = pin - along(pc); aw = big; }
     } else if constexpr (SOLVER == SOLVE_PCR) {
       R area;
       double pin;
@@ -858,13 +935,19 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     }
     if (pos == 1) { a0 = lb0; a1 = lb1; a2 = lb2; rr = lbr; }
     if (pos == n) { a0 = hb0; a1 = hb1; a2 = hb2; rr = hbr; }
+    if constexpr (DELTA) {
+      // boundary rows in increments (a0 = 0 on the first row, a2 = 0 on the last)
+      keep_phi[e] = along(pr);
+      if (pos == 1) rr = rr - a1 * keep_phi[e] - a2 * along(2);
+      if (pos == n) rr = rr - a0 * along(n - 1) - a1 * keep_phi[e];
+    }
     keep_aw[e] = aw;
     keep_b[e] = bval;
-    if (NORM) keep_old[e] = A.snap ? (double)AT(fld<R>(A.snap), iir, jjr) : 0.0;
+    if (NORM) keep_old[e] = A.snap ? (double)AT(fld<S>(A.snap), iir, jjr) : 0.0;
     if (other) {
       const int io = DIR == DIR_X ? pr : other;
       const int jo = DIR == DIR_X ? other : pr;
-      carry[e] = (EQ == EQ_P && A.in_zero) ? R(0) : AT(src, io, jo);
+      carry[e] = (EQ == EQ_P && A.in_zero) ? S(0) : AT(src, io, jo);
     }
     if (k < n) {
       sa[k] = R(a0);
@@ -884,13 +967,13 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     const int jj = DIR == DIR_X ? line : pos;
     if (pos >= 2 && pos <= hi) {
       if (MOM && DIR == DIR_X) {
-        AT(acoef, ii, jj) = keep_aw[e];
-        AO(acoef2, ii, jj) = keep_aw[e];
+        AT(acoef, ii, jj) = S(keep_aw[e]);
+        AO(acoef2, ii, jj) = S(keep_aw[e]);
       }
       if (EQ == EQ_P && DIR == DIR_X) AT(A.bo, ii, jj) = keep_b[e];
     } else if (MOM && (pos == 1 ? lo_typed : hi_typed)) {
-      AT(acoef, ii, jj) = big;
-      AO(acoef2, ii, jj) = big;
+      AT(acoef, ii, jj) = bigs;
+      AO(acoef2, ii, jj) = bigs;
     }
   }
   __syncthreads();
@@ -908,7 +991,9 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
-    const R x = sr[k];
+    S x;
+    if constexpr (DELTA) x = keep_phi[e] + (double)sr[k];
+    else x = sr[k];
     AT(out, ii, jj) = x;
     AO(out2, ii, jj) = x;
     if (NORM && pos >= 2 && pos <= mi) {
@@ -1238,7 +1323,7 @@ __global__ void __launch_bounds__(BS) k_multi(MultiArgs M) {
   } else if constexpr (BS == kBlock) {
     if (b < M.off[3]) {
       const int w = b - M.off[2];
-      if (w < M.cnt[2]) residual_row<R>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
+      if (w < M.cnt[2]) residual_row<typename StoreOf<SOLVER, R>::T>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
     } else {
       if (b == M.off[3] && M.cnt[3]) finalize_body(M.f, lds);
     }

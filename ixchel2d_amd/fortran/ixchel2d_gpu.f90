@@ -9,7 +9,8 @@
 ! exact .bin dumps when postprocesar is true.  The mesh extents come from the
 ! files (no recompilation per mesh).  Environment: IXCHEL2D_SOLVER = pcr
 ! (default) | thomas selects the line solver, IXCHEL2D_PRECISION = fp64
-! (default) | fp32_mixed the arithmetic of the sweeps (PCR only).
+! (default) | fp32_mixed | fp32_refined the arithmetic of the sweeps (PCR
+! only).
 !
 program ixchel2d_gpu
   use iso_c_binding
@@ -128,6 +129,7 @@ program ixchel2d_gpu
   if (trim(solver_env) == 'thomas') prm % solver = IXH_SOLVER_THOMAS
   call get_environment_variable('IXCHEL2D_PRECISION', solver_env)
   if (trim(solver_env) == 'fp32_mixed') prm % precision = IXH_FP32_MIXED
+  if (trim(solver_env) == 'fp32_refined') prm % precision = IXH_FP32_REFINED
   call ixh_check(ixh_create(grid, prm, estado), 'ixh_create')
   call ixh_check(ixh_upload_field(estado, IXH_F_U_ANT, u_ant, mi), 'upload u_ant')
   call ixh_check(ixh_upload_field(estado, IXH_F_V_ANT, v_ant, mi+1), 'upload v_ant')
