@@ -458,24 +458,27 @@ __device__ __forceinline__ R fast_rhs(const SweepArgs &A, int ii, int jj, R head
 }
 
 // FP32-refined sweeps: the unrelaxed FP64 residual of row (ii,jj) of the
-// current FP64 fields against the FP32 coefficients c,
-//   r = -ai phi_W - ad phi_E - bs phi_S - bn phi_N - ap phi_P
-//       + td (phi_ant - phi_P) + sources,
-// which is the right-hand side of the row's increment equation.  The time
-// term vanishes exactly where phi = phi_ant, so the iteration's fixed point is
-// the steady state of the FP64 fields for these coefficients; the increments
-// only need FP32 accuracy.
+// current FP64 fields against the FP32 neighbour coefficients c,
+//   r = -ai (phi_W - phi_P) - ad (phi_E - phi_P) - bs (phi_S - phi_P)
+//       - bn (phi_N - phi_P) + area Sp phi_P + td (phi_ant - phi_P) + sources,
+// which is the right-hand side of the row's increment equation.  a_P enters
+// only through the neighbour differences (a_P = -sum a_nb - area Sp + td), so
+// its FP32 rounding, about 6e-8 of a_P phi_P, does not reach the residual;
+// the sources and Sp use the FP64 cell area.  The time term vanishes exactly
+// where phi = phi_ant, so the iteration's fixed point is the steady state of
+// the FP64 fields for these neighbour coefficients; the increments only need
+// FP32 accuracy.
 template <int EQ, bool TR, typename R>
-__device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, const CoefT<R> &c,
-                                            R area) {
+__device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, const CoefT<R> &c) {
   const int P = A.P;
   const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
   const double *ant = EQ == EQ_U ? A.uant : EQ == EQ_V ? A.vant : A.tant;
   const double pP = AT(phi, ii, jj);
-  double r = -(double)c.ai * AT(phi, ii - 1, jj) - (double)c.ad * AT(phi, ii + 1, jj);
-  r = r - (double)c.bs * AT(phi, ii, jj - 1) - (double)c.bn * AT(phi, ii, jj + 1);
-  r = r - (double)c.ap * pP + (double)c.td * (AT(ant, ii, jj) - pP);
-  const double ar = (double)area;
+  const double ar = EQ == EQ_U ? A.g.dxu[ii] * A.g.dyu[jj]
+                  : EQ == EQ_V ? A.g.dxv[ii] * A.g.dyv[jj] : A.g.dxp[ii] * A.g.dyp[jj];
+  double r = -(double)c.ai * (AT(phi, ii - 1, jj) - pP) - (double)c.ad * (AT(phi, ii + 1, jj) - pP);
+  r = r - (double)c.bs * (AT(phi, ii, jj - 1) - pP) - (double)c.bn * (AT(phi, ii, jj + 1) - pP);
+  r = r + (double)c.td * (AT(ant, ii, jj) - pP);
   if (EQ == EQ_U) {
     if (A.ph.ri_x != 0.0) {
       const double tint = A.g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - A.g.fexp[ii]) * AT(A.t, ii, jj);
@@ -490,6 +493,7 @@ __device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, 
     r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * A.g.dxv[ii];
   }
   if (A.sc) r = r + ar * AT(A.sc, ii, jj);
+  if (A.sp) r = r + ar * AT(A.sp, ii, jj) * pP;
   return r;
 }
 
@@ -894,7 +898,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
       a0 = DIR == DIR_X ? c.ai : c.bs;
       a2 = DIR == DIR_X ? c.ad : c.bn;
       a1 = c.ac;
-      rr = delta_res<EQ, TR, R>(A, ii, jj, c, area);
+      rr = delta_res<EQ, TR, R>(A, ii, jj, c);
       aw = c.ac * R(A.ph.rel_vel);
       if (pin == pin) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = pin - along(pc); aw = big; }
     } else if constexpr (SOLVER == SOLVE_PCR) {
