@@ -156,13 +156,23 @@ constexpr double kPenaltyRow = 1.0e30;
 // (y contiguous, pitch >= nj+1) for the y-sweeps, so that the lanes of a wave
 // always read consecutive addresses.  TR (compile time, in scope) selects the
 // transposed one; P is the pitch of the layout in use.
-#define AT(ptr, i, j)                                                             \
-  (ptr)[TR ? (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P)                 \
-           : (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P)]
+#define IDX(i, j)                                                                 \
+  (TR ? (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P)                      \
+      : (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P))
+#define AT(ptr, i, j) (ptr)[IDX(i, j)]
 // the other layout (pitch P2), for the second copy a sweep writes
 #define AO(ptr, i, j)                                                             \
   (ptr)[TR ? (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P2)                \
            : (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P2)]
+
+// Optional operands without branches.  An absent source field (null) or an
+// unused buoyancy term reads element 0 of a zero block, or of the field,
+// instead of element (i,j) (`on` is uniform).  The load then goes out with
+// all the others instead of after a branch, the term it feeds is zero, and
+// x + 0 = x leaves the arithmetic of the present terms unchanged.
+__constant__ double kZeroBlock[8];
+__device__ __forceinline__ const double *or_zero(const double *p) { return p ? p : kZeroBlock; }
+#define ATZ(ptr, on, i, j) (ptr)[(on) ? IDX(i, j) : (size_t)0]
 
 __device__ __forceinline__ double pow5(double t) { return t * (t * (t * (t * t))); }
 __device__ __forceinline__ double dmax0(double x) { return x < 0.0 ? 0.0 : x; }
@@ -207,7 +217,7 @@ __device__ __forceinline__ Coef u_coef(const SweepArgs &A, int ii, int jj) {
   c.bs = nbc(gs, dx, ds, vs, false);
   c.bn = nbc(gn, dx, dn, vn, true);
   double acc = -c.ai - c.ad - c.bs - c.bn;
-  if (A.sp) acc = acc - dx * dy * AT(A.sp, ii, jj);
+  acc = acc - dx * dy * ATZ(or_zero(A.sp), A.sp != nullptr, ii, jj);
   c.ac = (acc + dx * dy / A.ph.dt) / A.ph.rel_vel;
   return c;
 }
@@ -218,11 +228,10 @@ __device__ __forceinline__ double u_rhs(const SweepArgs &A, int ii, int jj, doub
   const Geo &g = A.g;
   double dx = g.dxu[ii], dy = g.dyu[jj];
   double r = head;
-  if (A.ph.ri_x != 0.0) {
-    double tint = g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - g.fexp[ii]) * AT(A.t, ii, jj);
-    r = r - dx * dy * A.ph.ri_x * tint;
-  }
-  if (A.sc) r = r + dx * dy * AT(A.sc, ii, jj);
+  const bool buoy = A.ph.ri_x != 0.0;
+  double tint = g.fexp[ii] * ATZ(A.t, buoy, ii + 1, jj) + (1.0 - g.fexp[ii]) * ATZ(A.t, buoy, ii, jj);
+  r = r - dx * dy * A.ph.ri_x * tint;
+  r = r + dx * dy * ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj);
   r = r + dx * dy * AT(A.uant, ii, jj) / A.ph.dt;
   r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * dy;
   r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.u, ii, jj);
@@ -253,7 +262,7 @@ __device__ __forceinline__ Coef v_coef(const SweepArgs &A, int ii, int jj) {
   c.bs = nbc(gm, dxv, ds, vs, false);
   c.bn = nbc(gm, dxv, dn, vn, true);
   double acc = -c.ai - c.ad - c.bs - c.bn;
-  if (A.sp) acc = acc - dxv * dyv * AT(A.sp, ii, jj);
+  acc = acc - dxv * dyv * ATZ(or_zero(A.sp), A.sp != nullptr, ii, jj);
   c.ac = (acc + dxv * dyv / A.ph.dt) / A.ph.rel_vel;
   return c;
 }
@@ -264,11 +273,10 @@ __device__ __forceinline__ double v_rhs(const SweepArgs &A, int ii, int jj, doub
   const Geo &g = A.g;
   double dxv = g.dxv[ii], dyv = g.dyv[jj];
   double r = head;
-  if (A.ph.ri_y != 0.0) {
-    double tint = g.feyp[jj] * AT(A.t, ii, jj + 1) + (1.0 - g.feyp[jj]) * AT(A.t, ii, jj);
-    r = r - dxv * dyv * A.ph.ri_y * tint;
-  }
-  if (A.sc) r = r + dxv * dyv * AT(A.sc, ii, jj);
+  const bool buoy = A.ph.ri_y != 0.0;
+  double tint = g.feyp[jj] * ATZ(A.t, buoy, ii, jj + 1) + (1.0 - g.feyp[jj]) * ATZ(A.t, buoy, ii, jj);
+  r = r - dxv * dyv * A.ph.ri_y * tint;
+  r = r + dxv * dyv * ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj);
   r = r + dxv * dyv * AT(A.vant, ii, jj) / A.ph.dt;
   r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * dxv;
   r = r + ac * (1.0 - A.ph.rel_vel) * AT(A.v, ii, jj);
@@ -326,7 +334,7 @@ __device__ __forceinline__ Coef t_coef(const SweepArgs &A, int ii, int jj) {
   c.bs = nbc(gs, dxp, ds, vs, false);
   c.bn = nbc(gn, dxp, dn, vn, true);
   double acc = -c.ai - c.ad - c.bs - c.bn;
-  if (A.sp) acc = acc - dxp * dyp * AT(A.sp, ii, jj);
+  acc = acc - dxp * dyp * ATZ(or_zero(A.sp), A.sp != nullptr, ii, jj);
   c.ac = (acc + dxp * dyp / A.ph.dt) / A.ph.rel_ener;
   return c;
 }
@@ -337,7 +345,7 @@ __device__ __forceinline__ double t_rhs(const SweepArgs &A, int ii, int jj, doub
   const Geo &g = A.g;
   double dxp = g.dxp[ii], dyp = g.dyp[jj];
   double r = head;
-  if (A.sc) r = r + dxp * dyp * AT(A.sc, ii, jj);
+  r = r + dxp * dyp * ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj);
   r = r + dxp * dyp * AT(A.tant, ii, jj) / A.ph.dt;
   r = r + ac * (1.0 - A.ph.rel_ener) * AT(A.t, ii, jj);
   return r;
@@ -406,14 +414,16 @@ __device__ __forceinline__ CoefT<R> fast_coef(const SweepArgs &A, int ii, int jj
   area = lx * ly;
   R acc = -c.ai - c.ad - c.bs - c.bn;
   pin = __builtin_nan("");
-  if (A.sp) {
-    const double spv = AT(A.sp, ii, jj);
-    if constexpr (sizeof(R) == 8) {
-      acc = acc - area * spv;
-    } else {
-      if (spv < -kPenaltyRow) pin = AT(A.sc, ii, jj) / -spv;
-      else acc = acc - area * R(spv);
-    }
+  const double spv = ATZ(or_zero(A.sp), A.sp != nullptr, ii, jj);
+  if constexpr (sizeof(R) == 8) {
+    acc = acc - area * spv;
+  } else {
+    const double scv = ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj);
+    const bool pen = spv < -kPenaltyRow;
+    // no branch (it would take the Sc load with it): an unpenalised row
+    // divides by NaN
+    pin = scv / (pen ? -spv : __builtin_nan(""));
+    acc = pen ? acc : acc - area * R(spv);
   }
   c.ap = acc;
   c.td = area * F.inv_dt;
@@ -427,30 +437,29 @@ __device__ __forceinline__ R fast_rhs(const SweepArgs &A, int ii, int jj, R head
   const GeoT<R> &g = geo_of<R>(A);
   const R inv_dt = fast_of<R>(A).inv_dt;
   R r = head;
+  const R scv = R(ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj));
   if (EQ == EQ_U) {
     const R *u = fld<R>(A.u);
-    if (A.ph.ri_x != 0.0) {
-      const R *t = fld<R>(A.t);
-      R tint = g.fexp[ii] * AT(t, ii + 1, jj) + (R(1.0) - g.fexp[ii]) * AT(t, ii, jj);
-      r = r - area * R(A.ph.ri_x) * tint;
-    }
-    if (A.sc) r = r + area * R(AT(A.sc, ii, jj));
+    const R *t = fld<R>(A.t);
+    const bool buoy = A.ph.ri_x != 0.0;
+    R tint = g.fexp[ii] * ATZ(t, buoy, ii + 1, jj) + (R(1.0) - g.fexp[ii]) * ATZ(t, buoy, ii, jj);
+    r = r - area * R(A.ph.ri_x) * tint;
+    r = r + area * scv;
     r = r + area * AT(fld<R>(A.uant), ii, jj) * inv_dt;
     r = r + R((AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * A.g.dyu[jj]);
     r = r + ac * R(1.0 - A.ph.rel_vel) * AT(u, ii, jj);
   } else if (EQ == EQ_V) {
     const R *v = fld<R>(A.v);
-    if (A.ph.ri_y != 0.0) {
-      const R *t = fld<R>(A.t);
-      R tint = g.feyp[jj] * AT(t, ii, jj + 1) + (R(1.0) - g.feyp[jj]) * AT(t, ii, jj);
-      r = r - area * R(A.ph.ri_y) * tint;
-    }
-    if (A.sc) r = r + area * R(AT(A.sc, ii, jj));
+    const R *t = fld<R>(A.t);
+    const bool buoy = A.ph.ri_y != 0.0;
+    R tint = g.feyp[jj] * ATZ(t, buoy, ii, jj + 1) + (R(1.0) - g.feyp[jj]) * ATZ(t, buoy, ii, jj);
+    r = r - area * R(A.ph.ri_y) * tint;
+    r = r + area * scv;
     r = r + area * AT(fld<R>(A.vant), ii, jj) * inv_dt;
     r = r + R((AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * A.g.dxv[ii]);
     r = r + ac * R(1.0 - A.ph.rel_vel) * AT(v, ii, jj);
   } else {
-    if (A.sc) r = r + area * R(AT(A.sc, ii, jj));
+    r = r + area * scv;
     r = r + area * AT(fld<R>(A.tant), ii, jj) * inv_dt;
     r = r + ac * R(1.0 - A.ph.rel_ener) * AT(fld<R>(A.t), ii, jj);
   }
@@ -480,20 +489,18 @@ __device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, 
   r = r - (double)c.bs * (AT(phi, ii, jj - 1) - pP) - (double)c.bn * (AT(phi, ii, jj + 1) - pP);
   r = r + (double)c.td * (AT(ant, ii, jj) - pP);
   if (EQ == EQ_U) {
-    if (A.ph.ri_x != 0.0) {
-      const double tint = A.g.fexp[ii] * AT(A.t, ii + 1, jj) + (1.0 - A.g.fexp[ii]) * AT(A.t, ii, jj);
-      r = r - ar * A.ph.ri_x * tint;
-    }
+    const bool buoy = A.ph.ri_x != 0.0;
+    const double tint = A.g.fexp[ii] * ATZ(A.t, buoy, ii + 1, jj) + (1.0 - A.g.fexp[ii]) * ATZ(A.t, buoy, ii, jj);
+    r = r - ar * A.ph.ri_x * tint;
     r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * A.g.dyu[jj];
   } else if (EQ == EQ_V) {
-    if (A.ph.ri_y != 0.0) {
-      const double tint = A.g.feyp[jj] * AT(A.t, ii, jj + 1) + (1.0 - A.g.feyp[jj]) * AT(A.t, ii, jj);
-      r = r - ar * A.ph.ri_y * tint;
-    }
+    const bool buoy = A.ph.ri_y != 0.0;
+    const double tint = A.g.feyp[jj] * ATZ(A.t, buoy, ii, jj + 1) + (1.0 - A.g.feyp[jj]) * ATZ(A.t, buoy, ii, jj);
+    r = r - ar * A.ph.ri_y * tint;
     r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * A.g.dxv[ii];
   }
-  if (A.sc) r = r + ar * AT(A.sc, ii, jj);
-  if (A.sp) r = r + ar * AT(A.sp, ii, jj) * pP;
+  r = r + ar * ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj);
+  r = r + ar * ATZ(or_zero(A.sp), A.sp != nullptr, ii, jj) * pP;
   return r;
 }
 
@@ -755,23 +762,23 @@ __device__ __forceinline__ int xcd_line(int w, int n) {
   return x * q + (x < r ? x : r) + k;
 }
 
-// boundary row of a sweep: side low (a/b) or high (c/d); returns false when
-// the node is untyped
+// boundary row of a sweep: side low (a/b) or high (c/d); an untyped node
+// keeps the row passed in (a0 = a2 = 0) and returns false.  Selects, not
+// branches, so both loads go out with the rest of the assembly's.
 __device__ __forceinline__ bool bc_row(const Side &sd, int node, bool high, double &a0, double &a1,
                                        double &a2, double &rr) {
-  // both loads issued together (one memory round trip)
+  // (plain selects: a nested or short-circuit condition is emitted as
+  // branches, into which the loads are then sunk)
   const int t = sd.type[node];
   const double v = sd.val[node];
-  if (t == 0) return false;
-  if (t == 1) {
-    a0 = 0.0; a1 = 1.0; a2 = 0.0;
-  } else if (!high) {
-    a0 = 0.0; a1 = -1.0; a2 = 1.0;
-  } else {
-    a0 = -1.0; a1 = 1.0; a2 = 0.0;
-  }
-  rr = v;
-  return true;
+  const bool none = t == 0;
+  const bool nlo = (t == 2) & !high, nhi = (t == 2) & high;
+  const double typed_a1 = nlo ? -1.0 : 1.0;
+  a0 = nhi ? -1.0 : 0.0;
+  a1 = none ? a1 : typed_a1;
+  a2 = nlo ? 1.0 : 0.0;
+  rr = none ? rr : v;
+  return !none;
 }
 
 // ---------------------------------------------------------- sweep kernel
@@ -807,8 +814,19 @@ struct Layout {
 template <typename R, int EPT, int BS = kBlock>
 constexpr int line_lds() { return (4 * EPT * BS * (int)sizeof(R)) / 8 + kBlock; }
 
+// A zero the compiler must treat as a per-lane value.  Indices built on it
+// make uniform table lookups vector loads: vector loads return in order, so
+// each can be waited for alone, while a scalar load's value is usable only
+// once every outstanding scalar load has returned.
+__device__ __forceinline__ int lane_zero() {
+  int z;
+  asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
-__device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, double *lds) {
+__device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s, double *lds) {
+  const int line = line_s + lane_zero();
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
                 "the FP32 sweeps use the PCR solver");
   static_assert(SOLVER != SOLVE_DELTA || sizeof(R) == 4, "the increment sweeps are FP32");
@@ -844,7 +862,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     hi_typed = bc_row(hi_side, line, true, hb0, hb1, hb2, hbr);
   }
   // the first and last solved lines also carry the lines next to them
-  const int other = line == A.line0 ? A.line0 - 1 : (line == lastl ? lastl + 1 : 0);
+  const int other_hi = line == lastl ? lastl + 1 : 0;
+  const int other = line == A.line0 ? A.line0 - 1 : other_hi;
   const S *src = fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc);
   // the field at position q of this line (increment sweeps)
   auto along = [&](int q) -> double {
@@ -855,12 +874,16 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
   R keep_aw[EPT];
   S carry[EPT];
   double keep_old[EPT], keep_b[EPT], keep_phi[EPT];
+  // FP32 fields read a zero block as zeros too
+  const S *snap = A.snap ? fld<S>(A.snap) : reinterpret_cast<const S *>(kZeroBlock);
+  const int oline = other ? other : line;  // a valid line to load from when none is carried
+  // the field next to each end of the line (boundary rows of the increment sweeps)
+  const double phi_2 = DELTA ? along(2) : 0.0, phi_n1 = DELTA ? along(n - 1) : 0.0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    if (e * BS >= n) break;  // uniform: no row of this pass lies on the line
     const int k = threadIdx.x + e * BS;
     const int pos = k + 1;                                    // 1-based position along the line
-    const int pc = pos < 2 ? 2 : (pos > hi ? hi : pos);       // assembled position
+    const int pc = min(max(pos, 2), hi);                     // assembled position
     const int pr = pos > n ? n : pos;                         // position for the plain loads
     const int ii = DIR == DIR_X ? pc : line;
     const int jj = DIR == DIR_X ? line : pc;
@@ -870,26 +893,36 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
     R aw = R(0);
     if (EQ == EQ_P) {
       const PCoefT<R> c = p_coef<TR, R, S>(A, ii, jj);
+      // p' = 0 at the first pass: the loads still go out (the buffer is
+      // allocated), their values are dropped
       const S *pcf = fld<S>(A.pc);
-      double pcP = A.in_zero ? 0.0 : (double)AT(pcf, ii, jj);
+      const bool z = A.in_zero != 0;
+      const double pcv = (double)AT(pcf, ii, jj);
+      double pcP = z ? 0.0 : pcv;
       double head;
       if (DIR == DIR_X) {
         a0 = c.ai; a2 = c.ad;
-        double pS = A.in_zero ? 0.0 : (double)AT(pcf, ii, jj - 1);
-        double pN = A.in_zero ? 0.0 : (double)AT(pcf, ii, jj + 1);
+        const double s0 = (double)AT(pcf, ii, jj - 1), n0 = (double)AT(pcf, ii, jj + 1);
+        double pS = z ? 0.0 : s0;
+        double pN = z ? 0.0 : n0;
         head = -(double)c.bs * pS - (double)c.bn * pN;
         bval = c.b;
       } else {
         a0 = c.bs; a2 = c.bn;
-        double pW = A.in_zero ? 0.0 : (double)AT(pcf, ii - 1, jj);
-        double pE = A.in_zero ? 0.0 : (double)AT(pcf, ii + 1, jj);
+        const double w0 = (double)AT(pcf, ii - 1, jj), e0 = (double)AT(pcf, ii + 1, jj);
+        double pW = z ? 0.0 : w0;
+        double pE = z ? 0.0 : e0;
         head = -(double)c.ai * pW - (double)c.ad * pE;
       }
       a1 = c.ac;
       rr = head + c.b + (1.0 - A.ph.rel_pres) * (double)c.ac * pcP;
       if constexpr (sizeof(R) == 4) {
         // every neighbour behind an infinite a_u, a_v (inside a body): p' = 0
-        if (c.ac == R(0)) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = 0.0; }
+        const bool shut = c.ac == R(0);
+        a0 = shut ? 0.0 : a0;
+        a1 = shut ? 1.0 : a1;
+        a2 = shut ? 0.0 : a2;
+        rr = shut ? 0.0 : rr;
       }
     } else if constexpr (SOLVER == SOLVE_DELTA) {
       R area;
@@ -900,7 +933,13 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
       a1 = c.ac;
       rr = delta_res<EQ, TR, R>(A, ii, jj, c);
       aw = c.ac * R(A.ph.rel_vel);
-      if (pin == pin) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = pin - along(pc); aw = big; }
+      const double phic = along(pc);
+      const bool pinned = pin == pin;
+      a0 = pinned ? 0.0 : a0;
+      a1 = pinned ? 1.0 : a1;
+      a2 = pinned ? 0.0 : a2;
+      rr = pinned ? pin - phic : rr;
+      aw = pinned ? big : aw;
     } else if constexpr (SOLVER == SOLVE_PCR) {
       R area;
       double pin;
@@ -918,7 +957,12 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
       rr = fast_rhs<EQ, TR, R>(A, ii, jj, head, c.ac, area);
       aw = c.ac * R(A.ph.rel_vel);
       if constexpr (sizeof(R) == 4) {
-        if (pin == pin) { a0 = 0.0; a1 = 1.0; a2 = 0.0; rr = pin; aw = big; }
+        const bool pinned = pin == pin;
+        a0 = pinned ? 0.0 : a0;
+        a1 = pinned ? 1.0 : a1;
+        a2 = pinned ? 0.0 : a2;
+        rr = pinned ? pin : rr;
+        aw = pinned ? big : aw;
       }
     } else {
       const Coef c = EQ == EQ_U ? u_coef<TR>(A, ii, jj) : EQ == EQ_V ? v_coef<TR>(A, ii, jj)
@@ -937,28 +981,40 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line, d
          : EQ == EQ_V ? v_rhs<TR>(A, ii, jj, head, c.ac) : t_rhs<TR>(A, ii, jj, head, c.ac);
       aw = R(c.ac * A.ph.rel_vel);
     }
-    if (pos == 1) { a0 = lb0; a1 = lb1; a2 = lb2; rr = lbr; }
-    if (pos == n) { a0 = hb0; a1 = hb1; a2 = hb2; rr = hbr; }
+    // the boundary rows, as selects: a branch would take the boundary
+    // loads into it and split the assembly into blocks
+    const bool lo = pos == 1, hi_row = pos == n;
+    a0 = hi_row ? hb0 : a0;
+    a1 = hi_row ? hb1 : a1;
+    a2 = hi_row ? hb2 : a2;
+    rr = hi_row ? hbr : rr;
+    a0 = lo ? lb0 : a0;
+    a1 = lo ? lb1 : a1;
+    a2 = lo ? lb2 : a2;
+    rr = lo ? lbr : rr;
     if constexpr (DELTA) {
       // boundary rows in increments (a0 = 0 on the first row, a2 = 0 on the last)
       keep_phi[e] = along(pr);
-      if (pos == 1) rr = rr - a1 * keep_phi[e] - a2 * along(2);
-      if (pos == n) rr = rr - a0 * along(n - 1) - a1 * keep_phi[e];
+      const double r_lo = rr - a1 * keep_phi[e] - a2 * phi_2;
+      const double r_hi = rr - a0 * phi_n1 - a1 * keep_phi[e];
+      rr = hi_row ? r_hi : rr;
+      rr = lo ? r_lo : rr;
     }
     keep_aw[e] = aw;
     keep_b[e] = bval;
-    if (NORM) keep_old[e] = A.snap ? (double)AT(fld<S>(A.snap), iir, jjr) : 0.0;
-    if (other) {
-      const int io = DIR == DIR_X ? pr : other;
-      const int jo = DIR == DIR_X ? other : pr;
-      carry[e] = (EQ == EQ_P && A.in_zero) ? S(0) : AT(src, io, jo);
+    if (NORM) keep_old[e] = (double)ATZ(snap, A.snap != nullptr, iir, jjr);
+    {
+      const int io = DIR == DIR_X ? pr : oline;
+      const int jo = DIR == DIR_X ? oline : pr;
+      const S cv = AT(src, io, jo);
+      carry[e] = (EQ == EQ_P && A.in_zero) ? S(0) : cv;
     }
-    if (k < n) {
-      sa[k] = R(a0);
-      sb[k] = R(a1);
-      sc_[k] = R(a2);
-      sr[k] = R(rr);
-    }
+    // rows past the line fill LDS the solver does not read (no branch, so
+    // the next row's loads are not held behind this one's stores)
+    sa[k] = R(a0);
+    sb[k] = R(a1);
+    sc_[k] = R(a2);
+    sr[k] = R(rr);
   }
   // global stores of the assembly: a_u / a_v (x-sweeps of u, v; typed
   // boundary faces in both sweeps), b_o (p' x-sweep, row layout)
@@ -1182,15 +1238,14 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, dou
       double bs = -nbc(gs, dx, ds, vs, false);
       double bn = -nbc(gn, dx, dn, vn, true);
       double ac = ai + ad + bs + bn;
-      if (A.sp) ac = ac - dx * dy * AT(A.sp, ii, jj);
+      ac = ac - dx * dy * ATZ(or_zero(A.sp), A.sp != nullptr, ii, jj);
       ac = ac + dx * dy / A.ph.dt;
       double r = ai * (double)AT(u, ii - 1, jj) - ac * (double)AT(u, ii, jj) + ad * (double)AT(u, ii + 1, jj) +
                  bs * (double)AT(u, ii, jj - 1) + bn * (double)AT(u, ii, jj + 1);
-      if (A.ph.ri_x != 0.0) {
-        double tint = g.fexp[ii] * (double)AT(t, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(t, ii, jj);
-        r = r - dx * dy * A.ph.ri_x * tint;
-      }
-      if (A.sc) r = r + dx * dy * AT(A.sc, ii, jj);
+      const bool buoy = A.ph.ri_x != 0.0;
+      double tint = g.fexp[ii] * (double)ATZ(t, buoy, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)ATZ(t, buoy, ii, jj);
+      r = r - dx * dy * A.ph.ri_x * tint;
+      r = r + dx * dy * ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj);
       r = r + dx * dy * (double)AT(uant, ii, jj) / A.ph.dt;
       r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * dy;
       rv[m] = r;

@@ -41,18 +41,20 @@ def test_fortran_host_reproduces_upstream_outputs(name, solver, tmp_path):
         np.testing.assert_allclose(out["nusselt"][:, :3], gold["nusselt"][:, :3], rtol=1e-12, atol=1e-16)
 
 
-def test_fortran_host_fp32_mixed(tmp_path):
-    """IXCHEL2D_PRECISION=fp32_mixed: the same outputs within the FP32-mixed
-    bound of the nc34 fixture (2e-4 relative after 1000 iterations)."""
+@pytest.mark.parametrize("precision,bound", [("fp32_mixed", 2e-4), ("fp32_refined", 5e-7)])
+def test_fortran_host_fp32_paths(precision, bound, tmp_path):
+    """IXCHEL2D_PRECISION selects an FP32 path: the same outputs within that
+    path's bound on the nc34 fixture after 1000 iterations (FP32-mixed 2e-4,
+    FP32-refined 5e-7 relative)."""
     assert os.path.exists(EXE), "Fortran host not built"
     c = K.nc34()
     gold = K.load_fixture("nc34")
     C.write_reference_case(c, str(tmp_path))
-    env = dict(os.environ, IXCHEL2D_SOLVER="pcr", IXCHEL2D_PRECISION="fp32_mixed")
+    env = dict(os.environ, IXCHEL2D_SOLVER="pcr", IXCHEL2D_PRECISION=precision)
     r = subprocess.run([EXE], cwd=tmp_path, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stdout[-2000:]
     out = C.read_reference_outputs(c, str(tmp_path))
     for f in "uvtp":
         rel = np.max(np.abs(out[f] - gold[f])) / max(np.max(np.abs(gold[f])), 1e-300)
-        assert rel <= 2e-4, (f, rel)
+        assert rel <= bound, (f, rel)
