@@ -14,6 +14,7 @@
 #include "../../include/ixchel2d.h"
 #include "simple_device.hpp"
 #include "adjoint_device.hpp"
+#include "sweep_launch.hpp"
 
 using namespace ixh;
 
@@ -192,16 +193,6 @@ Phys phys_of(const ixh_state *s) {
   return Phys{p.dt, p.rel_pres, p.rel_vel, p.rel_ener, p.gamma_momen, p.gamma_energ, p.ri_x, p.ri_y};
 }
 
-int ept_for(int n) {
-  const int e = (n + kBlock - 1) / kBlock;
-  if (e <= 1) return 1;
-  if (e <= 2) return 2;
-  if (e <= 3) return 3;
-  if (e <= 4) return 4;
-  if (e <= 6) return 6;
-  return 8;
-}
-
 // ---- kernel launch helpers (with optional event timing)
 struct Timed {
   ixh_state *s;
@@ -221,25 +212,12 @@ struct Timed {
   }
 };
 
-// A fused launch of two PCR sweeps whose lines have at most this many nodes
-// runs one 64-thread wave per line: at 302-node x-lines the T_x + u_x launch
-// takes 17.6 us that way against 22.8 us with 256 threads per line. A single
-// sweep of about 700 such lines is faster with 256 threads per line
-// (13.4 us against 16.6 us for v_x), so single sweeps keep that shape.
-constexpr int kWaveLine = 5 * 64;
+LaunchCtx ctx_of(const ixh_state *s) { return LaunchCtx{s->launch, s->t_start, s->t_stop}; }
 
+// the sweep kernels live in sweeps_*.hip, one translation unit per path
 template <int EQ, int DIR, int SOLVER, typename R>
 int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
-  dim3 grid(A.nlines), block(kBlock);
-  switch (ept_for(n)) {
-    case 1: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 1, R>), grid, block, A); break;
-    case 2: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 2, R>), grid, block, A); break;
-    case 3: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 3, R>), grid, block, A); break;
-    case 4: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 4, R>), grid, block, A); break;
-    case 6: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 6, R>), grid, block, A); break;
-    default: IXH_LAUNCH(s, (k_sweep<EQ, DIR, SOLVER, 8, R>), grid, block, A); break;
-  }
-  HIPCHK(hipGetLastError());
+  HIPCHK((launch_sweep_k<EQ, DIR, SOLVER, R>(ctx_of(s), A, n)));
   return 0;
 }
 
@@ -656,31 +634,7 @@ int fast_step(ixh_state *s) {
 // k_multi with sweeps (EQ0, DIR0) and (EQ1, DIR1) at a common EPT
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
 int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn) {
-  dim3 grid(nwg), block(kBlock);
-  // sweeps only, every line short: one wave per line
-  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
-    if (maxn <= kWaveLine && M.cnt[2] == 0 && M.cnt[3] == 0) {
-      const dim3 wave(64);
-      switch ((maxn + 63) / 64) {
-        case 1: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 1, R, 64>), grid, wave, M); break;
-        case 2: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R, 64>), grid, wave, M); break;
-        case 3: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 3, R, 64>), grid, wave, M); break;
-        case 4: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R, 64>), grid, wave, M); break;
-        default: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 5, R, 64>), grid, wave, M); break;
-      }
-      HIPCHK(hipGetLastError());
-      return 0;
-    }
-  }
-  switch (ept) {
-    case 1: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 1, R>), grid, block, M); break;
-    case 2: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R>), grid, block, M); break;
-    case 3: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 3, R>), grid, block, M); break;
-    case 4: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R>), grid, block, M); break;
-    case 6: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 6, R>), grid, block, M); break;
-    default: IXH_LAUNCH(s, (k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 8, R>), grid, block, M); break;
-  }
-  HIPCHK(hipGetLastError());
+  HIPCHK((launch_multi_k<EQ0, DIR0, EQ1, DIR1, SOLVER, R>(ctx_of(s), M, nwg, ept, maxn)));
   return 0;
 }
 
@@ -1646,29 +1600,8 @@ int batch_prepare(ixh_batch *b, BatchStep &E) {
 
 template <int EQ, int DIR, int SOLVER, typename R>
 int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
-  dim3 grid(b->nwg[ph]), block(kBlock);
-  const int2 *wg = b->d_wg + b->wg_off[ph];
-  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
-    if (b->wave_lines) {
-      const dim3 wave(64);
-      switch (b->ept64[ph]) {
-        case 1: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 1, R, 64>), grid, wave, 0, st, D, wg); break;
-        case 2: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 2, R, 64>), grid, wave, 0, st, D, wg); break;
-        case 3: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 3, R, 64>), grid, wave, 0, st, D, wg); break;
-        case 4: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 4, R, 64>), grid, wave, 0, st, D, wg); break;
-        default: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 5, R, 64>), grid, wave, 0, st, D, wg); break;
-      }
-      HIPCHK(hipGetLastError());
-      return 0;
-    }
-  }
-  switch (b->ept[ph]) {
-    case 1: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 1, R>), grid, block, 0, st, D, wg); break;
-    case 2: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 2, R>), grid, block, 0, st, D, wg); break;
-    case 3: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 3, R>), grid, block, 0, st, D, wg); break;
-    default: hipLaunchKernelGGL((k_sweep_batch<EQ, DIR, SOLVER, 4, R>), grid, block, 0, st, D, wg); break;
-  }
-  HIPCHK(hipGetLastError());
+  HIPCHK((launch_sweep_batch_k<EQ, DIR, SOLVER, R>(st, D, b->d_wg + b->wg_off[ph], b->nwg[ph], b->ept[ph],
+                                                   b->ept64[ph], b->wave_lines)));
   return 0;
 }
 

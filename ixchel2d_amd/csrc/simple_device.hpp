@@ -170,7 +170,7 @@ constexpr double kPenaltyRow = 1.0e30;
 // instead of element (i,j) (`on` is uniform).  The load then goes out with
 // all the others instead of after a branch, the term it feeds is zero, and
 // x + 0 = x leaves the arithmetic of the present terms unchanged.
-__constant__ double kZeroBlock[8];
+static __constant__ double kZeroBlock[8];
 __device__ __forceinline__ const double *or_zero(const double *p) { return p ? p : kZeroBlock; }
 #define ATZ(ptr, on, i, j) (ptr)[(on) ? IDX(i, j) : (size_t)0]
 
@@ -1329,14 +1329,14 @@ __device__ __forceinline__ void finalize_body(const FinArgs &A, double *lds) {
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) {
+static __global__ void __launch_bounds__(kBlock) k_finalize(FinArgs A) {
   __shared__ double lds[5 * (kBlock / 64)];
   finalize_body(A, lds);
 }
 
 // batch: one workgroup per cavity; the norms of step *slot go to
 // D[c].out + 8 * min(*slot, cap) (record `cap` is a discard row)
-__global__ void __launch_bounds__(kBlock) k_finalize_batch(const FinArgs *__restrict__ D, const int *slot,
+static __global__ void __launch_bounds__(kBlock) k_finalize_batch(const FinArgs *__restrict__ D, const int *slot,
                                                            int cap) {
   __shared__ double lds[5 * (kBlock / 64)];
   FinArgs A = D[blockIdx.x];
@@ -1347,7 +1347,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize_batch(const FinArgs *__rest
 }
 
 // advance the batch step counter once every cavity has read it
-__global__ void k_add_int(int *p, int v) {
+static __global__ void k_add_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = *p + v;
 }
 
@@ -1389,7 +1389,7 @@ __global__ void __launch_bounds__(BS) k_multi(MultiArgs M) {
   }
 }
 
-__global__ void k_set_int(int *p, int v) {
+static __global__ void k_set_int(int *p, int v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
