@@ -43,13 +43,39 @@ struct StoreOf<SOLVE_DELTA, R> {
   using T = double;
 };
 
+// Global-memory pointers.  A pointer the kernel reads from memory (the
+// argument tables of the batch kernels) is generic to the compiler, and a
+// generic (flat) access counts against both the vector-memory and the
+// LDS/scalar counters, so each use waits for every access in flight.  gp()
+// types a pointer as global at the access; GPtr does it for the geometry
+// tables, whose entries are indexed directly.
+template <typename T>
+__device__ __forceinline__ __attribute__((address_space(1))) T *gp(T *p) {
+  return (__attribute__((address_space(1))) T *)p;
+}
+template <typename T>
+struct GPtr {
+  T *p;
+  __host__ __device__ GPtr &operator=(T *q) {
+    p = q;
+    return *this;
+  }
+  __host__ __device__ operator T *() const { return p; }
+  __device__ __forceinline__ __attribute__((address_space(1))) T &operator[](size_t i) const {
+    return gp(p)[i];
+  }
+  __device__ __forceinline__ __attribute__((address_space(1))) T &operator[](int i) const {
+    return gp(p)[i];
+  }
+};
+
 // 1-based geometry arrays (index 0 unused), built as in
 // source/mod_malla.f90:167-217; the FP32-mixed sweeps read FP32 copies
 template <typename R>
 struct GeoT {
-  const R *dxp, *dyp, *dxu, *dyu, *dxv, *dyv;
-  const R *fexp, *feyp, *fexu, *feyv;
-  const R *xu, *yv, *xp, *yp;
+  GPtr<const R> dxp, dyp, dxu, dyu, dxv, dyv;
+  GPtr<const R> fexp, feyp, fexu, feyv;
+  GPtr<const R> xu, yv, xp, yp;
 };
 using Geo = GeoT<double>;
 using GeoF = GeoT<float>;
@@ -72,7 +98,7 @@ struct Side {
 // tables leave only multiplications (results move by a few ulp).
 template <typename R>
 struct FastGeoT {
-  const R *cw[3], *pw[3], *ce[3], *pe[3], *cs[3], *ps[3], *cn[3], *pn[3];
+  GPtr<const R> cw[3], pw[3], ce[3], pe[3], cs[3], ps[3], cn[3], pn[3];
   R inv_dt, inv_rel_vel, inv_rel_ener;
 };
 using FastGeo = FastGeoT<double>;
@@ -159,11 +185,11 @@ constexpr double kPenaltyRow = 1.0e30;
 #define IDX(i, j)                                                                 \
   (TR ? (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P)                      \
       : (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P))
-#define AT(ptr, i, j) (ptr)[IDX(i, j)]
+#define AT(ptr, i, j) (gp(ptr))[IDX(i, j)]
 // the other layout (pitch P2), for the second copy a sweep writes
 #define AO(ptr, i, j)                                                             \
-  (ptr)[TR ? (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P2)                \
-           : (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P2)]
+  (gp(ptr))[TR ? (((size_t)(i) - 1) + ((size_t)(j) - 1) * (size_t)P2)            \
+               : (((size_t)(j) - 1) + ((size_t)(i) - 1) * (size_t)P2)]
 
 // Optional operands without branches.  An absent source field (null) or an
 // unused buoyancy term reads element 0 of a zero block, or of the field,
@@ -172,7 +198,7 @@ constexpr double kPenaltyRow = 1.0e30;
 // x + 0 = x leaves the arithmetic of the present terms unchanged.
 static __constant__ double kZeroBlock[8];
 __device__ __forceinline__ const double *or_zero(const double *p) { return p ? p : kZeroBlock; }
-#define ATZ(ptr, on, i, j) (ptr)[(on) ? IDX(i, j) : (size_t)0]
+#define ATZ(ptr, on, i, j) (gp(ptr))[(on) ? IDX(i, j) : (size_t)0]
 
 __device__ __forceinline__ double pow5(double t) { return t * (t * (t * (t * t))); }
 __device__ __forceinline__ double dmax0(double x) { return x < 0.0 ? 0.0 : x; }
@@ -769,8 +795,8 @@ __device__ __forceinline__ bool bc_row(const Side &sd, int node, bool high, doub
                                        double &a2, double &rr) {
   // (plain selects: a nested or short-circuit condition is emitted as
   // branches, into which the loads are then sunk)
-  const int t = sd.type[node];
-  const double v = sd.val[node];
+  const int t = gp(sd.type)[node];
+  const double v = gp(sd.val)[node];
   const bool none = t == 0;
   const bool nlo = (t == 2) & !high, nhi = (t == 2) & high;
   const double typed_a1 = nlo ? -1.0 : 1.0;
@@ -1078,8 +1104,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       s0 = (wave_sum(acc0[0]) + wave_sum(acc0[1])) + (wave_sum(acc0[2]) + wave_sum(acc0[3]));
       if (EQ == EQ_P) s1 = (wave_sum(acc1[0]) + wave_sum(acc1[1])) + (wave_sum(acc1[2]) + wave_sum(acc1[3]));
     }
-    if (EQ == EQ_P && threadIdx.x == 0) A.part1[line] = s1;
-    if (threadIdx.x == 0) A.part0[line] = s0;
+    if (EQ == EQ_P && threadIdx.x == 0) gp(A.part1)[line] = s1;
+    if (threadIdx.x == 0) gp(A.part0)[line] = s0;
   }
 }
 
@@ -1167,9 +1193,9 @@ __device__ __forceinline__ void correct_tile(const CorrArgs &A, const int bx, co
     const int ii = i0 + r, jj = j0 + tx;  // lanes run along jj: coalesced in the transposed copy
     if (ii < 2 || ii > mi || jj < 2 || jj > nj) continue;
     const size_t o = ((size_t)jj - 1) + ((size_t)ii - 1) * (size_t)Q;
-    A.pT[o] = tp[tx][r];
-    if (ii <= mi - 1) uT[o] = tu[tx][r];
-    if (jj <= nj - 1) vT[o] = tv[tx][r];
+    gp(A.pT)[o] = tp[tx][r];
+    if (ii <= mi - 1) gp(uT)[o] = tu[tx][r];
+    if (jj <= nj - 1) gp(vT)[o] = tv[tx][r];
   }
 }
 
@@ -1264,7 +1290,7 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, dou
     s = block_sum(acc[0], red);
   else
     s = (wave_sum(acc[0]) + wave_sum(acc[1])) + (wave_sum(acc[2]) + wave_sum(acc[3]));
-  if (threadIdx.x == 0) A.part[jj] = s;
+  if (threadIdx.x == 0) gp(A.part)[jj] = s;
 }
 
 template <typename R>
@@ -1303,11 +1329,11 @@ __device__ __forceinline__ void finalize_body(const FinArgs &A, double *lds) {
   const int slot = A.slot ? *A.slot : 0;  // loaded with the partials
   double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = 0.0;
   for (int j = 2 + (int)threadIdx.x; j <= A.nj; j += kBlock) {
-    if ((A.which & 1) && j <= A.nj - 1) a0 = a0 + A.pv[j];
-    if (A.which & 2) a1 = a1 + A.pp[j];
-    if (A.which & 4) a2 = a2 + A.pb[j];
-    if (A.which & 8) a3 = a3 + A.pt[j];
-    if (A.which & 16) a4 = a4 + A.pr[j];
+    if ((A.which & 1) && j <= A.nj - 1) a0 = a0 + gp(A.pv)[j];
+    if (A.which & 2) a1 = a1 + gp(A.pp)[j];
+    if (A.which & 4) a2 = a2 + gp(A.pb)[j];
+    if (A.which & 8) a3 = a3 + gp(A.pt)[j];
+    if (A.which & 16) a4 = a4 + gp(A.pr)[j];
   }
   a0 = wave_sum(a0);
   a1 = wave_sum(a1);
@@ -1320,7 +1346,7 @@ __device__ __forceinline__ void finalize_body(const FinArgs &A, double *lds) {
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double *o = A.out + 8 * (size_t)slot;
+    __attribute__((address_space(1))) double *o = gp(A.out) + 8 * (size_t)slot;
     double r[5];
     for (int k = 0; k < 5; ++k) r[k] = (red[k][0] + red[k][1]) + (red[k][2] + red[k][3]);
     for (int k = 0; k < 5; ++k)
