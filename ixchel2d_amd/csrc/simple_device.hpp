@@ -836,9 +836,10 @@ struct Layout {
 // memory once; rows past the line or at its ends assemble at a clamped
 // interior position and are then replaced.
 // LDS of one line workgroup of BS threads, in doubles: the four line arrays
-// and the reduction scratch
+// and, for 256 threads, the reduction scratch (one wave reduces in registers;
+// leaving it out raises the batch kernels' occupancy)
 template <typename R, int EPT, int BS = kBlock>
-constexpr int line_lds() { return (4 * EPT * BS * (int)sizeof(R)) / 8 + kBlock; }
+constexpr int line_lds() { return (4 * EPT * BS * (int)sizeof(R)) / 8 + (BS == kBlock ? kBlock : 0); }
 
 // A zero the compiler must treat as a per-lane value.  Indices built on it
 // make uniform table lookups vector loads: vector loads return in order, so
