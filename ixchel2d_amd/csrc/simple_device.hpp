@@ -165,6 +165,15 @@ __device__ __forceinline__ double qdiv(double a, double b) {
   return fma(fma(-b, q, a), r, q);
 }
 __device__ __forceinline__ float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+// reciprocal of the wave solver's pivots: in FP64 the hardware reciprocal
+// and two Newton steps (error about 2^-96 before rounding, so within an ulp
+// of 1/b); in FP32 the hardware reciprocal (1 ulp)
+__device__ __forceinline__ double rcpn(double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(fma(-b, r, 1.0), r, r);
+  return fma(fma(-b, r, 1.0), r, r);
+}
+__device__ __forceinline__ float rcpn(float b) { return __builtin_amdgcn_rcpf(b); }
 // quotient of the p' assembly, shared by every solver: IEEE division in FP64
 // (the Thomas path matches upstream bit for bit), the reciprocal in FP32
 __device__ __forceinline__ double adiv(double a, double b) { return a / b; }
@@ -637,15 +646,21 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
       }
     }
     const bool hasL = A[0] != T(0);
-    // downward: L_i = spike on x[s-1]
+    // downward: L_i = spike on x[s-1].  The pivots B_i are final after this
+    // pass, so their reciprocals serve the upward pass and the back
+    // substitution too: only this pass has a reciprocal in its dependent
+    // chain.
+    T IB[MMAX];
     L[0] = A[0];
+    IB[0] = rcpn(B[0]);
 #pragma unroll
     for (int i = 1; i < MMAX; ++i) {
       if (i < M) {
-        const T f = qdiv(A[i], B[i - 1]);
+        const T f = A[i] * IB[i - 1];
         B[i] = B[i] - f * C[i - 1];
         R[i] = R[i] - f * R[i - 1];
         L[i] = -f * L[i - 1];
+        IB[i] = rcpn(B[i]);
       }
     }
     // upward: Cp_i = spike on x[e+1]
@@ -654,7 +669,7 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
       if (i == M - 1) {
         Cp[i] = C[i];
       } else if (i < M - 1) {
-        const T g = qdiv(C[i], B[i + 1]);
+        const T g = C[i] * IB[i + 1];
         R[i] = R[i] - g * R[i + 1];
         L[i] = L[i] - g * L[i + 1];
         Cp[i] = -g * Cp[i + 1];
@@ -740,7 +755,7 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
         T x;
         if (i == 0) x = zs;
         else if (i == M - 1) x = ze;
-        else x = qdiv(R[i] - (l > 0 ? L[i] * xl : T(0)) - (l < 63 ? Cp[i] * xr : T(0)), B[i]);
+        else x = (R[i] - (l > 0 ? L[i] * xl : T(0)) - (l < 63 ? Cp[i] * xr : T(0))) * IB[i];
         if (k < n) r[k] = x;
       }
     }
