@@ -695,59 +695,42 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
     } else {
       ra[1] = T(0); rb[1] = Be; rc[1] = Ce; rr[1] = Re;
     }
-    // PCR over the 128 reduced rows k = 2l + p
-    for (int st = 1; st < 128; st <<= 1) {
-      const T ib0 = qdiv(T(1), rb[0]), ib1 = qdiv(T(1), rb[1]);
-      T na[2], nb[2], nc[2], nr[2];
-      if (st == 1) {
-        // row 2l: k-1 = 2l-1 (lane l-1, p=1), k+1 = 2l+1 (own p=1)
-        // row 2l+1: k-1 = 2l (own p=0), k+1 = 2l+2 (lane l+1, p=0)
-        const T pa = __shfl_up(ra[1], 1, 64), pc = __shfl_up(rc[1], 1, 64);
-        const T pr = __shfl_up(rr[1], 1, 64), pib = __shfl_up(ib1, 1, 64);
-        const T qa = __shfl_down(ra[0], 1, 64), qc = __shfl_down(rc[0], 1, 64);
-        const T qr = __shfl_down(rr[0], 1, 64), qib = __shfl_down(ib0, 1, 64);
-        {
-          const T al = l > 0 ? -ra[0] * pib : T(0);
-          const T ga = -rc[0] * ib1;
-          na[0] = l > 0 ? al * pa : T(0);
-          nc[0] = ga * rc[1];
-          nb[0] = rb[0] + (l > 0 ? al * pc : T(0)) + ga * ra[1];
-          nr[0] = rr[0] + (l > 0 ? al * pr : T(0)) + ga * rr[1];
-        }
-        {
-          const T al = -ra[1] * ib0;
-          const T ga = l < 63 ? -rc[1] * qib : T(0);
-          na[1] = al * ra[0];
-          nc[1] = l < 63 ? ga * qc : T(0);
-          nb[1] = rb[1] + al * rc[0] + (l < 63 ? ga * qa : T(0));
-          nr[1] = rr[1] + al * rr[0] + (l < 63 ? ga * qr : T(0));
-        }
-      } else {
-        const int d = st >> 1;
-#pragma unroll
-        for (int p = 0; p < 2; ++p) {
-          const T ib = p ? ib1 : ib0;
-          const T pa = __shfl_up(ra[p], d, 64), pc = __shfl_up(rc[p], d, 64);
-          const T pr = __shfl_up(rr[p], d, 64), pib = __shfl_up(ib, d, 64);
-          const T qa = __shfl_down(ra[p], d, 64), qc = __shfl_down(rc[p], d, 64);
-          const T qr = __shfl_down(rr[p], d, 64), qib = __shfl_down(ib, d, 64);
-          const bool lo = l - d >= 0, hi = l + d <= 63;
-          const T al = lo ? -ra[p] * pib : T(0);
-          const T ga = hi ? -rc[p] * qib : T(0);
-          na[p] = lo ? al * pa : T(0);
-          nc[p] = hi ? ga * qc : T(0);
-          nb[p] = rb[p] + (lo ? al * pc : T(0)) + (hi ? ga * qa : T(0));
-          nr[p] = rr[p] + (lo ? al * pr : T(0)) + (hi ? ga * qr : T(0));
-        }
-      }
-#pragma unroll
-      for (int p = 0; p < 2; ++p) {
-        ra[p] = na[p]; rb[p] = nb[p]; rc[p] = nc[p]; rr[p] = nr[p];
-      }
+    // The 128 reduced rows: one cyclic-reduction step removes the odd
+    // unknowns z[2l+1] from the even rows, PCR solves the 64 even rows (one
+    // per lane), and the odd unknowns follow from their own rows.  Each row
+    // sends its coefficients already scaled by its pivot reciprocal, three
+    // shuffles per neighbour.
+    const T ib1 = rcpn(rb[1]);
+    const T a1s = ra[1] * ib1, c1s = rc[1] * ib1, r1s = rr[1] * ib1;
+    T A0, B0, C0, R0;
+    {
+      // even row 2l: z[2l-1] from lane l-1's odd row, z[2l+1] from its own
+      const T pa = __shfl_up(a1s, 1, 64), pc = __shfl_up(c1s, 1, 64), pr = __shfl_up(r1s, 1, 64);
+      const T al = l > 0 ? -ra[0] : T(0);
+      const T ga = -rc[0];
+      A0 = al * pa;
+      C0 = ga * c1s;
+      B0 = rb[0] + al * pc + ga * a1s;
+      R0 = rr[0] + al * pr + ga * r1s;
     }
-    const T zs = qdiv(rr[0], rb[0]), ze = qdiv(rr[1], rb[1]);
-    const T xl = __shfl_up(ze, 1, 64);    // x[s-1]
-    const T xr = __shfl_down(zs, 1, 64);  // x[e+1]
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const T ib = rcpn(B0);
+      const T as = A0 * ib, cs = C0 * ib, rs = R0 * ib;
+      const T ua = __shfl_up(as, d, 64), uc = __shfl_up(cs, d, 64), ur = __shfl_up(rs, d, 64);
+      const T da = __shfl_down(as, d, 64), dc = __shfl_down(cs, d, 64), dr = __shfl_down(rs, d, 64);
+      const T al = l - d >= 0 ? -A0 : T(0);
+      const T ga = l + d <= 63 ? -C0 : T(0);
+      const T nA = al * ua, nC = ga * dc;
+      B0 = B0 + al * uc + ga * da;
+      R0 = R0 + al * ur + ga * dr;
+      A0 = nA;
+      C0 = nC;
+    }
+    const T zs = R0 * rcpn(B0);                        // z[2l] = x[s]
+    const T xr = __shfl_down(zs, 1, 64);               // x[e+1] = z[2l+2]
+    const T ze = (rr[1] - ra[1] * zs - (l < 63 ? rc[1] * xr : T(0))) * ib1;  // x[e]
+    const T xl = __shfl_up(ze, 1, 64);                 // x[s-1]
 #pragma unroll
     for (int i = 0; i < MMAX; ++i) {
       if (i < M) {
