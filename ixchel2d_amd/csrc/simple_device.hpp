@@ -1108,8 +1108,14 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   }
 }
 
+// At least 3 waves per SIMD: the FP64 sweeps of 702-node lines need 173
+// registers unbounded (the wave solver's rows), which leaves room for 2
+// line workgroups per CU; bounded they fit in 168 without spilling, so a
+// fused launch of ~1000 lines runs in one round instead of two.
+constexpr int kMinWaves = 3;
+
 template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
-__global__ void __launch_bounds__(BS) k_sweep(SweepArgs A) {
+__global__ void __launch_bounds__(BS, kMinWaves) k_sweep(SweepArgs A) {
   __shared__ double lds[line_lds<R, EPT, BS>()];
   sweep_line<EQ, DIR, SOLVER, EPT, R, BS>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines), lds);
 }
@@ -1391,7 +1397,7 @@ struct MultiArgs {
 
 // BS = 64: one wave per line, for launches of sweeps only (no residual, no fold)
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, int EPT, typename R, int BS = kBlock>
-__global__ void __launch_bounds__(BS) k_multi(MultiArgs M) {
+__global__ void __launch_bounds__(BS, kMinWaves) k_multi(MultiArgs M) {
   __shared__ double lds[line_lds<R, EPT, BS>()];
   const int b = (int)blockIdx.x;
   if (b < M.off[1]) {
