@@ -153,7 +153,7 @@ def test_immersed_cylinder_sources():
 def test_tridiagonal_against_dense_solve(solver):
     from ixchel2d_amd.simple import tridiagonal
     rng = np.random.default_rng(3)
-    for n in (1, 2, 7, 256, 257, 702, 2048):
+    for n in (1, 2, 7, 64, 65, 128, 129, 256, 257, 702, 1024, 1025, 2048):
         nsys = 5
         a = rng.uniform(-1, 0, (nsys, n))
         c = rng.uniform(-1, 0, (nsys, n))
