@@ -631,19 +631,18 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
     const int M = (n + 63) >> 6 < 2 ? 2 : (n + 63) >> 6;
     const int s0 = l * M;
     T A[MMAX], B[MMAX], C[MMAX], R[MMAX], L[MMAX], Cp[MMAX];
+    // every read goes out before the first use (selects, no branches);
+    // rows past the line are identity rows, slots past the chunk unused
 #pragma unroll
     for (int i = 0; i < MMAX; ++i) {
-      if (i < M) {
-        const int k = s0 + i;
-        if (k < n) {
-          A[i] = k == 0 ? T(0) : a[k];
-          B[i] = b[k];
-          C[i] = k == n - 1 ? T(0) : c[k];
-          R[i] = r[k];
-        } else {
-          A[i] = T(0); B[i] = T(1); C[i] = T(0); R[i] = T(0);
-        }
-      }
+      const int k = s0 + i;
+      const bool in = (i < M) & (k < n);
+      const int kc = in ? k : 0;
+      const T av = a[kc], bv = b[kc], cv = c[kc], rv = r[kc];
+      A[i] = (in & (k != 0)) ? av : T(0);
+      B[i] = in ? bv : T(1);
+      C[i] = (in & (k != n - 1)) ? cv : T(0);
+      R[i] = in ? rv : T(0);
     }
     const bool hasL = A[0] != T(0);
     // downward: L_i = spike on x[s-1].  The pivots B_i are final after this
