@@ -619,8 +619,10 @@ __device__ __forceinline__ void pcr_lds(T *a, T *b, T *c, T *r, int n) {
 //     so that every row reads  L_i x[s-1] + B_i x_i + C_i x[e+1] = R_i ;
 //  2. the chunk end rows give two equations per lane in the chunk-boundary
 //     unknowns z[2l] = x_s, z[2l+1] = x_e; combined without divisions they
-//     form a tridiagonal system of 128 rows, solved by parallel cyclic
-//     reduction across lanes with shuffles (7 steps);
+//     form a tridiagonal system of 128 rows.  One cyclic-reduction step
+//     removes the odd unknowns, parallel cyclic reduction across lanes with
+//     shuffles solves the 64 even rows (6 steps), and each odd unknown then
+//     follows from its own row;
 //  3. each lane back-substitutes its interior rows from x[s-1] = z[2l-1] and
 //     x[e+1] = z[2l+2].
 // MMAX bounds M at compile time (registers); M itself is uniform per wave.
