@@ -747,18 +747,17 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
   __syncthreads();
 }
 
-// BS: threads of the line's workgroup (256, or 64 for one wave per line,
-// where EPT = ceil(n / 64) is also the wave solver's chunk length)
+// BS: threads of the line's workgroup (256, 128, or 64 for one wave per
+// line); the wave solver's chunk length is EPT * BS / 64 rows per lane
 template <int SOLVER, int EPT, typename T, int BS = kBlock>
 __device__ __forceinline__ void solve_line(T *a, T *b, T *c, T *r, int n) {
+  constexpr int MW = EPT * BS / 64;
   if (SOLVER == SOLVE_THOMAS)
     thomas_lds(a, b, c, r, n);
-  else if (BS == 64)
-    wpt_solve<(EPT < 2 ? 2 : EPT)>(a, b, c, r, n);
-  else if (EPT <= 4)
-    wpt_solve<4 * EPT>(a, b, c, r, n);  // SOLVE_PCR and SOLVE_PCR_EXACT
+  else if (MW <= 16)
+    wpt_solve<(MW < 2 ? 2 : MW)>(a, b, c, r, n);  // SOLVE_PCR and SOLVE_PCR_EXACT
   else
-    pcr_lds<EPT>(a, b, c, r, n);        // lines longer than 1024
+    pcr_lds<EPT>(a, b, c, r, n);                   // 256 threads, lines longer than 1024
 }
 
 // deterministic block sum: butterfly within each wave64, then the four wave
@@ -775,6 +774,32 @@ __device__ __forceinline__ double block_sum(double x, double *scratch) {
   if ((threadIdx.x & 63) == 0) scratch[threadIdx.x >> 6] = x;
   __syncthreads();
   const double s = (scratch[0] + scratch[1]) + (scratch[2] + scratch[3]);
+  __syncthreads();
+  return s;
+}
+
+// The sum of a row's terms in the order of a 256-thread workgroup: term k
+// belongs to thread k % 256, each thread adds its terms in increasing k,
+// then block_sum.  A workgroup of BS < 256 threads keeps the partial of
+// owner wave (k % 256) / 64 in acc[that wave]; with 256 threads only acc[0]
+// is used.  Every thread gets the sum; scratch: 8 doubles of LDS.
+template <int BS>
+__device__ __forceinline__ double owner_sum(const double acc[4], double *scratch) {
+  static_assert(BS == kBlock || BS == 128 || BS == 64, "workgroups of 256, 128 or 64 threads");
+  if (BS == kBlock) return block_sum(acc[0], scratch);
+  double w[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) w[i] = wave_sum(acc[i]);
+  if (BS == 64) return (w[0] + w[1]) + (w[2] + w[3]);
+  // two waves: wave v holds owners v and v + 2, exact zeros elsewhere
+  // (adding an exact zero changes no bit)
+  if ((threadIdx.x & 63) == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) scratch[(threadIdx.x >> 6) * 4 + i] = w[i];
+  }
+  __syncthreads();
+  const double s = ((scratch[0] + scratch[4]) + (scratch[1] + scratch[5])) +
+                   ((scratch[2] + scratch[6]) + (scratch[3] + scratch[7]));
   __syncthreads();
   return s;
 }
@@ -838,7 +863,9 @@ struct Layout {
 // and, for 256 threads, the reduction scratch (one wave reduces in registers;
 // leaving it out raises the batch kernels' occupancy)
 template <typename R, int EPT, int BS = kBlock>
-constexpr int line_lds() { return (4 * EPT * BS * (int)sizeof(R)) / 8 + (BS == kBlock ? kBlock : 0); }
+constexpr int line_lds() {
+  return (4 * EPT * BS * (int)sizeof(R)) / 8 + (BS == kBlock ? kBlock : (BS == 128 ? 8 : 0));
+}
 
 // A zero the compiler must treat as a per-lane value.  Indices built on it
 // make uniform table lookups vector loads: vector loads return in order, so
@@ -856,7 +883,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
                 "the FP32 sweeps use the PCR solver");
   static_assert(SOLVER != SOLVE_DELTA || sizeof(R) == 4, "the increment sweeps are FP32");
-  static_assert(BS == kBlock || BS == 64, "line workgroups of 256 threads or one wave");
+  static_assert(BS == kBlock || BS == 128 || BS == 64, "line workgroups of 256, 128 or 64 threads");
+  static_assert(BS == kBlock || SOLVER != SOLVE_THOMAS, "smaller line workgroups use the wave solver");
   using S = typename StoreOf<SOLVER, R>::T;      // stored type of the fields
   constexpr bool DELTA = SOLVER == SOLVE_DELTA && EQ != EQ_P;
   R *sa = reinterpret_cast<R *>(lds), *sb = sa + EPT * BS, *sc_ = sb + EPT * BS, *sr = sc_ + EPT * BS;
@@ -1066,9 +1094,9 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   solve_line<SOLVER, EPT, R, BS>(sa, sb, sc_, sr, n);
 
   // write the line and the carried lines, accumulate the x-sweep norms.
-  // With one wave per line, row k is summed into the partial of the thread
-  // of a 256-thread workgroup that would own it (wave (k % 256) / 64), so the
-  // norms equal the 256-thread ones bit for bit.
+  // With fewer than 256 threads per line, row k is summed into the partial
+  // of the thread of a 256-thread workgroup that would own it (wave
+  // (k % 256) / 64), so the norms equal the 256-thread ones bit for bit.
   double acc0[4] = {0.0, 0.0, 0.0, 0.0}, acc1[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
@@ -1097,13 +1125,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   }
   if (NORM) {
     double s0, s1 = 0.0;
-    if (BS == kBlock) {
-      s0 = block_sum(acc0[0], red);
-      if (EQ == EQ_P) s1 = block_sum(acc1[0], red);
-    } else {
-      s0 = (wave_sum(acc0[0]) + wave_sum(acc0[1])) + (wave_sum(acc0[2]) + wave_sum(acc0[3]));
-      if (EQ == EQ_P) s1 = (wave_sum(acc1[0]) + wave_sum(acc1[1])) + (wave_sum(acc1[2]) + wave_sum(acc1[3]));
-    }
+    s0 = owner_sum<BS>(acc0, red);
+    if (EQ == EQ_P) s1 = owner_sum<BS>(acc1, red);
     if (EQ == EQ_P && threadIdx.x == 0) gp(A.part1)[line] = s1;
     if (threadIdx.x == 0) gp(A.part0)[line] = s0;
   }
@@ -1235,24 +1258,27 @@ struct ResArgs {
   double *resu, *part;
 };
 
-// BS: threads per row (256, or 64 for one wave per row; the wave then sums
-// row positions into the partials of the 256-thread owners so the norm is
-// the same bit for bit)
-template <typename R, int BS = kBlock>
+// RB: threads per row (256, 128, or 64 for one wave per row; the whole
+// workgroup runs the row).  Fewer than 256 threads sum row positions into
+// the partials of the 256-thread owners, so the norm is the same bit for
+// bit.  Two positions per thread per pass, all loads of a pass before its
+// stores.
+template <typename R, int RB = kBlock>
 __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, double *red) {
   constexpr bool TR = false;
+  constexpr int MR = 2;
   const int P = A.P;
+  const int t = (int)threadIdx.x;
   // the stored fields in their real type, read as FP64
-  const R *u = fld<R>(A.u), *v = fld<R>(A.v), *t = fld<R>(A.t), *uant = fld<R>(A.uant);
+  const R *u = fld<R>(A.u), *v = fld<R>(A.v), *t_ = fld<R>(A.t), *uant = fld<R>(A.uant);
   const Geo &g = A.g;
   const double gm = A.ph.gm;
-  // two rows per thread per pass: all loads before the stores
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  for (int base = 2; base <= A.mi - 1; base += 2 * BS) {
-    double rv[2];
+  for (int base = 2; base <= A.mi - 1; base += MR * RB) {
+    double rv[MR];
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      const int ii = min(base + (int)threadIdx.x + m * BS, A.mi - 1);
+    for (int m = 0; m < MR; ++m) {
+      const int ii = min(base + t + m * RB, A.mi - 1);
       double ud = g.fexu[ii] * (double)AT(u, ii + 1, jj) + (1.0 - g.fexu[ii]) * (double)AT(u, ii, jj);
       double ui = g.fexu[ii - 1] * (double)AT(u, ii, jj) + (1.0 - g.fexu[ii - 1]) * (double)AT(u, ii - 1, jj);
       double vn = g.fexp[ii] * (double)AT(v, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)AT(v, ii, jj);
@@ -1275,7 +1301,7 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, dou
       double r = ai * (double)AT(u, ii - 1, jj) - ac * (double)AT(u, ii, jj) + ad * (double)AT(u, ii + 1, jj) +
                  bs * (double)AT(u, ii, jj - 1) + bn * (double)AT(u, ii, jj + 1);
       const bool buoy = A.ph.ri_x != 0.0;
-      double tint = g.fexp[ii] * (double)ATZ(t, buoy, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)ATZ(t, buoy, ii, jj);
+      double tint = g.fexp[ii] * (double)ATZ(t_, buoy, ii + 1, jj) + (1.0 - g.fexp[ii]) * (double)ATZ(t_, buoy, ii, jj);
       r = r - dx * dy * A.ph.ri_x * tint;
       r = r + dx * dy * ATZ(or_zero(A.sc), A.sc != nullptr, ii, jj);
       r = r + dx * dy * (double)AT(uant, ii, jj) / A.ph.dt;
@@ -1283,20 +1309,16 @@ __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, dou
       rv[m] = r;
     }
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-      const int ii = base + (int)threadIdx.x + m * BS;
+    for (int m = 0; m < MR; ++m) {
+      const int ii = base + t + m * RB;
       if (ii > A.mi - 1) continue;
       AT(A.resu, ii, jj) = rv[m];
-      const int w = BS == kBlock ? 0 : ((ii - 2) & 255) >> 6;
+      const int w = RB == kBlock ? 0 : ((ii - 2) & 255) >> 6;
       acc[w] = acc[w] + rv[m] * rv[m];
     }
   }
-  double s;
-  if (BS == kBlock)
-    s = block_sum(acc[0], red);
-  else
-    s = (wave_sum(acc[0]) + wave_sum(acc[1])) + (wave_sum(acc[2]) + wave_sum(acc[3]));
-  if (threadIdx.x == 0) gp(A.part)[jj] = s;
+  const double s = owner_sum<RB>(acc, red);
+  if (t == 0) gp(A.part)[jj] = s;
 }
 
 template <typename R>
@@ -1396,7 +1418,8 @@ struct MultiArgs {
   int off[4], cnt[4];
 };
 
-// BS = 64: one wave per line, for launches of sweeps only (no residual, no fold)
+// BS = 64: one wave per line, for launches of sweeps only (no residual, no
+// fold); BS = 128: two waves per line and per residual row (no fold)
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, int EPT, typename R, int BS = kBlock>
 __global__ void __launch_bounds__(BS, kMinWaves) k_multi(MultiArgs M) {
   __shared__ double lds[line_lds<R, EPT, BS>()];
@@ -1411,11 +1434,12 @@ __global__ void __launch_bounds__(BS, kMinWaves) k_multi(MultiArgs M) {
       if (w < M.cnt[1])
         sweep_line<EQ1, DIR1, SOLVER, EPT, R, BS>(M.s[1], M.s[1].line0 + xcd_line(w, M.s[1].nlines), lds);
     }
-  } else if constexpr (BS == kBlock) {
+  } else if constexpr (BS >= 128) {
     if (b < M.off[3]) {
       const int w = b - M.off[2];
-      if (w < M.cnt[2]) residual_row<typename StoreOf<SOLVER, R>::T>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
-    } else {
+      if (w < M.cnt[2])
+        residual_row<typename StoreOf<SOLVER, R>::T, BS>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
+    } else if constexpr (BS == kBlock) {
       if (b == M.off[3] && M.cnt[3]) finalize_body(M.f, lds);
     }
   }
