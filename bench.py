@@ -291,8 +291,7 @@ LAUNCH_PARTS = {
     "multi_v_y+fold": ("sweep_v_y", "finalize"),
     "step_sweep_v_x": ("sweep_v_x",),
     "step_sweep_p_y": ("sweep_p_y",),
-    "step_sweep_p_x": ("sweep_p_x",),
-    "step_correct": ("correct",),
+    "step_p_x+correct": ("sweep_p_x", "correct"),
     "multi_t_y+residual+u_y": ("sweep_t_y", "residual_u", "sweep_u_y"),
     "multi_t_x+u_x": ("sweep_t_x", "sweep_u_x"),
 }
@@ -337,7 +336,7 @@ def convergence_study(args):
 
 def time_path(case, solver, precision, steps, warmup):
     """Whole-step throughput of one cavity and the average kernel duration of
-    each of the step's 7 launches over the same timed steps."""
+    each of the step's 6 launches over the same timed steps."""
     import torch
     from ixchel2d_amd.simple import Cavity
     m = case.mesh

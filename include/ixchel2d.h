@@ -152,7 +152,7 @@ int ixh_lag(ixh_state *s);
 
 /* nsteps time steps of main.f90:391-1270 with upstream's inner-loop and
  * SIMPLE exit rules.  When every loop is single-pass (iter_ecuaci_max < 0 and
- * iter_simple_max < 0) the steps run as 7 launches per step on the state's
+ * iter_simple_max < 0) the steps run as 6 launches per step on the state's
  * stream (independent phases of consecutive steps share a launch) with no
  * host synchronisation; otherwise the errors are read back after every pass.
  * One record per SIMPLE iteration goes to hist (up to cap); *nrec receives
@@ -176,7 +176,7 @@ int ixh_tridiagonal(const double *a, double *b, const double *c, double *r, int3
  * on (ixh_set_profiling resets it): for each kernel id the total milliseconds
  * and the launch count, measured with HIP events on the state's stream.  Ids
  * 0-10 are the single kernels of the sequential step (events around each
- * launch); ids 11-17 the 7 launches of the fused step, whose events carry
+ * launch); ids 11-16 the 6 launches of the fused step, whose events carry
  * the kernels' own start and end (hipExtLaunchKernelGGL) and are read back
  * from a ring without stalling the launches. */
 int ixh_set_profiling(ixh_state *s, int32_t on);

@@ -60,14 +60,14 @@ int fail(int code, const char *fmt, ...) {
 
 enum KernelId {
   K_U_Y = 0, K_U_X, K_V_Y, K_V_X, K_P_Y, K_P_X, K_CORR, K_T_Y, K_T_X, K_RES, K_FIN,
-  K_L1, K_L2, K_L3, K_L4, K_L5, K_L6, K_L7,  // the 7 launches of the fused step
+  K_L1, K_L2, K_L3, K_L4, K_L5, K_L6,  // the 6 launches of the fused step
   K_NUM
 };
 const char *kKernelNames[K_NUM] = {"sweep_u_y", "sweep_u_x", "sweep_v_y", "sweep_v_x", "sweep_p_y",
                                    "sweep_p_x", "correct",   "sweep_t_y", "sweep_t_x", "residual_u",
                                    "finalize",  "multi_v_y+fold", "step_sweep_v_x", "step_sweep_p_y",
-                                   "step_sweep_p_x", "step_correct", "multi_t_y+residual+u_y",
-                                   "multi_t_x+u_x"};
+                                   "step_p_x+correct", "multi_t_y+residual+u_y", "multi_t_x+u_x"};
+constexpr int kParts = 6;  // launches of one fused step
 
 constexpr int kNbuf = 4;  // rotation buffers per solved field
 
@@ -219,6 +219,21 @@ template <int EQ, int DIR, int SOLVER, typename R>
 int launch_sweep_t(ixh_state *s, const SweepArgs &A, int n) {
   HIPCHK((launch_sweep_k<EQ, DIR, SOLVER, R>(ctx_of(s), A, n)));
   return 0;
+}
+
+template <int SOLVER, typename R>
+int launch_pcorr_t(ixh_state *s, const SweepArgs &A, const CorrLine &K, int n) {
+  HIPCHK((launch_pcorr_k<SOLVER, R>(ctx_of(s), A, K, n)));
+  return 0;
+}
+
+// the p' x-sweep with the correction of its lines
+int launch_pcorr(ixh_state *s, const SweepArgs &A, const CorrLine &K, int n) {
+  if (s->f32) return launch_pcorr_t<SOLVE_PCR, float>(s, A, K, n);
+  if (s->delta) return launch_pcorr_t<SOLVE_DELTA, float>(s, A, K, n);
+  if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_pcorr_t<SOLVE_THOMAS, double>(s, A, K, n);
+  if (s->prm.solver == IXH_SOLVER_PCR_EXACT) return launch_pcorr_t<SOLVE_PCR_EXACT, double>(s, A, K, n);
+  return launch_pcorr_t<SOLVE_PCR, double>(s, A, K, n);
 }
 
 template <int EQ, int DIR>
@@ -532,6 +547,22 @@ int prep_correct(ixh_state *s, CorrArgs &A, bool &active) {
   return 0;
 }
 
+// the correction as the p' x-sweep does it: corrected u and v into free
+// buffers (the sweep's other workgroups still read the old ones while it
+// runs), p in place (only its own line's workgroup reads p)
+void prep_correct_line(ixh_state *s, CorrLine &K) {
+  const int nu = pick(s->u_cur, s->u_ant, s->u_hold);
+  const int nv = pick(s->v_cur, s->v_ant, s->v_snap);
+  K.u = s->ub[nu];
+  K.uT = s->ubT[nu];
+  K.v = s->vb[nv];
+  K.vT = s->vbT[nv];
+  K.p = s->p;
+  K.pT = s->pT;
+  s->u_cur = nu;
+  s->v_cur = nv;
+}
+
 int corr_tiles_x(const ixh_state *s) { return (s->mi + 1 + kTile - 1) / kTile; }
 int corr_tiles_y(const ixh_state *s) { return (s->nj + 1 + kTile - 1) / kTile; }
 
@@ -630,7 +661,7 @@ int fast_step(ixh_state *s) {
   return 0;
 }
 
-// ---- fused step: 7 launches on the state's stream per time step
+// ---- fused step: 6 launches on the state's stream per time step
 // k_multi with sweeps (EQ0, DIR0) and (EQ1, DIR1) at a common EPT
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
 int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn) {
@@ -661,9 +692,10 @@ int multi_layout(MultiArgs &M, int n0, int n1, int nres, int nfin) {
   return M.off[3] + nfin;
 }
 
-// The 7 launches of one fused time step k (the u equation must not read T):
-//   1: v_y(k) [+ fold of step k-1]   2: v_x(k)   3: p'_y(k)   4: p'_x(k)
-//   5: correction(k)   6: T_y(k) + residual(k) + u_y(k+1)   7: T_x(k) + u_x(k+1)
+// The 6 launches of one fused time step k (the u equation must not read T):
+//   1: v_y(k) [+ fold of step k-1]   2: v_x(k)   3: p'_y(k)
+//   4: p'_x(k) + correction(k)   5: T_y(k) + residual(k) + u_y(k+1)
+//   6: T_x(k) + u_x(k+1)
 // The u sweeps of the first step run before the first step, the fold of the
 // last step after it.  u_ant of step k+1 is u of step k; the u buffer the
 // residual of step k reads as its u_ant is held out of the rotation while
@@ -671,12 +703,12 @@ int multi_layout(MultiArgs &M, int n0, int n1, int nres, int nfin) {
 struct StepLaunches {
   MultiArgs m1, m6, m7;
   SweepArgs vx, py, px;
-  CorrArgs corr;
+  CorrLine corr;
   int nwg1, nwg6, nwg7, ept1, ept6, ept7;
   bool last;
 };
 
-// arguments of the 7 launches of one step (advances the rotation as the
+// arguments of the launches of one step (advances the rotation as the
 // step does); `fold` null: no fold in launch 1 (first step)
 int build_step(ixh_state *s, bool first, bool last, StepLaunches &L) {
   memset(&L, 0, sizeof L);
@@ -689,9 +721,7 @@ int build_step(ixh_state *s, bool first, bool last, StepLaunches &L) {
   s->pc_zero = true;
   CHK(prep_p(s, IXH_DIR_Y, L.py));
   CHK(prep_p(s, IXH_DIR_X, L.px));
-  bool active = false;
-  CHK(prep_correct(s, L.corr, active));
-  if (!active) return fail(IXH_ERR_ARG, "fused step: p' is zero after its sweeps");
+  prep_correct_line(s, L.corr);
   // the T sweeps and the residual of step k read u of step k: build their
   // arguments before the u sweeps of step k+1 advance the rotation
   CHK(prep_t(s, IXH_DIR_Y, L.m6.s[0]));
@@ -719,26 +749,17 @@ int build_step(ixh_state *s, bool first, bool last, StepLaunches &L) {
   return 0;
 }
 
-// launch part k (1..7) of a built step
+// launch part k (1..kParts) of a built step
 int launch_part(ixh_state *s, const StepLaunches &L, int k) {
   switch (k) {
     case 1: return launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, L.m1, L.nwg1, L.ept1, line_len(s, EQ_V, IXH_DIR_Y));
     case 2: return launch_sweep<EQ_V, DIR_X>(s, L.vx, line_len(s, EQ_V, IXH_DIR_X));
     case 3: return launch_sweep<EQ_P, DIR_Y>(s, L.py, line_len(s, EQ_P, IXH_DIR_Y));
-    case 4: return launch_sweep<EQ_P, DIR_X>(s, L.px, line_len(s, EQ_P, IXH_DIR_X));
-    case 5: {
-      dim3 block(kBlock), grid(corr_tiles_x(s), corr_tiles_y(s));
-      if (s->f32)
-        IXH_LAUNCH(s, k_correct<float>, grid, block, L.corr);
-      else
-        IXH_LAUNCH(s, k_correct<double>, grid, block, L.corr);
-      HIPCHK(hipGetLastError());
-      return 0;
-    }
-    case 6:
+    case 4: return launch_pcorr(s, L.px, L.corr, line_len(s, EQ_P, IXH_DIR_X));
+    case 5:
       if (L.last) return launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, L.m6, L.nwg6, L.ept6, s->nj + 1);
       return launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, L.m6, L.nwg6, L.ept6, s->nj + 1);
-    case 7:
+    case 6:
       if (L.last) return launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, L.m7, L.nwg7, L.ept7, s->mi + 1);
       return launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, L.m7, L.nwg7, L.ept7, s->mi + 1);
   }
@@ -747,11 +768,11 @@ int launch_part(ixh_state *s, const StepLaunches &L, int k) {
 
 constexpr int kRing = 32;  // steps of launch events in flight
 
-// add the durations of the 7 launches of ring slot `slot`
+// add the durations of the launches of ring slot `slot`
 int harvest_slot(ixh_state *s, int slot) {
-  hipEvent_t *e = &s->lev[(size_t)slot * 14];
-  HIPCHK(hipEventSynchronize(e[13]));
-  for (int k = 0; k < 7; ++k) {
+  hipEvent_t *e = &s->lev[(size_t)slot * 2 * kParts];
+  HIPCHK(hipEventSynchronize(e[2 * kParts - 1]));
+  for (int k = 0; k < kParts; ++k) {
     float ms = 0.f;
     HIPCHK(hipEventElapsedTime(&ms, e[2 * k], e[2 * k + 1]));
     s->kms[K_L1 + k] += ms;
@@ -769,15 +790,15 @@ int fused_step(ixh_state *s, bool first, bool last, int k) {
   const int slot = k % kRing;
   if (timed) {
     if (s->lev.empty()) {
-      s->lev.resize((size_t)kRing * 14);
+      s->lev.resize((size_t)kRing * 2 * kParts);
       for (hipEvent_t &e : s->lev) HIPCHK(hipEventCreate(&e));
     }
     if (k >= kRing) CHK(harvest_slot(s, slot));
   }
-  for (int p = 1; p <= 7; ++p) {
+  for (int p = 1; p <= kParts; ++p) {
     if (timed) {
-      s->t_start = s->lev[(size_t)slot * 14 + 2 * (p - 1)];
-      s->t_stop = s->lev[(size_t)slot * 14 + 2 * (p - 1) + 1];
+      s->t_start = s->lev[(size_t)slot * 2 * kParts + 2 * (p - 1)];
+      s->t_stop = s->lev[(size_t)slot * 2 * kParts + 2 * (p - 1) + 1];
     }
     const int rc = launch_part(s, L, p);
     s->t_start = s->t_stop = nullptr;

@@ -130,6 +130,14 @@ struct SweepArgs {
   int in_zero;            // p' input identically zero (first pass)
 };
 
+// Outputs of the velocity and pressure correction (main.f90:931-962) when
+// the p' x-sweep does it for its own line: corrected u and v into buffers of
+// their own (row layout and transposed copy), p in place.  The sweep reads
+// u, v, a_u, a_v and p through its SweepArgs.
+struct CorrLine {
+  double *u, *v, *uT, *vT, *p, *pT;
+};
+
 // a field pointer seen as R
 template <typename R>
 __device__ __forceinline__ const R *fld(const double *p) {
@@ -543,16 +551,19 @@ __device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, 
 // Thomas without pivoting in upstream's order (mod_solucionador.f90:25-47),
 // one lane; a[0] and c[n-1] are not read.
 template <typename T>
-__device__ __forceinline__ void thomas_lds(T *a, T *b, T *c, T *r, int n) {
-  if (threadIdx.x == 0) {
-    for (int k = 1; k < n; ++k) {
-      T m = a[k] / b[k - 1];
-      r[k] = r[k] - m * r[k - 1];
-      b[k] = b[k] - m * c[k - 1];
-    }
-    r[n - 1] = r[n - 1] / b[n - 1];
-    for (int k = n - 2; k >= 0; --k) r[k] = (r[k] - c[k] * r[k + 1]) / b[k];
+__device__ __forceinline__ void thomas_rows(T *a, T *b, T *c, T *r, int n) {
+  for (int k = 1; k < n; ++k) {
+    T m = a[k] / b[k - 1];
+    r[k] = r[k] - m * r[k - 1];
+    b[k] = b[k] - m * c[k - 1];
   }
+  r[n - 1] = r[n - 1] / b[n - 1];
+  for (int k = n - 2; k >= 0; --k) r[k] = (r[k] - c[k] * r[k + 1]) / b[k];
+}
+
+template <typename T>
+__device__ __forceinline__ void thomas_lds(T *a, T *b, T *c, T *r, int n) {
+  if (threadIdx.x == 0) thomas_rows(a, b, c, r, n);
   __syncthreads();
 }
 
@@ -627,9 +638,8 @@ __device__ __forceinline__ void pcr_lds(T *a, T *b, T *c, T *r, int n) {
 //     x[e+1] = z[2l+2].
 // MMAX bounds M at compile time (registers); M itself is uniform per wave.
 template <int MMAX, typename T>
-__device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
-  if (threadIdx.x < 64) {
-    const int l = threadIdx.x;
+__device__ __forceinline__ void wpt_body(T *a, T *b, T *c, T *r, int n, const int l) {
+  {
     const int M = (n + 63) >> 6 < 2 ? 2 : (n + 63) >> 6;
     const int s0 = l * M;
     T A[MMAX], B[MMAX], C[MMAX], R[MMAX], L[MMAX], Cp[MMAX];
@@ -744,6 +754,12 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
       }
     }
   }
+}
+
+// the wave solver run by the first wave of the workgroup
+template <int MMAX, typename T>
+__device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
+  if (threadIdx.x < 64) wpt_body<MMAX>(a, b, c, r, n, (int)threadIdx.x);
   __syncthreads();
 }
 
@@ -758,6 +774,27 @@ __device__ __forceinline__ void solve_line(T *a, T *b, T *c, T *r, int n) {
     wpt_solve<(MW < 2 ? 2 : MW)>(a, b, c, r, n);  // SOLVE_PCR and SOLVE_PCR_EXACT
   else
     pcr_lds<EPT>(a, b, c, r, n);                   // 256 threads, lines longer than 1024
+}
+
+// two lines of n rows at once in a 256-thread workgroup (sets a..r and
+// a2..r2): the first wave solves the first, the second wave the second;
+// lines longer than 1024 rows, one after the other
+template <int SOLVER, int EPT, typename T>
+__device__ __forceinline__ void solve_two(T *a, T *b, T *c, T *r, T *a2, T *b2, T *c2, T *r2, int n) {
+  constexpr int MW = EPT * kBlock / 64;
+  if (SOLVER == SOLVE_THOMAS) {
+    if (threadIdx.x == 0) thomas_rows(a, b, c, r, n);
+    if (threadIdx.x == 64) thomas_rows(a2, b2, c2, r2, n);
+    __syncthreads();
+  } else if (MW <= 16) {
+    const int w = (int)threadIdx.x >> 6, l = (int)threadIdx.x & 63;
+    if (w == 0) wpt_body<(MW < 2 ? 2 : MW)>(a, b, c, r, n, l);
+    if (w == 1) wpt_body<(MW < 2 ? 2 : MW)>(a2, b2, c2, r2, n, l);
+    __syncthreads();
+  } else {
+    pcr_lds<EPT>(a, b, c, r, n);
+    pcr_lds<EPT>(a2, b2, c2, r2, n);
+  }
 }
 
 // deterministic block sum: butterfly within each wave64, then the four wave
@@ -877,8 +914,16 @@ __device__ __forceinline__ int lane_zero() {
   return z;
 }
 
-template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
-__device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s, double *lds) {
+// CORR (p' x-sweep only, 256 threads): the workgroup also assembles and
+// solves line + 1 (its second wave, beside the first), then corrects u, v
+// and p on its own line from p' of both lines, so the correction needs no
+// launch of its own.  Line + 1's solve repeats the one its own workgroup
+// does, on the same inputs, so both give the same bits; only the owner
+// writes p' and the norms.  LDS: twice line_lds.
+template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock, bool CORR = false>
+__device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s, double *lds,
+                                           const CorrLine *K = nullptr) {
+  static_assert(!CORR || (EQ == EQ_P && DIR == DIR_X && BS == kBlock), "the correction rides on the p' x-sweep");
   const int line = line_s + lane_zero();
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
                 "the FP32 sweeps use the PCR solver");
@@ -889,6 +934,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   constexpr bool DELTA = SOLVER == SOLVE_DELTA && EQ != EQ_P;
   R *sa = reinterpret_cast<R *>(lds), *sb = sa + EPT * BS, *sc_ = sb + EPT * BS, *sr = sc_ + EPT * BS;
   double *red = lds + (4 * EPT * BS * (int)sizeof(R)) / 8;
+  // line + 1 (CORR): after the first set and its reduction scratch
+  R *sa2 = reinterpret_cast<R *>(red + kBlock), *sb2 = sa2 + EPT * BS, *sc2 = sb2 + EPT * BS, *sr2 = sc2 + EPT * BS;
   constexpr bool TR = DIR == DIR_Y;
   constexpr bool MOM = EQ == EQ_U || EQ == EQ_V;
   constexpr bool NORM = DIR == DIR_X && EQ != EQ_U;
@@ -925,9 +972,50 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     return DIR == DIR_X ? AT(f, q, line) : AT(f, line, q);
   };
 
+  // the p' row at (ii, jj): stencil, right-hand side, mass residual b_o
+  auto p_row = [&](int ii, int jj, double &a0, double &a1, double &a2, double &rr, double &bval) {
+    const PCoefT<R> c = p_coef<TR, R, S>(A, ii, jj);
+    // p' = 0 at the first pass: the loads still go out (the buffer is
+    // allocated), their values are dropped
+    const S *pcf = fld<S>(A.pc);
+    const bool z = A.in_zero != 0;
+    const double pcv = (double)AT(pcf, ii, jj);
+    double pcP = z ? 0.0 : pcv;
+    double head;
+    if (DIR == DIR_X) {
+      a0 = c.ai; a2 = c.ad;
+      const double s0 = (double)AT(pcf, ii, jj - 1), n0 = (double)AT(pcf, ii, jj + 1);
+      double pS = z ? 0.0 : s0;
+      double pN = z ? 0.0 : n0;
+      head = -(double)c.bs * pS - (double)c.bn * pN;
+      bval = c.b;
+    } else {
+      a0 = c.bs; a2 = c.bn;
+      const double w0 = (double)AT(pcf, ii - 1, jj), e0 = (double)AT(pcf, ii + 1, jj);
+      double pW = z ? 0.0 : w0;
+      double pE = z ? 0.0 : e0;
+      head = -(double)c.ai * pW - (double)c.ad * pE;
+    }
+    a1 = c.ac;
+    rr = head + c.b + (1.0 - A.ph.rel_pres) * (double)c.ac * pcP;
+    if constexpr (sizeof(R) == 4) {
+      // every neighbour behind an infinite a_u, a_v (inside a body): p' = 0
+      const bool shut = c.ac == R(0);
+      a0 = shut ? 0.0 : a0;
+      a1 = shut ? 1.0 : a1;
+      a2 = shut ? 0.0 : a2;
+      rr = shut ? 0.0 : rr;
+    }
+  };
+
   R keep_aw[EPT];
   S carry[EPT];
   double keep_old[EPT], keep_b[EPT], keep_phi[EPT];
+  // CORR: u, v, p, a_u, a_v and dx_v at this line's positions (pre-correction),
+  // u and v of the carried boundary line
+  double keep_u[EPT], keep_v[EPT], keep_p[EPT], keep_au[EPT], keep_av[EPT], keep_dxv[EPT];
+  S keep_ub[EPT], keep_vb[EPT];
+  const double dyu_line = CORR ? (double)A.g.dyu[line] : 0.0;
   // FP32 fields read a zero block as zeros too
   const S *snap = A.snap ? fld<S>(A.snap) : reinterpret_cast<const S *>(kZeroBlock);
   const int oline = other ? other : line;  // a valid line to load from when none is carried
@@ -946,38 +1034,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0, bval = 0.0;
     R aw = R(0);
     if (EQ == EQ_P) {
-      const PCoefT<R> c = p_coef<TR, R, S>(A, ii, jj);
-      // p' = 0 at the first pass: the loads still go out (the buffer is
-      // allocated), their values are dropped
-      const S *pcf = fld<S>(A.pc);
-      const bool z = A.in_zero != 0;
-      const double pcv = (double)AT(pcf, ii, jj);
-      double pcP = z ? 0.0 : pcv;
-      double head;
-      if (DIR == DIR_X) {
-        a0 = c.ai; a2 = c.ad;
-        const double s0 = (double)AT(pcf, ii, jj - 1), n0 = (double)AT(pcf, ii, jj + 1);
-        double pS = z ? 0.0 : s0;
-        double pN = z ? 0.0 : n0;
-        head = -(double)c.bs * pS - (double)c.bn * pN;
-        bval = c.b;
-      } else {
-        a0 = c.bs; a2 = c.bn;
-        const double w0 = (double)AT(pcf, ii - 1, jj), e0 = (double)AT(pcf, ii + 1, jj);
-        double pW = z ? 0.0 : w0;
-        double pE = z ? 0.0 : e0;
-        head = -(double)c.ai * pW - (double)c.ad * pE;
-      }
-      a1 = c.ac;
-      rr = head + c.b + (1.0 - A.ph.rel_pres) * (double)c.ac * pcP;
-      if constexpr (sizeof(R) == 4) {
-        // every neighbour behind an infinite a_u, a_v (inside a body): p' = 0
-        const bool shut = c.ac == R(0);
-        a0 = shut ? 0.0 : a0;
-        a1 = shut ? 1.0 : a1;
-        a2 = shut ? 0.0 : a2;
-        rr = shut ? 0.0 : rr;
-      }
+      p_row(ii, jj, a0, a1, a2, rr, bval);
     } else if constexpr (SOLVER == SOLVE_DELTA) {
       R area;
       double pin;
@@ -1063,12 +1120,52 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       const S cv = AT(src, io, jo);
       carry[e] = (EQ == EQ_P && A.in_zero) ? S(0) : cv;
     }
+    if constexpr (CORR) {
+      // the correction's operands (positions clamped into its ranges) and u,
+      // v of the whole line and of the boundary line it carries, which go to
+      // the new buffers unchanged
+      const int ic = min(max(pos, 2), mi), iu = min(ic, mi - 1), jv = min(line, nj - 1);
+      const int iul = min(pos, mi), ivl = min(pos, mi + 1);
+      const S *uo = fld<S>(A.u), *vo = fld<S>(A.v);
+      keep_p[e] = AT(A.p, ic, line);
+      keep_u[e] = (double)AT(uo, iul, line);
+      keep_au[e] = (double)AT(fld<S>(A.au), iu, line);
+      keep_v[e] = (double)AT(vo, ivl, line);
+      keep_av[e] = (double)AT(fld<S>(A.av), ic, jv);
+      keep_dxv[e] = A.g.dxv[ic];
+      keep_ub[e] = ATZ(uo, other != 0, iul, oline);
+      keep_vb[e] = ATZ(vo, other != 0, ivl, min(oline, nj));
+    }
     // rows past the line fill LDS the solver does not read (no branch, so
     // the next row's loads are not held behind this one's stores)
     sa[k] = R(a0);
     sb[k] = R(a1);
     sc_[k] = R(a2);
     sr[k] = R(rr);
+  }
+  if constexpr (CORR) {
+    // line + 1 (the last line repeats itself), rows as the owner builds them
+    const int line2 = min(line + 1, lastl);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int k = threadIdx.x + e * BS;
+      const int pos = k + 1;
+      double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0, bval = 0.0;
+      p_row(min(max(pos, 2), hi), line2, a0, a1, a2, rr, bval);
+      const bool lo = pos == 1, hi_row = pos == n;
+      a0 = hi_row ? hb0 : a0;
+      a1 = hi_row ? hb1 : a1;
+      a2 = hi_row ? hb2 : a2;
+      rr = hi_row ? hbr : rr;
+      a0 = lo ? lb0 : a0;
+      a1 = lo ? lb1 : a1;
+      a2 = lo ? lb2 : a2;
+      rr = lo ? lbr : rr;
+      sa2[k] = R(a0);
+      sb2[k] = R(a1);
+      sc2[k] = R(a2);
+      sr2[k] = R(rr);
+    }
   }
   // global stores of the assembly: a_u / a_v (x-sweeps of u, v; typed
   // boundary faces in both sweeps), b_o (p' x-sweep, row layout)
@@ -1091,7 +1188,10 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     }
   }
   __syncthreads();
-  solve_line<SOLVER, EPT, R, BS>(sa, sb, sc_, sr, n);
+  if constexpr (CORR)
+    solve_two<SOLVER, EPT, R>(sa, sb, sc_, sr, sa2, sb2, sc2, sr2, n);
+  else
+    solve_line<SOLVER, EPT, R, BS>(sa, sb, sc_, sr, n);
 
   // write the line and the carried lines, accumulate the x-sweep norms.
   // With fewer than 256 threads per line, row k is summed into the partial
@@ -1130,6 +1230,52 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     if (EQ == EQ_P && threadIdx.x == 0) gp(A.part1)[line] = s1;
     if (threadIdx.x == 0) gp(A.part0)[line] = s0;
   }
+  if constexpr (CORR) {
+    // p += 0.6 p' (ii 2..mi), u += dy_u (p'_P - p'_E) / a_u (ii 2..mi-1),
+    // v += dx_v (p'_P - p'_N) / a_v (ii 2..mi, jj <= nj-1), with p' as
+    // stored; p in place, u and v into the new buffers, which also receive
+    // every other u, v value of the line (and of the carried boundary line)
+    // unchanged
+    S *u1 = fldw<S>(K->u), *v1 = fldw<S>(K->v), *u1T = fldw<S>(K->uT), *v1T = fldw<S>(K->vT);
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int k = threadIdx.x + e * BS;
+      const int pos = k + 1;
+      if (pos > n) continue;
+      const double pcP = (double)S(sr[k]);
+      if (pos >= 2 && pos <= mi) {
+        const double pn = keep_p[e] + 0.6 * pcP;
+        AT(K->p, pos, line) = pn;
+        AO(K->pT, pos, line) = pn;
+      }
+      if (pos <= mi) {
+        S un = S(keep_u[e]);
+        if (pos >= 2 && pos <= mi - 1) {
+          const double pcE = (double)S(sr[k + 1]);
+          un = S(keep_u[e] + dyu_line * (pcP - pcE) / keep_au[e]);
+        }
+        AT(u1, pos, line) = un;
+        AO(u1T, pos, line) = un;
+      }
+      S vn = S(keep_v[e]);
+      if (pos >= 2 && pos <= mi && line <= nj - 1) {
+        const double pcN = (double)S(sr2[k]);
+        vn = S(keep_v[e] + keep_dxv[e] * (pcP - pcN) / keep_av[e]);
+      }
+      AT(v1, pos, line) = vn;
+      AO(v1T, pos, line) = vn;
+      if (other) {
+        if (pos <= mi) {
+          AT(u1, pos, other) = keep_ub[e];
+          AO(u1T, pos, other) = keep_ub[e];
+        }
+        if (other <= nj) {
+          AT(v1, pos, other) = keep_vb[e];
+          AO(v1T, pos, other) = keep_vb[e];
+        }
+      }
+    }
+  }
 }
 
 // At least 3 waves per SIMD: the FP64 sweeps of 702-node lines need 173
@@ -1142,6 +1288,13 @@ template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
 __global__ void __launch_bounds__(BS, kMinWaves) k_sweep(SweepArgs A) {
   __shared__ double lds[line_lds<R, EPT, BS>()];
   sweep_line<EQ, DIR, SOLVER, EPT, R, BS>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines), lds);
+}
+
+// p' x-sweep with the velocity and pressure correction of its lines
+template <int SOLVER, int EPT, typename R>
+__global__ void __launch_bounds__(kBlock, kMinWaves) k_pcorr_x(SweepArgs A, CorrLine K) {
+  __shared__ double lds[2 * line_lds<R, EPT>()];
+  sweep_line<EQ_P, DIR_X, SOLVER, EPT, R, kBlock, true>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines), lds, &K);
 }
 
 // A batch of cavities: workgroup b runs line wg[b].y of cavity wg[b].x, whose

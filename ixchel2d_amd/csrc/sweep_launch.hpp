@@ -31,13 +31,13 @@ constexpr int ept_for(int n) {
 // (13.4 us against 16.6 us for v_x), so single sweeps keep that shape.
 constexpr int kWaveLine = 5 * 64;
 
-// A fused launch with the u residual rows beside its sweeps (launch 6 of a
-// time step: ~300 + ~300 y-lines and ~700 rows at 301x701) runs 128 threads
+// A fused launch with the u residual rows beside its sweeps (the T y-sweep
+// launch of a time step: ~300 + ~300 y-lines and ~700 rows at 301x701) runs 128 threads
 // per line and per row when its lines have at most this many nodes.  Its
 // sweep kernels hold 168 registers, 3 waves per SIMD: that is 3 workgroups
 // of 256 threads per CU, 768 on the GPU, so 1300 workgroups ran in two
 // rounds; 6 of 128 threads per CU hold 1536.  All at once, each workgroup
-// runs longer (the residual rows 10 us instead of 4): FP32-mixed launch 6
+// runs longer (the residual rows 10 us instead of 4): FP32-mixed, that launch
 // takes 12.0 us instead of 13.7, FP64 15.9 instead of 16.0.
 constexpr int kPairLine = 1024;
 
@@ -57,6 +57,11 @@ hipError_t launch_sweep_k(const LaunchCtx &c, const SweepArgs &A, int n);
 // a fused launch of nwg workgroups (k_multi); maxn: longest line of its sweeps
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
 hipError_t launch_multi_k(const LaunchCtx &c, const MultiArgs &M, int nwg, int ept, int maxn);
+
+// the p' x-sweep of A.nlines lines of n nodes with the correction of those
+// lines (k_pcorr_x), one 256-thread workgroup per line
+template <int SOLVER, typename R>
+hipError_t launch_pcorr_k(const LaunchCtx &c, const SweepArgs &A, const CorrLine &K, int n);
 
 // one sweep phase of a batch: nwg workgroups mapped by wg to (cavity, line);
 // wave_lines: one 64-thread wave per line with ept64 rows per lane
@@ -84,6 +89,20 @@ hipError_t launch_sweep_k(const LaunchCtx &c, const SweepArgs &A, int n) {
     case 4: launch_on(c, k_sweep<EQ, DIR, SOLVER, 4, R>, grid, block, A); break;
     case 6: launch_on(c, k_sweep<EQ, DIR, SOLVER, 6, R>, grid, block, A); break;
     default: launch_on(c, k_sweep<EQ, DIR, SOLVER, 8, R>, grid, block, A); break;
+  }
+  return hipGetLastError();
+}
+
+template <int SOLVER, typename R>
+hipError_t launch_pcorr_k(const LaunchCtx &c, const SweepArgs &A, const CorrLine &K, int n) {
+  const dim3 grid(A.nlines), block(kBlock);
+  switch (ept_for(n)) {
+    case 1: launch_on(c, k_pcorr_x<SOLVER, 1, R>, grid, block, A, K); break;
+    case 2: launch_on(c, k_pcorr_x<SOLVER, 2, R>, grid, block, A, K); break;
+    case 3: launch_on(c, k_pcorr_x<SOLVER, 3, R>, grid, block, A, K); break;
+    case 4: launch_on(c, k_pcorr_x<SOLVER, 4, R>, grid, block, A, K); break;
+    case 6: launch_on(c, k_pcorr_x<SOLVER, 6, R>, grid, block, A, K); break;
+    default: launch_on(c, k_pcorr_x<SOLVER, 8, R>, grid, block, A, K); break;
   }
   return hipGetLastError();
 }
@@ -169,6 +188,7 @@ hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *
   template hipError_t launch_multi_k<EQ0, DIR0, EQ1, DIR1, SOLVER, R>(const LaunchCtx &, \
                                                                       const MultiArgs &, int, int, int);
 #define IXH_INSTANTIATE_SWEEPS(SOLVER, R)              \
+  template hipError_t launch_pcorr_k<SOLVER, R>(const LaunchCtx &, const SweepArgs &, const CorrLine &, int); \
   IXH_SWEEP_ONE(EQ_U, DIR_Y, SOLVER, R)                \
   IXH_SWEEP_ONE(EQ_U, DIR_X, SOLVER, R)                \
   IXH_SWEEP_ONE(EQ_V, DIR_Y, SOLVER, R)                \
