@@ -972,8 +972,10 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     return DIR == DIR_X ? AT(f, q, line) : AT(f, line, q);
   };
 
-  // the p' row at (ii, jj): stencil, right-hand side, mass residual b_o
-  auto p_row = [&](int ii, int jj, double &a0, double &a1, double &a2, double &rr, double &bval) {
+  // the p' row at (ii, jj): stencil, right-hand side, mass residual b_o;
+  // clamp_n: the north p' read stays on the last row of p' (row nj + 1)
+  auto p_row = [&](int ii, int jj, double &a0, double &a1, double &a2, double &rr, double &bval,
+                   bool clamp_n) {
     const PCoefT<R> c = p_coef<TR, R, S>(A, ii, jj);
     // p' = 0 at the first pass: the loads still go out (the buffer is
     // allocated), their values are dropped
@@ -984,7 +986,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     double head;
     if (DIR == DIR_X) {
       a0 = c.ai; a2 = c.ad;
-      const double s0 = (double)AT(pcf, ii, jj - 1), n0 = (double)AT(pcf, ii, jj + 1);
+      const int jn = clamp_n ? min(jj + 1, nj + 1) : jj + 1;
+      const double s0 = (double)AT(pcf, ii, jj - 1), n0 = (double)AT(pcf, ii, jn);
       double pS = z ? 0.0 : s0;
       double pN = z ? 0.0 : n0;
       head = -(double)c.bs * pS - (double)c.bn * pN;
@@ -1016,6 +1019,9 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   double keep_u[EPT], keep_v[EPT], keep_p[EPT], keep_au[EPT], keep_av[EPT], keep_dxv[EPT];
   S keep_ub[EPT], keep_vb[EPT];
   const double dyu_line = CORR ? (double)A.g.dyu[line] : 0.0;
+  const double u_first = CORR ? (double)AT(fld<S>(A.u), 1, line) : 0.0;
+  const double v_first = CORR ? (double)AT(fld<S>(A.v), 1, line) : 0.0;
+  const double v_last = CORR ? (double)AT(fld<S>(A.v), mi + 1, line) : 0.0;
   // FP32 fields read a zero block as zeros too
   const S *snap = A.snap ? fld<S>(A.snap) : reinterpret_cast<const S *>(kZeroBlock);
   const int oline = other ? other : line;  // a valid line to load from when none is carried
@@ -1034,7 +1040,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0, bval = 0.0;
     R aw = R(0);
     if (EQ == EQ_P) {
-      p_row(ii, jj, a0, a1, a2, rr, bval);
+      p_row(ii, jj, a0, a1, a2, rr, bval, false);
     } else if constexpr (SOLVER == SOLVE_DELTA) {
       R area;
       double pin;
@@ -1121,18 +1127,20 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       carry[e] = (EQ == EQ_P && A.in_zero) ? S(0) : cv;
     }
     if constexpr (CORR) {
-      // the correction's operands (positions clamped into its ranges) and u,
-      // v of the whole line and of the boundary line it carries, which go to
-      // the new buffers unchanged
-      const int ic = min(max(pos, 2), mi), iu = min(ic, mi - 1), jv = min(line, nj - 1);
+      // the correction's operands and u, v of the whole line and of the
+      // boundary line it carries, which go to the new buffers unchanged.
+      // Interior positions read at the row's own (ii, jj), the same loads as
+      // the row's coefficients; the end positions add u(1), v(1), v(mi+1).
       const int iul = min(pos, mi), ivl = min(pos, mi + 1);
       const S *uo = fld<S>(A.u), *vo = fld<S>(A.v);
-      keep_p[e] = AT(A.p, ic, line);
-      keep_u[e] = (double)AT(uo, iul, line);
-      keep_au[e] = (double)AT(fld<S>(A.au), iu, line);
-      keep_v[e] = (double)AT(vo, ivl, line);
-      keep_av[e] = (double)AT(fld<S>(A.av), ic, jv);
-      keep_dxv[e] = A.g.dxv[ic];
+      keep_p[e] = AT(A.p, ii, line);
+      const double u_in = (double)AT(uo, ii, line), v_in = (double)AT(vo, ii, line);
+      keep_u[e] = pos == 1 ? u_first : u_in;
+      keep_v[e] = pos == 1 ? v_first : v_in;
+      keep_v[e] = pos == mi + 1 ? v_last : keep_v[e];
+      keep_au[e] = (double)AT(fld<S>(A.au), ii, line);
+      keep_av[e] = (double)AT(fld<S>(A.av), ii, line);
+      keep_dxv[e] = A.g.dxv[ii];
       keep_ub[e] = ATZ(uo, other != 0, iul, oline);
       keep_vb[e] = ATZ(vo, other != 0, ivl, min(oline, nj));
     }
@@ -1144,14 +1152,16 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     sr[k] = R(rr);
   }
   if constexpr (CORR) {
-    // line + 1 (the last line repeats itself), rows as the owner builds them
-    const int line2 = min(line + 1, lastl);
+    // line + 1, rows as its owner builds them; past the last line (row
+    // nj + 1, allocated in every field) the rows are not used, and their
+    // north p' read stays in the array
+    const int line2 = line + 1;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int k = threadIdx.x + e * BS;
       const int pos = k + 1;
       double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0, bval = 0.0;
-      p_row(min(max(pos, 2), hi), line2, a0, a1, a2, rr, bval);
+      p_row(min(max(pos, 2), hi), line2, a0, a1, a2, rr, bval, true);
       const bool lo = pos == 1, hi_row = pos == n;
       a0 = hi_row ? hb0 : a0;
       a1 = hi_row ? hb1 : a1;
