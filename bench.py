@@ -288,11 +288,11 @@ def adjoint_bench(args):
 
 
 LAUNCH_PARTS = {
-    "multi_v_y+fold": ("sweep_v_y", "finalize"),
-    "step_sweep_v_x": ("sweep_v_x",),
+    "multi_v_y+residual": ("sweep_v_y", "residual_u"),
+    "multi_v_x+fold": ("sweep_v_x", "finalize"),
     "step_sweep_p_y": ("sweep_p_y",),
     "step_p_x+correct": ("sweep_p_x", "correct"),
-    "multi_t_y+residual+u_y": ("sweep_t_y", "residual_u", "sweep_u_y"),
+    "multi_t_y+u_y": ("sweep_t_y", "sweep_u_y"),
     "multi_t_x+u_x": ("sweep_t_x", "sweep_u_x"),
 }
 

@@ -65,8 +65,8 @@ enum KernelId {
 };
 const char *kKernelNames[K_NUM] = {"sweep_u_y", "sweep_u_x", "sweep_v_y", "sweep_v_x", "sweep_p_y",
                                    "sweep_p_x", "correct",   "sweep_t_y", "sweep_t_x", "residual_u",
-                                   "finalize",  "multi_v_y+fold", "step_sweep_v_x", "step_sweep_p_y",
-                                   "step_p_x+correct", "multi_t_y+residual+u_y", "multi_t_x+u_x"};
+                                   "finalize",  "multi_v_y+residual", "multi_v_x+fold", "step_sweep_p_y",
+                                   "step_p_x+correct", "multi_t_y+u_y", "multi_t_x+u_x"};
 constexpr int kParts = 6;  // launches of one fused step
 
 constexpr int kNbuf = 4;  // rotation buffers per solved field
@@ -79,7 +79,9 @@ struct ixh_state {
   hipStream_t stream = nullptr;
   bool own_stream = false;
   hipStream_t launch = nullptr;  // the stream kernels go to (the state's stream)
-  int u_hold = -1;  // u buffer still read by a residual running beside the next u sweeps
+  int u_hold = -1;  // u buffer still read by a residual running after the next u sweeps
+  int pv_alt = 0;   // which of the two v norm partial slots the v x-sweep writes
+  ResArgs res_next{};  // fused steps: the residual of the last built step
   ixh_params prm{};
   // FP32-mixed: u, v, T, p' (all copies) and a_u, a_v are stored as float in
   // the same buffers (same element pitch); p, b_o, Res_u stay FP64
@@ -404,7 +406,7 @@ int prep_v(ixh_state *s, int dir, SweepArgs &A) {
   } else {
     out = pick(s->v_cur, s->v_ant, s->v_snap);
     A.snap = s->v_snap >= 0 ? s->vb[s->v_snap] : s->vb[s->v_cur];
-    A.part0 = part_of(s, 0);
+    A.part0 = part_of(s, s->pv_alt ? 5 : 0);
     A.nlines = s->nj - 2;
   }
   set_out(A, dir, s->vb, s->vbT, out);
@@ -613,7 +615,7 @@ int do_residual(ixh_state *s) {
 
 void prep_finalize(ixh_state *s, FinArgs &F, int which, double *out, int *slot) {
   F.nj = s->nj;
-  F.pv = part_of(s, 0);
+  F.pv = part_of(s, s->pv_alt ? 5 : 0);
   F.pp = part_of(s, 1);
   F.pb = part_of(s, 2);
   F.pt = part_of(s, 3);
@@ -693,57 +695,64 @@ int multi_layout(MultiArgs &M, int n0, int n1, int nres, int nfin) {
 }
 
 // The 6 launches of one fused time step k (the u equation must not read T):
-//   1: v_y(k) [+ fold of step k-1]   2: v_x(k)   3: p'_y(k)
-//   4: p'_x(k) + correction(k)   5: T_y(k) + residual(k) + u_y(k+1)
+//   1: v_y(k) [+ residual of step k-1]   2: v_x(k) [+ fold of step k-1]
+//   3: p'_y(k)   4: p'_x(k) + correction(k)   5: T_y(k) + u_y(k+1)
 //   6: T_x(k) + u_x(k+1)
-// The u sweeps of the first step run before the first step, the fold of the
-// last step after it.  u_ant of step k+1 is u of step k; the u buffer the
-// residual of step k reads as its u_ant is held out of the rotation while
-// the u sweeps of step k+1 run beside it.
+// The u sweeps of the first step run before the first step, the residual
+// and the fold of the last step after it.  u_ant of step k+1 is u of step
+// k; the u buffer the residual of step k reads as its u_ant (u of step k-1)
+// is held out of the rotation while the u sweeps of step k+1 run, and
+// nothing writes u, v or p between those sweeps and the residual.  The fold
+// of step k runs beside v_x of step k+1, so the v norm partials alternate
+// between two slots.
 struct StepLaunches {
-  MultiArgs m1, m6, m7;
-  SweepArgs vx, py, px;
+  MultiArgs m1, m2, m5, m6;
+  SweepArgs py, px;
   CorrLine corr;
-  int nwg1, nwg6, nwg7, ept1, ept6, ept7;
+  int nwg1, nwg2, nwg5, nwg6, ept1, ept2, ept5, ept6;
   bool last;
 };
 
 // arguments of the launches of one step (advances the rotation as the
-// step does); `fold` null: no fold in launch 1 (first step)
+// step does); the residual of step k is kept in the state for step k+1
 int build_step(ixh_state *s, bool first, bool last, StepLaunches &L) {
   memset(&L, 0, sizeof L);
   L.last = last;
   CHK(prep_v(s, IXH_DIR_Y, L.m1.s[0]));
-  if (!first) prep_finalize(s, L.m1.f, 31, s->d_hist, s->d_slot);
-  L.nwg1 = multi_layout(L.m1, L.m1.s[0].nlines, 0, 0, first ? 0 : 1);
+  if (!first) L.m1.r = s->res_next;
+  L.nwg1 = multi_layout(L.m1, L.m1.s[0].nlines, 0, first ? 0 : s->nj - 1, 0);
   L.ept1 = ept_for(line_len(s, EQ_V, IXH_DIR_Y));
-  CHK(prep_v(s, IXH_DIR_X, L.vx));
+  if (!first) prep_finalize(s, L.m2.f, 31, s->d_hist, s->d_slot);  // step k-1's v partials
+  s->pv_alt ^= 1;
+  CHK(prep_v(s, IXH_DIR_X, L.m2.s[0]));
+  L.nwg2 = multi_layout(L.m2, L.m2.s[0].nlines, 0, 0, first ? 0 : 1);
+  L.ept2 = ept_for(line_len(s, EQ_V, IXH_DIR_X));
   s->pc_zero = true;
   CHK(prep_p(s, IXH_DIR_Y, L.py));
   CHK(prep_p(s, IXH_DIR_X, L.px));
   prep_correct_line(s, L.corr);
   // the T sweeps and the residual of step k read u of step k: build their
   // arguments before the u sweeps of step k+1 advance the rotation
-  CHK(prep_t(s, IXH_DIR_Y, L.m6.s[0]));
-  CHK(prep_t(s, IXH_DIR_X, L.m7.s[0]));
-  prep_residual(s, L.m6.r);
+  CHK(prep_t(s, IXH_DIR_Y, L.m5.s[0]));
+  CHK(prep_t(s, IXH_DIR_X, L.m6.s[0]));
+  prep_residual(s, s->res_next);
   const int held = s->u_ant;
   s->u_ant = s->u_cur;
-  L.ept6 = ept_for(line_len(s, EQ_T, IXH_DIR_Y));
-  L.ept7 = ept_for(line_len(s, EQ_T, IXH_DIR_X));
-  int nu6 = 0, nu7 = 0;
+  L.ept5 = ept_for(line_len(s, EQ_T, IXH_DIR_Y));
+  L.ept6 = ept_for(line_len(s, EQ_T, IXH_DIR_X));
+  int nu5 = 0, nu6 = 0;
   if (!last) {
     s->u_hold = held;
-    CHK(prep_u(s, IXH_DIR_Y, L.m6.s[1]));
-    CHK(prep_u(s, IXH_DIR_X, L.m7.s[1]));
+    CHK(prep_u(s, IXH_DIR_Y, L.m5.s[1]));
+    CHK(prep_u(s, IXH_DIR_X, L.m6.s[1]));
     s->u_hold = -1;
+    nu5 = L.m5.s[1].nlines;
     nu6 = L.m6.s[1].nlines;
-    nu7 = L.m7.s[1].nlines;
-    L.ept6 = std::max(L.ept6, ept_for(line_len(s, EQ_U, IXH_DIR_Y)));
-    L.ept7 = std::max(L.ept7, ept_for(line_len(s, EQ_U, IXH_DIR_X)));
+    L.ept5 = std::max(L.ept5, ept_for(line_len(s, EQ_U, IXH_DIR_Y)));
+    L.ept6 = std::max(L.ept6, ept_for(line_len(s, EQ_U, IXH_DIR_X)));
   }
-  L.nwg6 = multi_layout(L.m6, L.m6.s[0].nlines, nu6, s->nj - 1, 0);
-  L.nwg7 = multi_layout(L.m7, L.m7.s[0].nlines, nu7, 0, 0);
+  L.nwg5 = multi_layout(L.m5, L.m5.s[0].nlines, nu5, 0, 0);
+  L.nwg6 = multi_layout(L.m6, L.m6.s[0].nlines, nu6, 0, 0);
   s->v_ant = s->v_cur;
   s->t_ant = s->t_cur;
   return 0;
@@ -753,15 +762,15 @@ int build_step(ixh_state *s, bool first, bool last, StepLaunches &L) {
 int launch_part(ixh_state *s, const StepLaunches &L, int k) {
   switch (k) {
     case 1: return launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, L.m1, L.nwg1, L.ept1, line_len(s, EQ_V, IXH_DIR_Y));
-    case 2: return launch_sweep<EQ_V, DIR_X>(s, L.vx, line_len(s, EQ_V, IXH_DIR_X));
+    case 2: return launch_multi<EQ_V, DIR_X, -1, DIR_X>(s, L.m2, L.nwg2, L.ept2, line_len(s, EQ_V, IXH_DIR_X));
     case 3: return launch_sweep<EQ_P, DIR_Y>(s, L.py, line_len(s, EQ_P, IXH_DIR_Y));
     case 4: return launch_pcorr(s, L.px, L.corr, line_len(s, EQ_P, IXH_DIR_X));
     case 5:
-      if (L.last) return launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, L.m6, L.nwg6, L.ept6, s->nj + 1);
-      return launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, L.m6, L.nwg6, L.ept6, s->nj + 1);
+      if (L.last) return launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, L.m5, L.nwg5, L.ept5, s->nj + 1);
+      return launch_multi<EQ_T, DIR_Y, EQ_U, DIR_Y>(s, L.m5, L.nwg5, L.ept5, s->nj + 1);
     case 6:
-      if (L.last) return launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, L.m7, L.nwg7, L.ept7, s->mi + 1);
-      return launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, L.m7, L.nwg7, L.ept7, s->mi + 1);
+      if (L.last) return launch_multi<EQ_T, DIR_X, -1, DIR_X>(s, L.m6, L.nwg6, L.ept6, s->mi + 1);
+      return launch_multi<EQ_T, DIR_X, EQ_U, DIR_X>(s, L.m6, L.nwg6, L.ept6, s->mi + 1);
   }
   return fail(IXH_ERR_ARG, "launch part %d", k);
 }
@@ -804,7 +813,15 @@ int fused_step(ixh_state *s, bool first, bool last, int k) {
     s->t_start = s->t_stop = nullptr;
     CHK(rc);
   }
-  if (last) CHK(do_finalize(s, 31, s->d_hist, s->d_slot));
+  if (last) {
+    // the residual and the fold of the last step
+    if (s->f32)
+      hipLaunchKernelGGL(k_residual_u<float>, dim3(s->nj - 1), dim3(kBlock), 0, s->launch, s->res_next);
+    else
+      hipLaunchKernelGGL(k_residual_u<double>, dim3(s->nj - 1), dim3(kBlock), 0, s->launch, s->res_next);
+    HIPCHK(hipGetLastError());
+    CHK(do_finalize(s, 31, s->d_hist, s->d_slot));
+  }
   return 0;
 }
 
@@ -918,7 +935,7 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
       return bail(fail(IXH_ERR_HIP, "a_u/a_v initialisation failed"));
   }
   s->plen = (mi > nj ? mi : nj) + 4;
-  if ((rc = dmalloc(&s->part, (size_t)5 * s->plen)) || (rc = dmalloc(&s->d_out, 8))) return bail(rc);
+  if ((rc = dmalloc(&s->part, (size_t)6 * s->plen)) || (rc = dmalloc(&s->d_out, 8))) return bail(rc);
   if (hipMalloc((void **)&s->d_slot, sizeof(int)) != hipSuccess) return bail(fail(IXH_ERR_NOMEM, "slot"));
   if ((rc = build_fast(s))) return bail(rc);
   for (int v = 0; v < 3; ++v)

@@ -1582,7 +1582,7 @@ struct MultiArgs {
 };
 
 // BS = 64: one wave per line, for launches of sweeps only (no residual, no
-// fold); BS = 128: two waves per line and per residual row (no fold)
+// fold)
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, int EPT, typename R, int BS = kBlock>
 __global__ void __launch_bounds__(BS, kMinWaves) k_multi(MultiArgs M) {
   __shared__ double lds[line_lds<R, EPT, BS>()];
@@ -1597,12 +1597,12 @@ __global__ void __launch_bounds__(BS, kMinWaves) k_multi(MultiArgs M) {
       if (w < M.cnt[1])
         sweep_line<EQ1, DIR1, SOLVER, EPT, R, BS>(M.s[1], M.s[1].line0 + xcd_line(w, M.s[1].nlines), lds);
     }
-  } else if constexpr (BS >= 128) {
+  } else if constexpr (BS == kBlock) {
     if (b < M.off[3]) {
       const int w = b - M.off[2];
       if (w < M.cnt[2])
-        residual_row<typename StoreOf<SOLVER, R>::T, BS>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
-    } else if constexpr (BS == kBlock) {
+        residual_row<typename StoreOf<SOLVER, R>::T>(M.r, 2 + xcd_line(w, M.r.nj - 1), lds);
+    } else {
       if (b == M.off[3] && M.cnt[3]) finalize_body(M.f, lds);
     }
   }

@@ -31,23 +31,12 @@ constexpr int ept_for(int n) {
 // (13.4 us against 16.6 us for v_x), so single sweeps keep that shape.
 constexpr int kWaveLine = 5 * 64;
 
-// A fused launch with the u residual rows beside its sweeps (the T y-sweep
-// launch of a time step: ~300 + ~300 y-lines and ~700 rows at 301x701) runs 128 threads
-// per line and per row when its lines have at most this many nodes.  Its
-// sweep kernels hold 168 registers, 3 waves per SIMD: that is 3 workgroups
-// of 256 threads per CU, 768 on the GPU, so 1300 workgroups ran in two
-// rounds; 6 of 128 threads per CU hold 1536.  All at once, each workgroup
-// runs longer (the residual rows 10 us instead of 4): FP32-mixed, that launch
-// takes 12.0 us instead of 13.7, FP64 15.9 instead of 16.0.
-constexpr int kPairLine = 1024;
-
-// threads per workgroup of a fused launch whose first sweep runs in
-// direction dir0 over lines of at most maxn nodes
-inline int multi_block(int solver, int dir0, int maxn, bool residual, bool fold) {
+// threads per workgroup of a fused launch over lines of at most maxn
+// nodes: one wave per line for sweeps only (no residual, no fold) on short
+// lines with the wave solver, else 256
+inline int multi_block(int solver, int maxn, bool residual, bool fold) {
   const bool wave_solver = solver == SOLVE_PCR || solver == SOLVE_DELTA;
-  if (wave_solver && !residual && !fold && maxn <= kWaveLine) return 64;
-  if (wave_solver && dir0 == DIR_Y && residual && !fold && maxn <= kPairLine) return 128;
-  return kBlock;
+  return wave_solver && !residual && !fold && maxn <= kWaveLine ? 64 : kBlock;
 }
 
 // one sweep of A.nlines lines of n nodes, one 256-thread workgroup per line
@@ -110,7 +99,7 @@ hipError_t launch_pcorr_k(const LaunchCtx &c, const SweepArgs &A, const CorrLine
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
 hipError_t launch_multi_k(const LaunchCtx &c, const MultiArgs &M, int nwg, int ept, int maxn) {
   const dim3 grid(nwg), block(kBlock);
-  const int bs = multi_block(SOLVER, DIR0, maxn, M.cnt[2] > 0, M.cnt[3] > 0);
+  const int bs = multi_block(SOLVER, maxn, M.cnt[2] > 0, M.cnt[3] > 0);
   // sweeps only, every line short: one wave per line
   if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
     if (bs == 64) {
@@ -122,22 +111,6 @@ hipError_t launch_multi_k(const LaunchCtx &c, const MultiArgs &M, int nwg, int e
         case 4: launch_on(c, k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R, 64>, grid, wave, M); break;
         default: launch_on(c, k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 5, R, 64>, grid, wave, M); break;
       }
-      return hipGetLastError();
-    }
-  }
-  // sweeps and the residual, lines up to 1024 nodes: two waves per line
-  if constexpr ((SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) && DIR0 == DIR_Y) {
-    if (bs == 128) {
-      const dim3 pair(128);
-      const int e = (maxn + 127) / 128;
-      if (e <= 2)
-        launch_on(c, k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 2, R, 128>, grid, pair, M);
-      else if (e <= 4)
-        launch_on(c, k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 4, R, 128>, grid, pair, M);
-      else if (e <= 6)
-        launch_on(c, k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 6, R, 128>, grid, pair, M);
-      else
-        launch_on(c, k_multi<EQ0, DIR0, EQ1, DIR1, SOLVER, 8, R, 128>, grid, pair, M);
       return hipGetLastError();
     }
   }
@@ -198,6 +171,7 @@ hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *
   IXH_SWEEP_ONE(EQ_T, DIR_Y, SOLVER, R)                \
   IXH_SWEEP_ONE(EQ_T, DIR_X, SOLVER, R)                \
   IXH_MULTI_ONE(EQ_V, DIR_Y, -1, DIR_Y, SOLVER, R)     \
+  IXH_MULTI_ONE(EQ_V, DIR_X, -1, DIR_X, SOLVER, R)     \
   IXH_MULTI_ONE(EQ_T, DIR_Y, -1, DIR_Y, SOLVER, R)     \
   IXH_MULTI_ONE(EQ_T, DIR_Y, EQ_U, DIR_Y, SOLVER, R)   \
   IXH_MULTI_ONE(EQ_T, DIR_X, -1, DIR_X, SOLVER, R)     \

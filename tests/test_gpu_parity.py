@@ -221,8 +221,8 @@ def test_fused_step_launch_timing():
     g.run(100)
     kt = g.kernel_times()
     g.set_profiling(False)
-    names = ["multi_v_y+fold", "step_sweep_v_x", "step_sweep_p_y", "step_p_x+correct",
-             "multi_t_y+residual+u_y", "multi_t_x+u_x"]
+    names = ["multi_v_y+residual", "multi_v_x+fold", "step_sweep_p_y", "step_p_x+correct",
+             "multi_t_y+u_y", "multi_t_x+u_x"]
     for n in names:
         ms, cnt = kt[n]
         assert cnt == 100 and ms > 0.0, (n, kt[n])
