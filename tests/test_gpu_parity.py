@@ -123,6 +123,24 @@ def test_long_lines_against_oracle(mi, nj):
         assert relerr(g.get(f), o.get(f)) <= REL, f"{f}: rel err {relerr(g.get(f), o.get(f)):.2e}"
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed", "fp32_refined"])
+def test_fused_steps_split_across_calls(precision):
+    """A fused step runs the residual of the step before it beside its v
+    y-sweep, folds the norms beside its v x-sweep and alternates the v norm
+    slots; runs split over several calls (5 + 1 + 6 steps) must give the
+    fields and the history of one 12-step run bit for bit."""
+    from ixchel2d_amd.simple import Cavity
+    case = K.mixed22x17()
+    whole = Cavity(case, solver="pcr", precision=precision)
+    hw = whole.run(12)
+    split = Cavity(case, solver="pcr", precision=precision)
+    parts = [split.run(n) for n in (5, 1, 6)]
+    for f in "uvtp":
+        assert np.array_equal(split.get(f), whole.get(f)), f
+    for k in ("err_v", "err_p", "maxbo", "err_t", "residuo"):
+        assert np.array_equal(np.concatenate([h[k] for h in parts]), hw[k]), k
+
+
 def test_control_flow_pass_counts_match_oracle():
     case = K.ctrl18x14()
     g = _cavity(case, "thomas")
