@@ -763,8 +763,8 @@ __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
   __syncthreads();
 }
 
-// BS: threads of the line's workgroup (256, 128, or 64 for one wave per
-// line); the wave solver's chunk length is EPT * BS / 64 rows per lane
+// BS: threads of the line's workgroup (256, or 64 for one wave per line);
+// the wave solver's chunk length is EPT * BS / 64 rows per lane
 template <int SOLVER, int EPT, typename T, int BS = kBlock>
 __device__ __forceinline__ void solve_line(T *a, T *b, T *c, T *r, int n) {
   constexpr int MW = EPT * BS / 64;
@@ -817,28 +817,14 @@ __device__ __forceinline__ double block_sum(double x, double *scratch) {
 
 // The sum of a row's terms in the order of a 256-thread workgroup: term k
 // belongs to thread k % 256, each thread adds its terms in increasing k,
-// then block_sum.  A workgroup of BS < 256 threads keeps the partial of
-// owner wave (k % 256) / 64 in acc[that wave]; with 256 threads only acc[0]
-// is used.  Every thread gets the sum; scratch: 8 doubles of LDS.
+// then block_sum.  One wave (BS = 64) keeps the partial of owner wave
+// (k % 256) / 64 in acc[that wave]; with 256 threads only acc[0] is used.
+// Every thread gets the sum; scratch: 4 doubles of LDS.
 template <int BS>
 __device__ __forceinline__ double owner_sum(const double acc[4], double *scratch) {
-  static_assert(BS == kBlock || BS == 128 || BS == 64, "workgroups of 256, 128 or 64 threads");
+  static_assert(BS == kBlock || BS == 64, "workgroups of 256 threads or one wave");
   if (BS == kBlock) return block_sum(acc[0], scratch);
-  double w[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) w[i] = wave_sum(acc[i]);
-  if (BS == 64) return (w[0] + w[1]) + (w[2] + w[3]);
-  // two waves: wave v holds owners v and v + 2, exact zeros elsewhere
-  // (adding an exact zero changes no bit)
-  if ((threadIdx.x & 63) == 0) {
-#pragma unroll
-    for (int i = 0; i < 4; ++i) scratch[(threadIdx.x >> 6) * 4 + i] = w[i];
-  }
-  __syncthreads();
-  const double s = ((scratch[0] + scratch[4]) + (scratch[1] + scratch[5])) +
-                   ((scratch[2] + scratch[6]) + (scratch[3] + scratch[7]));
-  __syncthreads();
-  return s;
+  return (wave_sum(acc[0]) + wave_sum(acc[1])) + (wave_sum(acc[2]) + wave_sum(acc[3]));
 }
 
 // Workgroup w runs on XCD w % 8; give every XCD a contiguous block of lines
@@ -900,9 +886,7 @@ struct Layout {
 // and, for 256 threads, the reduction scratch (one wave reduces in registers;
 // leaving it out raises the batch kernels' occupancy)
 template <typename R, int EPT, int BS = kBlock>
-constexpr int line_lds() {
-  return (4 * EPT * BS * (int)sizeof(R)) / 8 + (BS == kBlock ? kBlock : (BS == 128 ? 8 : 0));
-}
+constexpr int line_lds() { return (4 * EPT * BS * (int)sizeof(R)) / 8 + (BS == kBlock ? kBlock : 0); }
 
 // A zero the compiler must treat as a per-lane value.  Indices built on it
 // make uniform table lookups vector loads: vector loads return in order, so
@@ -928,7 +912,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
                 "the FP32 sweeps use the PCR solver");
   static_assert(SOLVER != SOLVE_DELTA || sizeof(R) == 4, "the increment sweeps are FP32");
-  static_assert(BS == kBlock || BS == 128 || BS == 64, "line workgroups of 256, 128 or 64 threads");
+  static_assert(BS == kBlock || BS == 64, "line workgroups of 256 threads or one wave");
   static_assert(BS == kBlock || SOLVER != SOLVE_THOMAS, "smaller line workgroups use the wave solver");
   using S = typename StoreOf<SOLVER, R>::T;      // stored type of the fields
   constexpr bool DELTA = SOLVER == SOLVE_DELTA && EQ != EQ_P;
@@ -1204,9 +1188,9 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     solve_line<SOLVER, EPT, R, BS>(sa, sb, sc_, sr, n);
 
   // write the line and the carried lines, accumulate the x-sweep norms.
-  // With fewer than 256 threads per line, row k is summed into the partial
-  // of the thread of a 256-thread workgroup that would own it (wave
-  // (k % 256) / 64), so the norms equal the 256-thread ones bit for bit.
+  // With one wave per line, row k is summed into the partial of the thread
+  // of a 256-thread workgroup that would own it (wave (k % 256) / 64), so
+  // the norms equal the 256-thread ones bit for bit.
   double acc0[4] = {0.0, 0.0, 0.0, 0.0}, acc1[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
@@ -1421,11 +1405,10 @@ struct ResArgs {
   double *resu, *part;
 };
 
-// RB: threads per row (256, 128, or 64 for one wave per row; the whole
-// workgroup runs the row).  Fewer than 256 threads sum row positions into
-// the partials of the 256-thread owners, so the norm is the same bit for
-// bit.  Two positions per thread per pass, all loads of a pass before its
-// stores.
+// RB: threads per row (256, or 64 for one wave per row).  One wave sums row
+// positions into the partials of the 256-thread owners, so the norm is the
+// same bit for bit.  Two positions per thread per pass, all loads of a pass
+// before its stores.
 template <typename R, int RB = kBlock>
 __device__ __forceinline__ void residual_row(const ResArgs &A, const int jj, double *red) {
   constexpr bool TR = false;
