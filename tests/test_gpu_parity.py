@@ -141,6 +141,26 @@ def test_fused_steps_split_across_calls(precision):
         assert np.array_equal(np.concatenate([h[k] for h in parts]), hw[k]), k
 
 
+@pytest.mark.parametrize("name", ["nc18x14", "mixed22x17", "nc34"])
+def test_fused_step_equals_kernel_by_kernel(name):
+    """With convergence thresholds of 1e30 every loop stops after one pass
+    whatever its pass limit; non-negative limits make the driver run the
+    kernels one after the other (upstream order, norms read back after each
+    pass), negative ones select the fused 6-launch step.  Both must give the
+    same fields and norms bit for bit."""
+    case = K.CASES[name]()
+    case.params.itermax = 12
+    fused = _cavity(case, "pcr")
+    hf = fused.run(12)
+    case.params.iter_ecuaci_max, case.params.iter_simple_max = 3, 3
+    seq = _cavity(case, "pcr")
+    hs = seq.run(12)
+    for f in "uvtp":
+        assert np.array_equal(fused.get(f), seq.get(f)), f
+    for k in ("err_v", "err_p", "maxbo", "err_t", "residuo"):
+        assert np.array_equal(hf[k], hs[k]), k
+
+
 def test_control_flow_pass_counts_match_oracle():
     case = K.ctrl18x14()
     g = _cavity(case, "thomas")
