@@ -825,6 +825,19 @@ int fused_step(ixh_state *s, bool first, bool last, int k) {
   return 0;
 }
 
+// nsteps fused steps (the u equation must not read T): the u sweeps of the
+// first step before it, the residual and the fold of the last after it; the
+// norms of each step go to the history slot d_slot points at
+int run_fused(ixh_state *s, int nsteps) {
+  if (nsteps <= 0) return 0;
+  CHK(sweep_u(s, IXH_DIR_Y));
+  CHK(sweep_u(s, IXH_DIR_X));
+  for (int k = 0; k < nsteps; ++k) CHK(fused_step(s, k == 0, k == nsteps - 1, k));
+  if (s->profiling)
+    for (int k = std::max(0, nsteps - kRing); k < nsteps; ++k) CHK(harvest_slot(s, k % kRing));
+  return 0;
+}
+
 int ensure_hist(ixh_state *s, int n) {
   if (n <= s->hist_cap) return 0;
   if (s->d_hist) HIPCHK(hipFree(s->d_hist));
@@ -1254,16 +1267,12 @@ int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t 
       // a u equation that reads T: upstream order, one kernel after the
       // other (per-kernel event timing with profiling on)
       for (int k = 0; k < nsteps; ++k) CHK(fast_step(s));
-    } else if (nsteps > 0) {
+    } else {
       // fused launches on one stream: work on separate streams did not
       // overlap on this stack (the cross-stream waits cost more than the
       // overlap won), and a captured HIP graph of the step ran slower than
       // direct launches
-      CHK(sweep_u(s, IXH_DIR_Y));  // prologue: u sweeps of the first step
-      CHK(sweep_u(s, IXH_DIR_X));
-      for (int k = 0; k < nsteps; ++k) CHK(fused_step(s, k == 0, k == nsteps - 1, k));
-      if (s->profiling)
-        for (int k = std::max(0, nsteps - kRing); k < nsteps; ++k) CHK(harvest_slot(s, k % kRing));
+      CHK(run_fused(s, nsteps));
     }
     const int keep = nsteps < cap ? nsteps : cap;
     std::vector<double> h((size_t)8 * (keep > 0 ? keep : 1));
@@ -2026,9 +2035,18 @@ int ixh_adjoint_record(ixh_state *s, int32_t nsteps, int32_t every) {
   CHK(ensure_hist(s, nsteps));
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, s->stream, s->d_slot, 0);
   HIPCHK(hipGetLastError());
-  for (int k = 0; k < nsteps; ++k) {
-    if (k % every == 0) CHK(ck_save(s, s->adj_ck[k / every]));
-    CHK(fast_step(s));
+  if (s->prm.ri_x == 0.0) {
+    // fused steps between checkpoints (bit for bit the kernel-by-kernel
+    // step); each chunk ends on a whole step
+    for (int k0 = 0; k0 < nsteps; k0 += every) {
+      CHK(ck_save(s, s->adj_ck[k0 / every]));
+      CHK(run_fused(s, std::min(every, nsteps - k0)));
+    }
+  } else {
+    for (int k = 0; k < nsteps; ++k) {
+      if (k % every == 0) CHK(ck_save(s, s->adj_ck[k / every]));
+      CHK(fast_step(s));
+    }
   }
   CHK(ck_save(s, s->adj_ck[nck - 1]));
   HIPCHK(hipStreamSynchronize(s->stream));
