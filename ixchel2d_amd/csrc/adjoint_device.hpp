@@ -437,6 +437,21 @@ __global__ void __launch_bounds__(kBlock) k_adj2_correct(AdjCorrArgs C) {
   AT(C.pcb, I, J) = s;
 }
 
+// up to kCopyMax buffer copies of n doubles in one launch (grid.y = copy)
+constexpr int kCopyMax = 12;
+struct CopyList {
+  const double *src[kCopyMax];
+  double *dst[kCopyMax];
+  int n;
+};
+
+__global__ void __launch_bounds__(kBlock) k_copy_many(CopyList L) {
+  const double *src = L.src[blockIdx.y];
+  double *dst = L.dst[blockIdx.y];
+  for (int i = (int)(blockIdx.x * kBlock + threadIdx.x); i < L.n; i += (int)(gridDim.x * kBlock))
+    gp(dst)[i] = gp(src)[i];
+}
+
 // y += x over the whole buffer (n doubles)
 __global__ void k_axpy1(double *y, const double *x, int n) {
   const int i = (int)(blockIdx.x * kBlock + threadIdx.x);

@@ -2129,35 +2129,59 @@ int dcopy(ixh_state *s, double *dst, const double *src) {
   return 0;
 }
 
+// the listed buffers copied in one launch (whole buffers)
+int copy_many(ixh_state *s, std::initializer_list<std::pair<double *, const double *>> pairs) {
+  CopyList L;
+  memset(&L, 0, sizeof L);
+  int c = 0;
+  for (const auto &pr : pairs) {
+    if (c == kCopyMax) return fail(IXH_ERR_ARG, "copy_many: more than %d copies", kCopyMax);
+    L.dst[c] = pr.first;
+    L.src[c] = pr.second;
+    ++c;
+  }
+  L.n = (int)s->fsize;
+  const dim3 grid((unsigned)std::min<size_t>((s->fsize + kBlock - 1) / kBlock, 64), (unsigned)c);
+  hipLaunchKernelGGL(k_copy_many, grid, dim3(kBlock), 0, s->stream, L);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
 // one time step in the order of fast_step, every produced field copied to
-// the tape (the residual and the norms do not change fields and are skipped)
+// the tape (the residual and the norms do not change fields and are
+// skipped).  The copies go out in three launches, each before any of its
+// buffers can be overwritten: the sweeps write buffers other than the ones
+// they read (u0 is held out of the rotation for the u sweeps: after a
+// checkpoint is loaded u_ant is another buffer, and the u x-sweep could
+// otherwise pick u0's), so u0, v0 (the v snapshot), T0, u1, u2, v1, v2,
+// a_u, a_v and p0 (which the correction updates in place) are intact after
+// the v x-sweep; p'1 and p'2 after the p' x-sweep; u3, v3 (the corrected
+// u2, v2, in place) and T1, T2 after the T x-sweep.
 int tape_step(ixh_state *s, double *const *tp) {
-  CHK(dcopy(s, tp[TP_U0], s->ub[s->u_cur]));
-  CHK(dcopy(s, tp[TP_V0], s->vb[s->v_cur]));
-  CHK(dcopy(s, tp[TP_T0], s->tb[s->t_cur]));
-  CHK(dcopy(s, tp[TP_P0], s->p));
+  double *u0 = s->ub[s->u_cur], *v0 = s->vb[s->v_cur], *t0 = s->tb[s->t_cur];
+  s->u_hold = s->u_cur;
   CHK(sweep_u(s, IXH_DIR_Y));
-  CHK(dcopy(s, tp[TP_U1], s->ub[s->u_cur]));
+  double *u1 = s->ub[s->u_cur];
   CHK(sweep_u(s, IXH_DIR_X));
-  CHK(dcopy(s, tp[TP_U2], s->ub[s->u_cur]));
-  CHK(dcopy(s, tp[TP_AU], s->au));
+  s->u_hold = -1;
+  double *u2 = s->ub[s->u_cur];
   CHK(sweep_v(s, IXH_DIR_Y));
-  CHK(dcopy(s, tp[TP_V1], s->vb[s->v_cur]));
+  double *v1 = s->vb[s->v_cur];
   CHK(sweep_v(s, IXH_DIR_X));
-  CHK(dcopy(s, tp[TP_V2], s->vb[s->v_cur]));
-  CHK(dcopy(s, tp[TP_AV], s->av));
+  double *v2 = s->vb[s->v_cur];
+  CHK(copy_many(s, {{tp[TP_U0], u0}, {tp[TP_V0], v0}, {tp[TP_T0], t0}, {tp[TP_P0], s->p}, {tp[TP_U1], u1},
+                    {tp[TP_U2], u2}, {tp[TP_AU], s->au}, {tp[TP_V1], v1}, {tp[TP_V2], v2}, {tp[TP_AV], s->av}}));
   s->pc_zero = true;
   CHK(sweep_p(s, IXH_DIR_Y));
-  CHK(dcopy(s, tp[TP_PC1], s->pcb[s->pc_cur]));
+  double *pc1 = s->pcb[s->pc_cur];
   CHK(sweep_p(s, IXH_DIR_X));
-  CHK(dcopy(s, tp[TP_PC2], s->pcb[s->pc_cur]));
+  CHK(copy_many(s, {{tp[TP_PC1], pc1}, {tp[TP_PC2], s->pcb[s->pc_cur]}}));
   CHK(do_correct(s));
-  CHK(dcopy(s, tp[TP_U3], s->ub[s->u_cur]));
-  CHK(dcopy(s, tp[TP_V3], s->vb[s->v_cur]));
+  double *u3 = s->ub[s->u_cur], *v3 = s->vb[s->v_cur];
   CHK(sweep_t(s, IXH_DIR_Y));
-  CHK(dcopy(s, tp[TP_T1], s->tb[s->t_cur]));
+  double *t1 = s->tb[s->t_cur];
   CHK(sweep_t(s, IXH_DIR_X));
-  CHK(dcopy(s, tp[TP_T2], s->tb[s->t_cur]));
+  CHK(copy_many(s, {{tp[TP_U3], u3}, {tp[TP_V3], v3}, {tp[TP_T1], t1}, {tp[TP_T2], s->tb[s->t_cur]}}));
   s->u_ant = s->u_cur;
   s->v_ant = s->v_cur;
   s->t_ant = s->t_cur;

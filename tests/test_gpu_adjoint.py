@@ -162,6 +162,24 @@ def test_coupled_adjoint_against_central_differences_with_buoyancy(buoyant):
     assert err < err_frozen
 
 
+def test_coupled_checkpoint_spacing_with_buoyancy(buoyant):
+    """With buoyancy the velocity adjoints feed back into T, so every entry
+    of the recomputed tape (u, v at each phase of each step) reaches the
+    gradient; another checkpoint spacing (40 = 5 * 7 + 5 steps) moves the
+    segment starts, where the state comes from a checkpoint, and must leave
+    the gradient unchanged."""
+    c = _controls(6, seed=11)
+    buoyant.forward(c)
+    g15 = buoyant.gradient()
+    buoyant.every = 7
+    try:
+        buoyant.forward(c)
+        g7 = buoyant.gradient()
+    finally:
+        buoyant.every = 15
+    np.testing.assert_allclose(g7, g15, rtol=1e-12, atol=1e-15)
+
+
 def test_coupled_initial_temperature_with_buoyancy(buoyant):
     c = _controls(6, seed=9)
     buoyant.forward(c)
