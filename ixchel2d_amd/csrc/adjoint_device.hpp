@@ -452,10 +452,20 @@ __global__ void __launch_bounds__(kBlock) k_copy_many(CopyList L) {
     gp(dst)[i] = gp(src)[i];
 }
 
-// y += x over the whole buffer (n doubles)
-__global__ void k_axpy1(double *y, const double *x, int n) {
-  const int i = (int)(blockIdx.x * kBlock + threadIdx.x);
-  if (i < n) y[i] = y[i] + x[i];
+// dst[c] = 0 (src unused) for every listed buffer c, in one launch
+__global__ void __launch_bounds__(kBlock) k_zero_many(CopyList L) {
+  double *dst = L.dst[blockIdx.y];
+  for (int i = (int)(blockIdx.x * kBlock + threadIdx.x); i < L.n; i += (int)(gridDim.x * kBlock))
+    gp(dst)[i] = 0.0;
 }
+
+// dst[c] += src[c] for every listed pair c, in one launch
+__global__ void __launch_bounds__(kBlock) k_add_many(CopyList L) {
+  const double *src = L.src[blockIdx.y];
+  double *dst = L.dst[blockIdx.y];
+  for (int i = (int)(blockIdx.x * kBlock + threadIdx.x); i < L.n; i += (int)(gridDim.x * kBlock))
+    gp(dst)[i] = gp(dst)[i] + gp(src)[i];
+}
+
 
 }  // namespace ixh

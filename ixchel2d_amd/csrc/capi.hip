@@ -2129,22 +2129,34 @@ int dcopy(ixh_state *s, double *dst, const double *src) {
   return 0;
 }
 
-// the listed buffers copied in one launch (whole buffers)
-int copy_many(ixh_state *s, std::initializer_list<std::pair<double *, const double *>> pairs) {
+enum ManyOp { MANY_COPY, MANY_ZERO, MANY_ADD };
+
+// one launch over whole buffers: dst = src, dst = 0 or dst += src for every
+// listed (dst, src) pair
+int many(ixh_state *s, ManyOp op, std::initializer_list<std::pair<double *, const double *>> pairs) {
   CopyList L;
   memset(&L, 0, sizeof L);
   int c = 0;
   for (const auto &pr : pairs) {
-    if (c == kCopyMax) return fail(IXH_ERR_ARG, "copy_many: more than %d copies", kCopyMax);
+    if (c == kCopyMax) return fail(IXH_ERR_ARG, "many: more than %d buffers", kCopyMax);
     L.dst[c] = pr.first;
     L.src[c] = pr.second;
     ++c;
   }
   L.n = (int)s->fsize;
   const dim3 grid((unsigned)std::min<size_t>((s->fsize + kBlock - 1) / kBlock, 64), (unsigned)c);
-  hipLaunchKernelGGL(k_copy_many, grid, dim3(kBlock), 0, s->stream, L);
+  if (op == MANY_COPY)
+    hipLaunchKernelGGL(k_copy_many, grid, dim3(kBlock), 0, s->stream, L);
+  else if (op == MANY_ZERO)
+    hipLaunchKernelGGL(k_zero_many, grid, dim3(kBlock), 0, s->stream, L);
+  else
+    hipLaunchKernelGGL(k_add_many, grid, dim3(kBlock), 0, s->stream, L);
   HIPCHK(hipGetLastError());
   return 0;
+}
+
+int copy_many(ixh_state *s, std::initializer_list<std::pair<double *, const double *>> pairs) {
+  return many(s, MANY_COPY, pairs);
 }
 
 // one time step in the order of fast_step, every produced field copied to
@@ -2224,9 +2236,8 @@ Adj2Args adj2_args(ixh_state *s, int var, double *const *W) {
 // W_UB3, W_VB3, W_TB2, W_PB, become those of its start in W_UB0, W_VB0,
 // W_TB0, W_PB; boundary adjoints of T accumulate in adj_g
 int adj2_step(ixh_state *s, double *const *tp, double *const *W) {
-  const size_t fb = s->fsize * sizeof(double);
-  for (int k : {W_AUB, W_AVB, W_UANTB, W_VANTB, W_TANTB})
-    HIPCHK(hipMemsetAsync(W[k], 0, fb, s->stream));
+  CHK(many(s, MANY_ZERO, {{W[W_AUB], nullptr}, {W[W_AVB], nullptr}, {W[W_UANTB], nullptr},
+                          {W[W_VANTB], nullptr}, {W[W_TANTB], nullptr}}));
   const int mi = s->mi, nj = s->nj;
   // T x-sweep, then T y-sweep
   {
@@ -2285,12 +2296,7 @@ int adj2_step(ixh_state *s, double *const *tp, double *const *W) {
     CHK((adj2_sweep<EQ_U, DIR_Y>(s, G, mi - 2, nj + 1)));
   }
   // the lagged copies are the fields of the step's start
-  const int nf = (int)s->fsize;
-  const dim3 fg((nf + kBlock - 1) / kBlock), block(kBlock);
-  hipLaunchKernelGGL(k_axpy1, fg, block, 0, s->stream, W[W_UB0], (const double *)W[W_UANTB], nf);
-  hipLaunchKernelGGL(k_axpy1, fg, block, 0, s->stream, W[W_VB0], (const double *)W[W_VANTB], nf);
-  hipLaunchKernelGGL(k_axpy1, fg, block, 0, s->stream, W[W_TB0], (const double *)W[W_TANTB], nf);
-  HIPCHK(hipGetLastError());
+  CHK(many(s, MANY_ADD, {{W[W_UB0], W[W_UANTB]}, {W[W_VB0], W[W_VANTB]}, {W[W_TB0], W[W_TANTB]}}));
   return 0;
 }
 
