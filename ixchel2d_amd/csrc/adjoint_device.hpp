@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_adj2_correct(AdjCorrArgs C) {
 }
 
 // up to kCopyMax buffer copies of n doubles in one launch (grid.y = copy)
-constexpr int kCopyMax = 12;
+constexpr int kCopyMax = 16;
 struct CopyList {
   const double *src[kCopyMax];
   double *dst[kCopyMax];

@@ -1925,6 +1925,36 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
 // steps from its checkpoint.
 namespace {
 
+enum ManyOp { MANY_COPY, MANY_ZERO, MANY_ADD };
+
+// one launch over whole buffers: dst = src, dst = 0 or dst += src for every
+// listed (dst, src) pair
+int many(ixh_state *s, ManyOp op, std::initializer_list<std::pair<double *, const double *>> pairs) {
+  CopyList L;
+  memset(&L, 0, sizeof L);
+  int c = 0;
+  for (const auto &pr : pairs) {
+    if (c == kCopyMax) return fail(IXH_ERR_ARG, "many: more than %d buffers", kCopyMax);
+    L.dst[c] = pr.first;
+    L.src[c] = pr.second;
+    ++c;
+  }
+  L.n = (int)s->fsize;
+  const dim3 grid((unsigned)std::min<size_t>((s->fsize + kBlock - 1) / kBlock, 64), (unsigned)c);
+  if (op == MANY_COPY)
+    hipLaunchKernelGGL(k_copy_many, grid, dim3(kBlock), 0, s->stream, L);
+  else if (op == MANY_ZERO)
+    hipLaunchKernelGGL(k_zero_many, grid, dim3(kBlock), 0, s->stream, L);
+  else
+    hipLaunchKernelGGL(k_add_many, grid, dim3(kBlock), 0, s->stream, L);
+  HIPCHK(hipGetLastError());
+  return 0;
+}
+
+int copy_many(ixh_state *s, std::initializer_list<std::pair<double *, const double *>> pairs) {
+  return many(s, MANY_COPY, pairs);
+}
+
 constexpr int kCkArrays = 14;  // u, v, T, p and the lagged u, v, T, both layouts
 
 void ck_fields(ixh_state *s, double *f[kCkArrays]) {
@@ -1938,10 +1968,12 @@ void ck_fields(ixh_state *s, double *f[kCkArrays]) {
 int ck_save(ixh_state *s, double *dst) {
   double *f[kCkArrays];
   ck_fields(s, f);
-  for (int k = 0; k < kCkArrays; ++k)
-    HIPCHK(hipMemcpyAsync(dst + (size_t)k * s->fsize, f[k], s->fsize * sizeof(double), hipMemcpyDeviceToDevice,
-                          s->stream));
-  return 0;
+  const size_t n = s->fsize;
+  return many(s, MANY_COPY,
+              {{dst, f[0]}, {dst + n, f[1]}, {dst + 2 * n, f[2]}, {dst + 3 * n, f[3]}, {dst + 4 * n, f[4]},
+               {dst + 5 * n, f[5]}, {dst + 6 * n, f[6]}, {dst + 7 * n, f[7]}, {dst + 8 * n, f[8]},
+               {dst + 9 * n, f[9]}, {dst + 10 * n, f[10]}, {dst + 11 * n, f[11]}, {dst + 12 * n, f[12]},
+               {dst + 13 * n, f[13]}});
 }
 
 // restore a checkpoint as the state at a step boundary (p' zero, no snapshots)
@@ -1954,10 +1986,12 @@ int ck_load(ixh_state *s, const double *src) {
   s->u_hold = -1;
   double *f[kCkArrays];
   ck_fields(s, f);
-  for (int k = 0; k < kCkArrays; ++k)
-    HIPCHK(hipMemcpyAsync(f[k], src + (size_t)k * s->fsize, s->fsize * sizeof(double), hipMemcpyDeviceToDevice,
-                          s->stream));
-  return 0;
+  const size_t n = s->fsize;
+  return many(s, MANY_COPY,
+              {{f[0], src}, {f[1], src + n}, {f[2], src + 2 * n}, {f[3], src + 3 * n}, {f[4], src + 4 * n},
+               {f[5], src + 5 * n}, {f[6], src + 6 * n}, {f[7], src + 7 * n}, {f[8], src + 8 * n},
+               {f[9], src + 9 * n}, {f[10], src + 10 * n}, {f[11], src + 11 * n}, {f[12], src + 12 * n},
+               {f[13], src + 13 * n}});
 }
 
 int adj_ept(int n) {
@@ -2129,35 +2163,7 @@ int dcopy(ixh_state *s, double *dst, const double *src) {
   return 0;
 }
 
-enum ManyOp { MANY_COPY, MANY_ZERO, MANY_ADD };
 
-// one launch over whole buffers: dst = src, dst = 0 or dst += src for every
-// listed (dst, src) pair
-int many(ixh_state *s, ManyOp op, std::initializer_list<std::pair<double *, const double *>> pairs) {
-  CopyList L;
-  memset(&L, 0, sizeof L);
-  int c = 0;
-  for (const auto &pr : pairs) {
-    if (c == kCopyMax) return fail(IXH_ERR_ARG, "many: more than %d buffers", kCopyMax);
-    L.dst[c] = pr.first;
-    L.src[c] = pr.second;
-    ++c;
-  }
-  L.n = (int)s->fsize;
-  const dim3 grid((unsigned)std::min<size_t>((s->fsize + kBlock - 1) / kBlock, 64), (unsigned)c);
-  if (op == MANY_COPY)
-    hipLaunchKernelGGL(k_copy_many, grid, dim3(kBlock), 0, s->stream, L);
-  else if (op == MANY_ZERO)
-    hipLaunchKernelGGL(k_zero_many, grid, dim3(kBlock), 0, s->stream, L);
-  else
-    hipLaunchKernelGGL(k_add_many, grid, dim3(kBlock), 0, s->stream, L);
-  HIPCHK(hipGetLastError());
-  return 0;
-}
-
-int copy_many(ixh_state *s, std::initializer_list<std::pair<double *, const double *>> pairs) {
-  return many(s, MANY_COPY, pairs);
-}
 
 // one time step in the order of fast_step, every produced field copied to
 // the tape (the residual and the norms do not change fields and are
