@@ -2173,8 +2173,8 @@ int dcopy(ixh_state *s, double *dst, const double *src) {
 // checkpoint is loaded u_ant is another buffer, and the u x-sweep could
 // otherwise pick u0's), so u0, v0 (the v snapshot), T0, u1, u2, v1, v2,
 // a_u, a_v and p0 (which the correction updates in place) are intact after
-// the v x-sweep; p'1 and p'2 after the p' x-sweep; u3, v3 (the corrected
-// u2, v2, in place) and T1, T2 after the T x-sweep.
+// the v x-sweep; p'1 and p'2 after the p' x-sweep and correction; u3, v3
+// and T1, T2 after the T x-sweep.
 int tape_step(ixh_state *s, double *const *tp) {
   double *u0 = s->ub[s->u_cur], *v0 = s->vb[s->v_cur], *t0 = s->tb[s->t_cur];
   s->u_hold = s->u_cur;
@@ -2192,9 +2192,14 @@ int tape_step(ixh_state *s, double *const *tp) {
   s->pc_zero = true;
   CHK(sweep_p(s, IXH_DIR_Y));
   double *pc1 = s->pcb[s->pc_cur];
-  CHK(sweep_p(s, IXH_DIR_X));
+  // the p' x-sweep with the correction: u3, v3 into free buffers (u0, u1,
+  // v0, v1 are on the tape already), p in place, p'1 and p'2 untouched
+  SweepArgs px;
+  CHK(prep_p(s, IXH_DIR_X, px));
+  CorrLine corr;
+  prep_correct_line(s, corr);
+  CHK(launch_pcorr(s, px, corr, line_len(s, EQ_P, IXH_DIR_X)));
   CHK(copy_many(s, {{tp[TP_PC1], pc1}, {tp[TP_PC2], s->pcb[s->pc_cur]}}));
-  CHK(do_correct(s));
   double *u3 = s->ub[s->u_cur], *v3 = s->vb[s->v_cur];
   CHK(sweep_t(s, IXH_DIR_Y));
   double *t1 = s->tb[s->t_cur];
