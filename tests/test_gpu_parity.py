@@ -123,6 +123,26 @@ def test_long_lines_against_oracle(mi, nj):
         assert relerr(g.get(f), o.get(f)) <= REL, f"{f}: rel err {relerr(g.get(f), o.get(f)):.2e}"
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32_mixed", 5e-5), ("fp32_refined", 5e-6)])
+@pytest.mark.parametrize("mi,nj", [(20, 900), (20, 1100), (1100, 20), (700, 30)])
+def test_long_lines_fp32_paths(mi, nj, precision, tol):
+    """The FP32 paths through the same long-line launch shapes (FP32 wave
+    solver up to 1024 nodes, FP32 LDS PCR beyond), 30 fused iterations
+    against the FP64 oracle.  Measured: FP32-mixed 2e-6..6e-6, FP32-refined
+    1e-8..5e-7 of each field's maximum."""
+    case = C.natural_convection_case(mi, nj, height=nj / mi, Ra=1e4, iters=30, dt=0.05, seed=5)
+    case.params.paq_itera = 10
+    from ixchel2d_amd.simple import Cavity
+    g = Cavity(case, solver="pcr", precision=precision)
+    g.run(30)
+    o = Oracle(case)
+    o.run(30)
+    for f in "uvtp":
+        a = g.get(f)
+        assert np.all(np.isfinite(a)), f
+        assert relerr(a, o.get(f)) <= tol, f"{f}: rel err {relerr(a, o.get(f)):.2e}"
+
+
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed", "fp32_refined"])
 def test_fused_steps_split_across_calls(precision):
     """A fused step runs the residual of the step before it beside its v
