@@ -438,7 +438,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
-              "(the cavities of a batch would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(a batch would shard across GPUs by cavity; this build runs on one GPU)", file=sys.stderr)
         sys.exit(2)
 
     import torch
