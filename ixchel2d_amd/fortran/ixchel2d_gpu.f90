@@ -5,8 +5,8 @@
 ! Reads the same case directory upstream reads (parametros.dat, the three
 ! mesh/initial-field files, cond_frontera{u,v,t}.dat) and writes the same
 ! outputs: fort.102 ('SIMPLE', iteration, maxbo, residuo), the nuss_sim file
-! every 1000 iterations, and per package the out_*.dat restart files and the
-! exact .bin dumps when postprocesar is true.  The mesh extents come from the
+! every 1000 iterations, and per package the out_*.dat restart files, the
+! exact .bin dumps and the t_*.vtk sample when postprocesar is true.  The mesh extents come from the
 ! files (no recompilation per mesh).  Environment: IXCHEL2D_SOLVER = pcr
 ! (default) | thomas selects the line solver, IXCHEL2D_PRECISION = fp64
 ! (default) | fp32_mixed | fp32_refined the arithmetic of the sweeps (PCR
@@ -86,6 +86,9 @@ program ixchel2d_gpu
      error stop 'adimen must be natural or mixta'
   end if
   if (paq_itera < 1) error stop 'paq_itera must be positive'
+  ! the VTK sample tag counts tenths of a package (upstream divides by
+  ! paq_itera/10 there)
+  if (postprocesar .and. paq_itera < 10) error stop 'paq_itera must be at least 10 to name the VTK samples'
 
   call dimensiones_malla(entrada_u, entrada_tp)
   mic = entero_caracter(mi)
@@ -191,6 +194,8 @@ program ixchel2d_gpu
         prefijo = 'out_n'//trim(njc)//'m'//trim(mic)//'_R'//trim(Rec)
         call postproceso_dat(trim(prefijo), u, v, temp, pres, placa_min, placa_max, itera_total, ao)
         call postproceso_bin(trim(prefijo), u, v, temp, pres)
+        call postproceso_vtk('n'//trim(njc)//'m'//trim(mic)//'R'//trim(Rec)//'/t_'// &
+             muestra_vtk(itera_total, paq_itera)//'.vtk', u, v, temp, pres)
      end if
   end do
   close(102)

@@ -1,8 +1,8 @@
 !
 ! postproceso: Nusselt numbers and output files (upstream module
 ! source/mod_postproceso.f90).  The Nusselt integrals run on the GPU; the
-! writers produce upstream's text (.dat, which are also restart files) and
-! stream (.bin) formats.
+! writers produce upstream's text (.dat, which are also restart files),
+! stream (.bin) and big-endian VTK formats.
 !
 module postproceso
   use iso_c_binding
@@ -91,5 +91,75 @@ contains
     end do
     close(un)
   end subroutine postproceso_bin
+
+  ! Node velocities of the VTK output (main.f90:1295-1313).  Interior p nodes
+  ! take the mean of their two faces; the side columns take the boundary
+  ! face of u and the ghost column of v; the bottom and top rows take the
+  ! wall faces of v, while u there stays 0 apart from the corners, as
+  ! upstream leaves it.
+  subroutine velocidad_nodal(u, v, uf, vf)
+    real(DBL), intent(in) :: u(mi, nj+1), v(mi+1, nj)
+    real(DBL), intent(out) :: uf(mi+1, nj+1), vf(mi+1, nj+1)
+    integer :: ii, jj
+    uf = 0.0_DBL
+    vf = 0.0_DBL
+    do jj = 2, nj
+       do ii = 2, mi
+          uf(ii, jj) = (u(ii, jj) + u(ii-1, jj)) / 2._DBL
+          vf(ii, jj) = (v(ii, jj) + v(ii, jj-1)) / 2._DBL
+       end do
+       vf(1, jj) = v(1, jj)
+       vf(mi+1, jj) = v(mi+1, jj)
+    end do
+    uf(1, :) = u(1, :)
+    uf(mi+1, :) = u(mi, :)
+    vf(:, 1) = v(:, 1)
+    vf(:, nj+1) = v(:, nj)
+  end subroutine velocidad_nodal
+
+  ! postproceso_vtk (mod_postproceso.f90:225-300): legacy VTK, binary and
+  ! big-endian, a structured grid on the p nodes (ghosts included) carrying
+  ! the pressure, the temperature and the node velocities in FP32.  The
+  ! header bytes are upstream's: its integers in list-directed form, and no
+  ! line break between the POINT_DATA count and the next keyword (VTK
+  ! readers take the count and the keyword apart).
+  subroutine postproceso_vtk(archivo, u, v, temp, pres)
+    character(len=*), intent(in) :: archivo
+    real(DBL), intent(in) :: u(mi, nj+1), v(mi+1, nj), temp(mi+1, nj+1), pres(mi+1, nj+1)
+    real(DBL), allocatable :: uf(:, :), vf(:, :)
+    character(len=32) :: cx, cy, cz, cn
+    character(len=1) :: nl
+    integer :: ii, jj, un
+    allocate(uf(mi+1, nj+1), vf(mi+1, nj+1))
+    call velocidad_nodal(u, v, uf, vf)
+    nl = new_line('a')
+    write(cx, *) mi + 1
+    write(cy, *) nj + 1
+    write(cz, *) 1
+    write(cn, *) (mi + 1) * (nj + 1)
+    open(newunit=un, file=archivo, access='stream', status='replace', convert='big_endian')
+    write(un) '# vtk DataFile Version 2.3'//nl, '3D Mesh'//nl, 'BINARY'//nl, 'DATASET STRUCTURED_GRID'//nl, &
+         'DIMENSIONS '//trim(cx)//trim(cy)//trim(cz)//nl, 'POINTS '//trim(cn)//' float'//nl
+    write(un) ((real(xp(ii)), real(yp(jj)), 0.0, ii = 1, mi + 1), jj = 1, nj + 1)
+    write(un) nl//'POINT_DATA '//trim(cn), 'SCALARS PRESS float'//nl, 'LOOKUP_TABLE default'//nl
+    write(un) real(pres)
+    write(un) nl//'SCALARS TEMPER float'//nl, 'LOOKUP_TABLE default'//nl
+    write(un) real(temp)
+    write(un) nl//'VECTORS VELOCITY float'//nl
+    write(un) ((real(uf(ii, jj)), real(vf(ii, jj)), 0.0, ii = 1, mi + 1), jj = 1, nj + 1)
+    close(un)
+  end subroutine postproceso_vtk
+
+  ! Sample tag of the VTK file after itera_total iterations in packages of
+  ! paq (main.f90:1282-1294): the thousands, hundreds, tens and units digits
+  ! of the package count, then the tenth of a package, one I1 field each.
+  function muestra_vtk(itera_total, paq) result(s)
+    integer, intent(in) :: itera_total, paq
+    character(len=5) :: s
+    integer :: k, r
+    k = itera_total / paq
+    r = mod(itera_total, paq)
+    write(s, '(5I1)') k / 1000, mod(k, 1000) / 100, mod(k, 100) / 10, mod(k, 10), r / (paq / 10)
+  end function muestra_vtk
 
 end module postproceso

@@ -114,3 +114,24 @@ def load_fixture(name: str):
 
 def subsample(a: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(a[::SUB_I, ::SUB_J])
+
+
+def parse_vtk(data: bytes) -> dict:
+    """The sections of a legacy binary VTK structured grid in the layout of
+    upstream's postproceso_vtk: the text pieces (bytes) and the big-endian
+    FP32 blocks (points, p, t, vel) as arrays."""
+    d0 = data.index(b"DIMENSIONS")
+    dims = [int(x) for x in data[d0 + 10:data.index(b"\n", d0)].split()]
+    n = dims[0] * dims[1] * dims[2]
+    j = data.index(b"\n", data.index(b"POINTS ")) + 1
+    out = {"header": data[:j], "points": np.frombuffer(data, ">f4", 3 * n, j)}
+    j += 12 * n
+    for key, marker, cnt in (("p", b"LOOKUP_TABLE default\n", n), ("t", b"LOOKUP_TABLE default\n", n),
+                             ("vel", b"VECTORS VELOCITY float\n", 3 * n)):
+        k = data.index(marker, j) + len(marker)
+        out[key + "_text"] = data[j:k]
+        out[key] = np.frombuffer(data, ">f4", cnt, k)
+        j = k + 4 * cnt
+    if j != len(data):
+        raise ValueError("trailing bytes after the VTK velocity block")
+    return out

@@ -5,8 +5,8 @@ upstream binaries with oracle/build_ref.sh).  For each case of
 tests/golden/cases.py it writes the case files in upstream's formats into a
 scratch directory, runs the serial flang build there, and stores the exact
 FP64 fields (from upstream's stream-access .bin dumps), the fort.102 history
-(maxbo, residuo per logged SIMPLE iteration) and the Nusselt records as
-tests/golden/<case>.npz.  The full-size case stores a subsample of each field
+(maxbo, residuo per logged SIMPLE iteration), the Nusselt records and the
+bytes of the last big-endian VTK sample as tests/golden/<case>.npz.  The full-size case stores a subsample of each field
 plus its centre lines and norms.
 
 usage: python tests/golden/make_golden.py [case ...]
@@ -42,8 +42,17 @@ def run_case(c: C.Case, workdir: str) -> dict:
     t0 = time.time()
     with open(os.path.join(workdir, "stdout.txt"), "w") as log:
         subprocess.run([exe], cwd=workdir, stdout=log, stderr=subprocess.STDOUT, check=True)
+    wall = time.time() - t0
     out = C.read_reference_outputs(c, workdir)
-    out["wall_seconds"] = np.array(time.time() - t0)
+    out["wall_seconds"] = np.array(wall)
+    # the VTK sample of the last package (upstream names them t_<tag>.vtk,
+    # tags increasing with the iteration count)
+    vdir = os.path.join(workdir, f"n{c.mesh.nj:05d}m{c.mesh.mi:05d}R{c.params.rec()}")
+    vtks = sorted(f for f in os.listdir(vdir) if f.endswith(".vtk")) if os.path.isdir(vdir) else []
+    if vtks:
+        with open(os.path.join(vdir, vtks[-1]), "rb") as fh:
+            out["vtk"] = np.frombuffer(fh.read(), dtype=np.uint8)
+        out["vtk_name"] = np.array(vtks[-1])
     return out
 
 
@@ -67,6 +76,9 @@ def make(name: str) -> None:
     else:
         for f in ("u", "v", "t", "p"):
             store[f] = out[f]
+        if "vtk" in out:
+            store["vtk"] = out["vtk"]
+            store["vtk_name"] = out["vtk_name"]
     np.savez_compressed(K.fixture_path(name), **store)
     print(f"{name}: {K.fixture_path(name)} ({float(out['wall_seconds']):.1f} s upstream)", flush=True)
 

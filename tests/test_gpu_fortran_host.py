@@ -1,7 +1,7 @@
 """The Fortran host (ixchel2d_amd/fortran/ixchel2d_gpu) as a drop-in for the
-upstream program: the same case directory in, the same fort.102 history and
-the same exact .bin field dumps out (Thomas line solver: bit-identical; PCR:
-1e-10 relative)."""
+upstream program: the same case directory in, the same fort.102 history,
+the same exact .bin field dumps and the same VTK sample out (Thomas line
+solver: bit-identical; PCR: 1e-10 relative)."""
 import os
 import subprocess
 
@@ -39,6 +39,22 @@ def test_fortran_host_reproduces_upstream_outputs(name, solver, tmp_path):
     np.testing.assert_allclose(out["history"], gold["history"], rtol=1e-9, atol=1e-15)
     if gold["nusselt"].size:
         np.testing.assert_allclose(out["nusselt"][:, :3], gold["nusselt"][:, :3], rtol=1e-12, atol=1e-16)
+    # the last VTK sample: upstream's bytes (Thomas); with PCR the FP32
+    # values within one FP32 rounding of upstream's
+    vdir = os.path.join(tmp_path, f"n{c.mesh.nj:05d}m{c.mesh.mi:05d}R{c.params.rec()}")
+    with open(os.path.join(vdir, str(gold["vtk_name"])), "rb") as fh:
+        data = fh.read()
+    ref = gold["vtk"].tobytes()
+    if solver == "thomas":
+        assert data == ref
+    else:
+        a, b = K.parse_vtk(data), K.parse_vtk(ref)
+        assert a.keys() == b.keys()
+        for k in a:
+            if k in ("points", "p", "t", "vel"):
+                np.testing.assert_allclose(a[k], b[k], rtol=2.5e-7, atol=1e-9 * float(np.max(np.abs(b[k]))), err_msg=k)
+            else:
+                assert a[k] == b[k], k
 
 
 @pytest.mark.parametrize("precision,bound", [("fp32_mixed", 2e-4), ("fp32_refined", 5e-7)])
