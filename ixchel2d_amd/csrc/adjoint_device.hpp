@@ -79,14 +79,15 @@ __device__ __forceinline__ CoefD coef_d(const SweepArgs &A, int ii, int jj) {
     ly = g.dyp[jj];
   }
   CoefD c;
-  c.ai = nbf<double>(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
-  c.ad = nbf<double>(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
-  c.bs = nbf<double>(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
-  c.bn = nbf<double>(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
-  c.dai = nbf_dw(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
-  c.dad = nbf_dw(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
-  c.dbs = nbf_dw(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
-  c.dbn = nbf_dw(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
+  const FaceT<double> fx = face(F.fx[e], ii), fy = face(F.fy[e], jj);
+  c.ai = nbf<double>(fx.c_lo, ly, fx.p_lo, ui, false);
+  c.ad = nbf<double>(fx.c_hi, ly, fx.p_hi, ud, true);
+  c.bs = nbf<double>(fy.c_lo, lx, fy.p_lo, vs, false);
+  c.bn = nbf<double>(fy.c_hi, lx, fy.p_hi, vn, true);
+  c.dai = nbf_dw(fx.c_lo, ly, fx.p_lo, ui, false);
+  c.dad = nbf_dw(fx.c_hi, ly, fx.p_hi, ud, true);
+  c.dbs = nbf_dw(fy.c_lo, lx, fy.p_lo, vs, false);
+  c.dbn = nbf_dw(fy.c_hi, lx, fy.p_hi, vn, true);
   c.area = lx * ly;
   double acc = -c.ai - c.ad - c.bs - c.bn;
   if (A.sp) acc = acc - c.area * AT(A.sp, ii, jj);

@@ -252,41 +252,46 @@ int build_fast(ixh_state *s) {
   const int mi = s->mi, nj = s->nj, gx = mi + 2, gy = nj + 2;
   const std::vector<double> &dxp = s->h_dx[0], &dyp = s->h_dx[1], &dxu = s->h_dx[2], &dyv = s->h_dx[5];
   const double gm = s->prm.gamma_momen, gt = s->prm.gamma_energ;
+  // per equation: gx x-records (west, east) then gy y-records (south,
+  // north), each {conductance, Peclet factor} of its low and high face
   const size_t per = 4 * (size_t)gx + 4 * (size_t)gy;
   std::vector<double> h(3 * per, 0.0);
   auto q = [](double a, double b) { return b != 0.0 ? a / b : 0.0; };
+  auto rec = [&](double *r, double G, double dlo, double dhi) {
+    r[0] = q(G, dlo); r[1] = q(dlo, G);
+    r[2] = q(G, dhi); r[3] = q(dhi, G);
+  };
   for (int e = 0; e < 3; ++e) {
-    double *cw = &h[e * per], *pw = cw + gx, *ce = pw + gx, *pe = ce + gx;
-    double *cs = pe + gx, *ps = cs + gy, *cn = ps + gy, *pn = cn + gy;
+    double *fx = &h[e * per], *fy = fx + 4 * (size_t)gx;
     const double G = e == 2 ? gt : gm;
     for (int i = 2; i <= mi; ++i) {
       // west/east distances: u between u-nodes (dxp), v and T between p-nodes (dxu)
       const double dw = e == 0 ? dxp[i] : dxu[i - 1];
       const double de = e == 0 ? (i + 1 <= mi ? dxp[i + 1] : 0.0) : dxu[i];
-      cw[i] = q(G, dw); pw[i] = q(dw, G);
-      ce[i] = q(G, de); pe[i] = q(de, G);
+      rec(fx + 4 * (size_t)i, G, dw, de);
     }
     for (int j = 2; j <= nj; ++j) {
       // south/north distances: u and T between p-rows (dyv), v between v-rows (dyp)
       const double ds = e == 1 ? dyp[j] : dyv[j - 1];
       const double dn = e == 1 ? (j + 1 <= nj ? dyp[j + 1] : 0.0) : dyv[j];
-      cs[j] = q(G, ds); ps[j] = q(ds, G);
-      cn[j] = q(G, dn); pn[j] = q(dn, G);
+      rec(fy + 4 * (size_t)j, G, ds, dn);
     }
   }
   if (!s->d_fast) CHK(dmalloc(&s->d_fast, 3 * per));
   HIPCHK(hipMemcpy(s->d_fast, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+  static_assert(sizeof(FaceT<double>) == 4 * sizeof(double) && sizeof(FaceT<float>) == 4 * sizeof(float),
+                "face records are four packed values");
   for (int e = 0; e < 3; ++e) {
     const double *b = s->d_fast + e * per;
-    s->fast.cw[e] = b; s->fast.pw[e] = b + gx; s->fast.ce[e] = b + 2 * gx; s->fast.pe[e] = b + 3 * gx;
-    b += 4 * (size_t)gx;
-    s->fast.cs[e] = b; s->fast.ps[e] = b + gy; s->fast.cn[e] = b + 2 * gy; s->fast.pn[e] = b + 3 * gy;
+    s->fast.fx[e] = reinterpret_cast<const FaceT<double> *>(b);
+    s->fast.fy[e] = reinterpret_cast<const FaceT<double> *>(b + 4 * (size_t)gx);
   }
   s->fast.inv_dt = 1.0 / s->prm.dt;
   s->fast.inv_rel_vel = 1.0 / s->prm.rel_vel;
   s->fast.inv_rel_ener = 1.0 / s->prm.rel_ener;
-  // FP32 copies: the 1-D geometry arrays (Geo order) followed by the tables
-  const size_t ngeo = 7 * (size_t)gx + 7 * (size_t)gy;
+  // FP32 copies: the 1-D geometry arrays (Geo order) followed by the tables,
+  // which start on a 16-byte boundary (one record per access)
+  const size_t ngeo = (7 * (size_t)gx + 7 * (size_t)gy + 3) & ~(size_t)3;
   std::vector<float> f(ngeo + h.size());
   {
     const std::vector<double> *xs[] = {&s->h_dx[0], &s->h_dx[2], &s->h_dx[4], &s->h_fx[0], &s->h_fx[1],
@@ -309,12 +314,11 @@ int build_fast(ixh_state *s) {
   d += 7 * (size_t)gx;
   s->geof.dyp = d; s->geof.dyu = d + gy; s->geof.dyv = d + 2 * gy; s->geof.feyp = d + 3 * gy;
   s->geof.feyv = d + 4 * gy; s->geof.yv = d + 5 * gy; s->geof.yp = d + 6 * gy;
-  d += 7 * (size_t)gy;
+  d = s->d_f32 + ngeo;
   for (int e = 0; e < 3; ++e) {
     const float *b = d + e * per;
-    s->fastf.cw[e] = b; s->fastf.pw[e] = b + gx; s->fastf.ce[e] = b + 2 * gx; s->fastf.pe[e] = b + 3 * gx;
-    b += 4 * (size_t)gx;
-    s->fastf.cs[e] = b; s->fastf.ps[e] = b + gy; s->fastf.cn[e] = b + 2 * gy; s->fastf.pn[e] = b + 3 * gy;
+    s->fastf.fx[e] = reinterpret_cast<const FaceT<float> *>(b);
+    s->fastf.fy[e] = reinterpret_cast<const FaceT<float> *>(b + 4 * (size_t)gx);
   }
   s->fastf.inv_dt = (float)s->fast.inv_dt;
   s->fastf.inv_rel_vel = (float)s->fast.inv_rel_vel;

@@ -93,14 +93,28 @@ struct Side {
 
 // Geometry-only factors of the division-free assembly, one set per equation
 // (0 = u, 1 = v, 2 = T): for each face the conductance G/d and the Peclet
-// factor d/G (constant G), west/east faces indexed by ii, south/north by jj.
-// Upstream computes G*L/d and w*d/G with divisions in every cell; these
-// tables leave only multiplications (results move by a few ulp).
+// factor d/G (constant G).  Upstream computes G*L/d and w*d/G with divisions
+// in every cell; these tables leave only multiplications (results move by a
+// few ulp).  A node's two faces along one direction form one record (low:
+// west or south, high: east or north), which a row reads in one access:
+// fx indexed by ii, fy by jj.
+template <typename R>
+struct alignas(4 * sizeof(R)) FaceT {
+  R c_lo, p_lo, c_hi, p_hi;
+};
 template <typename R>
 struct FastGeoT {
-  GPtr<const R> cw[3], pw[3], ce[3], pe[3], cs[3], ps[3], cn[3], pn[3];
+  GPtr<const FaceT<R>> fx[3], fy[3];
   R inv_dt, inv_rel_vel, inv_rel_ener;
 };
+// record i of a face table, read as one vector (two 16-byte loads in FP64,
+// one in FP32)
+template <typename R>
+__device__ __forceinline__ FaceT<R> face(const GPtr<const FaceT<R>> &t, int i) {
+  typedef R R4 __attribute__((ext_vector_type(4)));
+  const R4 v = *reinterpret_cast<const __attribute__((address_space(1))) R4 *>(&t[i]);
+  return FaceT<R>{v.x, v.y, v.z, v.w};
+}
 using FastGeo = FastGeoT<double>;
 using FastGeoF = FastGeoT<float>;
 
@@ -450,10 +464,11 @@ __device__ __forceinline__ CoefT<R> fast_coef(const SweepArgs &A, int ii, int jj
     ly = g.dyp[jj];
   }
   CoefT<R> c;
-  c.ai = nbf<R>(F.cw[e][ii], ly, F.pw[e][ii], ui, false);
-  c.ad = nbf<R>(F.ce[e][ii], ly, F.pe[e][ii], ud, true);
-  c.bs = nbf<R>(F.cs[e][jj], lx, F.ps[e][jj], vs, false);
-  c.bn = nbf<R>(F.cn[e][jj], lx, F.pn[e][jj], vn, true);
+  const FaceT<R> fx = face(F.fx[e], ii), fy = face(F.fy[e], jj);
+  c.ai = nbf<R>(fx.c_lo, ly, fx.p_lo, ui, false);
+  c.ad = nbf<R>(fx.c_hi, ly, fx.p_hi, ud, true);
+  c.bs = nbf<R>(fy.c_lo, lx, fy.p_lo, vs, false);
+  c.bn = nbf<R>(fy.c_hi, lx, fy.p_hi, vn, true);
   area = lx * ly;
   R acc = -c.ai - c.ad - c.bs - c.bn;
   pin = __builtin_nan("");
