@@ -104,6 +104,17 @@ def cpu_baseline(case, budget_s: float = 20.0):
             "sample": f"C oracle (OpenMP), {n} iterations of the 301x701 case"}
 
 
+def batch_traffic(args):
+    """HBM bytes of one step of the default 256-cavity FP64 batch, from the
+    FETCH_SIZE / WRITE_SIZE counter runs summarised in profiles/ (None for
+    other batches and precisions, which were not counted)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_batch256_{args.precision}.json")
+    if args.batch != 256 or args.solver != "pcr" or not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh).get("hbm_bytes_per_step")
+
+
 def batch_bench(args):
     """Aggregate throughput of the mixed-size batch (256 cavities of 65^2,
     129^2, 257^2 and 129x301 cells, lid-driven and natural convection, 2 of
@@ -161,7 +172,7 @@ def batch_bench(args):
                    "autotune_ms_per_step": {f"{k[0]}{'+graph' if k[1] else ''}": round(v, 4)
                                             for k, v in table.items()}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "whole step",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": batch_traffic(args), "kernel": "whole step",
                      "iteration_alg_bytes": iter_bytes},
         "cpu_baseline": cpu,
     }
