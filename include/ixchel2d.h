@@ -223,8 +223,11 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
  * in ixh_set_bc) and dJ/dT of the initial temperature (with T_ant taken equal
  * to it).  The adjoint holds the velocities of each step fixed: it is the
  * exact gradient when the flow does not depend on T (ri_x = ri_y = 0), and
- * the frozen-flow part of it under buoyancy.  The state ends as after the
- * recorded run.  FP64 path only. */
+ * the frozen-flow part of it under buoyancy.  Afterwards u, v, T, p and the
+ * lagged u_ant, v_ant, T_ant are those at the end of the recorded run, which
+ * is all a later time step reads; p' reads as zero, and a_u, a_v, b_o and
+ * Res_u hold values of the recomputation (every step recomputes them before
+ * use).  FP64 path only. */
 int ixh_adjoint_record(ixh_state *s, int32_t nsteps, int32_t every);
 int ixh_adjoint_energy(ixh_state *s, const double *tbar, int32_t ld, double *grad_a, double *grad_b,
                        double *grad_c, double *grad_d, double *grad_t0, int32_t ld0);

@@ -93,22 +93,36 @@ class HotWallProblem:
         if np.any(self.types_a[1:-1] != BC_DIRI):
             raise ValueError("the hot wall (side a) must be Dirichlet for T")
         self.start = start if start is not None else {f: self.cav.get(f) for f in ("u", "v", "t", "p")}
+        self._recorded: Optional[np.ndarray] = None  # controls of the run the cavity holds
 
     def wall_values(self, controls) -> np.ndarray:
         return self.B @ np.asarray(controls, dtype=np.float64)
 
     def forward(self, controls) -> float:
         c = self.cav
+        controls = np.array(controls, dtype=np.float64)
+        self._recorded = None
         c.set_bc("t", "a", self.types_a, self.wall_values(controls))
         for f, arr in self.start.items():
             c.set(f, arr)
             if f in ("u", "v", "t"):
                 c.set(f + "_ant", arr)
         c.adjoint_record(self.steps, self.every)
+        self._recorded = controls
         return c.nusselt_x()[0]
 
-    def gradient(self, scale: float = 1.0) -> np.ndarray:
-        """dJ/dc of the last forward evaluation, times ``scale``."""
+    def gradient(self, scale: float = 1.0, controls=None) -> np.ndarray:
+        """dJ/dc times ``scale`` at ``controls``, or at the controls of the
+        last forward evaluation when None.  The adjoint runs backward over
+        the run the cavity recorded last, so when that run belongs to other
+        controls (another forward evaluation came in between) the run is
+        recorded again at ``controls`` first."""
+        if controls is not None:
+            controls = np.asarray(controls, dtype=np.float64)
+            if self._recorded is None or not np.array_equal(controls, self._recorded):
+                self.forward(controls)
+        elif self._recorded is None:
+            raise RuntimeError("gradient() needs a forward evaluation first")
         g, _ = self.cav.adjoint_energy(scale * self.W, coupled=self.coupled)
         return self.B.T @ g["a"]
 
@@ -122,13 +136,17 @@ try:
 
         @staticmethod
         def forward(ctx, controls, problem: HotWallProblem):
-            nu = problem.forward(controls.detach().cpu().double().numpy())
+            c = controls.detach().cpu().double().numpy().copy()
+            nu = problem.forward(c)
             ctx.problem = problem
+            ctx.controls = c
             return controls.new_tensor(nu, dtype=torch.float64)
 
         @staticmethod
         def backward(ctx, grad_out):
-            g = ctx.problem.gradient(float(grad_out))
+            # at this evaluation's own controls, even when other forward
+            # evaluations of the same problem ran since
+            g = ctx.problem.gradient(float(grad_out), controls=ctx.controls)
             return torch.from_numpy(g).to(device=grad_out.device, dtype=torch.float64), None
 except ImportError:  # torch is optional for the numpy interface
     pass

@@ -853,6 +853,52 @@ int ensure_hist(ixh_state *s, int n) {
 
 }  // namespace
 
+namespace {
+
+// ---- batched tridiagonal solves of host arrays (ixh_tridiagonal*)
+int check_tridiag(const double *a, const double *b, const double *c, const double *r, int n, int nsys, int solver) {
+  if (!a || !b || !c || !r || n < 1 || nsys < 0) return fail(IXH_ERR_ARG, "bad tridiagonal arguments");
+  if (n > kNmax) return fail(IXH_ERR_UNSUPP, "systems longer than %d are not built", kNmax);
+  if (solver != IXH_SOLVER_PCR && solver != IXH_SOLVER_THOMAS) return fail(IXH_ERR_ARG, "unknown solver");
+  return 0;
+}
+
+// nsys systems of n rows through device memory d (4 n nsys doubles): copy
+// a, b, c, r in, solve on the null stream, copy r and b back
+int tridiag_on(double *d, const double *a, double *b, const double *c, double *r, int n, int nsys, int solver) {
+  const size_t tot = (size_t)n * nsys;
+  double *da = d, *db = d + tot, *dc = d + 2 * tot, *dr = d + 3 * tot;
+  hipError_t e = hipSuccess;
+  if ((e = hipMemcpy(da, a, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(db, b, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dc, c, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
+      (e = hipMemcpy(dr, r, tot * 8, hipMemcpyHostToDevice)) != hipSuccess)
+    return fail(IXH_ERR_HIP, "tridiagonal upload: %s", hipGetErrorString(e));
+  const int ept = ept_for(n);
+#define IXH_TRI(S, E) hipLaunchKernelGGL((k_tridiag<S, E>), dim3(nsys), dim3(kBlock), 0, 0, da, db, dc, dr, n)
+  if (solver == IXH_SOLVER_THOMAS) {
+    switch (ept) {
+      case 1: IXH_TRI(SOLVE_THOMAS, 1); break; case 2: IXH_TRI(SOLVE_THOMAS, 2); break;
+      case 3: IXH_TRI(SOLVE_THOMAS, 3); break; case 4: IXH_TRI(SOLVE_THOMAS, 4); break;
+      case 6: IXH_TRI(SOLVE_THOMAS, 6); break; default: IXH_TRI(SOLVE_THOMAS, 8); break;
+    }
+  } else {
+    switch (ept) {
+      case 1: IXH_TRI(SOLVE_PCR, 1); break; case 2: IXH_TRI(SOLVE_PCR, 2); break;
+      case 3: IXH_TRI(SOLVE_PCR, 3); break; case 4: IXH_TRI(SOLVE_PCR, 4); break;
+      case 6: IXH_TRI(SOLVE_PCR, 6); break; default: IXH_TRI(SOLVE_PCR, 8); break;
+    }
+  }
+#undef IXH_TRI
+  e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(r, dr, tot * 8, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(b, db, tot * 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(IXH_ERR_HIP, "tridiagonal: %s", hipGetErrorString(e));
+  return 0;
+}
+
+}  // namespace
+
 // =================================================================== C-ABI
 extern "C" {
 
@@ -1016,6 +1062,9 @@ int ixh_set_params(ixh_state *s, const ixh_params *params) {
   CHK(check_params(params));
   if (params->precision != s->prm.precision)
     return fail(IXH_ERR_ARG, "the precision of a state is fixed when it is created");
+  // the factor tables are rewritten with synchronous copies: work still
+  // queued on the state's stream (possibly a non-blocking one) reads them
+  HIPCHK(hipStreamSynchronize(s->stream));
   s->prm = *params;
   return build_fast(s);
 }
@@ -1159,6 +1208,8 @@ int ixh_set_bc(ixh_state *s, int32_t var, int32_t side, const int32_t *type, con
     t[k] = (int8_t)ty;
     v[k] = value[k - 1];
   }
+  // queued sweeps of the state may still read the old tables
+  HIPCHK(hipStreamSynchronize(s->stream));
   HIPCHK(hipMemcpy(s->bct[var][side], t.data(), n + 2, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(s->bcv[var][side], v.data(), (n + 2) * sizeof(double), hipMemcpyHostToDevice));
   s->bc_set[var][side] = true;
@@ -1167,6 +1218,8 @@ int ixh_set_bc(ixh_state *s, int32_t var, int32_t side, const int32_t *type, con
 
 int ixh_set_sources(ixh_state *s, int32_t var, const double *sc, const double *sp) {
   if (!s || var < 0 || var > 2) return fail(IXH_ERR_ARG, "bad source arguments");
+  // queued sweeps of the state may still read the old sources
+  HIPCHK(hipStreamSynchronize(s->stream));
   if (!sc && !sp) {
     s->has_src[var] = false;
     return 0;
@@ -1412,43 +1465,12 @@ int ixh_nusselt_x(ixh_state *s, double *nu_w, double *nu_e) {
 
 int ixh_tridiagonal(const double *a, double *b, const double *c, double *r, int32_t n, int32_t nsys,
                     int32_t solver) {
-  if (!a || !b || !c || !r || n < 1 || nsys < 0) return fail(IXH_ERR_ARG, "bad tridiagonal arguments");
-  if (n > kNmax) return fail(IXH_ERR_UNSUPP, "systems longer than %d are not built", kNmax);
-  if (solver != IXH_SOLVER_PCR && solver != IXH_SOLVER_THOMAS) return fail(IXH_ERR_ARG, "unknown solver");
+  CHK(check_tridiag(a, b, c, r, n, nsys, solver));
   if (nsys == 0) return 0;
   const size_t tot = (size_t)n * nsys;
   double *d = nullptr;
   HIPCHK(hipMalloc((void **)&d, 4 * tot * sizeof(double)));
-  double *da = d, *db = d + tot, *dc = d + 2 * tot, *dr = d + 3 * tot;
-  int rc = 0;
-  hipError_t e = hipSuccess;
-  if ((e = hipMemcpy(da, a, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(db, b, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(dc, c, tot * 8, hipMemcpyHostToDevice)) != hipSuccess ||
-      (e = hipMemcpy(dr, r, tot * 8, hipMemcpyHostToDevice)) != hipSuccess) {
-    (void)hipFree(d);
-    return fail(IXH_ERR_HIP, "tridiagonal upload: %s", hipGetErrorString(e));
-  }
-  const int ept = ept_for(n);
-#define IXH_TRI(S, E) hipLaunchKernelGGL((k_tridiag<S, E>), dim3(nsys), dim3(kBlock), 0, 0, da, db, dc, dr, n)
-  if (solver == IXH_SOLVER_THOMAS) {
-    switch (ept) {
-      case 1: IXH_TRI(SOLVE_THOMAS, 1); break; case 2: IXH_TRI(SOLVE_THOMAS, 2); break;
-      case 3: IXH_TRI(SOLVE_THOMAS, 3); break; case 4: IXH_TRI(SOLVE_THOMAS, 4); break;
-      case 6: IXH_TRI(SOLVE_THOMAS, 6); break; default: IXH_TRI(SOLVE_THOMAS, 8); break;
-    }
-  } else {
-    switch (ept) {
-      case 1: IXH_TRI(SOLVE_PCR, 1); break; case 2: IXH_TRI(SOLVE_PCR, 2); break;
-      case 3: IXH_TRI(SOLVE_PCR, 3); break; case 4: IXH_TRI(SOLVE_PCR, 4); break;
-      case 6: IXH_TRI(SOLVE_PCR, 6); break; default: IXH_TRI(SOLVE_PCR, 8); break;
-    }
-  }
-#undef IXH_TRI
-  e = hipGetLastError();
-  if (e == hipSuccess) e = hipMemcpy(r, dr, tot * 8, hipMemcpyDeviceToHost);
-  if (e == hipSuccess) e = hipMemcpy(b, db, tot * 8, hipMemcpyDeviceToHost);
-  if (e != hipSuccess) rc = fail(IXH_ERR_HIP, "tridiagonal: %s", hipGetErrorString(e));
+  const int rc = tridiag_on(d, a, b, c, r, n, nsys, solver);
   (void)hipFree(d);
   return rc;
 }

@@ -14,14 +14,19 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU run")
 
 
-def pytest_collection_modifyitems(config, items):
-    # print the case before every GPU test so a crash names it
-    for it in items:
-        if "gpu" in it.keywords:
-            it.add_marker(pytest.mark.usefixtures("_announce"))
-
-
-@pytest.fixture
-def _announce(request):
-    print(f"\n[gpu-test] {request.node.nodeid}", flush=True)
-    yield
+@pytest.hookimpl(tryfirst=True)
+def pytest_runtest_setup(item):
+    # name every GPU test on the terminal before it runs, with output
+    # capturing suspended and the line flushed, so that the tail of a run
+    # that took the device down says which test it was
+    if item.get_closest_marker("gpu") is None:
+        return
+    capman = item.config.pluginmanager.getplugin("capturemanager")
+    line = f"\n[gpu-test] {item.nodeid}\n"
+    if capman is None:
+        sys.stderr.write(line)
+        sys.stderr.flush()
+        return
+    with capman.global_and_fixture_disabled():
+        sys.stderr.write(line)
+        sys.stderr.flush()

@@ -197,3 +197,83 @@ def test_coupled_initial_temperature_with_buoyancy(buoyant):
         buoyant.start = dict(buoyant.start, t=saved)
     fd = (jp - jm) / (2 * eps)
     assert abs(fd - float(np.sum(gt0 * d))) <= 1e-5 * abs(fd)
+
+
+def test_autograd_backward_after_a_second_forward(problem):
+    """Two evaluations, then the backward pass of the first: its gradient is
+    that of its own controls, not of the run the cavity recorded last."""
+    torch = pytest.importorskip("torch")
+    c1 = torch.tensor(_controls(8, seed=12), dtype=torch.float64, requires_grad=True)
+    c2 = torch.tensor(_controls(8, seed=13), dtype=torch.float64, requires_grad=True)
+    nu1 = A.HotWallNusselt.apply(c1, problem)
+    nu2 = A.HotWallNusselt.apply(c2, problem)
+    nu1.backward()
+    problem.forward(c1.detach().numpy())
+    g1 = problem.gradient()
+    np.testing.assert_allclose(c1.grad.numpy(), g1, rtol=1e-14, atol=0)
+    nu2.backward()
+    problem.forward(c2.detach().numpy())
+    g2 = problem.gradient()
+    np.testing.assert_allclose(c2.grad.numpy(), g2, rtol=1e-14, atol=0)
+    assert not np.allclose(g1, g2, rtol=1e-6, atol=0)
+
+
+# ---- the adjoint workload at its own size: natural convection on 129^2
+# cells (uniform mesh), Ra = 1e5, Pr = 0.71, 64 piecewise-linear hot-wall
+# controls 1 +- U(0.1) (seed 0), J = mean hot-wall Nu after K = 400
+# iterations from a converged start, checkpoints every 20
+
+@pytest.fixture(scope="module")
+def nc129():
+    from ixchel2d_amd.simple import Cavity
+    case = C.natural_convection_case(130, 130, height=1.0, Ra=1e5, iters=400, dt=0.01, seed=0,
+                                     mesh_kind="unif")
+    warm = Cavity(case, solver="pcr")
+    warm.run(20000, cap=1)
+    h = warm.run(1)
+    start = {f: warm.get(f) for f in ("u", "v", "t", "p")}
+    warm.close()
+    assert h["residuo"][-1] < 1e-5, h["residuo"][-1]
+    prob = A.HotWallProblem(case, ncontrols=64, steps=400, every=20, start=start, coupled=True)
+    c = 1.0 + np.random.default_rng(0).uniform(-0.1, 0.1, 64)
+    prob.forward(c)
+    return prob, c, prob.gradient()
+
+
+FD_CONTROLS = (0, 13, 31, 50, 63)
+
+
+def test_adjoint129_against_gpu_central_differences(nc129):
+    prob, c, g = nc129
+    h = 1e-6
+    fd = []
+    for i in FD_CONTROLS:
+        e = np.zeros(64)
+        e[i] = h
+        fd.append((prob.forward(c + e) - prob.forward(c - e)) / (2 * h))
+    fd = np.array(fd)
+    err = np.max(np.abs(g[list(FD_CONTROLS)] - fd)) / np.max(np.abs(fd))
+    assert err <= 1e-5, (err, g[list(FD_CONTROLS)], fd)
+
+
+def test_adjoint129_against_oracle_central_difference(nc129):
+    """The FP64 reference: central differences of the oracle's own K-step
+    run (the C restatement on the CPU) for one control."""
+    prob, c, g = nc129
+    i = 31
+    h = 1e-6
+
+    def J(cc):
+        o = Oracle(prob.case)
+        o.set_bc("t", "a", prob.types_a, prob.wall_values(cc))
+        for f, arr in prob.start.items():
+            o.set(f, arr)
+            if f in ("u", "v", "t"):
+                o.set(f + "_ant", arr)
+        o.run(400)
+        return o.nusselt_x()[0]
+
+    e = np.zeros(64)
+    e[i] = h
+    ref = (J(c + e) - J(c - e)) / (2 * h)
+    assert abs(g[i] - ref) <= 1e-5 * abs(ref), (g[i], ref)
