@@ -171,6 +171,17 @@ int ixh_nusselt_x(ixh_state *s, double *nu_west, double *nu_east);
  * device; b and r are overwritten as upstream does (r = solution). */
 int ixh_tridiagonal(const double *a, double *b, const double *c, double *r, int32_t n,
                     int32_t nsys, int32_t solver);
+/* ixh_tridiagonal without any device allocation: no upstream counterpart;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, per-line calls such as the Fortran
+ * tridiagonal wrapper).  The systems go through a device scratch array of
+ * 4 x 262144 doubles (8 MiB per GPU, held until the process ends), so
+ * n * nsys must be at most 262144 (IXH_ERR_UNSUPP otherwise).  Same
+ * arguments, results and errors as ixh_tridiagonal.  Every call on one GPU
+ * uses that GPU's one array: calls that overlap on one GPU (other host
+ * threads) race. */
+int ixh_tridiagonal_noalloc(const double *a, double *b, const double *c, double *r, int32_t n,
+                            int32_t nsys, int32_t solver);
 
 /* per-kernel timing of the ixh_simple_iterations calls made with profiling
  * on (ixh_set_profiling resets it): for each kernel id the total milliseconds

@@ -27,7 +27,7 @@ EXPORTS = [
     "ixh_synchronize", "ixh_upload_field", "ixh_download_field", "ixh_set_bc", "ixh_set_sources",
     "ixh_momentum_u", "ixh_momentum_v", "ixh_pcorr_zero", "ixh_pcorr", "ixh_correct", "ixh_energy",
     "ixh_residual_u", "ixh_lag", "ixh_simple_iterations", "ixh_nusselt_y", "ixh_nusselt_x",
-    "ixh_tridiagonal", "ixh_set_profiling", "ixh_kernel_times", "ixh_kernel_name",
+    "ixh_tridiagonal", "ixh_tridiagonal_noalloc", "ixh_set_profiling", "ixh_kernel_times", "ixh_kernel_name",
     "ixh_batch_create", "ixh_batch_destroy", "ixh_batch_set_schedule", "ixh_batch_iterations",
     "ixh_batch_autotune", "ixh_adjoint_record", "ixh_adjoint_energy",
     "ixh_adjoint_coupled",
@@ -100,6 +100,7 @@ def load(path: str = LIB_PATH):
     L.ixh_nusselt_y.argtypes = [vp, C.c_int32, C.c_int32, dp, dp]
     L.ixh_nusselt_x.argtypes = [vp, dp, dp]
     L.ixh_tridiagonal.argtypes = [dp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32]
+    L.ixh_tridiagonal_noalloc.argtypes = [dp, dp, dp, dp, C.c_int32, C.c_int32, C.c_int32]
     L.ixh_set_profiling.argtypes = [vp, C.c_int32]
     L.ixh_kernel_times.argtypes = [vp, dp, ip, C.c_int32, ip]
     L.ixh_batch_create.argtypes = [C.POINTER(vp), C.c_int32, C.POINTER(vp)]

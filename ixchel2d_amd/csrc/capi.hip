@@ -9,6 +9,7 @@
 #include <cstring>
 #include <string>
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "../../include/ixchel2d.h"
@@ -853,6 +854,17 @@ int ensure_hist(ixh_state *s, int n) {
 
 }  // namespace
 
+// values per array of ixh_tridiagonal_noalloc's scratch
+constexpr size_t kTriScratchPerArray = (size_t)1 << 18;
+// Device scratch of ixh_tridiagonal_noalloc: the four arrays of the systems
+// of one call, 8 MiB in all, held by every GPU the library runs on until the
+// process ends.  In host code `scratch` names a host-side stand-in, not the
+// device array: its device address is looked up once per device with
+// hipGetSymbolAddress (with that device current) and only that address is
+// used.  Calls on one GPU share its array, so calls that overlap on one GPU
+// race; each call copies in every value it reads, so nothing needs clearing.
+static __device__ __align__(256) double scratch[4 * kTriScratchPerArray];
+
 namespace {
 
 // ---- batched tridiagonal solves of host arrays (ixh_tridiagonal*)
@@ -894,6 +906,24 @@ int tridiag_on(double *d, const double *a, double *b, const double *c, double *r
   if (e == hipSuccess) e = hipMemcpy(r, dr, tot * 8, hipMemcpyDeviceToHost);
   if (e == hipSuccess) e = hipMemcpy(b, db, tot * 8, hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail(IXH_ERR_HIP, "tridiagonal: %s", hipGetErrorString(e));
+  return 0;
+}
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxDevices] = {};
+
+int tri_scratch(double **out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) return fail(IXH_ERR_UNSUPP, "device %d beyond %d", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double *>(p);
+  }
+  *out = g_scratch_ptr[dev];
   return 0;
 }
 
@@ -1473,6 +1503,18 @@ int ixh_tridiagonal(const double *a, double *b, const double *c, double *r, int3
   const int rc = tridiag_on(d, a, b, c, r, n, nsys, solver);
   (void)hipFree(d);
   return rc;
+}
+
+int ixh_tridiagonal_noalloc(const double *a, double *b, const double *c, double *r, int32_t n, int32_t nsys,
+                            int32_t solver) {
+  CHK(check_tridiag(a, b, c, r, n, nsys, solver));
+  if (nsys == 0) return 0;
+  if ((size_t)n * nsys > kTriScratchPerArray)
+    return fail(IXH_ERR_UNSUPP, "ixh_tridiagonal_noalloc: n*nsys = %zu exceeds its scratch of %zu values per array",
+                (size_t)n * nsys, kTriScratchPerArray);
+  double *d = nullptr;
+  CHK(tri_scratch(&d));
+  return tridiag_on(d, a, b, c, r, n, nsys, solver);
 }
 
 int ixh_set_profiling(ixh_state *s, int32_t on) {

@@ -165,6 +165,13 @@ module ixchel_hip
        real(c_double), intent(inout) :: b(*), r(*)
        integer(c_int32_t), value :: n, nsys, solver
      end function ixh_tridiagonal
+     integer(c_int) function ixh_tridiagonal_noalloc(a, b, c, r, n, nsys, solver) &
+          bind(C, name='ixh_tridiagonal_noalloc')
+       import :: c_int, c_int32_t, c_double
+       real(c_double), intent(in) :: a(*), c(*)
+       real(c_double), intent(inout) :: b(*), r(*)
+       integer(c_int32_t), value :: n, nsys, solver
+     end function ixh_tridiagonal_noalloc
      integer(c_int) function ixh_set_profiling(state, on) bind(C, name='ixh_set_profiling')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: state
