@@ -203,9 +203,10 @@ __device__ __forceinline__ float adiv(float a, float b) { return a * __builtin_a
 __device__ __forceinline__ double rabs(double x) { return fabs(x); }
 __device__ __forceinline__ float rabs(float x) { return fabsf(x); }
 
-// an FP32-mixed momentum or energy row whose source is an immersed-body
-// penalty (|Sp| above this) is replaced by phi = Sc / -Sp: the penalty
-// itself (1e51) does not fit FP32
+// an FP32 momentum or energy row whose source is an immersed-body penalty
+// (|Sp| above this) is replaced by phi = alpha Sc / -Sp + (1 - alpha) phi_old,
+// the limit of upstream's under-relaxed penalty row: the penalty itself
+// (1e51) does not fit FP32
 constexpr double kPenaltyRow = 1.0e30;
 
 // Field element (i,j), 1-based.  Every field is kept twice: row layout
@@ -952,6 +953,12 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   // and of a penalised FP32 row (infinite)
   const R big = sizeof(R) == 8 ? R(1.e40) : R(__builtin_huge_valf());
   const S bigs = sizeof(S) == 8 ? S(1.e40) : S(__builtin_huge_valf());
+  // Under-relaxation of a penalised row.  Upstream's row of an immersed-body
+  // node carries the 1e51 penalty in a_P and Sc and is under-relaxed like
+  // any other, so its value moves as phi <- alpha Sc/-Sp + (1 - alpha) phi
+  // (source/mod_ec_momento.f90:528-537, mod_ec_energia.f90:260-266); the
+  // FP32 rows replacing it follow the same update.
+  const double relx = EQ == EQ_T ? A.ph.rel_ener : A.ph.rel_vel;
   S *acoef = fldw<S>(A.acoef), *acoef2 = fldw<S>(A.acoef2);
   S *out = fldw<S>(A.out), *out2 = fldw<S>(A.out2);
   // the boundary rows at both ends (p': Dirichlet 0, main.f90:789-800, 845-856)
@@ -1054,7 +1061,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       a0 = pinned ? 0.0 : a0;
       a1 = pinned ? 1.0 : a1;
       a2 = pinned ? 0.0 : a2;
-      rr = pinned ? pin - phic : rr;
+      rr = pinned ? relx * (pin - phic) : rr;
       aw = pinned ? big : aw;
     } else if constexpr (SOLVER == SOLVE_PCR) {
       R area;
@@ -1074,10 +1081,11 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       aw = c.ac * R(A.ph.rel_vel);
       if constexpr (sizeof(R) == 4) {
         const bool pinned = pin == pin;
+        const double phio = (double)AT(phi, ii, jj);
         a0 = pinned ? 0.0 : a0;
         a1 = pinned ? 1.0 : a1;
         a2 = pinned ? 0.0 : a2;
-        rr = pinned ? pin : rr;
+        rr = pinned ? relx * pin + (1.0 - relx) * phio : rr;
         aw = pinned ? big : aw;
       }
     } else {

@@ -3,14 +3,16 @@ assembly in FP32 with FP32 storage of u, v, T, p', a_u, a_v; mass residual,
 p' right-hand side, p, the norms, the u residual and the Nusselt integrals in
 FP64.  Its error against the FP64 upstream fixtures is bounded by the FP32
 rounding of the fields, accumulated over the iterations; the bounds below are
-about 5x the errors measured on MI355X (nc301x701: 5x the field errors,
-5x the Nusselt error):
+a few times the errors measured on MI355X:
   - nc18x14, mixed22x17 (20-30 iterations): fields 1e-6 .. 2e-6;
   - lid66 (2000 iterations): 4e-6;
   - nc34 (1000 iterations, transient): 4e-5;
-  - nc301x701 (5000 iterations, transient): 2e-4, hot-wall Nu 5e-4.
-Immersed bodies become rows phi = Sc/-Sp (the 1e51 penalty does not fit
-FP32): the body temperature must be exact."""
+  - nc301x701 (5000 iterations, transient): fields 1.6e-4 (bound 4e-4),
+    hot-wall Nu 4.5e-4 (bound 1e-3) -- above the 1e-4 Nu target, which the
+    FP32-refined path meets.
+Immersed bodies become rows phi = alpha Sc/-Sp + (1 - alpha) phi_old, the
+limit of upstream's under-relaxed 1e51 penalty row (which does not fit
+FP32): the body follows the FP64 path's values."""
 import numpy as np
 import pytest
 
@@ -62,14 +64,18 @@ def test_fp32_mixed_full_size_5000_iterations():
     gold = K.load_fixture("nc301x701")
     g = _cavity(case)
     g.run(5000)
+    # measured on MI355X: fields within 1.2e-4 .. 1.6e-4 of their maxima,
+    # hot-wall Nu 4.5e-4 from FP64.  This path does not meet the 1e-4 Nu
+    # bound (FP32 storage of fields that change by less than their FP32
+    # spacing per step); the FP32-refined path does and is tested to it
     for f in "uvtp":
         a = g.get(f)
-        assert relerr(K.subsample(a), gold[f + "_sub"]) <= 1e-3, f
-        assert relerr(a[a.shape[0] // 2, :], gold[f + "_xmid"]) <= 1e-3, f
+        assert relerr(K.subsample(a), gold[f + "_sub"]) <= 4e-4, f
+        assert relerr(a[a.shape[0] // 2, :], gold[f + "_xmid"]) <= 4e-4, f
     g64 = _cavity(case, "fp64")
     g64.run(5000)
     nu32, nu64 = g.nusselt_x()[0], g64.nusselt_x()[0]
-    assert abs(nu32 - nu64) / abs(nu64) <= 2.5e-3
+    assert abs(nu32 - nu64) / abs(nu64) <= 1e-3
 
 
 def test_fp32_mixed_nusselt_is_fp64_integral_of_fp32_field():
@@ -98,11 +104,13 @@ def test_fp32_mixed_immersed_cylinder():
     inside = (X - 0.55) ** 2 + (Y - 0.6) ** 2 <= 0.15 ** 2
     for f in "uvtp":
         assert np.all(np.isfinite(g.get(f))), f
-    assert np.max(np.abs(g.get("t")[inside] - 1.0)) == 0.0
+    # a penalised row relaxes toward the body value as upstream's
+    # under-relaxed penalty row does: T inside follows the FP64 path's
+    assert np.max(np.abs(g.get("t")[inside] - g64.get("t")[inside])) <= 1e-6
     for f in "uvt":
-        assert relerr(g.get(f), g64.get(f)) <= 5e-3, f
+        assert relerr(g.get(f), g64.get(f)) <= 2e-5, f
     p32, p64 = g.get("p"), g64.get("p")
-    assert relerr(p32[~inside], p64[~inside]) <= 5e-3
+    assert relerr(p32[~inside], p64[~inside]) <= 2e-5
 
 
 def test_fp32_mixed_batch_matches_single_cavity():
