@@ -10,7 +10,10 @@
 ! files (no recompilation per mesh).  Environment: IXCHEL2D_SOLVER = pcr
 ! (default) | thomas selects the line solver, IXCHEL2D_PRECISION = fp64
 ! (default) | fp32_mixed | fp32_refined the arithmetic of the sweeps (PCR
-! only).
+! only).  IXCHEL2D_LOOP = phases runs every time step through the
+! phase-level procedures of ec_momento, ec_energia and residuos with
+! upstream's own loops, shared pass counter and exit rules (as a main.f90
+! that keeps its control flow would), instead of ixh_simple_iterations.
 !
 program ixchel2d_gpu
   use iso_c_binding
@@ -26,7 +29,7 @@ program ixchel2d_gpu
 
   character(len=28) :: entrada_u, entrada_v, entrada_tp
   character(len=8) :: adimen, flujo_ini, tempe_ini
-  character(len=16) :: solver_env
+  character(len=16) :: solver_env, loop_env
   character(len=6) :: Rec
   character(len=64) :: prefijo, nussf
   logical :: postprocesar
@@ -34,6 +37,8 @@ program ixchel2d_gpu
   real(DBL) :: nusselt0, nusselt1, temp_med, residuo_ult, maxbo_ult
   integer :: paq_itera, itermax, iter_ecuaci_max, iter_simple_max
   integer :: placa_min, placa_max, itera_inicial, itera, itera_total, ll, resto, n, k, cap, un, ic
+  integer :: iter_ecuaci, iter_simple
+  logical :: por_fases
   type(ixh_grid_t) :: grid
   type(ixh_params_t) :: prm
   type(ixh_hist_t), allocatable :: hist(:)
@@ -133,6 +138,8 @@ program ixchel2d_gpu
   call get_environment_variable('IXCHEL2D_PRECISION', solver_env)
   if (trim(solver_env) == 'fp32_mixed') prm % precision = IXH_FP32_MIXED
   if (trim(solver_env) == 'fp32_refined') prm % precision = IXH_FP32_REFINED
+  call get_environment_variable('IXCHEL2D_LOOP', loop_env)
+  por_fases = trim(loop_env) == 'phases'
   call ixh_check(ixh_create(grid, prm, estado), 'ixh_create')
   call ixh_check(ixh_upload_field(estado, IXH_F_U_ANT, u_ant, mi), 'upload u_ant')
   call ixh_check(ixh_upload_field(estado, IXH_F_V_ANT, v_ant, mi+1), 'upload v_ant')
@@ -154,20 +161,28 @@ program ixchel2d_gpu
   itera = 0
   residuo_ult = 0.0_DBL
   maxbo_ult = 0.0_DBL
+  ! pass counters, shared by the inner loops (main.f90:313-314)
+  iter_ecuaci = 0
+  iter_simple = 0
   do ll = 1, itermax / paq_itera
      resto = paq_itera
      do while (resto > 0)
         ! stop at every multiple of 500 (progress line) and 1000 (Nusselt)
-        n = min(resto, 500 - mod(itera, 500), cap / max(1, iter_simple_max + 2))
-        call ixh_check(ixh_simple_iterations(estado, int(n, c_int32_t), hist, int(cap, c_int32_t), nrec), &
-             'ixh_simple_iterations')
-        do k = 1, min(int(nrec), cap)
-           if (hist(k) % logged /= 0) write(102, *) 'SIMPLE', hist(k) % simple_iter, hist(k) % maxbo, &
-                hist(k) % residuo
-        end do
-        if (nrec > 0) then
-           residuo_ult = hist(min(int(nrec), cap)) % residuo
-           maxbo_ult = hist(min(int(nrec), cap)) % maxbo
+        if (por_fases) then
+           n = 1
+           call paso_por_fases(maxbo_ult, residuo_ult)
+        else
+           n = min(resto, 500 - mod(itera, 500), cap / max(1, iter_simple_max + 2))
+           call ixh_check(ixh_simple_iterations(estado, int(n, c_int32_t), hist, int(cap, c_int32_t), nrec), &
+                'ixh_simple_iterations')
+           do k = 1, min(int(nrec), cap)
+              if (hist(k) % logged /= 0) write(102, *) 'SIMPLE', hist(k) % simple_iter, hist(k) % maxbo, &
+                   hist(k) % residuo
+           end do
+           if (nrec > 0) then
+              residuo_ult = hist(min(int(nrec), cap)) % residuo
+              maxbo_ult = hist(min(int(nrec), cap)) % maxbo
+           end if
         end if
         itera = itera + n
         resto = resto - n
@@ -200,4 +215,72 @@ program ixchel2d_gpu
   end do
   close(102)
   call ixh_check(ixh_destroy(estado), 'ixh_destroy')
+
+contains
+
+  ! One time step of main.f90:395-1270 through the phase procedures: the
+  ! momentum loop (u then v sweeps, error of v), p' = 0 and the p' loop
+  ! (error of p', maxbo), the correction, the energy loop, residuo_u, the
+  ! SIMPLE exit test with its fort.102 line, and the lagged copies.  The
+  ! inner loops exit on error < conv or after iter_ecuaci_max + 2 passes;
+  ! iter_ecuaci is shared by all three and reset on every exit.
+  subroutine paso_por_fases(maxbo_o, residuo_o)
+    real(DBL), intent(out) :: maxbo_o, residuo_o
+    real(c_double) :: error, mb, residuo
+    simple: do
+       momento: do
+          call ecuacion_velu(IXH_DIR_Y)
+          call ecuacion_velu(IXH_DIR_X)
+          call ecuacion_velv(IXH_DIR_Y, error)
+          call ecuacion_velv(IXH_DIR_X, error)
+          if (salida(error, conv_u, 'ADVER. MOMEN: convergencia no alcanzada ')) exit
+       end do momento
+       call inicializa_corrector_presion()
+       presion: do
+          call ecuacion_corr_pres(IXH_DIR_Y, error, mb)
+          call ecuacion_corr_pres(IXH_DIR_X, error, mb)
+          maxbo = mb
+          if (salida(error, conv_p, 'ADVER. PRES: convergencia no alcanzada ')) exit
+       end do presion
+       call corrige_presion_velocidad()
+       energia: do
+          call ecuacion_energia(IXH_DIR_Y, error)
+          call ecuacion_energia(IXH_DIR_X, error)
+          if (salida(error, conv_t, 'Adver. ENERG: convergencia no alcanzada ')) exit
+       end do energia
+       call residuo_u(residuo)
+       if (maxbo < conv_paso .and. residuo < conv_resi) then
+          iter_simple = 0
+          write(102, *) 'SIMPLE', iter_simple, maxbo, residuo
+          exit
+       else if (iter_simple > iter_simple_max) then
+          iter_simple = 0
+          exit
+       else
+          iter_simple = iter_simple + 1
+          write(102, *) 'SIMPLE', iter_simple, maxbo, residuo
+       end if
+    end do simple
+    call ixh_check(ixh_lag(estado), 'ixh_lag')
+    maxbo_o = maxbo
+    residuo_o = residuo
+  end subroutine paso_por_fases
+
+  ! exit test of an inner loop (main.f90:729-740, 913-924, 1144-1156)
+  logical function salida(error, conv, aviso)
+    real(c_double), intent(in) :: error
+    real(DBL), intent(in) :: conv
+    character(len=*), intent(in) :: aviso
+    salida = .true.
+    if (error < conv) then
+       iter_ecuaci = 0
+    else if (iter_ecuaci > iter_ecuaci_max) then
+       iter_ecuaci = 0
+       write(*, *) aviso, error
+    else
+       iter_ecuaci = iter_ecuaci + 1
+       salida = .false.
+    end if
+  end function salida
+
 end program ixchel2d_gpu

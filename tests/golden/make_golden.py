@@ -5,9 +5,11 @@ upstream binaries with oracle/build_ref.sh).  For each case of
 tests/golden/cases.py it writes the case files in upstream's formats into a
 scratch directory, runs the serial flang build there, and stores the exact
 FP64 fields (from upstream's stream-access .bin dumps), the fort.102 history
-(maxbo, residuo per logged SIMPLE iteration), the Nusselt records and the
-bytes of the last big-endian VTK sample as tests/golden/<case>.npz.  The full-size case stores a subsample of each field
-plus its centre lines and norms.
+(maxbo, residuo per logged SIMPLE iteration), the Nusselt records, the
+bytes of the last package's out_*.dat restart files and of the last
+big-endian VTK sample as tests/golden/<case>.npz.  The
+full-size case stores a subsample of each field plus its centre lines and
+norms, and the temperature on the three columns and rows next to each wall.
 
 usage: python tests/golden/make_golden.py [case ...]
 """
@@ -45,6 +47,12 @@ def run_case(c: C.Case, workdir: str) -> dict:
     wall = time.time() - t0
     out = C.read_reference_outputs(c, workdir)
     out["wall_seconds"] = np.array(wall)
+    # the restart files of the last package (main.f90:1329-1354), as bytes
+    base = os.path.join(workdir, f"out_n{c.mesh.nj:05d}m{c.mesh.mi:05d}_R{c.params.rec()}")
+    for f in "uvp":
+        if os.path.exists(base + f + ".dat"):
+            with open(base + f + ".dat", "rb") as fh:
+                out["dat_" + f] = np.frombuffer(fh.read(), dtype=np.uint8)
     # the VTK sample of the last package (upstream names them t_<tag>.vtk,
     # tags increasing with the iteration count)
     vdir = os.path.join(workdir, f"n{c.mesh.nj:05d}m{c.mesh.mi:05d}R{c.params.rec()}")
@@ -73,12 +81,22 @@ def make(name: str) -> None:
             store[f + "_ymid"] = np.ascontiguousarray(a[:, a.shape[1] // 2])
             store[f + "_l2"] = np.array(np.sqrt(np.sum(a * a)))
             store[f + "_sum"] = np.array(np.sum(a))
+        # the temperature next to each wall, which the Nusselt integrals read:
+        # columns i = 1..3 and mi-1..mi+1, rows j = 1..3 and nj-1..nj+1
+        t = out["t"]
+        store["t_cols_w"] = np.asfortranarray(t[:3, :])
+        store["t_cols_e"] = np.asfortranarray(t[-3:, :])
+        store["t_rows_s"] = np.asfortranarray(t[:, :3])
+        store["t_rows_n"] = np.asfortranarray(t[:, -3:])
     else:
         for f in ("u", "v", "t", "p"):
             store[f] = out[f]
         if "vtk" in out:
             store["vtk"] = out["vtk"]
             store["vtk_name"] = out["vtk_name"]
+        for f in "uvp":
+            if "dat_" + f in out:
+                store["dat_" + f] = out["dat_" + f]
     np.savez_compressed(K.fixture_path(name), **store)
     print(f"{name}: {K.fixture_path(name)} ({float(out['wall_seconds']):.1f} s upstream)", flush=True)
 
