@@ -16,6 +16,7 @@ Prints one JSON line.
   python bench.py --workload batch256   # 256 mixed-size cavities, one launch per phase
   python bench.py --workload adjoint129 # hot-wall Nu gradient through the adjoint
   python bench.py --workload fp32_convergence  # FP64 vs both FP32 paths toward the steady state
+  python bench.py --workload barrier --steps 2000  # grid barrier vs kernel boundary (probe)
 """
 from __future__ import annotations
 
@@ -65,43 +66,83 @@ def build_case():
     return K.nc301x701()
 
 
-def cpu_baseline(case, budget_s: float = 20.0):
-    """Upstream's serial CPU build (oracle/_ref) when present, else the C
-    oracle with OpenMP, on a bounded number of iterations of the same case."""
+def host_threads() -> int:
+    """Host threads the CPU baselines use: OMP_NUM_THREADS when set, else the
+    cores this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def oracle_timing(case_expr: str, iters_min: int, iters_max: int, budget_s: float, threads: int) -> dict:
+    """Seconds per SIMPLE iteration of the C oracle (OpenMP over the loop
+    nests) on the case `case_expr` builds, in a child process started with
+    OMP_NUM_THREADS = threads and OMP_PROC_BIND = close (the OpenMP runtime
+    reads them when it starts; this process has torch's loaded already)."""
+    code = (
+        "import json, sys, time\n"
+        f"sys.path.insert(0, {ROOT!r}); sys.path.insert(0, {os.path.join(ROOT, 'tests', 'golden')!r})\n"
+        "import cases as K\n"
+        "from ixchel2d_amd import case as C\n"
+        "from oracle.pyoracle import Oracle\n"
+        f"o = Oracle({case_expr})\n"
+        "t = time.perf_counter(); o.run(2); per = (time.perf_counter() - t) / 2\n"
+        f"n = int(max({iters_min}, min({iters_max}, {budget_s} / per)))\n"
+        "t = time.perf_counter(); o.run(n); per = (time.perf_counter() - t) / n\n"
+        "print(json.dumps({'per': per, 'n': n}))\n")
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close")
+    r = subprocess.run([sys.executable, "-c", code], env=env, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                       text=True, check=True)
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def reference_serial(case, budget_s: float = 15.0):
+    """Upstream's own CPU build (oracle/_ref, flang -O2, serial) on a bounded
+    number of iterations of the case, or None where it is not built."""
     m = case.mesh
-    cells = m.mi * m.nj
     ref = os.path.join(ROOT, "oracle", "_ref", f"ixchel2d_ref_m{m.mi}_n{m.nj}")
-    if os.path.exists(ref):
-        from ixchel2d_amd import case as C
-        work = tempfile.mkdtemp(prefix="ixh_cpu_")
-        def run(iters):
-            case.params.itermax = iters
-            case.params.paq_itera = 10
-            C.write_reference_case(case, work, postprocesar=False)
-            t = time.perf_counter()
-            subprocess.run([ref], cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
-            return time.perf_counter() - t
-        t0 = run(0)
-        t1 = run(10)
-        per = max((t1 - t0) / 10.0, 1e-6)
-        n = int(max(10, min(400, (budget_s / per) // 10 * 10)))
-        tn = run(n)
-        per = (tn - t0) / n
-        return {"value": cells / per / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": 1, "kind": "reference",
-                "sample": f"upstream serial flang build, {n} iterations of the 301x701 case minus a "
-                          f"0-iteration run (setup and file reads)"}
-    from oracle.pyoracle import Oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    o = Oracle(case)
-    t = time.perf_counter()
-    o.run(5)
-    per = (time.perf_counter() - t) / 5
-    n = int(max(5, min(500, budget_s / per)))
-    t = time.perf_counter()
-    o.run(n)
-    per = (time.perf_counter() - t) / n
-    return {"value": cells / per / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": threads, "kind": "port",
-            "sample": f"C oracle (OpenMP), {n} iterations of the 301x701 case"}
+    if not os.path.exists(ref):
+        return None
+    from ixchel2d_amd import case as C
+    work = tempfile.mkdtemp(prefix="ixh_cpu_")
+
+    def run(iters):
+        case.params.itermax = iters
+        case.params.paq_itera = 10
+        C.write_reference_case(case, work, postprocesar=False)
+        t = time.perf_counter()
+        subprocess.run([ref], cwd=work, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, check=True)
+        return time.perf_counter() - t
+    t0 = run(0)
+    t1 = run(10)
+    per = max((t1 - t0) / 10.0, 1e-6)
+    n = int(max(10, min(400, (budget_s / per) // 10 * 10)))
+    per = (run(n) - t0) / n
+    return {"value": m.mi * m.nj / per / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": 1, "kind": "reference",
+            "sample": f"upstream serial flang build, {n} iterations of the 301x701 case minus a "
+                      f"0-iteration run (setup and file reads)"}
+
+
+def cpu_baseline(case, budget_s: float = 15.0):
+    """The multicore CPU baseline: the C oracle (the FP64 restatement of the
+    same loop nests, OpenMP) on every host thread, on a bounded number of
+    iterations of the same case; upstream's serial build beside it.
+    Upstream's multicore build cannot be made: its parallelism is OpenACC,
+    which flang rejects, there is no gfortran or nvfortran here, and the
+    source has no OpenMP."""
+    m = case.mesh
+    threads = host_threads()
+    t = oracle_timing("K.nc301x701()", 5, 2000, budget_s, threads)
+    out = {"value": m.mi * m.nj / t["per"] / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": threads,
+           "kind": "port",
+           "sample": f"C oracle (FP64 restatement, OpenMP, OMP_PROC_BIND=close), {t['n']} iterations of the "
+                     "301x701 case on every host thread",
+           "note": "upstream's multicore build does not exist here: its parallelism is OpenACC (flang rejects "
+                   "it, no gfortran/nvfortran), and its source has no OpenMP; the serial upstream build is "
+                   "timed beside"}
+    out["reference_serial"] = reference_serial(case)
+    return out
 
 
 def batch_traffic(args):
@@ -126,9 +167,17 @@ def batch_bench(args):
     cases = C.batch_cases(args.batch, seed=0)
     cav = [Cavity(c, solver=args.solver, precision=args.precision) for c in cases]
     b = Batch(cav)
-    order, graph, table = b.autotune(trial_steps=3)
+    trial = 3
+    order, graph, table = b.autotune(trial_steps=trial)
     b.run(args.warmup, cap=1)
     torch.cuda.synchronize()
+    warm_steps = 0
+    t0 = time.perf_counter()
+    while time.perf_counter() - t0 < args.warm_seconds:
+        b.run(20, cap=1)
+        torch.cuda.synchronize()
+        warm_steps += 20
+    warm_s = time.perf_counter() - t0
     t0 = time.perf_counter()
     b.run(args.steps, cap=1)
     torch.cuda.synchronize()
@@ -146,11 +195,34 @@ def batch_bench(args):
         if c.sources:
             iter_bytes += 2 * 7 * 8 * npts
     gbs = iter_bytes / (ms_per_step * 1e-3) / 1e9
+    total_steps = 6 * (trial + 4) + args.warmup + warm_steps + args.steps
+    nu_err = None
+    if args.precision != "fp64":
+        # every cavity's hot-wall Nusselt number against the same batch run
+        # on the FP64 path for the same steps
+        import numpy as np
+        nu32 = [c.nusselt_x()[0] for c in cav]
+        b.close()
+        for c in cav:
+            c.close()
+        ref = [Cavity(c, solver=args.solver, precision="fp64") for c in cases]
+        rb = Batch(ref, order="xcd")
+        rb.run(total_steps, cap=1)
+        rb.close()
+        errs = []
+        for k, r in enumerate(ref):
+            nu64 = r.nusselt_x()[0]
+            if abs(nu64) > 1e-12:
+                errs.append(abs(nu32[k] - nu64) / abs(nu64))
+            r.close()
+        e = np.array(errs)
+        nu_err = {"cavities_with_heat_flow": len(errs), "steps": total_steps, "max": float(e.max()),
+                  "median": float(np.median(e)), "p90": float(np.quantile(e, 0.9))}
     cpu = None
     if not args.no_cpu:
         from oracle.pyoracle import Oracle
         sample = cases[:16]
-        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        threads = host_threads()
         t = time.perf_counter()
         for c in sample:
             Oracle(c).run(10)
@@ -174,6 +246,8 @@ def batch_bench(args):
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": batch_traffic(args), "kernel": "whole step",
                      "iteration_alg_bytes": iter_bytes},
+        "warm_untimed": {"steps": warm_steps, "seconds": round(warm_s, 3)},
+        "nu_rel_err_vs_fp64": nu_err,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
@@ -345,7 +419,7 @@ def convergence_study(args):
                                   "toward the steady state", "chunk": args.conv_chunk, "rows": rows}))
 
 
-def time_path(case, solver, precision, steps, warmup):
+def time_path(case, solver, precision, steps, warmup, warm_s=1.0):
     """Whole-step throughput of one cavity and the average kernel duration of
     each of the step's 6 launches over the same timed steps."""
     import torch
@@ -354,8 +428,17 @@ def time_path(case, solver, precision, steps, warmup):
     cells = m.mi * m.nj
     npts = (m.mi + 1) * (m.nj + 1)
     gpu = Cavity(case, solver=solver, precision=precision)
-    gpu.run(warmup)
+    gpu.run(warmup, cap=1)
     torch.cuda.synchronize()
+    # untimed steps until warm_s seconds have passed, so that the timed
+    # steps run at steady clocks however few they are
+    warm_steps = 0
+    tw = time.perf_counter()
+    while time.perf_counter() - tw < warm_s:
+        gpu.run(200, cap=1)
+        torch.cuda.synchronize()
+        warm_steps += 200
+    warm_took = time.perf_counter() - tw
     t0 = time.perf_counter()
     gpu.run(steps)
     torch.cuda.synchronize()
@@ -399,28 +482,82 @@ def time_path(case, solver, precision, steps, warmup):
                 "iteration_frac": round(it_gbs / HBM_PEAK_GBS, 4),
                 "sum_launch_us": round(sum(v["avg_us"] for v in launches.values()), 2)}
     return {"value": round(cells * steps / dt / 1e6, 2), "ms_per_step": round(ms_per_step, 5),
-            "ms_per_step_with_events": round(ms_events, 5), "roofline": roofline, "launches": launches}
+            "ms_per_step_with_events": round(ms_events, 5), "roofline": roofline, "launches": launches,
+            "warm_untimed": {"steps": warm_steps, "seconds": round(warm_took, 3)}}
 
 
 def accuracy(case, solver, steps, precision):
     """One FP32 path against FP64 from the same start after `steps`
-    iterations: relative error of the hot-wall mean Nusselt number and
-    max-norm errors of u, v, T (relative to the FP64 field's max)."""
+    iterations (a transient): relative error of the hot-wall mean Nusselt
+    number and max-norm errors of u, v, T (relative to the FP64 field's max)."""
     from ixchel2d_amd.simple import Cavity
     import numpy as np
     res = {}
     for prec in ("fp64", precision):
         g = Cavity(case, solver=solver, precision=prec)
-        h = g.run(steps)
+        h = g.run(steps, cap=1)
+        h = g.run(1)
         res[prec] = ({f: g.get(f) for f in "uvt"}, g.nusselt_x()[0], float(h["residuo"][-1]))
         g.close()
     f64, nu64, r64 = res["fp64"]
     f32, nu32, r32 = res[precision]
-    out = {"iterations": steps, "nu_hot_fp64": nu64, f"nu_hot_{precision}": nu32,
+    out = {"iterations": steps + 1, "nu_hot_fp64": nu64, f"nu_hot_{precision}": nu32,
            "nu_rel_err": abs(nu32 - nu64) / abs(nu64), "residuo_fp64": r64, f"residuo_{precision}": r32}
     for f in "uvt":
         out[f"linf_rel_err_{f}"] = float(np.max(np.abs(f32[f] - f64[f])) / np.max(np.abs(f64[f])))
     return out
+
+
+def run_to_residual(case, solver, precision, tol, chunk, max_iters):
+    """Iterate until the FP64 u residual norm (residuo_u of the path's own
+    fields, computed in FP64 on every path) is at most tol, checked every
+    `chunk` iterations, or until max_iters.  Returns the cavity, the
+    iterations, the last residual and maxbo and the wall time."""
+    import torch
+    from ixchel2d_amd.simple import Cavity
+    g = Cavity(case, solver=solver, precision=precision)
+    it = 0
+    t = time.perf_counter()
+    while True:
+        g.run(chunk - 1, cap=1)
+        h = g.run(1)
+        it += chunk
+        r, mb = float(h["residuo"][-1]), float(h["maxbo"][-1])
+        if r <= tol or it >= max_iters:
+            break
+    torch.cuda.synchronize()
+    return g, it, r, mb, time.perf_counter() - t
+
+
+def accuracy_converged(ref, case, solver, precision, tol, chunk, max_iters):
+    """An FP32 path run to the FP64-measured residual tol, against the FP64
+    path run to the same tol (`ref`: cavity, iterations, residual, maxbo,
+    seconds): hot-wall Nusselt number, max-norm field errors."""
+    import numpy as np
+    g64, it64, r64, mb64, s64 = ref
+    g, it, r, mb, sec = run_to_residual(case, solver, precision, tol, chunk, max_iters)
+    nu64, nu = g64.nusselt_x()[0], g.nusselt_x()[0]
+    out = {"tol": tol, "iterations": it, "residuo": r, "maxbo": mb, "reached": r <= tol, "seconds": round(sec, 3),
+           "fp64_iterations": it64, "fp64_residuo": r64, "fp64_seconds": round(s64, 3),
+           "nu_hot_fp64": nu64, f"nu_hot_{precision}": nu, "nu_rel_err": abs(nu - nu64) / abs(nu64)}
+    for f in "uvt":
+        a, b = g.get(f), g64.get(f)
+        out[f"linf_rel_err_{f}"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    g.close()
+    return out
+
+
+def barrier_probe(args):
+    """Cost of a grid-wide barrier in one cooperative launch against a kernel
+    boundary, at the grid shapes of a persistent fused step
+    (ixchel2d_amd/csrc/probe_barrier.hip); prints the probe's JSON line."""
+    exe = os.path.join(ROOT, "ixchel2d_amd", "probe_barrier")
+    if not os.path.exists(exe):
+        raise SystemExit(f"{exe} is not built; run __graft_entry__.build()")
+    for per in (1, 8):
+        r = subprocess.run([exe, str(args.steps), str(per)], stdout=subprocess.PIPE, text=True, check=True,
+                           timeout=300)
+        print(r.stdout.strip())
 
 
 def main():
@@ -430,7 +567,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--solver", default="pcr", choices=["pcr", "thomas"])
     ap.add_argument("--workload", default="nc301x701",
-                    choices=["nc301x701", "batch256", "adjoint129", "fp32_convergence"])
+                    choices=["nc301x701", "batch256", "adjoint129", "fp32_convergence", "barrier"])
     ap.add_argument("--conv-chunk", type=int, default=30000, help="iterations per row of the convergence study")
     ap.add_argument("--conv-chunks", type=int, default=12, help="rows of the convergence study")
     ap.add_argument("--adj-dt", type=float, default=0.01, help="time step of the adjoint workload")
@@ -442,7 +579,13 @@ def main():
                     help="path of the batch workload")
     ap.add_argument("--fp64-only", action="store_true", help="skip the FP32 lines of the 301x701 workload")
     ap.add_argument("--accuracy-steps", type=int, default=5000,
-                    help="iterations of the FP32 vs FP64 accuracy comparisons")
+                    help="iterations of the transient FP32 vs FP64 accuracy comparisons")
+    ap.add_argument("--converge-tol", type=float, default=1e-8,
+                    help="FP64-measured residual the FP32 paths and FP64 are run to for their accuracy "
+                         "(0: skip)")
+    ap.add_argument("--converge-chunk", type=int, default=10000, help="iterations between residual checks")
+    ap.add_argument("--warm-seconds", type=float, default=1.0,
+                    help="untimed steps for this long after the warm-up steps, before the timed steps")
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="first run one device-to-device copy of this many MB (reference kernel for "
                          "the FETCH_SIZE / WRITE_SIZE counter scale)")
@@ -465,6 +608,9 @@ def main():
     if args.workload == "fp32_convergence":
         convergence_study(args)
         return
+    if args.workload == "barrier":
+        barrier_probe(args)
+        return
     if args.calibrate_copy_mb:
         n = args.calibrate_copy_mb * (1 << 20) // 4
         src = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -476,14 +622,22 @@ def main():
     case = build_case()
     m = case.mesh
     cells = m.mi * m.nj
-    r64 = time_path(case, args.solver, "fp64", args.steps, args.warmup)
+    r64 = time_path(case, args.solver, "fp64", args.steps, args.warmup, args.warm_seconds)
     reduced = {"fp32_mixed": None, "fp32_refined": None}
     if args.solver == "pcr" and not args.fp64_only:
+        ref = None
+        if args.converge_tol > 0:
+            ref = run_to_residual(build_case(), "pcr", "fp64", args.converge_tol, args.converge_chunk, 400000)
         for prec in reduced:
-            r = time_path(build_case(), "pcr", prec, args.steps, args.warmup)
+            r = time_path(build_case(), "pcr", prec, args.steps, args.warmup, args.warm_seconds)
             r["speedup_vs_fp64"] = round(r["value"] / r64["value"], 3)
             r["accuracy"] = accuracy(build_case(), "pcr", args.accuracy_steps, prec)
+            if ref is not None:
+                r["accuracy_converged"] = accuracy_converged(ref, build_case(), "pcr", prec, args.converge_tol,
+                                                             args.converge_chunk, int(1.5 * ref[1]))
             reduced[prec] = r
+        if ref is not None:
+            ref[0].close()
     cpu = None if args.no_cpu else cpu_baseline(build_case())
     out = {
         "metric": "Mcell-SIMPLE-iters/s @301x701",
@@ -502,6 +656,7 @@ def main():
                                "Pr=0.71, dt=0.003), FP64 plain path, one SIMPLE iteration per step",
                    "mesh": [m.mi, m.nj], "cells": cells, "solver": args.solver, "path": "fp64"},
         "roofline": r64["roofline"],
+        "warm_untimed": r64["warm_untimed"],
         "cpu_baseline": cpu,
         "launches": r64["launches"],
         "fp32_mixed": reduced["fp32_mixed"],

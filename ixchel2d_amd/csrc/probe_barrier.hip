@@ -1,0 +1,144 @@
+// Cost of a grid-wide barrier inside one cooperative launch, against the
+// cost of a kernel boundary, at the grid shapes a persistent fused time step
+// would run (256-thread workgroups, up to the number that fit on the GPU at
+// once).  Each phase writes `per` doubles per thread (a fused sweep launch
+// at 301x701 writes about 5-15 MB), then either waits at a grid barrier
+// (one cooperative launch of `reps` phases) or ends the kernel (`reps`
+// launches back to back on one stream).  Prints one JSON line.
+//
+// The barrier: thread 0 of every workgroup adds 1 to an arrival counter
+// (agent scope, acquire-release) and waits until a generation word reaches
+// its epoch; the last arrival publishes the epoch.  The wait is bounded: past
+// `spin_limit` polls a workgroup sets the timeout flag and every workgroup
+// leaves the kernel at its next check, so the launch always drains.
+//
+//   probe_barrier [reps] [per]
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#define CHECK(x)                                                                                \
+  do {                                                                                          \
+    hipError_t e_ = (x);                                                                        \
+    if (e_ != hipSuccess) {                                                                     \
+      fprintf(stderr, "%s failed: %s (line %d)\n", #x, hipGetErrorString(e_), __LINE__);       \
+      exit(1);                                                                                  \
+    }                                                                                           \
+  } while (0)
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ void phase_work(double *sink, int per, int k) {
+  const size_t base = ((size_t)blockIdx.x * kThreads + threadIdx.x) * (size_t)per;
+  for (int i = 0; i < per; ++i) sink[base + i] = (double)(k + i);
+}
+
+__device__ __forceinline__ bool grid_barrier(unsigned *count, unsigned *gen, int *timeout, unsigned nblocks,
+                                             unsigned epoch, long spin_limit) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned arrived =
+        __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    if (arrived == epoch * nblocks) {
+      __hip_atomic_store(gen, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        if (++spins > spin_limit) {
+          __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  __syncthreads();
+  return __hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+}
+
+__global__ void __launch_bounds__(kThreads) k_phases(unsigned *count, unsigned *gen, int *timeout, double *sink,
+                                                     int per, int reps, long spin_limit) {
+  for (int k = 0; k < reps; ++k) {
+    phase_work(sink, per, k);
+    if (!grid_barrier(count, gen, timeout, gridDim.x, (unsigned)k + 1u, spin_limit)) return;
+  }
+}
+
+__global__ void __launch_bounds__(kThreads) k_phase(double *sink, int per, int k) { phase_work(sink, per, k); }
+
+int main(int argc, char **argv) {
+  const int reps = argc > 1 ? atoi(argv[1]) : 2000;
+  const int per = argc > 2 ? atoi(argv[2]) : 8;
+  hipDeviceProp_t prop;
+  CHECK(hipGetDeviceProperties(&prop, 0));
+  int coop = 0;
+  CHECK(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, 0));
+  int per_cu = 0;
+  CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_phases, kThreads, 0));
+  const int cus = prop.multiProcessorCount;
+  const int max_blocks = per_cu * cus;
+  unsigned *count = nullptr, *gen = nullptr;
+  int *timeout = nullptr;
+  double *sink = nullptr;
+  const size_t sink_n = (size_t)max_blocks * kThreads * (size_t)per;
+  CHECK(hipMalloc(&count, sizeof(unsigned)));
+  CHECK(hipMalloc(&gen, sizeof(unsigned)));
+  CHECK(hipMalloc(&timeout, sizeof(int)));
+  CHECK(hipMalloc(&sink, sink_n * sizeof(double)));
+  hipStream_t st;
+  CHECK(hipStreamCreate(&st));
+  hipEvent_t e0, e1;
+  CHECK(hipEventCreate(&e0));
+  CHECK(hipEventCreate(&e1));
+  const long spin_limit = 2000000;  // polls of about 64 cycles each: well under a second
+  printf("{\"probe\": \"grid barrier vs kernel boundary\", \"cus\": %d, \"blocks_per_cu\": %d, "
+         "\"cooperative\": %d, \"reps\": %d, \"doubles_per_thread_per_phase\": %d, \"rows\": [",
+         cus, per_cu, coop, reps, per);
+  bool first = true;
+  for (int g : {256, 512, 768, 1024, 2048}) {
+    if (g > max_blocks || !coop) continue;
+    const double mb = (double)g * kThreads * per * 8.0 / 1e6;
+    // kernel boundaries: reps launches of one phase each
+    for (int w = 0; w < 50; ++w) hipLaunchKernelGGL(k_phase, dim3(g), dim3(kThreads), 0, st, sink, per, w);
+    CHECK(hipGetLastError());
+    CHECK(hipEventRecord(e0, st));
+    for (int k = 0; k < reps; ++k) hipLaunchKernelGGL(k_phase, dim3(g), dim3(kThreads), 0, st, sink, per, k);
+    CHECK(hipGetLastError());
+    CHECK(hipEventRecord(e1, st));
+    CHECK(hipEventSynchronize(e1));
+    float ms_launch = 0.f;
+    CHECK(hipEventElapsedTime(&ms_launch, e0, e1));
+    // one cooperative launch of reps phases separated by grid barriers
+    float ms_coop = 0.f;
+    int to = 0;
+    for (int trial = 0; trial < 2; ++trial) {
+      CHECK(hipMemsetAsync(count, 0, sizeof(unsigned), st));
+      CHECK(hipMemsetAsync(gen, 0, sizeof(unsigned), st));
+      CHECK(hipMemsetAsync(timeout, 0, sizeof(int), st));
+      int r = reps;
+      long lim = spin_limit;
+      int pv = per;
+      void *args[] = {&count, &gen, &timeout, &sink, &pv, &r, &lim};
+      CHECK(hipEventRecord(e0, st));
+      CHECK(hipLaunchCooperativeKernel((const void *)k_phases, dim3(g), dim3(kThreads), args, 0, st));
+      CHECK(hipEventRecord(e1, st));
+      CHECK(hipEventSynchronize(e1));
+      CHECK(hipEventElapsedTime(&ms_coop, e0, e1));
+      CHECK(hipMemcpy(&to, timeout, sizeof(int), hipMemcpyDeviceToHost));
+      if (to) break;
+    }
+    printf("%s{\"workgroups\": %d, \"MB_per_phase\": %.2f, \"us_per_kernel_boundary_phase\": %.3f, "
+           "\"us_per_barrier_phase\": %.3f, \"timeout\": %d}",
+           first ? "" : ", ", g, mb, ms_launch * 1e3 / reps, ms_coop * 1e3 / reps, to);
+    first = false;
+  }
+  printf("]}\n");
+  CHECK(hipFree(count));
+  CHECK(hipFree(gen));
+  CHECK(hipFree(timeout));
+  CHECK(hipFree(sink));
+  CHECK(hipStreamDestroy(st));
+  return 0;
+}
