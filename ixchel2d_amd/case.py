@@ -331,6 +331,38 @@ def cylinder_sources(case: Case, xc: float, yc: float, radius: float,
         put("t", (mi + 1, nj + 1), t_body)
 
 
+def upstream_body_sources(case: Case, opcion: str) -> None:
+    """The immersed bodies upstream's definir_cuerpo builds in
+    (source/mod_front_inmersa.f90:54-179): 'cuadr', the square of side 0.1
+    centred at (6.0, 0.05), and 'trian', the triangle 0 <= y <= 0.1 between
+    the lines x = 2y + 5.8 and x = 6.2 - 2y (recta_mxb1 / recta_mxb2,
+    :192-252).  Both force u and v only: inside, Sp = -1e51 and
+    Sc = 1e51 * 1e-11.  The mask is tested at the pressure-node coordinates
+    (xp(ii), yp(jj)) for both arrays, with the same floating-point
+    expressions as upstream's tests."""
+    m = case.mesh
+    mi, nj = m.mi, m.nj
+    X, Y = np.meshgrid(m.xp, m.yp, indexing="ij")  # (mi+1, nj+1)
+    if opcion == "cuadr":
+        yv, xv, hh = 0.05, 6.0, 0.1
+        inside = (yv - hh / 2.0 <= Y) & (Y <= yv + hh / 2.0) & (xv - hh / 2.0 <= X) & (X <= xv + hh / 2.0)
+    elif opcion == "trian":
+        yv, hh = 0.1, 0.1
+        m1, m2 = 1.0 / 2.0, -1.0 / 2.0
+        b1 = -(0.1 - m1 * 6.0) / m1
+        b2 = -(0.1 - m2 * 6.0) / m2
+        inside = (yv - hh <= Y) & (Y <= yv) & (Y / m1 + b1 <= X) & (X <= Y / m2 + b2)
+    else:
+        raise ValueError(f"body {opcion!r}: 'cuadr' or 'trian'")
+    for var, shape in (("u", (mi, nj + 1)), ("v", (mi + 1, nj))):
+        msk = inside[: shape[0], : shape[1]]
+        sc = np.zeros(shape, order="F")
+        sp = np.zeros(shape, order="F")
+        sp[msk] = -PENALTY
+        sc[msk] = PENALTY * 10.0e-12
+        case.sources[var] = (np.asfortranarray(sc), np.asfortranarray(sp))
+
+
 # --------------------------------------------------------------------------
 # batches of uneven cavities
 # --------------------------------------------------------------------------

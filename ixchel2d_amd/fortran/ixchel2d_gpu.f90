@@ -14,6 +14,8 @@
 ! phase-level procedures of ec_momento, ec_energia and residuos with
 ! upstream's own loops, shared pass counter and exit rules (as a main.f90
 ! that keeps its control flow would), instead of ixh_simple_iterations.
+! IXCHEL2D_CUERPO = trian | cuadr | chime passes that immersed-body option
+! to definir_cuerpo in place of upstream's 'sincu'.
 !
 program ixchel2d_gpu
   use iso_c_binding
@@ -29,7 +31,7 @@ program ixchel2d_gpu
 
   character(len=28) :: entrada_u, entrada_v, entrada_tp
   character(len=8) :: adimen, flujo_ini, tempe_ini
-  character(len=16) :: solver_env, loop_env
+  character(len=16) :: solver_env, loop_env, cuerpo_env
   character(len=6) :: Rec
   character(len=64) :: prefijo, nussf
   logical :: postprocesar
@@ -150,7 +152,10 @@ program ixchel2d_gpu
   call ixh_check(ixh_upload_field(estado, IXH_F_P, pres, mi+1), 'upload pres')
   call ini_frontera_uv()
   call ini_frontera_t()
-  call definir_cuerpo('sincu')
+  ! the driver's body option (upstream passes 'sincu', main.f90:327)
+  call get_environment_variable('IXCHEL2D_CUERPO', cuerpo_env)
+  if (len_trim(cuerpo_env) == 0) cuerpo_env = 'sincu'
+  call definir_cuerpo(trim(cuerpo_env))
   write(*,'(1X,A,I6,A,I6,A,A)') 'mi=', mi, ', nj=', nj, ', Re=', Rec
 
   ! SIMPLE records per time step are at most iter_simple_max + 2

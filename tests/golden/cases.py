@@ -96,8 +96,41 @@ def nc301x701(iters: int = 5000):
     return c
 
 
+def ib161x41():
+    """Mixed convection in a long lid-driven cavity (8 x 1, 161 x 41 uniform
+    mesh, Re = 100, Ri = 0.5, hot wall a, cold wall c, adiabatic b and d)
+    around upstream's square body 'cuadr' (side 0.1 centred at (6.0, 0.05),
+    on the bottom wall).  The case files carry no body: the reference builds
+    it in definir_cuerpo (its fixture comes from the binary built with BODY =
+    cuadr), and the oracle and the GPU get it from with_body()."""
+    mesh = C.make_mesh(161, 41, 8.0, 1.0, "unif")
+    prm = C.Params(adimen="mixta", Ra=100.0, Pr=0.71, Ri_1=0.5, dt=0.1, paq_itera=100, itermax=200,
+                   rel_pres=0.3, rel_vel=0.7, rel_ener=0.7, flujo_ini="nulo", tempe_ini="archivo")
+    bcs = {
+        "u": C.uniform_bc("u", ("diri", 0.0), ("diri", 0.0), ("diri", 0.0), ("diri", 1.0)),
+        "v": C.uniform_bc("v", ("diri", 0.0), ("diri", 0.0), ("diri", 0.0), ("diri", 0.0)),
+        "t": C.uniform_bc("t", ("diri", 1.0), ("neum", 0.0), ("diri", 0.0), ("neum", 0.0)),
+    }
+    mi, nj = mesh.mi, mesh.nj
+    rng = np.random.default_rng(11)
+    t0 = np.asfortranarray(0.5 + rng.uniform(-1e-3, 1e-3, size=(mi + 1, nj + 1)))
+    z = lambda *sh: np.zeros(sh, order="F")  # noqa: E731
+    return C.Case(mesh, prm, bcs, z(mi, nj + 1), z(mi + 1, nj), t0, z(mi + 1, nj + 1), name="ib161x41")
+
+
+# the definir_cuerpo option of the reference build behind a fixture
+REF_BODY = {"ib161x41": "cuadr"}
+
+
+def with_body(case):
+    """The case with the immersed body its fixture's reference build defines."""
+    if case.name in REF_BODY:
+        C.upstream_body_sources(case, REF_BODY[case.name])
+    return case
+
+
 CASES = {"nc18x14": nc18x14, "mixed22x17": mixed22x17, "ctrl18x14": ctrl18x14, "lid66": lid66,
-         "nc34": nc34, "nc301x701": nc301x701}
+         "nc34": nc34, "nc301x701": nc301x701, "ib161x41": ib161x41}
 
 # subsampling of the full-size fixture (every SUB_I-th node in x, SUB_J-th in y)
 SUB_I, SUB_J = 6, 10

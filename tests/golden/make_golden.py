@@ -32,15 +32,16 @@ ROOT = K.ROOT
 from ixchel2d_amd import case as C  # noqa: E402
 
 
-def ref_binary(mi: int, nj: int) -> str:
+def ref_binary(mi: int, nj: int, body: str = "") -> str:
+    env = dict(os.environ, BODY=body)
     out = subprocess.run(["bash", os.path.join(ROOT, "oracle", "build_ref.sh"), str(mi), str(nj)],
-                         check=True, stdout=subprocess.PIPE, text=True).stdout.strip().splitlines()[-1]
+                         check=True, stdout=subprocess.PIPE, text=True, env=env).stdout.strip().splitlines()[-1]
     return out
 
 
 def run_case(c: C.Case, workdir: str) -> dict:
     C.write_reference_case(c, workdir)
-    exe = ref_binary(c.mesh.mi, c.mesh.nj)
+    exe = ref_binary(c.mesh.mi, c.mesh.nj, K.REF_BODY.get(c.name, ""))
     t0 = time.time()
     with open(os.path.join(workdir, "stdout.txt"), "w") as log:
         subprocess.run([exe], cwd=workdir, stdout=log, stderr=subprocess.STDOUT, check=True)

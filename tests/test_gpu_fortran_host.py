@@ -208,3 +208,26 @@ def test_fortran_host_restart_round_trip(tmp_path):
     # the resumed run's last restart files carry the total count
     with open(second / (tag + "u.dat")) as fh:
         assert int(fh.readline().split()[2]) == 40
+
+
+def test_fortran_host_immersed_square_against_upstream(tmp_path):
+    """IXCHEL2D_CUERPO=cuadr: definir_cuerpo('cuadr') of the Fortran host
+    (ixchel2d_amd/fortran/mod_front_inmersa.f90) against the reference built
+    with the same option (mod_front_inmersa.f90:99-128, main.f90:327): with
+    the Thomas solver the fields, the restart files and the VTK sample equal
+    upstream's bit for bit."""
+    assert os.path.exists(EXE), "Fortran host not built"
+    c = K.ib161x41()
+    gold = K.load_fixture("ib161x41")
+    C.write_reference_case(c, str(tmp_path))
+    out = _run_host(c, tmp_path, IXCHEL2D_CUERPO="cuadr")
+    for f in "uvtp":
+        assert np.array_equal(out[f], gold[f]), f
+    np.testing.assert_allclose(out["history"], gold["history"], rtol=1e-12, atol=1e-300)
+    base = os.path.join(tmp_path, f"out_n{c.mesh.nj:05d}m{c.mesh.mi:05d}_R{c.params.rec()}")
+    for f in "uvp":
+        with open(base + f + ".dat", "rb") as fh:
+            assert fh.read() == gold["dat_" + f].tobytes(), f + ".dat"
+    vdir = os.path.join(tmp_path, f"n{c.mesh.nj:05d}m{c.mesh.mi:05d}R{c.params.rec()}")
+    with open(os.path.join(vdir, str(gold["vtk_name"])), "rb") as fh:
+        assert fh.read() == gold["vtk"].tobytes()

@@ -218,12 +218,61 @@ def test_immersed_cylinder_sources():
     o.run(30)
     for f in "uvtp":
         assert np.array_equal(g.get(f), o.get(f)), f
-    # inside the body the penalty pins T to the body temperature
+    # inside the body the penalty pins T to the body temperature and u, v
+    # to the penalty value 1e-11
     m = case.mesh
     X, Y = np.meshgrid(m.xp, m.yp, indexing="ij")
     inside = (X - 0.55) ** 2 + (Y - 0.6) ** 2 <= 0.15 ** 2
     t = g.get("t")
     assert np.max(np.abs(t[inside] - 1.0)) < 1e-12
+    mu = case.sources["u"][1] != 0.0
+    mv = case.sources["v"][1] != 0.0
+    assert mu.sum() > 10 and mv.sum() > 10
+    assert np.max(np.abs(g.get("u")[mu])) <= 1e-10
+    assert np.max(np.abs(g.get("v")[mv])) <= 1e-10
+
+
+def _square_inside(mesh):
+    """p-nodes of upstream's 'cuadr' body (mod_front_inmersa.f90:99-128)."""
+    X, Y = np.meshgrid(mesh.xp, mesh.yp, indexing="ij")
+    return (0.05 - 0.05 <= Y) & (Y <= 0.05 + 0.05) & (6.0 - 0.05 <= X) & (X <= 6.0 + 0.05)
+
+
+@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+def test_immersed_square_against_upstream(solver):
+    """Upstream's own immersed body: the reference built with
+    definir_cuerpo(..., 'cuadr') (main.f90:327), 200 iterations of a long
+    lid-driven mixed-convection cavity.  Thomas: every field bit for bit,
+    the penalty-amplified pressure inside the body included; PCR: 1e-10
+    relative, the pressure compared outside the body (inside it is rounding
+    noise amplified by the 1e51 penalty).  u and v inside sit at the
+    penalty value, about 1e-11."""
+    case = K.with_body(K.ib161x41())
+    gold = K.load_fixture("ib161x41")
+    g = _cavity(case, solver)
+    h = g.run(case.params.itermax)
+    outside = ~_square_inside(case.mesh)
+    for f in "uvtp":
+        a, b = g.get(f), gold[f]
+        if solver == "thomas":
+            assert np.array_equal(a, b), f
+        else:
+            if f == "p":
+                a, b = a[outside], b[outside]
+            assert relerr(a, b) <= REL, f"{f}: {relerr(a, b):.2e}"
+    mu = case.sources["u"][1] != 0.0
+    mv = case.sources["v"][1] != 0.0
+    assert np.max(np.abs(g.get("u")[mu])) <= 1e-10
+    assert np.max(np.abs(g.get("v")[mv])) <= 1e-10
+    logged = h["logged"].astype(bool)
+    hist = np.c_[h["maxbo"], h["residuo"]][logged]
+    assert hist.shape == gold["history"].shape
+    np.testing.assert_allclose(hist[:, 0], gold["history"][:, 0], rtol=1e-9, atol=1e-15)
+    # residuo here (about 1e28) is the u residual of the penalised rows:
+    # the rounding of u inside the body times the 1e51 penalty, so the
+    # line solver's rounding shows in it (measured with PCR: 3.6e-7)
+    rtol = 1e-12 if solver == "thomas" else 1e-6
+    np.testing.assert_allclose(hist[:, 1], gold["history"][:, 1], rtol=rtol, atol=1e-15)
 
 
 @pytest.mark.parametrize("solver", ["thomas", "pcr"])

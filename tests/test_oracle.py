@@ -10,12 +10,15 @@ import cases as K
 from ixchel2d_amd import case as C
 from oracle import pyoracle as O
 
-SMALL = ["nc18x14", "mixed22x17", "ctrl18x14", "lid66", "nc34"]
+SMALL = ["nc18x14", "mixed22x17", "ctrl18x14", "lid66", "nc34", "ib161x41"]
 
 
 @pytest.mark.parametrize("name", SMALL)
 def test_oracle_matches_upstream_bit_for_bit(name):
-    c = K.CASES[name]()
+    """ib161x41 carries upstream's square body ('cuadr' in definir_cuerpo,
+    source/mod_front_inmersa.f90:54-179): the oracle gets it as sources
+    from case.upstream_body_sources, the reference from its own build."""
+    c = K.with_body(K.CASES[name]())
     g = K.load_fixture(name)
     o = O.Oracle(c)
     h = o.run(c.params.itermax)
