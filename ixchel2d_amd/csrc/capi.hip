@@ -239,6 +239,22 @@ int launch_pcorr(ixh_state *s, const SweepArgs &A, const CorrLine &K, int n) {
   return launch_pcorr_t<SOLVE_PCR, double>(s, A, K, n);
 }
 
+// the p' y-sweep of a fused step (p' zero at entry, row layout out)
+template <int SOLVER, typename R>
+int launch_sweep_pz_t(ixh_state *s, const SweepArgs &A, int n) {
+  HIPCHK((launch_sweep_pz_k<SOLVER, R>(ctx_of(s), A, n)));
+  return 0;
+}
+
+int launch_sweep_pz(ixh_state *s, const SweepArgs &A, int n) {
+  if (!A.in_zero) return fail(IXH_ERR_ARG, "the zero-input p' y-sweep needs p' = 0 at entry");
+  if (s->f32) return launch_sweep_pz_t<SOLVE_PCR, float>(s, A, n);
+  if (s->delta) return launch_sweep_pz_t<SOLVE_DELTA, float>(s, A, n);
+  if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_pz_t<SOLVE_THOMAS, double>(s, A, n);
+  if (s->prm.solver == IXH_SOLVER_PCR_EXACT) return launch_sweep_pz_t<SOLVE_PCR_EXACT, double>(s, A, n);
+  return launch_sweep_pz_t<SOLVE_PCR, double>(s, A, n);
+}
+
 template <int EQ, int DIR>
 int launch_sweep(ixh_state *s, const SweepArgs &A, int n) {
   if (s->f32) return launch_sweep_t<EQ, DIR, SOLVE_PCR, float>(s, A, n);
@@ -768,7 +784,7 @@ int launch_part(ixh_state *s, const StepLaunches &L, int k) {
   switch (k) {
     case 1: return launch_multi<EQ_V, DIR_Y, -1, DIR_Y>(s, L.m1, L.nwg1, L.ept1, line_len(s, EQ_V, IXH_DIR_Y));
     case 2: return launch_multi<EQ_V, DIR_X, -1, DIR_X>(s, L.m2, L.nwg2, L.ept2, line_len(s, EQ_V, IXH_DIR_X));
-    case 3: return launch_sweep<EQ_P, DIR_Y>(s, L.py, line_len(s, EQ_P, IXH_DIR_Y));
+    case 3: return launch_sweep_pz(s, L.py, line_len(s, EQ_P, IXH_DIR_Y));
     case 4: return launch_pcorr(s, L.px, L.corr, line_len(s, EQ_P, IXH_DIR_X));
     case 5:
       if (L.last) return launch_multi<EQ_T, DIR_Y, -1, DIR_Y>(s, L.m5, L.nwg5, L.ept5, s->nj + 1);

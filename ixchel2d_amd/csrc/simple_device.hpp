@@ -920,10 +920,14 @@ __device__ __forceinline__ int lane_zero() {
 // launch of its own.  Line + 1's solve repeats the one its own workgroup
 // does, on the same inputs, so both give the same bits; only the owner
 // writes p' and the norms.  LDS: twice line_lds.
-template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock, bool CORR = false>
+// PZ (p' y-sweep only): p' is zero at entry, so it is not read at all, and
+// only the other (row) layout of the result is written: the p' x-sweep that
+// follows reads that layout and writes both itself.
+template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock, bool CORR = false, bool PZ = false>
 __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s, double *lds,
                                            const CorrLine *K = nullptr) {
   static_assert(!CORR || (EQ == EQ_P && DIR == DIR_X && BS == kBlock), "the correction rides on the p' x-sweep");
+  static_assert(!PZ || (EQ == EQ_P && DIR == DIR_Y), "the zero-input form is the p' y-sweep's");
   const int line = line_s + lane_zero();
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
                 "the FP32 sweeps use the PCR solver");
@@ -986,8 +990,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     // p' = 0 at the first pass: the loads still go out (the buffer is
     // allocated), their values are dropped
     const S *pcf = fld<S>(A.pc);
-    const bool z = A.in_zero != 0;
-    const double pcv = (double)AT(pcf, ii, jj);
+    const bool z = PZ || A.in_zero != 0;
+    const double pcv = PZ ? 0.0 : (double)AT(pcf, ii, jj);
     double pcP = z ? 0.0 : pcv;
     double head;
     if (DIR == DIR_X) {
@@ -1000,7 +1004,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       bval = c.b;
     } else {
       a0 = c.bs; a2 = c.bn;
-      const double w0 = (double)AT(pcf, ii - 1, jj), e0 = (double)AT(pcf, ii + 1, jj);
+      const double w0 = PZ ? 0.0 : (double)AT(pcf, ii - 1, jj), e0 = PZ ? 0.0 : (double)AT(pcf, ii + 1, jj);
       double pW = z ? 0.0 : w0;
       double pE = z ? 0.0 : e0;
       head = -(double)c.ai * pW - (double)c.ad * pE;
@@ -1130,7 +1134,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     {
       const int io = DIR == DIR_X ? pr : oline;
       const int jo = DIR == DIR_X ? oline : pr;
-      const S cv = AT(src, io, jo);
+      const S cv = PZ ? S(0) : AT(src, io, jo);
       carry[e] = (EQ == EQ_P && A.in_zero) ? S(0) : cv;
     }
     if constexpr (CORR) {
@@ -1225,7 +1229,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     S x;
     if constexpr (DELTA) x = keep_phi[e] + (double)sr[k];
     else x = sr[k];
-    AT(out, ii, jj) = x;
+    if (!PZ) AT(out, ii, jj) = x;
     AO(out2, ii, jj) = x;
     if (NORM && pos >= 2 && pos <= mi) {
       const int w = BS == kBlock ? 0 : (k & 255) >> 6;
@@ -1236,7 +1240,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     if (other) {
       const int io = DIR == DIR_X ? pos : other;
       const int jo = DIR == DIR_X ? other : pos;
-      AT(out, io, jo) = carry[e];
+      if (!PZ) AT(out, io, jo) = carry[e];
       AO(out2, io, jo) = carry[e];
     }
   }
@@ -1305,6 +1309,14 @@ template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock>
 __global__ void __launch_bounds__(BS, kMinWaves) k_sweep(SweepArgs A) {
   __shared__ double lds[line_lds<R, EPT, BS>()];
   sweep_line<EQ, DIR, SOLVER, EPT, R, BS>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines), lds);
+}
+
+// the p' y-sweep of the fused time step: p' zero at entry, row layout out
+template <int SOLVER, int EPT, typename R>
+__global__ void __launch_bounds__(kBlock, kMinWaves) k_sweep_pz(SweepArgs A) {
+  __shared__ double lds[line_lds<R, EPT, kBlock>()];
+  sweep_line<EQ_P, DIR_Y, SOLVER, EPT, R, kBlock, false, true>(A, A.line0 + xcd_line((int)blockIdx.x, A.nlines),
+                                                              lds);
 }
 
 // p' x-sweep with the velocity and pressure correction of its lines

@@ -47,6 +47,11 @@ hipError_t launch_sweep_k(const LaunchCtx &c, const SweepArgs &A, int n);
 template <int EQ0, int DIR0, int EQ1, int DIR1, int SOLVER, typename R>
 hipError_t launch_multi_k(const LaunchCtx &c, const MultiArgs &M, int nwg, int ept, int maxn);
 
+// the p' y-sweep of a fused step (k_sweep_pz): p' zero at entry, only the
+// row layout written
+template <int SOLVER, typename R>
+hipError_t launch_sweep_pz_k(const LaunchCtx &c, const SweepArgs &A, int n);
+
 // the p' x-sweep of A.nlines lines of n nodes with the correction of those
 // lines (k_pcorr_x), one 256-thread workgroup per line
 template <int SOLVER, typename R>
@@ -78,6 +83,20 @@ hipError_t launch_sweep_k(const LaunchCtx &c, const SweepArgs &A, int n) {
     case 4: launch_on(c, k_sweep<EQ, DIR, SOLVER, 4, R>, grid, block, A); break;
     case 6: launch_on(c, k_sweep<EQ, DIR, SOLVER, 6, R>, grid, block, A); break;
     default: launch_on(c, k_sweep<EQ, DIR, SOLVER, 8, R>, grid, block, A); break;
+  }
+  return hipGetLastError();
+}
+
+template <int SOLVER, typename R>
+hipError_t launch_sweep_pz_k(const LaunchCtx &c, const SweepArgs &A, int n) {
+  const dim3 grid(A.nlines), block(kBlock);
+  switch (ept_for(n)) {
+    case 1: launch_on(c, k_sweep_pz<SOLVER, 1, R>, grid, block, A); break;
+    case 2: launch_on(c, k_sweep_pz<SOLVER, 2, R>, grid, block, A); break;
+    case 3: launch_on(c, k_sweep_pz<SOLVER, 3, R>, grid, block, A); break;
+    case 4: launch_on(c, k_sweep_pz<SOLVER, 4, R>, grid, block, A); break;
+    case 6: launch_on(c, k_sweep_pz<SOLVER, 6, R>, grid, block, A); break;
+    default: launch_on(c, k_sweep_pz<SOLVER, 8, R>, grid, block, A); break;
   }
   return hipGetLastError();
 }
@@ -162,6 +181,7 @@ hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *
                                                                       const MultiArgs &, int, int, int);
 #define IXH_INSTANTIATE_SWEEPS(SOLVER, R)              \
   template hipError_t launch_pcorr_k<SOLVER, R>(const LaunchCtx &, const SweepArgs &, const CorrLine &, int); \
+  template hipError_t launch_sweep_pz_k<SOLVER, R>(const LaunchCtx &, const SweepArgs &, int); \
   IXH_SWEEP_ONE(EQ_U, DIR_Y, SOLVER, R)                \
   IXH_SWEEP_ONE(EQ_U, DIR_X, SOLVER, R)                \
   IXH_SWEEP_ONE(EQ_V, DIR_Y, SOLVER, R)                \

@@ -168,12 +168,7 @@ class Cavity:
         hist = (_lib.Hist * cap)()
         n = C.c_int32()
         _lib.check(self.L.ixh_simple_iterations(self.h, nsteps, hist, cap, C.byref(n)))
-        k = min(n.value, cap)
-        rec = {f: np.array([getattr(hist[i], f) for i in range(k)])
-               for f in ("err_v", "err_p", "maxbo", "err_t", "residuo")}
-        for f in ("passes_mom", "passes_p", "passes_t", "simple_iter", "logged"):
-            rec[f] = np.array([getattr(hist[i], f) for i in range(k)], dtype=np.int32)
-        return rec
+        return _records(hist, 0, min(n.value, cap))
 
     def nusselt_y(self, i0: Optional[int] = None, i1: Optional[int] = None):
         m = self.case.mesh
@@ -222,11 +217,18 @@ class Cavity:
         return {self.L.ixh_kernel_name(k).decode(): (ms[k], cnt[k]) for k in range(nk.value)}
 
 
+# ixh_hist as a numpy record (5 doubles, 6 int32: 64 bytes)
+_HIST_DTYPE = np.dtype([(f, "<f8") for f in ("err_v", "err_p", "maxbo", "err_t", "residuo")] +
+                       [(f, "<i4") for f in ("passes_mom", "passes_p", "passes_t", "simple_iter", "logged", "pad_")])
+assert _HIST_DTYPE.itemsize == C.sizeof(_lib.Hist)
+
+
 def _records(hist, k0: int, k: int) -> Dict[str, np.ndarray]:
-    rec = {f: np.array([getattr(hist[k0 + i], f) for i in range(k)])
-           for f in ("err_v", "err_p", "maxbo", "err_t", "residuo")}
+    """Records k0 .. k0+k-1 of an ixh_hist array as one array per field."""
+    a = np.frombuffer(hist, dtype=_HIST_DTYPE, count=k, offset=k0 * _HIST_DTYPE.itemsize)
+    rec = {f: a[f].astype(np.float64) for f in ("err_v", "err_p", "maxbo", "err_t", "residuo")}
     for f in ("passes_mom", "passes_p", "passes_t", "simple_iter", "logged"):
-        rec[f] = np.array([getattr(hist[k0 + i], f) for i in range(k)], dtype=np.int32)
+        rec[f] = a[f].astype(np.int32)
     return rec
 
 
