@@ -10,7 +10,9 @@
 // (agent scope, acquire-release) and waits until a generation word reaches
 // its epoch; the last arrival publishes the epoch.  The wait is bounded: past
 // `spin_limit` polls a workgroup sets the timeout flag and every workgroup
-// leaves the kernel at its next check, so the launch always drains.
+// leaves the kernel at its next check, so the launch always drains.  A
+// two-level form spreads the arrivals over one counter per group of 32
+// workgroups before the root counter.
 //
 //   probe_barrier [reps] [per]
 #include <hip/hip_runtime.h>
@@ -58,11 +60,49 @@ __device__ __forceinline__ bool grid_barrier(unsigned *count, unsigned *gen, int
   return __hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
 }
 
+// Two-level form: workgroups arrive in groups of kGroup on counters of their
+// own (one per 256-byte line, count[64 * (1 + group)]); the last arrival of
+// a group arrives at the root counter count[0], and the last root arrival
+// publishes the epoch.
+constexpr int kGroup = 32;
+
+__device__ __forceinline__ bool tree_barrier(unsigned *count, unsigned *gen, int *timeout, unsigned nblocks,
+                                             unsigned epoch, long spin_limit) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const unsigned g = blockIdx.x / kGroup;
+    const unsigned ngroups = (nblocks + kGroup - 1) / kGroup;
+    const unsigned in_group = (g + 1) * kGroup <= nblocks ? kGroup : nblocks - g * kGroup;
+    const unsigned a = __hip_atomic_fetch_add(count + 64 * (1 + g), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+    bool publish = false;
+    if (a == epoch * in_group) {
+      const unsigned r = __hip_atomic_fetch_add(count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
+      publish = r == epoch * ngroups;
+    }
+    if (publish) {
+      __hip_atomic_store(gen, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      long spins = 0;
+      while (__hip_atomic_load(gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < epoch) {
+        if (++spins > spin_limit) {
+          __hip_atomic_store(timeout, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
+  }
+  __syncthreads();
+  return __hip_atomic_load(timeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+}
+
 __global__ void __launch_bounds__(kThreads) k_phases(unsigned *count, unsigned *gen, int *timeout, double *sink,
-                                                     int per, int reps, long spin_limit) {
+                                                     int per, int reps, long spin_limit, int tree) {
   for (int k = 0; k < reps; ++k) {
     phase_work(sink, per, k);
-    if (!grid_barrier(count, gen, timeout, gridDim.x, (unsigned)k + 1u, spin_limit)) return;
+    const bool ok = tree ? tree_barrier(count, gen, timeout, gridDim.x, (unsigned)k + 1u, spin_limit)
+                         : grid_barrier(count, gen, timeout, gridDim.x, (unsigned)k + 1u, spin_limit);
+    if (!ok) return;
   }
 }
 
@@ -83,7 +123,9 @@ int main(int argc, char **argv) {
   int *timeout = nullptr;
   double *sink = nullptr;
   const size_t sink_n = (size_t)max_blocks * kThreads * (size_t)per;
-  CHECK(hipMalloc(&count, sizeof(unsigned)));
+  // the root counter and one 256-byte line per group of the two-level form
+  const size_t ncount = 64 * (1 + (size_t)(max_blocks + kGroup - 1) / kGroup);
+  CHECK(hipMalloc(&count, ncount * sizeof(unsigned)));
   CHECK(hipMalloc(&gen, sizeof(unsigned)));
   CHECK(hipMalloc(&timeout, sizeof(int)));
   CHECK(hipMalloc(&sink, sink_n * sizeof(double)));
@@ -110,28 +152,31 @@ int main(int argc, char **argv) {
     CHECK(hipEventSynchronize(e1));
     float ms_launch = 0.f;
     CHECK(hipEventElapsedTime(&ms_launch, e0, e1));
-    // one cooperative launch of reps phases separated by grid barriers
-    float ms_coop = 0.f;
+    // one cooperative launch of reps phases separated by grid barriers,
+    // flat (tree = 0) and two-level (tree = 1)
+    float ms_coop[2] = {0.f, 0.f};
     int to = 0;
-    for (int trial = 0; trial < 2; ++trial) {
-      CHECK(hipMemsetAsync(count, 0, sizeof(unsigned), st));
-      CHECK(hipMemsetAsync(gen, 0, sizeof(unsigned), st));
-      CHECK(hipMemsetAsync(timeout, 0, sizeof(int), st));
-      int r = reps;
-      long lim = spin_limit;
-      int pv = per;
-      void *args[] = {&count, &gen, &timeout, &sink, &pv, &r, &lim};
-      CHECK(hipEventRecord(e0, st));
-      CHECK(hipLaunchCooperativeKernel((const void *)k_phases, dim3(g), dim3(kThreads), args, 0, st));
-      CHECK(hipEventRecord(e1, st));
-      CHECK(hipEventSynchronize(e1));
-      CHECK(hipEventElapsedTime(&ms_coop, e0, e1));
-      CHECK(hipMemcpy(&to, timeout, sizeof(int), hipMemcpyDeviceToHost));
-      if (to) break;
+    for (int tree = 0; tree < 2 && !to; ++tree) {
+      for (int trial = 0; trial < 2; ++trial) {
+        CHECK(hipMemsetAsync(count, 0, ncount * sizeof(unsigned), st));
+        CHECK(hipMemsetAsync(gen, 0, sizeof(unsigned), st));
+        CHECK(hipMemsetAsync(timeout, 0, sizeof(int), st));
+        int r = reps, tr = tree;
+        long lim = spin_limit;
+        int pv = per;
+        void *args[] = {&count, &gen, &timeout, &sink, &pv, &r, &lim, &tr};
+        CHECK(hipEventRecord(e0, st));
+        CHECK(hipLaunchCooperativeKernel((const void *)k_phases, dim3(g), dim3(kThreads), args, 0, st));
+        CHECK(hipEventRecord(e1, st));
+        CHECK(hipEventSynchronize(e1));
+        CHECK(hipEventElapsedTime(&ms_coop[tree], e0, e1));
+        CHECK(hipMemcpy(&to, timeout, sizeof(int), hipMemcpyDeviceToHost));
+        if (to) break;
+      }
     }
     printf("%s{\"workgroups\": %d, \"MB_per_phase\": %.2f, \"us_per_kernel_boundary_phase\": %.3f, "
-           "\"us_per_barrier_phase\": %.3f, \"timeout\": %d}",
-           first ? "" : ", ", g, mb, ms_launch * 1e3 / reps, ms_coop * 1e3 / reps, to);
+           "\"us_per_barrier_phase\": %.3f, \"us_per_two_level_barrier_phase\": %.3f, \"timeout\": %d}",
+           first ? "" : ", ", g, mb, ms_launch * 1e3 / reps, ms_coop[0] * 1e3 / reps, ms_coop[1] * 1e3 / reps, to);
     first = false;
   }
   printf("]}\n");

@@ -10,8 +10,8 @@
  * association follows the upstream source term by term (Fortran evaluates
  * left to right, `x**5` is lowered by flang as x*(x*(x*(x*x))), and the
  * x86-64 baseline has no FMA), so with -ffp-contract=off the fields agree
- * with upstream's serial CPU build bit for bit; tests/test_oracle_golden.py
- * checks this against fixtures written by that build.
+ * with upstream's serial CPU build bit for bit; tests/test_oracle.py checks
+ * this against the fixtures that build writes (tests/golden/).
  *
  * The OpenMP pragmas only split independent cells and independent lines, so
  * the fields do not depend on the thread count; the reductions are serial in
