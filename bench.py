@@ -168,7 +168,7 @@ def batch_bench(args):
     cav = [Cavity(c, solver=args.solver, precision=args.precision) for c in cases]
     b = Batch(cav)
     trial = 3
-    order, graph, table = b.autotune(trial_steps=trial)
+    sched, table = b.autotune(trial_steps=trial)
     b.run(args.warmup, cap=1)
     torch.cuda.synchronize()
     warm_steps = 0
@@ -195,7 +195,7 @@ def batch_bench(args):
         if c.sources:
             iter_bytes += 2 * 7 * 8 * npts
     gbs = iter_bytes / (ms_per_step * 1e-3) / 1e9
-    total_steps = 6 * (trial + 4) + args.warmup + warm_steps + args.steps
+    total_steps = len(table) * (trial + 4) + args.warmup + warm_steps + args.steps
     nu_err = None
     if args.precision != "fp64":
         # every cavity's hot-wall Nusselt number against the same batch run
@@ -240,9 +240,10 @@ def batch_bench(args):
         "config": {"workload": f"{args.batch} cavities: sizes 65^2/129^2/257^2/129x301 cells, half lid-driven "
                                "Re 100..3200, half natural convection Ra 1e3..1e6, 2 of 16 with a cylinder",
                    "cavities": args.batch, "cells": cells, "solver": args.solver, "path": args.precision,
-                   "schedule": {"order": order, "graph": graph},
-                   "autotune_ms_per_step": {f"{k[0]}{'+graph' if k[1] else ''}": round(v, 4)
-                                            for k, v in table.items()}},
+                   "schedule": sched,
+                   "autotune_ms_per_step": {
+                       f"{k[0]}{'+graph' if k[1] else ''}/{k[2]}t{'/res-with-T' if k[3] else ''}": round(v, 4)
+                       for k, v in table.items()}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": batch_traffic(args), "kernel": "whole step",
                      "iteration_alg_bytes": iter_bytes},

@@ -28,11 +28,13 @@ EXPORTS = [
     "ixh_momentum_u", "ixh_momentum_v", "ixh_pcorr_zero", "ixh_pcorr", "ixh_correct", "ixh_energy",
     "ixh_residual_u", "ixh_lag", "ixh_simple_iterations", "ixh_nusselt_y", "ixh_nusselt_x",
     "ixh_tridiagonal", "ixh_tridiagonal_noalloc", "ixh_set_profiling", "ixh_kernel_times", "ixh_kernel_name",
-    "ixh_batch_create", "ixh_batch_destroy", "ixh_batch_set_schedule", "ixh_batch_iterations",
+    "ixh_batch_create", "ixh_batch_destroy", "ixh_batch_set_schedule", "ixh_batch_set_shape",
+    "ixh_batch_iterations",
     "ixh_batch_autotune", "ixh_adjoint_record", "ixh_adjoint_energy",
     "ixh_adjoint_coupled",
 ]
 ORDER_ID = {"cavity": 0, "largest": 1, "xcd": 2}
+BATCH_SCHEDULES = 24  # entries of ixh_batch_autotune's timing table
 
 
 class Grid(C.Structure):
@@ -107,7 +109,8 @@ def load(path: str = LIB_PATH):
     L.ixh_batch_destroy.argtypes = [vp]
     L.ixh_batch_set_schedule.argtypes = [vp, C.c_int32, C.c_int32]
     L.ixh_batch_iterations.argtypes = [vp, C.c_int32, C.POINTER(Hist), C.c_int32]
-    L.ixh_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, dp]
+    L.ixh_batch_set_shape.argtypes = [vp, C.c_int32, C.c_int32]
+    L.ixh_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, ip, ip, dp]
     L.ixh_adjoint_record.argtypes = [vp, C.c_int32, C.c_int32]
     L.ixh_adjoint_energy.argtypes = [vp, dp, C.c_int32, dp, dp, dp, dp, dp, C.c_int32]
     L.ixh_adjoint_coupled.argtypes = [vp, dp, C.c_int32, dp, dp, dp, dp, dp, C.c_int32]

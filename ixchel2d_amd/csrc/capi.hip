@@ -1597,6 +1597,7 @@ void rot_set(ixh_state *s, const int *r) {
 struct BatchStep {
   std::vector<int> key, post;  // rotation states before and after the step
   char *d = nullptr;           // SweepArgs[8][n], CorrArgs[n], ResArgs[n]
+  bool res_with_t = false;     // the u residual rides on the T x-sweep's launch
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -1610,8 +1611,11 @@ struct ixh_batch {
   int ept[8] = {};
   // one 64-thread workgroup per line (PCR, every line at most 320 nodes):
   // four times as many lines resident per CU as with 256 threads
-  bool wave_lines = false;
+  bool wave_ok = false, wave_lines = false;
   int ept64[8] = {};
+  // the u residual in the T x-sweep's launch, where no cavity's u equation
+  // reads T
+  bool res_with_t = false;
   int order = IXH_ORDER_XCD;
   bool graph = false;
   hipStream_t stream = nullptr;
@@ -1699,6 +1703,7 @@ int batch_prepare(ixh_batch *b, BatchStep &E) {
   std::vector<SweepArgs> sw(8 * (size_t)n);
   std::vector<CorrArgs> co(n);
   std::vector<ResArgs> re(n);
+  bool all_ri0 = true;
   for (int c = 0; c < n; ++c) {
     ixh_state *s = b->st[c];
     CHK(prep_u(s, IXH_DIR_Y, sw[PH_U_Y * (size_t)n + c]));
@@ -1716,6 +1721,7 @@ int batch_prepare(ixh_batch *b, BatchStep &E) {
     CHK(prep_t(s, IXH_DIR_Y, sw[PH_T_Y * (size_t)n + c]));
     CHK(prep_t(s, IXH_DIR_X, sw[PH_T_X * (size_t)n + c]));
     prep_residual(s, re[c]);
+    all_ri0 = all_ri0 && s->prm.ri_x == 0.0;
     s->u_ant = s->u_cur;
     s->v_ant = s->v_cur;
     s->t_ant = s->t_cur;
@@ -1726,25 +1732,42 @@ int batch_prepare(ixh_batch *b, BatchStep &E) {
   HIPCHK(hipMemcpy(E.d, sw.data(), bs, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E.d + bs, co.data(), bc, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E.d + bs + bc, re.data(), br, hipMemcpyHostToDevice));
+  E.res_with_t = b->res_with_t && all_ri0;
   return 0;
 }
 
+// One sweep phase.  The p' y-sweep reads no p' (zero at entry in every
+// cavity: batch_prepare starts each step's p' from zero).  The T x-sweep
+// also runs the u residual rows when re is not null.
 template <int EQ, int DIR, int SOLVER, typename R>
-int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
-  HIPCHK((launch_sweep_batch_k<EQ, DIR, SOLVER, R>(st, D, b->d_wg + b->wg_off[ph], b->nwg[ph], b->ept[ph],
-                                                   b->ept64[ph], b->wave_lines)));
+int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st, const ResArgs *re) {
+  const int2 *wg = b->d_wg + b->wg_off[ph];
+  if constexpr (EQ == EQ_P && DIR == DIR_Y) {
+    HIPCHK((launch_sweep_pz_batch_k<SOLVER, R>(st, D, wg, b->nwg[ph], b->ept[ph], b->ept64[ph], b->wave_lines)));
+  } else if constexpr (EQ == EQ_T && DIR == DIR_X) {
+    if (re)
+      HIPCHK((launch_sweep_t_res_batch_k<SOLVER, R>(st, D, wg, b->nwg[ph], re, b->d_wg + b->wg_off[PH_RES],
+                                                    b->nwg[PH_RES], b->ept[ph], b->ept64[ph], b->wave_lines)));
+    else
+      HIPCHK((launch_sweep_batch_k<EQ, DIR, SOLVER, R>(st, D, wg, b->nwg[ph], b->ept[ph], b->ept64[ph],
+                                                       b->wave_lines)));
+  } else {
+    HIPCHK((launch_sweep_batch_k<EQ, DIR, SOLVER, R>(st, D, wg, b->nwg[ph], b->ept[ph], b->ept64[ph],
+                                                     b->wave_lines)));
+  }
   return 0;
 }
 
 template <int EQ, int DIR>
-int batch_sweep(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st) {
-  if (b->nwg[ph] == 0) return 0;
+int batch_sweep(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st, const ResArgs *re = nullptr) {
+  if (b->nwg[ph] == 0 && !re) return 0;
   D += (size_t)ph * b->n;
-  if (b->f32) return batch_sweep_t<EQ, DIR, SOLVE_PCR, float>(b, ph, D, st);
-  if (b->delta) return batch_sweep_t<EQ, DIR, SOLVE_DELTA, float>(b, ph, D, st);
-  if (b->solver == IXH_SOLVER_THOMAS) return batch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(b, ph, D, st);
-  if (b->solver == IXH_SOLVER_PCR_EXACT) return batch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT, double>(b, ph, D, st);
-  return batch_sweep_t<EQ, DIR, SOLVE_PCR, double>(b, ph, D, st);
+  if (b->f32) return batch_sweep_t<EQ, DIR, SOLVE_PCR, float>(b, ph, D, st, re);
+  if (b->delta) return batch_sweep_t<EQ, DIR, SOLVE_DELTA, float>(b, ph, D, st, re);
+  if (b->solver == IXH_SOLVER_THOMAS) return batch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(b, ph, D, st, re);
+  if (b->solver == IXH_SOLVER_PCR_EXACT)
+    return batch_sweep_t<EQ, DIR, SOLVE_PCR_EXACT, double>(b, ph, D, st, re);
+  return batch_sweep_t<EQ, DIR, SOLVE_PCR, double>(b, ph, D, st, re);
 }
 
 // the launches of one step, in the sequential order of a single cavity
@@ -1766,9 +1789,11 @@ int batch_launch(ixh_batch *b, const BatchStep &E, hipStream_t st, int cap) {
     hipLaunchKernelGGL(k_correct_batch<double>, dim3(b->nwg[PH_CORR]), dim3(kBlock), 0, st, co, wgc);
   HIPCHK(hipGetLastError());
   CHK((batch_sweep<EQ_T, DIR_Y>(b, PH_T_Y, sw, st)));
-  CHK((batch_sweep<EQ_T, DIR_X>(b, PH_T_X, sw, st)));
+  CHK((batch_sweep<EQ_T, DIR_X>(b, PH_T_X, sw, st, E.res_with_t && b->nwg[PH_RES] > 0 ? re : nullptr)));
   const int2 *wgr = b->d_wg + b->wg_off[PH_RES];
-  if (b->wave_lines) {
+  if (E.res_with_t || b->nwg[PH_RES] == 0) {
+    // the residual rows ran with the T x-sweep (or there are none)
+  } else if (b->wave_lines) {
     if (b->f32)
       hipLaunchKernelGGL((k_residual_u_batch<float, 64>), dim3(b->nwg[PH_RES]), dim3(64), 0, st, re, wgr);
     else
@@ -1894,7 +1919,9 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
     b->ept64[ph] = (m + 63) / 64;
     longest = std::max(longest, m);
   }
-  b->wave_lines = solver == IXH_SOLVER_PCR && longest <= 5 * 64;
+  b->wave_ok = solver == IXH_SOLVER_PCR && longest <= 5 * 64;
+  b->wave_lines = b->wave_ok;
+  b->res_with_t = true;
   auto bail = [&](int code) {
     ixh_batch_destroy(b);
     return code;
@@ -1936,6 +1963,20 @@ int ixh_batch_set_schedule(ixh_batch *b, int32_t order, int32_t graph) {
   return batch_tables(b);
 }
 
+int ixh_batch_set_shape(ixh_batch *b, int32_t line_threads, int32_t residual_with_t) {
+  if (!b) return fail(IXH_ERR_ARG, "batch is NULL");
+  if (line_threads != 64 && line_threads != kBlock)
+    return fail(IXH_ERR_ARG, "batch: %d threads per line (64 or %d)", line_threads, kBlock);
+  if (line_threads == 64 && !b->wave_ok)
+    return fail(IXH_ERR_UNSUPP, "batch: one wave per line needs the pcr solver and lines of at most %d nodes",
+                5 * 64);
+  HIPCHK(hipStreamSynchronize(b->stream));
+  batch_clear_steps(b);
+  b->wave_lines = line_threads == 64;
+  b->res_with_t = residual_with_t != 0;
+  return 0;
+}
+
 int ixh_batch_iterations(ixh_batch *b, int32_t nsteps, ixh_hist *hist, int32_t cap) {
   if (!b || nsteps < 0 || cap < 0 || (cap > 0 && !hist)) return fail(IXH_ERR_ARG, "batch: bad arguments");
   for (ixh_state *s : b->st) {
@@ -1971,12 +2012,22 @@ int ixh_batch_iterations(ixh_batch *b, int32_t nsteps, ixh_hist *hist, int32_t c
   return 0;
 }
 
-int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_t *graph, double *ms_per_step) {
+int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_t *graph, int32_t *line_threads,
+                       int32_t *residual_with_t, double *ms_per_step) {
   if (!b || trial_steps < 1) return fail(IXH_ERR_ARG, "batch: bad arguments");
+  bool any_ri = false;
+  for (ixh_state *s : b->st) any_ri = any_ri || s->prm.ri_x != 0.0;
+  if (ms_per_step)
+    for (int k = 0; k < IXH_BATCH_SCHEDULES; ++k) ms_per_step[k] = -1.0;
   int best = -1;
   double best_ms = 0.0;
-  for (int cfg = 0; cfg < 6; ++cfg) {
-    CHK(ixh_batch_set_schedule(b, cfg >> 1, cfg & 1));
+  // index 8 order + 4 graph + 2 (256 threads per line) + residual with T
+  for (int cfg = 0; cfg < IXH_BATCH_SCHEDULES; ++cfg) {
+    const int o = cfg >> 3, g = (cfg >> 2) & 1, wide = (cfg >> 1) & 1, rt = cfg & 1;
+    if (!wide && !b->wave_ok) continue;
+    if (rt && any_ri) continue;
+    CHK(ixh_batch_set_schedule(b, o, g));
+    CHK(ixh_batch_set_shape(b, wide ? kBlock : 64, rt));
     // untimed steps build the arguments (and graphs) of the rotation states
     // the timed steps meet
     CHK(batch_run(b, 2, 0));
@@ -1995,9 +2046,13 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
       best_ms = per;
     }
   }
-  CHK(ixh_batch_set_schedule(b, best >> 1, best & 1));
-  if (order) *order = best >> 1;
-  if (graph) *graph = best & 1;
+  const int o = best >> 3, g = (best >> 2) & 1, lt = (best >> 1) & 1 ? kBlock : 64, rt = best & 1;
+  CHK(ixh_batch_set_schedule(b, o, g));
+  CHK(ixh_batch_set_shape(b, lt, rt));
+  if (order) *order = o;
+  if (graph) *graph = g;
+  if (line_threads) *line_threads = lt;
+  if (residual_with_t) *residual_with_t = rt;
   return 0;
 }
 

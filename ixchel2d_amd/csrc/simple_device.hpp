@@ -1338,6 +1338,17 @@ __global__ void __launch_bounds__(BS) k_sweep_batch(const SweepArgs *__restrict_
   sweep_line<EQ, DIR, SOLVER, EPT, R, BS>(D[w.x], w.y, lds);
 }
 
+// the p' y-sweep of a batch, every cavity's p' zero at entry (the PZ form of
+// sweep_line)
+template <int SOLVER, int EPT, typename R, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_sweep_pz_batch(const SweepArgs *__restrict__ D,
+                                                       const int2 *__restrict__ wg) {
+  __shared__ double lds[line_lds<R, EPT, BS>()];
+  const int2 w = wg[blockIdx.x];
+  if (w.x < 0) return;
+  sweep_line<EQ_P, DIR_Y, SOLVER, EPT, R, BS, false, true>(D[w.x], w.y, lds);
+}
+
 // ------------------------------------------------------------- correction
 // p += 0.6 p' on ii 2..mi, jj 2..nj; u += dyu (p'_P - p'_E)/a_u on ii 2..mi-1,
 // jj 2..nj; v += dxv (p'_P - p'_N)/a_v on ii 2..mi, jj 2..nj-1
@@ -1515,6 +1526,27 @@ __global__ void __launch_bounds__(BS) k_residual_u_batch(const ResArgs *__restri
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
   residual_row<R, BS>(D[w.x], w.y, red);
+}
+
+// The T x-sweep of a batch and, in the same launch, the u residual rows of
+// every cavity (workgroups nsweep on, after every line).  The residual reads
+// u, v, p after the correction and u of the step before, which the T sweeps
+// do not write; it reads T only where Ri != 0, and such a batch keeps the
+// residual's own launch.  The rows start as the last lines drain.
+template <int SOLVER, int EPT, typename R, int BS = kBlock>
+__global__ void __launch_bounds__(BS) k_sweep_t_res_batch(const SweepArgs *__restrict__ D,
+                                                          const int2 *__restrict__ wg, int nsweep,
+                                                          const ResArgs *__restrict__ DR,
+                                                          const int2 *__restrict__ wgr) {
+  constexpr int kl = line_lds<R, EPT, BS>();
+  __shared__ double lds[kl > kBlock ? kl : kBlock];
+  const int b = (int)blockIdx.x;
+  const int2 w = b < nsweep ? wg[b] : wgr[b - nsweep];
+  if (w.x < 0) return;
+  if (b < nsweep)
+    sweep_line<EQ_T, DIR_X, SOLVER, EPT, R, BS>(D[w.x], w.y, lds);
+  else
+    residual_row<typename StoreOf<SOLVER, R>::T, BS>(DR[w.x], w.y, lds);
 }
 
 // ---------------------------------------------------------------- finalize

@@ -1,4 +1,4 @@
-// Launchers of the line-sweep kernels (k_sweep, k_multi, k_sweep_batch).
+// Launchers of the line-sweep kernels (k_sweep, k_multi, k_sweep_batch, ...).
 // Each (solver, real type) pair is instantiated in its own translation unit
 // (sweeps_*.hip), so the library's many sweep kernels compile in parallel;
 // capi.hip sees the declarations only.  Every launcher returns the HIP error
@@ -62,6 +62,20 @@ hipError_t launch_pcorr_k(const LaunchCtx &c, const SweepArgs &A, const CorrLine
 template <int EQ, int DIR, int SOLVER, typename R>
 hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nwg, int ept,
                                 int ept64, bool wave_lines);
+
+// the p' y-sweep phase of a batch whose p' is zero at entry in every cavity
+// (k_sweep_pz_batch); shape as launch_sweep_batch_k
+template <int SOLVER, typename R>
+hipError_t launch_sweep_pz_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nwg, int ept,
+                                   int ept64, bool wave_lines);
+
+// the T x-sweep phase of a batch (nsweep workgroups of wg) and the u residual
+// rows (nres workgroups of wgr) in one launch (k_sweep_t_res_batch); every
+// cavity needs Ri = 0 in the u equation
+template <int SOLVER, typename R>
+hipError_t launch_sweep_t_res_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nsweep,
+                                      const ResArgs *DR, const int2 *wgr, int nres, int ept, int ept64,
+                                      bool wave_lines);
 
 #ifdef IXH_SWEEP_LAUNCH_DEFS
 
@@ -144,31 +158,69 @@ hipError_t launch_multi_k(const LaunchCtx &c, const MultiArgs &M, int nwg, int e
   return hipGetLastError();
 }
 
-template <int EQ, int DIR, int SOLVER, typename R>
-hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nwg, int ept,
-                                int ept64, bool wave_lines) {
+// A batch launch of nwg workgroups of kernel K::fn<EPT, BS>: one 64-thread
+// wave per line with ept64 rows per lane (wave solver only), else 256
+// threads per line with ept rows per thread.
+template <class K, int SOLVER, typename... Args>
+hipError_t launch_batch_shaped(hipStream_t st, int nwg, int ept, int ept64, bool wave_lines, Args... args) {
   const dim3 grid(nwg), block(kBlock);
   const LaunchCtx c{st, nullptr, nullptr};
   if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
     if (wave_lines) {
       const dim3 wave(64);
       switch (ept64) {
-        case 1: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 1, R, 64>, grid, wave, D, wg); break;
-        case 2: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 2, R, 64>, grid, wave, D, wg); break;
-        case 3: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 3, R, 64>, grid, wave, D, wg); break;
-        case 4: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 4, R, 64>, grid, wave, D, wg); break;
-        default: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 5, R, 64>, grid, wave, D, wg); break;
+        case 1: launch_on(c, K::template fn<1, 64>(), grid, wave, args...); break;
+        case 2: launch_on(c, K::template fn<2, 64>(), grid, wave, args...); break;
+        case 3: launch_on(c, K::template fn<3, 64>(), grid, wave, args...); break;
+        case 4: launch_on(c, K::template fn<4, 64>(), grid, wave, args...); break;
+        default: launch_on(c, K::template fn<5, 64>(), grid, wave, args...); break;
       }
       return hipGetLastError();
     }
   }
   switch (ept) {
-    case 1: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 1, R>, grid, block, D, wg); break;
-    case 2: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 2, R>, grid, block, D, wg); break;
-    case 3: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 3, R>, grid, block, D, wg); break;
-    default: launch_on(c, k_sweep_batch<EQ, DIR, SOLVER, 4, R>, grid, block, D, wg); break;
+    case 1: launch_on(c, K::template fn<1, kBlock>(), grid, block, args...); break;
+    case 2: launch_on(c, K::template fn<2, kBlock>(), grid, block, args...); break;
+    case 3: launch_on(c, K::template fn<3, kBlock>(), grid, block, args...); break;
+    default: launch_on(c, K::template fn<4, kBlock>(), grid, block, args...); break;
   }
   return hipGetLastError();
+}
+
+template <int EQ, int DIR, int SOLVER, typename R>
+struct SweepBatchK {
+  template <int EPT, int BS>
+  static auto fn() { return k_sweep_batch<EQ, DIR, SOLVER, EPT, R, BS>; }
+};
+template <int SOLVER, typename R>
+struct SweepPzBatchK {
+  template <int EPT, int BS>
+  static auto fn() { return k_sweep_pz_batch<SOLVER, EPT, R, BS>; }
+};
+template <int SOLVER, typename R>
+struct SweepTResBatchK {
+  template <int EPT, int BS>
+  static auto fn() { return k_sweep_t_res_batch<SOLVER, EPT, R, BS>; }
+};
+
+template <int EQ, int DIR, int SOLVER, typename R>
+hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nwg, int ept,
+                                int ept64, bool wave_lines) {
+  return launch_batch_shaped<SweepBatchK<EQ, DIR, SOLVER, R>, SOLVER>(st, nwg, ept, ept64, wave_lines, D, wg);
+}
+
+template <int SOLVER, typename R>
+hipError_t launch_sweep_pz_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nwg, int ept,
+                                   int ept64, bool wave_lines) {
+  return launch_batch_shaped<SweepPzBatchK<SOLVER, R>, SOLVER>(st, nwg, ept, ept64, wave_lines, D, wg);
+}
+
+template <int SOLVER, typename R>
+hipError_t launch_sweep_t_res_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nsweep,
+                                      const ResArgs *DR, const int2 *wgr, int nres, int ept, int ept64,
+                                      bool wave_lines) {
+  return launch_batch_shaped<SweepTResBatchK<SOLVER, R>, SOLVER>(st, nsweep + nres, ept, ept64, wave_lines, D,
+                                                                 wg, nsweep, DR, wgr);
 }
 
 // every launcher the library calls, for one (solver, real type)
@@ -182,6 +234,10 @@ hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *
 #define IXH_INSTANTIATE_SWEEPS(SOLVER, R)              \
   template hipError_t launch_pcorr_k<SOLVER, R>(const LaunchCtx &, const SweepArgs &, const CorrLine &, int); \
   template hipError_t launch_sweep_pz_k<SOLVER, R>(const LaunchCtx &, const SweepArgs &, int); \
+  template hipError_t launch_sweep_pz_batch_k<SOLVER, R>(hipStream_t, const SweepArgs *, const int2 *, int, int, \
+                                                          int, bool);                                         \
+  template hipError_t launch_sweep_t_res_batch_k<SOLVER, R>(hipStream_t, const SweepArgs *, const int2 *, int,    \
+                                                            const ResArgs *, const int2 *, int, int, int, bool); \
   IXH_SWEEP_ONE(EQ_U, DIR_Y, SOLVER, R)                \
   IXH_SWEEP_ONE(EQ_U, DIR_X, SOLVER, R)                \
   IXH_SWEEP_ONE(EQ_V, DIR_Y, SOLVER, R)                \

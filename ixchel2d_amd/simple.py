@@ -250,15 +250,29 @@ class Batch:
     def set_schedule(self, order: str = "xcd", graph: bool = False) -> None:
         _lib.check(self.L.ixh_batch_set_schedule(self.h, _lib.ORDER_ID[order], 1 if graph else 0))
 
+    def set_shape(self, line_threads: int = 64, residual_with_t: bool = True) -> None:
+        """Threads per line (64 or 256) and whether the u residual runs in
+        the T x-sweep's launch (``ixh_batch_set_shape``)."""
+        _lib.check(self.L.ixh_batch_set_shape(self.h, line_threads, 1 if residual_with_t else 0))
+
     def autotune(self, trial_steps: int = 10):
-        """Time every ordering with and without graphs, keep the fastest.
-        Returns (order, graph, {(order, graph): ms per step})."""
-        o, g = C.c_int32(), C.c_int32()
-        ms = (C.c_double * 6)()
-        _lib.check(self.L.ixh_batch_autotune(self.h, trial_steps, C.byref(o), C.byref(g), ms))
+        """Time every schedule the batch allows (ordering, graph, threads per
+        line, residual placement), keep the fastest.  Returns (schedule,
+        table): schedule = dict(order, graph, line_threads, residual_with_t),
+        table = {(order, graph, line_threads, residual_with_t): ms per step}
+        over the schedules tried."""
+        o, g, lt, rt = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        ms = (C.c_double * _lib.BATCH_SCHEDULES)()
+        _lib.check(self.L.ixh_batch_autotune(self.h, trial_steps, C.byref(o), C.byref(g), C.byref(lt),
+                                             C.byref(rt), ms))
         names = {v: k for k, v in _lib.ORDER_ID.items()}
-        table = {(names[k >> 1], bool(k & 1)): ms[k] for k in range(6)}
-        return names[o.value], bool(g.value), table
+        table = {}
+        for k in range(_lib.BATCH_SCHEDULES):
+            if ms[k] >= 0.0:
+                table[(names[k >> 3], bool((k >> 2) & 1), 256 if (k >> 1) & 1 else 64, bool(k & 1))] = ms[k]
+        sched = {"order": names[o.value], "graph": bool(g.value), "line_threads": lt.value,
+                 "residual_with_t": bool(rt.value)}
+        return sched, table
 
     def run(self, nsteps: int, cap: Optional[int] = None):
         """``nsteps`` single-pass time steps of every cavity; one history dict

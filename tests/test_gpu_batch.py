@@ -4,7 +4,9 @@ ixh_simple_iterations (bit for bit: per cavity the batch runs the same kernel
 bodies in the same order), and with the Thomas solver bit-identical to the
 oracle.  Covers every workgroup ordering, graph replay, consecutive calls
 (argument caches rebuilt per call), immersed cylinders, buoyancy in the v
-equation (natural and mixed convection) and the autotuner."""
+equation (natural and mixed convection), both launch shapes (threads per
+line, the u residual beside the T x-sweep or on its own) and the
+autotuner."""
 import numpy as np
 import pytest
 
@@ -102,16 +104,69 @@ def test_batch_autotune_then_run():
     cases = _small_batch()
     cav = [Cavity(c, solver="pcr") for c in cases]
     b = Batch(cav)
-    order, graph, table = b.autotune(trial_steps=1)
-    assert order in ("cavity", "largest", "xcd") and len(table) == 6
-    assert all(v > 0 for v in table.values())
+    sched, table = b.autotune(trial_steps=1)
+    # every line is short and no u equation reads T: all 24 schedules
+    assert len(table) == 24 and all(v > 0 for v in table.values())
+    assert {k[0] for k in table} == {"cavity", "largest", "xcd"}
+    assert {k[2] for k in table} == {64, 256} and {k[3] for k in table} == {False, True}
+    best = (sched["order"], sched["graph"], sched["line_threads"], sched["residual_with_t"])
+    assert table[best] == min(table.values())
     b.run(4)
-    steps = 6 * (1 + 4) + 4
+    steps = len(table) * (1 + 4) + 4
     for c, case in enumerate(cases):
         alone, _ = _run_alone(case, "pcr", steps)
         for f in "uvtp":
             assert np.array_equal(cav[c].get(f), alone[f]), f"cavity {c}: {f}"
     b.close()
+
+
+@pytest.mark.parametrize("solver,line_threads,residual_with_t",
+                         [("pcr", 64, False), ("pcr", 64, True), ("pcr", 256, False), ("pcr", 256, True),
+                          ("thomas", 256, True), ("thomas", 256, False)])
+def test_batch_shapes_match_single_cavity(solver, line_threads, residual_with_t):
+    """Threads per line and the residual's placement change the launch
+    shape only: every cavity still equals its run alone, bit for bit, over
+    calls that rebuild the arguments, and a cavity whose u equation reads T
+    (Ri_x != 0) sends the residual back to a launch of its own."""
+    from ixchel2d_amd.simple import Batch, Cavity
+    cases = _small_batch()
+    cav = [Cavity(c, solver=solver) for c in cases]
+    b = Batch(cav, order="xcd")
+    b.set_shape(line_threads, residual_with_t)
+    hb = b.run(5)
+    alone = [Cavity(c, solver=solver) for c in cases]
+    ha = [a.run(5) for a in alone]
+    for c in range(len(cases)):
+        for f in "uvtp":
+            assert np.array_equal(cav[c].get(f), alone[c].get(f)), f"cavity {c}: {f}"
+        for k in HIST:
+            assert np.array_equal(hb[c][k], ha[c][k]), f"cavity {c}: {k}"
+    # buoyancy in the u equation of one cavity, then 3 more steps
+    for x in (cav[3], alone[3]):
+        x.set_params(ri_x=0.4)
+    hb = b.run(3)
+    for c in range(len(cases)):
+        ha = alone[c].run(3)
+        for f in "uvtp":
+            assert np.array_equal(cav[c].get(f), alone[c].get(f)), f"cavity {c} after ri_x: {f}"
+        assert np.array_equal(hb[c]["residuo"], ha["residuo"]), f"cavity {c}: residuo"
+    b.close()
+    for x in cav + alone:
+        x.close()
+
+
+def test_batch_shape_rejects_one_wave_for_thomas():
+    from ixchel2d_amd import _lib
+    from ixchel2d_amd.simple import Batch, Cavity
+    cav = [Cavity(c, solver="thomas") for c in _small_batch()[:2]]
+    b = Batch(cav)
+    with pytest.raises(_lib.IxhError, match="one wave per line"):
+        b.set_shape(64, True)
+    with pytest.raises(_lib.IxhError, match="threads per line"):
+        b.set_shape(128, True)
+    b.close()
+    for x in cav:
+        x.close()
 
 
 def test_mixed_size_batch_of_32():

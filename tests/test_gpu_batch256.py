@@ -2,8 +2,9 @@
 129^2, 257^2 and 129 x 301 cells (seed 0), half lid-driven (Re 100..3200),
 half natural convection (Ra 1e3..1e6, Pr 0.71), 2 of every 16 with an
 immersed cylinder held at T = 1, advanced together on the FP32-mixed path
-with the schedule the autotuner picks and each step replayed as a captured
-HIP graph.
+for 50 steps, with the schedule (ordering, threads per line, residual
+placement) the autotuner picks on a copy of the batch and each step replayed
+as a captured HIP graph.
 
   - every cavity ends with the fields of the same cavity run alone through
     ixh_simple_iterations, bit for bit;
@@ -23,10 +24,8 @@ from oracle.pyoracle import Oracle
 pytestmark = pytest.mark.gpu
 
 N = 256
-STEPS = 20
+STEPS = 50
 TRIAL = 1
-# the autotuner's trial steps are real time steps: 6 schedules x (trial + 4)
-TOTAL = 6 * (TRIAL + 4) + STEPS
 NU_TOL = 1e-4
 ORACLE_TOL = 2e-5
 
@@ -43,13 +42,21 @@ def _gpu():
 def batch256():
     from ixchel2d_amd.simple import Batch, Cavity
     cases = C.batch_cases(N, seed=0)
+    # the autotuner's trial steps are real time steps (trial + 4 per
+    # schedule): tune on a copy of the batch, then run the tested one from
+    # the start under the chosen schedule
+    tune = [Cavity(c, solver="pcr", precision="fp32_mixed") for c in cases]
+    bt = Batch(tune)
+    sched, table = bt.autotune(trial_steps=TRIAL)
+    bt.close()
+    for c in tune:
+        c.close()
     cav = [Cavity(c, solver="pcr", precision="fp32_mixed") for c in cases]
-    b = Batch(cav)
-    order, graph, table = b.autotune(trial_steps=TRIAL)
-    b.set_schedule(order, graph=True)
+    b = Batch(cav, order=sched["order"], graph=True)
+    b.set_shape(sched["line_threads"], sched["residual_with_t"])
     hist = b.run(STEPS)
-    print(f"\n[batch256] autotuned order {order} (graph {graph}); timings {table}", flush=True)
-    yield cases, cav, hist
+    print(f"\n[batch256] autotuned {sched}; timings {table}", flush=True)
+    yield cases, cav, hist, STEPS
     b.close()
     for c in cav:
         c.close()
@@ -57,10 +64,10 @@ def batch256():
 
 def test_batch256_equals_each_cavity_alone(batch256):
     from ixchel2d_amd.simple import Cavity
-    cases, cav, hist = batch256
+    cases, cav, hist, total = batch256
     for c, case in enumerate(cases):
         g = Cavity(case, solver="pcr", precision="fp32_mixed")
-        h = g.run(TOTAL, cap=TOTAL)
+        h = g.run(total, cap=total)
         for f in "uvtp":
             assert np.array_equal(cav[c].get(f), g.get(f)), f"cavity {c} ({case.name}): {f}"
         assert np.array_equal(hist[c]["maxbo"], h["maxbo"][-STEPS:]), f"cavity {c}: maxbo"
@@ -80,19 +87,19 @@ def _nu_errors(cases, cav, ref):
     return errs
 
 
-def _run_batch(cases, precision):
+def _run_batch(cases, precision, total):
     from ixchel2d_amd.simple import Batch, Cavity
     cav = [Cavity(c, solver="pcr", precision=precision) for c in cases]
     b = Batch(cav, order="xcd")
-    b.run(TOTAL, cap=1)
+    b.run(total, cap=1)
     b.close()
     return cav
 
 
 def test_batch256_nusselt_against_fp64(batch256):
-    cases, cav, _ = batch256
-    ref = _run_batch(cases, "fp64")
-    refined = _run_batch(cases, "fp32_refined")
+    cases, cav, _, total = batch256
+    ref = _run_batch(cases, "fp64", total)
+    refined = _run_batch(cases, "fp32_refined", total)
     for name, got, tol in (("fp32_mixed", cav, NU_TOL), ("fp32_refined", refined, NU_TOL)):
         errs = _nu_errors(cases, got, ref)
         e = np.array([x for x, _ in errs])
@@ -105,12 +112,12 @@ def test_batch256_nusselt_against_fp64(batch256):
 
 
 def test_batch256_smallest_against_oracle(batch256):
-    cases, cav, _ = batch256
+    cases, cav, _, total = batch256
     small = sorted(range(N), key=lambda c: (cases[c].mesh.mi * cases[c].mesh.nj, c))[:16]
     for c in small:
         case = cases[c]
         o = Oracle(case)
-        o.run(TOTAL)
+        o.run(total)
         outside = np.ones((case.mesh.mi + 1, case.mesh.nj + 1), dtype=bool)
         if "t" in case.sources:
             outside = case.sources["t"][1] == 0.0

@@ -367,3 +367,38 @@ def test_fused_step_launch_timing():
     h.run(110)
     for f in "uvtp":
         assert np.array_equal(g.get(f), h.get(f)), f
+
+
+def test_state_on_a_torch_stream():
+    """ixh_set_stream onto a torch stream (non-blocking with respect to the
+    null stream): boundary conditions and sources changed between runs wait
+    for the work queued on that stream, and the fields equal a run on the
+    state's own stream bit for bit."""
+    import torch
+    case = C.natural_convection_case(34, 50, height=1.5, Ra=1e5, iters=20, dt=0.01, seed=3)
+    C.cylinder_sources(case, 0.5, 0.7, 0.1, t_body=1.0)
+    from ixchel2d_amd import _lib
+    typ, val = case.bcs["t"].expand(case.mesh)["a"]
+    hot = np.asarray(val) * 1.2
+
+    def drive(c):
+        # the T y-sweep is still queued when the boundary values change
+        c.run(15)
+        c.energy(_lib.DIR_Y)
+        c.set_bc("t", "a", typ, hot)
+        c.energy(_lib.DIR_X)
+        c.run(15)
+
+    ref = _cavity(case, "pcr")
+    drive(ref)
+    st = torch.cuda.Stream()
+    g = _cavity(case, "pcr")
+    g.set_stream(st.cuda_stream)
+    drive(g)
+    for f in "uvtp":
+        assert np.array_equal(g.get(f), ref.get(f)), f
+    g.set_stream(None)
+    g.run(2)
+    ref.run(2)
+    for f in "uvtp":
+        assert np.array_equal(g.get(f), ref.get(f)), f
