@@ -242,7 +242,8 @@ def batch_bench(args):
                    "cavities": args.batch, "cells": cells, "solver": args.solver, "path": args.precision,
                    "schedule": sched,
                    "autotune_ms_per_step": {
-                       f"{k[0]}{'+graph' if k[1] else ''}/{k[2]}t{'/res-with-T' if k[3] else ''}": round(v, 4)
+                       f"{k[0]}{'+graph' if k[1] else ''}/{k[2]}t{f'/{k[3]}lines' if k[3] > 1 else ''}"
+                       f"{'/res-with-T' if k[4] else ''}": round(v, 4)
                        for k, v in table.items()}},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": batch_traffic(args), "kernel": "whole step",

@@ -216,25 +216,32 @@ int ixh_batch_set_schedule(ixh_batch *b, int32_t order, int32_t graph);
 /* threads per line of the sweep and residual launches: 64 (one wave per line
  * with ceil(n/64) rows per lane; the pcr solver, every line at most 320
  * nodes, IXH_ERR_UNSUPP otherwise) or 256 (rows per thread from the longest
- * line); and whether the u residual rows run in the T x-sweep's launch (1) or
- * in a launch of their own (0).  A step in which some cavity's u equation
- * reads T (ri_x != 0) runs the residual on its own either way.  Defaults: 64
- * where allowed, 1.  Every shape gives the same fields and norms bit for bit. */
-int ixh_batch_set_shape(ixh_batch *b, int32_t line_threads, int32_t residual_with_t);
+ * line); lines per workgroup in the sweeps: 1, or IXH_BATCH_LINE_GROUP
+ * consecutive lines of a cavity, one wave each (64 threads per line only),
+ * whose copies in the other layout are stored as runs of neighbouring values
+ * through LDS; and whether the u residual rows run in the T x-sweep's launch
+ * (1; one line per workgroup only) or in a launch of their own (0).  A step in
+ * which some cavity's u equation reads T (ri_x != 0) runs the residual on its
+ * own either way.  Defaults: 64 threads and groups of IXH_BATCH_LINE_GROUP
+ * lines where allowed, else 256 threads; the residual in its own launch.
+ * Every shape gives the same fields and norms bit for bit. */
+#define IXH_BATCH_LINE_GROUP 4
+int ixh_batch_set_shape(ixh_batch *b, int32_t line_threads, int32_t lines_per_group, int32_t residual_with_t);
 /* nsteps single-pass time steps of every cavity (each needs
  * iter_ecuaci_max < 0 and iter_simple_max < 0); the records of the first
  * min(nsteps, cap) steps of cavity c go to hist[c*cap + k] */
 int ixh_batch_iterations(ixh_batch *b, int32_t nsteps, ixh_hist *hist, int32_t cap);
 /* time trial_steps steps under every schedule (ordering, graph or direct
- * launches, threads per line, residual placement) the batch allows, keep the
- * fastest and return it.  ms_per_step (NULL or IXH_BATCH_SCHEDULES entries,
- * index 8*order + 4*graph + 2*(line_threads == 256) + residual_with_t)
+ * launches, threads per line, lines per workgroup, residual placement) the
+ * batch allows, keep the fastest and return it.  ms_per_step (NULL or
+ * IXH_BATCH_SCHEDULES entries, index 16*order + 8*graph +
+ * 4*(line_threads == 256) + 2*(lines_per_group > 1) + residual_with_t)
  * receives the timings, -1 where a schedule was not allowed.  The trial steps
  * are real time steps of every cavity (trial_steps + 4 per schedule tried),
  * identical under every schedule. */
-#define IXH_BATCH_SCHEDULES 24
+#define IXH_BATCH_SCHEDULES 48
 int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_t *graph, int32_t *line_threads,
-                       int32_t *residual_with_t, double *ms_per_step);
+                       int32_t *lines_per_group, int32_t *residual_with_t, double *ms_per_step);
 
 /* ---- discrete adjoint of the energy transport.  No upstream counterpart.
  * ixh_adjoint_record runs nsteps single-pass time steps from the current

@@ -34,7 +34,8 @@ EXPORTS = [
     "ixh_adjoint_coupled",
 ]
 ORDER_ID = {"cavity": 0, "largest": 1, "xcd": 2}
-BATCH_SCHEDULES = 24  # entries of ixh_batch_autotune's timing table
+BATCH_SCHEDULES = 48  # entries of ixh_batch_autotune's timing table
+BATCH_LINE_GROUP = 4  # lines per workgroup of the grouped batch sweeps
 
 
 class Grid(C.Structure):
@@ -109,8 +110,8 @@ def load(path: str = LIB_PATH):
     L.ixh_batch_destroy.argtypes = [vp]
     L.ixh_batch_set_schedule.argtypes = [vp, C.c_int32, C.c_int32]
     L.ixh_batch_iterations.argtypes = [vp, C.c_int32, C.POINTER(Hist), C.c_int32]
-    L.ixh_batch_set_shape.argtypes = [vp, C.c_int32, C.c_int32]
-    L.ixh_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, ip, ip, dp]
+    L.ixh_batch_set_shape.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
+    L.ixh_batch_autotune.argtypes = [vp, C.c_int32, ip, ip, ip, ip, ip, dp]
     L.ixh_adjoint_record.argtypes = [vp, C.c_int32, C.c_int32]
     L.ixh_adjoint_energy.argtypes = [vp, dp, C.c_int32, dp, dp, dp, dp, dp, C.c_int32]
     L.ixh_adjoint_coupled.argtypes = [vp, dp, C.c_int32, dp, dp, dp, dp, dp, C.c_int32]

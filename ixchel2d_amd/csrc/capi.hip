@@ -1616,6 +1616,11 @@ struct ixh_batch {
   // the u residual in the T x-sweep's launch, where no cavity's u equation
   // reads T
   bool res_with_t = false;
+  static_assert(kLinesPerGroup == IXH_BATCH_LINE_GROUP, "line group size of the header");
+  // consecutive lines of a cavity per workgroup in the sweep phases (one
+  // wave each): 1, or kLinesPerGroup, whose other-layout stores leave as
+  // runs of that many neighbouring values
+  int group = 1;
   int order = IXH_ORDER_XCD;
   bool graph = false;
   hipStream_t stream = nullptr;
@@ -1665,18 +1670,20 @@ int batch_tables(ixh_batch *b) {
   std::vector<int2> all;
   for (int ph = 0; ph < PH_NUM; ++ph) {
     b->wg_off[ph] = all.size();
+    // sweep phases in line groups: one entry per group, its first line
+    const int stride = ph < PH_CORR ? b->group : 1;
     if (b->order != IXH_ORDER_XCD) {
       for (int c : byc) {
         int f, m;
         phase_units(b->st[c], ph, &f, &m);
-        for (int k = 0; k < m; ++k) all.push_back(make_int2(c, f + k));
+        for (int k = 0; k < m; k += stride) all.push_back(make_int2(c, f + k));
       }
     } else {
       std::vector<int2> lists[kXcd];
       for (int c : byc) {
         int f, m;
         phase_units(b->st[c], ph, &f, &m);
-        for (int k = 0; k < m; ++k) lists[bin[c]].push_back(make_int2(c, f + k));
+        for (int k = 0; k < m; k += stride) lists[bin[c]].push_back(make_int2(c, f + k));
       }
       size_t rounds = 0;
       for (int x = 0; x < kXcd; ++x) rounds = std::max(rounds, lists[x].size());
@@ -1732,7 +1739,7 @@ int batch_prepare(ixh_batch *b, BatchStep &E) {
   HIPCHK(hipMemcpy(E.d, sw.data(), bs, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E.d + bs, co.data(), bc, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E.d + bs + bc, re.data(), br, hipMemcpyHostToDevice));
-  E.res_with_t = b->res_with_t && all_ri0;
+  E.res_with_t = b->res_with_t && all_ri0 && b->group == 1;
   return 0;
 }
 
@@ -1742,7 +1749,9 @@ int batch_prepare(ixh_batch *b, BatchStep &E) {
 template <int EQ, int DIR, int SOLVER, typename R>
 int batch_sweep_t(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st, const ResArgs *re) {
   const int2 *wg = b->d_wg + b->wg_off[ph];
-  if constexpr (EQ == EQ_P && DIR == DIR_Y) {
+  if (b->group > 1) {
+    HIPCHK((launch_sweep_group_batch_k<EQ, DIR, SOLVER, R>(st, D, wg, b->nwg[ph], b->ept64[ph])));
+  } else if constexpr (EQ == EQ_P && DIR == DIR_Y) {
     HIPCHK((launch_sweep_pz_batch_k<SOLVER, R>(st, D, wg, b->nwg[ph], b->ept[ph], b->ept64[ph], b->wave_lines)));
   } else if constexpr (EQ == EQ_T && DIR == DIR_X) {
     if (re)
@@ -1921,7 +1930,8 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
   }
   b->wave_ok = solver == IXH_SOLVER_PCR && longest <= 5 * 64;
   b->wave_lines = b->wave_ok;
-  b->res_with_t = true;
+  b->group = b->wave_ok ? kLinesPerGroup : 1;
+  b->res_with_t = false;
   auto bail = [&](int code) {
     ixh_batch_destroy(b);
     return code;
@@ -1963,17 +1973,27 @@ int ixh_batch_set_schedule(ixh_batch *b, int32_t order, int32_t graph) {
   return batch_tables(b);
 }
 
-int ixh_batch_set_shape(ixh_batch *b, int32_t line_threads, int32_t residual_with_t) {
+int ixh_batch_set_shape(ixh_batch *b, int32_t line_threads, int32_t lines_per_group, int32_t residual_with_t) {
   if (!b) return fail(IXH_ERR_ARG, "batch is NULL");
   if (line_threads != 64 && line_threads != kBlock)
     return fail(IXH_ERR_ARG, "batch: %d threads per line (64 or %d)", line_threads, kBlock);
   if (line_threads == 64 && !b->wave_ok)
     return fail(IXH_ERR_UNSUPP, "batch: one wave per line needs the pcr solver and lines of at most %d nodes",
                 5 * 64);
+  if (lines_per_group != 1 && lines_per_group != kLinesPerGroup)
+    return fail(IXH_ERR_ARG, "batch: %d lines per group (1 or %d)", lines_per_group, kLinesPerGroup);
+  if (lines_per_group > 1 && line_threads != 64)
+    return fail(IXH_ERR_ARG, "batch: line groups hold one-wave lines (64 threads per line)");
+  if (lines_per_group > 1 && residual_with_t)
+    return fail(IXH_ERR_ARG, "batch: the residual rides on the T x-sweep of one-line workgroups only");
   HIPCHK(hipStreamSynchronize(b->stream));
   batch_clear_steps(b);
   b->wave_lines = line_threads == 64;
   b->res_with_t = residual_with_t != 0;
+  if (b->group != lines_per_group) {
+    b->group = lines_per_group;
+    CHK(batch_tables(b));
+  }
   return 0;
 }
 
@@ -2013,7 +2033,7 @@ int ixh_batch_iterations(ixh_batch *b, int32_t nsteps, ixh_hist *hist, int32_t c
 }
 
 int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_t *graph, int32_t *line_threads,
-                       int32_t *residual_with_t, double *ms_per_step) {
+                       int32_t *lines_per_group, int32_t *residual_with_t, double *ms_per_step) {
   if (!b || trial_steps < 1) return fail(IXH_ERR_ARG, "batch: bad arguments");
   bool any_ri = false;
   for (ixh_state *s : b->st) any_ri = any_ri || s->prm.ri_x != 0.0;
@@ -2021,13 +2041,15 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
     for (int k = 0; k < IXH_BATCH_SCHEDULES; ++k) ms_per_step[k] = -1.0;
   int best = -1;
   double best_ms = 0.0;
-  // index 8 order + 4 graph + 2 (256 threads per line) + residual with T
+  // index 8 (2 order + graph) + 4 (256 threads per line) + 2 (line groups)
+  // + residual with T
   for (int cfg = 0; cfg < IXH_BATCH_SCHEDULES; ++cfg) {
-    const int o = cfg >> 3, g = (cfg >> 2) & 1, wide = (cfg >> 1) & 1, rt = cfg & 1;
+    const int o = cfg >> 4, g = (cfg >> 3) & 1, wide = (cfg >> 2) & 1, grp = (cfg >> 1) & 1, rt = cfg & 1;
     if (!wide && !b->wave_ok) continue;
-    if (rt && any_ri) continue;
+    if (wide && grp) continue;
+    if (rt && (any_ri || grp)) continue;
     CHK(ixh_batch_set_schedule(b, o, g));
-    CHK(ixh_batch_set_shape(b, wide ? kBlock : 64, rt));
+    CHK(ixh_batch_set_shape(b, wide ? kBlock : 64, grp ? kLinesPerGroup : 1, rt));
     // untimed steps build the arguments (and graphs) of the rotation states
     // the timed steps meet
     CHK(batch_run(b, 2, 0));
@@ -2046,12 +2068,14 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
       best_ms = per;
     }
   }
-  const int o = best >> 3, g = (best >> 2) & 1, lt = (best >> 1) & 1 ? kBlock : 64, rt = best & 1;
+  const int o = best >> 4, g = (best >> 3) & 1, lt = (best >> 2) & 1 ? kBlock : 64;
+  const int lpg = (best >> 1) & 1 ? kLinesPerGroup : 1, rt = best & 1;
   CHK(ixh_batch_set_schedule(b, o, g));
-  CHK(ixh_batch_set_shape(b, lt, rt));
+  CHK(ixh_batch_set_shape(b, lt, lpg, rt));
   if (order) *order = o;
   if (graph) *graph = g;
   if (line_threads) *line_threads = lt;
+  if (lines_per_group) *lines_per_group = lpg;
   if (residual_with_t) *residual_with_t = rt;
   return 0;
 }

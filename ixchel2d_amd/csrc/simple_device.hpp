@@ -772,10 +772,11 @@ __device__ __forceinline__ void wpt_body(T *a, T *b, T *c, T *r, int n, const in
   }
 }
 
-// the wave solver run by the first wave of the workgroup
-template <int MMAX, typename T>
+// the wave solver, run by the first wave of a 256-thread line workgroup, or
+// by every wave where each wave holds a line of its own (BS = 64)
+template <int MMAX, typename T, int BS = kBlock>
 __device__ __forceinline__ void wpt_solve(T *a, T *b, T *c, T *r, int n) {
-  if (threadIdx.x < 64) wpt_body<MMAX>(a, b, c, r, n, (int)threadIdx.x);
+  if (BS == 64 || threadIdx.x < 64) wpt_body<MMAX>(a, b, c, r, n, (int)(threadIdx.x & 63));
   __syncthreads();
 }
 
@@ -787,7 +788,7 @@ __device__ __forceinline__ void solve_line(T *a, T *b, T *c, T *r, int n) {
   if (SOLVER == SOLVE_THOMAS)
     thomas_lds(a, b, c, r, n);
   else if (MW <= 16)
-    wpt_solve<(MW < 2 ? 2 : MW)>(a, b, c, r, n);  // SOLVE_PCR and SOLVE_PCR_EXACT
+    wpt_solve<(MW < 2 ? 2 : MW), T, BS>(a, b, c, r, n);  // SOLVE_PCR and SOLVE_PCR_EXACT
   else
     pcr_lds<EPT>(a, b, c, r, n);                   // 256 threads, lines longer than 1024
 }
@@ -923,12 +924,26 @@ __device__ __forceinline__ int lane_zero() {
 // PZ (p' y-sweep only): p' is zero at entry, so it is not read at all, and
 // only the other (row) layout of the result is written: the p' x-sweep that
 // follows reads that layout and writes both itself.
-template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock, bool CORR = false, bool PZ = false>
+// LPW > 1 (one wave per line only): the workgroup holds LPW consecutive
+// lines, wave w line line_s (= the group's first line + w), lds this wave's
+// own slice of line_lds doubles, the slices of the group in wave order.  A
+// wave past the cavity's last line assembles and solves that last line again
+// and stores nothing.  The other-layout copies of the results (and of
+// a_u / a_v) leave from LDS after the group's solves, LPW neighbouring
+// lines' values per position in one run of consecutive addresses, instead of
+// one value per lane, each in a cache line of its own.
+template <int EQ, int DIR, int SOLVER, int EPT, typename R, int BS = kBlock, bool CORR = false, bool PZ = false,
+          int LPW = 1>
 __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s, double *lds,
                                            const CorrLine *K = nullptr) {
   static_assert(!CORR || (EQ == EQ_P && DIR == DIR_X && BS == kBlock), "the correction rides on the p' x-sweep");
   static_assert(!PZ || (EQ == EQ_P && DIR == DIR_Y), "the zero-input form is the p' y-sweep's");
-  const int line = line_s + lane_zero();
+  static_assert(LPW == 1 || (BS == 64 && !CORR && (LPW & (LPW - 1)) == 0), "line groups of one-wave lines");
+  // this thread's position in its line's workgroup (in its wave, for LPW > 1)
+  const int tid = LPW > 1 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
+  const int lastl = Layout<EQ, DIR>::last_line(A.mi, A.nj);
+  const bool active = LPW == 1 || line_s <= lastl;
+  const int line = (LPW > 1 ? min(line_s, lastl) : line_s) + lane_zero();
   static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
                 "the FP32 sweeps use the PCR solver");
   static_assert(SOLVER != SOLVE_DELTA || sizeof(R) == 4, "the increment sweeps are FP32");
@@ -948,7 +963,6 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   const int mi = A.mi, nj = A.nj;
   const int n = Layout<EQ, DIR>::n(mi, nj);
   const int hi = Layout<EQ, DIR>::hi(mi, nj);  // = n - 1
-  const int lastl = Layout<EQ, DIR>::last_line(mi, nj);
   // sides holding the two ends of this line: x-lines end on a (low) and c,
   // y-lines on b (low) and d
   const Side &lo_side = DIR == DIR_X ? A.bc[0] : A.bc[1];
@@ -1039,7 +1053,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   const double phi_2 = DELTA ? along(2) : 0.0, phi_n1 = DELTA ? along(n - 1) : 0.0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int k = threadIdx.x + e * BS;
+    const int k = tid + e * BS;
     const int pos = k + 1;                                    // 1-based position along the line
     const int pc = min(max(pos, 2), hi);                     // assembled position
     const int pr = pos > n ? n : pos;                         // position for the plain loads
@@ -1169,7 +1183,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     const int line2 = line + 1;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int k = threadIdx.x + e * BS;
+      const int k = tid + e * BS;
       const int pos = k + 1;
       double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0, bval = 0.0;
       p_row(min(max(pos, 2), hi), line2, a0, a1, a2, rr, bval, true);
@@ -1192,15 +1206,15 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   // boundary faces in both sweeps), b_o (p' x-sweep, row layout)
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int k = threadIdx.x + e * BS;
+    const int k = tid + e * BS;
     const int pos = k + 1;
-    if (k >= n) continue;
+    if (k >= n || !active) continue;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
     if (pos >= 2 && pos <= hi) {
       if (MOM && DIR == DIR_X) {
         AT(acoef, ii, jj) = S(keep_aw[e]);
-        AO(acoef2, ii, jj) = S(keep_aw[e]);
+        if (LPW == 1) AO(acoef2, ii, jj) = S(keep_aw[e]);
       }
       if (EQ == EQ_P && DIR == DIR_X) AT(A.bo, ii, jj) = keep_b[e];
     } else if (MOM && (pos == 1 ? lo_typed : hi_typed)) {
@@ -1219,10 +1233,18 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   // of a 256-thread workgroup that would own it (wave (k % 256) / 64), so
   // the norms equal the 256-thread ones bit for bit.
   double acc0[4] = {0.0, 0.0, 0.0, 0.0}, acc1[4] = {0.0, 0.0, 0.0, 0.0};
+  // LPW > 1: the slots of this wave's slice that the other-layout stores
+  // read after the group's solves (sa, sb, sc are free once the line is
+  // solved): the result x in sr where it is stored in the solver's type,
+  // else (FP64 fields of FP32 solves) in sa-sb as FP64; a_u / a_v in sa, or
+  // in sc where sa-sb hold x
+  constexpr bool WIDE = sizeof(S) > sizeof(R);
+  S *tile_x = WIDE ? reinterpret_cast<S *>(sa) : reinterpret_cast<S *>(sr);
+  R *tile_aw = WIDE ? sc_ : sa;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int k = threadIdx.x + e * BS;
-    if (k >= n) continue;
+    const int k = tid + e * BS;
+    if (k >= n || !active) continue;
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
@@ -1230,7 +1252,12 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     if constexpr (DELTA) x = keep_phi[e] + (double)sr[k];
     else x = sr[k];
     if (!PZ) AT(out, ii, jj) = x;
-    AO(out2, ii, jj) = x;
+    if constexpr (LPW == 1) {
+      AO(out2, ii, jj) = x;
+    } else {
+      if constexpr (WIDE) tile_x[k] = x;
+      if constexpr (MOM && DIR == DIR_X) tile_aw[k] = keep_aw[e];
+    }
     if (NORM && pos >= 2 && pos <= mi) {
       const int w = BS == kBlock ? 0 : (k & 255) >> 6;
       double d = (double)x - keep_old[e];
@@ -1248,8 +1275,36 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     double s0, s1 = 0.0;
     s0 = owner_sum<BS>(acc0, red);
     if (EQ == EQ_P) s1 = owner_sum<BS>(acc1, red);
-    if (EQ == EQ_P && threadIdx.x == 0) gp(A.part1)[line] = s1;
-    if (threadIdx.x == 0) gp(A.part0)[line] = s0;
+    if (EQ == EQ_P && tid == 0 && active) gp(A.part1)[line] = s1;
+    if (tid == 0 && active) gp(A.part0)[line] = s0;
+  }
+  if constexpr (LPW > 1) {
+    // the other-layout stores of the group: thread t stores line
+    // first + t % LPW at positions t / LPW + 64 m, so LPW consecutive
+    // threads write LPW consecutive addresses
+    __syncthreads();
+    constexpr int kSlice = line_lds<R, EPT, 64>();
+    const int wv = (int)(threadIdx.x >> 6), t = (int)threadIdx.x;
+    const int w = t & (LPW - 1), p0 = t / LPW;
+    const int lw = line_s - wv + w;
+    const double *slice = lds + (w - wv) * kSlice;
+    const S *xw = reinterpret_cast<const S *>(reinterpret_cast<const char *>(tile_x) - reinterpret_cast<const char *>(lds) +
+                                              reinterpret_cast<const char *>(slice));
+    const R *aww = reinterpret_cast<const R *>(reinterpret_cast<const char *>(tile_aw) -
+                                               reinterpret_cast<const char *>(lds) +
+                                               reinterpret_cast<const char *>(slice));
+    if (lw <= lastl) {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int k = p0 + 64 * e;
+        if (k >= n) continue;
+        const int pos = k + 1;
+        const int ii = DIR == DIR_X ? pos : lw;
+        const int jj = DIR == DIR_X ? lw : pos;
+        AO(out2, ii, jj) = xw[k];
+        if (MOM && DIR == DIR_X && pos >= 2 && pos <= hi) AO(acoef2, ii, jj) = S(aww[k]);
+      }
+    }
   }
   if constexpr (CORR) {
     // p += 0.6 p' (ii 2..mi), u += dy_u (p'_P - p'_E) / a_u (ii 2..mi-1),
@@ -1260,7 +1315,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     S *u1 = fldw<S>(K->u), *v1 = fldw<S>(K->v), *u1T = fldw<S>(K->uT), *v1T = fldw<S>(K->vT);
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int k = threadIdx.x + e * BS;
+      const int k = tid + e * BS;
       const int pos = k + 1;
       if (pos > n) continue;
       const double pcP = (double)S(sr[k]);
@@ -1336,6 +1391,20 @@ __global__ void __launch_bounds__(BS) k_sweep_batch(const SweepArgs *__restrict_
   const int2 w = wg[blockIdx.x];
   if (w.x < 0) return;
   sweep_line<EQ, DIR, SOLVER, EPT, R, BS>(D[w.x], w.y, lds);
+}
+
+// A batch sweep with LPW consecutive lines of one cavity per workgroup, one
+// wave each (the LPW form of sweep_line): workgroup b runs lines wg[b].y ..
+// wg[b].y + LPW - 1 of cavity wg[b].x.  PZ: the zero-input p' y-sweep.
+template <int EQ, int DIR, int SOLVER, int EPT, typename R, int LPW, bool PZ = false>
+__global__ void __launch_bounds__(64 * LPW) k_sweep_group_batch(const SweepArgs *__restrict__ D,
+                                                                const int2 *__restrict__ wg) {
+  constexpr int kSlice = line_lds<R, EPT, 64>();
+  __shared__ double lds[LPW * kSlice];
+  const int2 w = wg[blockIdx.x];
+  if (w.x < 0) return;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  sweep_line<EQ, DIR, SOLVER, EPT, R, 64, false, PZ, LPW>(D[w.x], w.y + wave, lds + wave * kSlice);
 }
 
 // the p' y-sweep of a batch, every cavity's p' zero at entry (the PZ form of

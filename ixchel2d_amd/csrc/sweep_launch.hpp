@@ -69,6 +69,13 @@ template <int SOLVER, typename R>
 hipError_t launch_sweep_pz_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nwg, int ept,
                                    int ept64, bool wave_lines);
 
+// one sweep phase of a batch in groups of kLinesPerGroup consecutive lines
+// of a cavity per workgroup, one wave per line (k_sweep_group_batch; the
+// wave solver only): ngroups workgroups mapped by wg to (cavity, first line)
+constexpr int kLinesPerGroup = 4;
+template <int EQ, int DIR, int SOLVER, typename R>
+hipError_t launch_sweep_group_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int ngroups, int ept64);
+
 // the T x-sweep phase of a batch (nsweep workgroups of wg) and the u residual
 // rows (nres workgroups of wgr) in one launch (k_sweep_t_res_batch); every
 // cavity needs Ri = 0 in the u equation
@@ -209,6 +216,26 @@ hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *
   return launch_batch_shaped<SweepBatchK<EQ, DIR, SOLVER, R>, SOLVER>(st, nwg, ept, ept64, wave_lines, D, wg);
 }
 
+template <int EQ, int DIR, int SOLVER, typename R>
+hipError_t launch_sweep_group_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int ngroups, int ept64) {
+  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
+    constexpr int G = kLinesPerGroup;
+    constexpr bool PZ = EQ == EQ_P && DIR == DIR_Y;  // every batch step starts p' from zero
+    const dim3 grid(ngroups), block(64 * G);
+    const LaunchCtx c{st, nullptr, nullptr};
+    switch (ept64) {
+      case 1: launch_on(c, k_sweep_group_batch<EQ, DIR, SOLVER, 1, R, G, PZ>, grid, block, D, wg); break;
+      case 2: launch_on(c, k_sweep_group_batch<EQ, DIR, SOLVER, 2, R, G, PZ>, grid, block, D, wg); break;
+      case 3: launch_on(c, k_sweep_group_batch<EQ, DIR, SOLVER, 3, R, G, PZ>, grid, block, D, wg); break;
+      case 4: launch_on(c, k_sweep_group_batch<EQ, DIR, SOLVER, 4, R, G, PZ>, grid, block, D, wg); break;
+      default: launch_on(c, k_sweep_group_batch<EQ, DIR, SOLVER, 5, R, G, PZ>, grid, block, D, wg); break;
+    }
+    return hipGetLastError();
+  } else {
+    return hipErrorInvalidConfiguration;
+  }
+}
+
 template <int SOLVER, typename R>
 hipError_t launch_sweep_pz_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int nwg, int ept,
                                    int ept64, bool wave_lines) {
@@ -227,7 +254,9 @@ hipError_t launch_sweep_t_res_batch_k(hipStream_t st, const SweepArgs *D, const 
 #define IXH_SWEEP_ONE(EQ, DIR, SOLVER, R)                                                        \
   template hipError_t launch_sweep_k<EQ, DIR, SOLVER, R>(const LaunchCtx &, const SweepArgs &, int); \
   template hipError_t launch_sweep_batch_k<EQ, DIR, SOLVER, R>(hipStream_t, const SweepArgs *,      \
-                                                               const int2 *, int, int, int, bool);
+                                                               const int2 *, int, int, int, bool); \
+  template hipError_t launch_sweep_group_batch_k<EQ, DIR, SOLVER, R>(hipStream_t, const SweepArgs *, \
+                                                                     const int2 *, int, int);
 #define IXH_MULTI_ONE(EQ0, DIR0, EQ1, DIR1, SOLVER, R)                                  \
   template hipError_t launch_multi_k<EQ0, DIR0, EQ1, DIR1, SOLVER, R>(const LaunchCtx &, \
                                                                       const MultiArgs &, int, int, int);

@@ -207,18 +207,18 @@ module ixchel_hip
        integer(c_int32_t), value :: nsteps, cap
        type(ixh_hist_t), intent(out) :: hist(*)
      end function ixh_batch_iterations
-     integer(c_int) function ixh_batch_set_shape(batch, line_threads, residual_with_t) &
+     integer(c_int) function ixh_batch_set_shape(batch, line_threads, lines_per_group, residual_with_t) &
           bind(C, name='ixh_batch_set_shape')
        import :: c_int, c_ptr, c_int32_t
        type(c_ptr), value :: batch
-       integer(c_int32_t), value :: line_threads, residual_with_t
+       integer(c_int32_t), value :: line_threads, lines_per_group, residual_with_t
      end function ixh_batch_set_shape
-     integer(c_int) function ixh_batch_autotune(batch, trial_steps, order, graph, line_threads, residual_with_t, ms) &
-          bind(C, name='ixh_batch_autotune')
+     integer(c_int) function ixh_batch_autotune(batch, trial_steps, order, graph, line_threads, lines_per_group, &
+          residual_with_t, ms) bind(C, name='ixh_batch_autotune')
        import :: c_int, c_ptr, c_int32_t, c_double
        type(c_ptr), value :: batch
        integer(c_int32_t), value :: trial_steps
-       integer(c_int32_t), intent(out) :: order, graph, line_threads, residual_with_t
+       integer(c_int32_t), intent(out) :: order, graph, line_threads, lines_per_group, residual_with_t
        real(c_double), intent(out) :: ms(*)
      end function ixh_batch_autotune
      ! discrete adjoint of the energy transport (no upstream counterpart)

@@ -250,28 +250,32 @@ class Batch:
     def set_schedule(self, order: str = "xcd", graph: bool = False) -> None:
         _lib.check(self.L.ixh_batch_set_schedule(self.h, _lib.ORDER_ID[order], 1 if graph else 0))
 
-    def set_shape(self, line_threads: int = 64, residual_with_t: bool = True) -> None:
-        """Threads per line (64 or 256) and whether the u residual runs in
+    def set_shape(self, line_threads: int = 64, lines_per_group: int = _lib.BATCH_LINE_GROUP,
+                  residual_with_t: bool = False) -> None:
+        """Threads per line (64 or 256), lines per workgroup in the sweeps
+        (1 or ``_lib.BATCH_LINE_GROUP``) and whether the u residual runs in
         the T x-sweep's launch (``ixh_batch_set_shape``)."""
-        _lib.check(self.L.ixh_batch_set_shape(self.h, line_threads, 1 if residual_with_t else 0))
+        _lib.check(self.L.ixh_batch_set_shape(self.h, line_threads, lines_per_group, 1 if residual_with_t else 0))
 
     def autotune(self, trial_steps: int = 10):
         """Time every schedule the batch allows (ordering, graph, threads per
-        line, residual placement), keep the fastest.  Returns (schedule,
-        table): schedule = dict(order, graph, line_threads, residual_with_t),
-        table = {(order, graph, line_threads, residual_with_t): ms per step}
+        line, lines per workgroup, residual placement), keep the fastest.
+        Returns (schedule, table): schedule = dict(order, graph,
+        line_threads, lines_per_group, residual_with_t), table = {(order,
+        graph, line_threads, lines_per_group, residual_with_t): ms per step}
         over the schedules tried."""
-        o, g, lt, rt = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+        o, g, lt, lg, rt = (C.c_int32() for _ in range(5))
         ms = (C.c_double * _lib.BATCH_SCHEDULES)()
         _lib.check(self.L.ixh_batch_autotune(self.h, trial_steps, C.byref(o), C.byref(g), C.byref(lt),
-                                             C.byref(rt), ms))
+                                             C.byref(lg), C.byref(rt), ms))
         names = {v: k for k, v in _lib.ORDER_ID.items()}
         table = {}
         for k in range(_lib.BATCH_SCHEDULES):
             if ms[k] >= 0.0:
-                table[(names[k >> 3], bool((k >> 2) & 1), 256 if (k >> 1) & 1 else 64, bool(k & 1))] = ms[k]
+                table[(names[k >> 4], bool((k >> 3) & 1), 256 if (k >> 2) & 1 else 64,
+                       _lib.BATCH_LINE_GROUP if (k >> 1) & 1 else 1, bool(k & 1))] = ms[k]
         sched = {"order": names[o.value], "graph": bool(g.value), "line_threads": lt.value,
-                 "residual_with_t": bool(rt.value)}
+                 "lines_per_group": lg.value, "residual_with_t": bool(rt.value)}
         return sched, table
 
     def run(self, nsteps: int, cap: Optional[int] = None):

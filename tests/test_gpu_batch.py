@@ -105,11 +105,15 @@ def test_batch_autotune_then_run():
     cav = [Cavity(c, solver="pcr") for c in cases]
     b = Batch(cav)
     sched, table = b.autotune(trial_steps=1)
-    # every line is short and no u equation reads T: all 24 schedules
-    assert len(table) == 24 and all(v > 0 for v in table.values())
+    # every line is short and no u equation reads T: 3 orderings x graph or
+    # not x 5 shapes (64 threads: one line, or a group of lines; the residual
+    # beside T with one line; 256 threads, the residual either way)
+    assert len(table) == 30 and all(v > 0 for v in table.values())
     assert {k[0] for k in table} == {"cavity", "largest", "xcd"}
-    assert {k[2] for k in table} == {64, 256} and {k[3] for k in table} == {False, True}
-    best = (sched["order"], sched["graph"], sched["line_threads"], sched["residual_with_t"])
+    assert {k[2:] for k in table} == {(64, 1, False), (64, 1, True), (64, 4, False), (256, 1, False),
+                                      (256, 1, True)}
+    best = (sched["order"], sched["graph"], sched["line_threads"], sched["lines_per_group"],
+            sched["residual_with_t"])
     assert table[best] == min(table.values())
     b.run(4)
     steps = len(table) * (1 + 4) + 4
@@ -120,21 +124,24 @@ def test_batch_autotune_then_run():
     b.close()
 
 
-@pytest.mark.parametrize("solver,line_threads,residual_with_t",
-                         [("pcr", 64, False), ("pcr", 64, True), ("pcr", 256, False), ("pcr", 256, True),
-                          ("thomas", 256, True), ("thomas", 256, False)])
-def test_batch_shapes_match_single_cavity(solver, line_threads, residual_with_t):
-    """Threads per line and the residual's placement change the launch
-    shape only: every cavity still equals its run alone, bit for bit, over
-    calls that rebuild the arguments, and a cavity whose u equation reads T
-    (Ri_x != 0) sends the residual back to a launch of its own."""
+@pytest.mark.parametrize("solver,precision,line_threads,lines_per_group,residual_with_t",
+                         [("pcr", "fp64", 64, 1, False), ("pcr", "fp64", 64, 1, True), ("pcr", "fp64", 64, 4, False),
+                          ("pcr", "fp64", 256, 1, False), ("pcr", "fp64", 256, 1, True),
+                          ("thomas", "fp64", 256, 1, True), ("thomas", "fp64", 256, 1, False),
+                          ("pcr", "fp32_mixed", 64, 4, False), ("pcr", "fp32_refined", 64, 4, False)])
+def test_batch_shapes_match_single_cavity(solver, precision, line_threads, lines_per_group, residual_with_t):
+    """Threads per line, lines per workgroup and the residual's placement
+    change the launch shape only: every cavity still equals its run alone,
+    bit for bit, over calls that rebuild the arguments, and a cavity whose u
+    equation reads T (Ri_x != 0) sends the residual back to a launch of its
+    own."""
     from ixchel2d_amd.simple import Batch, Cavity
     cases = _small_batch()
-    cav = [Cavity(c, solver=solver) for c in cases]
+    cav = [Cavity(c, solver=solver, precision=precision) for c in cases]
     b = Batch(cav, order="xcd")
-    b.set_shape(line_threads, residual_with_t)
+    b.set_shape(line_threads, lines_per_group, residual_with_t)
     hb = b.run(5)
-    alone = [Cavity(c, solver=solver) for c in cases]
+    alone = [Cavity(c, solver=solver, precision=precision) for c in cases]
     ha = [a.run(5) for a in alone]
     for c in range(len(cases)):
         for f in "uvtp":
@@ -161,9 +168,13 @@ def test_batch_shape_rejects_one_wave_for_thomas():
     cav = [Cavity(c, solver="thomas") for c in _small_batch()[:2]]
     b = Batch(cav)
     with pytest.raises(_lib.IxhError, match="one wave per line"):
-        b.set_shape(64, True)
+        b.set_shape(64, 1, True)
     with pytest.raises(_lib.IxhError, match="threads per line"):
-        b.set_shape(128, True)
+        b.set_shape(128, 1, True)
+    with pytest.raises(_lib.IxhError, match="one-wave lines"):
+        b.set_shape(256, 4, False)
+    with pytest.raises(_lib.IxhError, match="lines per group"):
+        b.set_shape(256, 3, False)
     b.close()
     for x in cav:
         x.close()

@@ -53,7 +53,7 @@ def batch256():
         c.close()
     cav = [Cavity(c, solver="pcr", precision="fp32_mixed") for c in cases]
     b = Batch(cav, order=sched["order"], graph=True)
-    b.set_shape(sched["line_threads"], sched["residual_with_t"])
+    b.set_shape(sched["line_threads"], sched["lines_per_group"], sched["residual_with_t"])
     hist = b.run(STEPS)
     print(f"\n[batch256] autotuned {sched}; timings {table}", flush=True)
     yield cases, cav, hist, STEPS
