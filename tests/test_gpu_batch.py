@@ -144,7 +144,7 @@ def test_batch_shapes_match_single_cavity(solver, precision, line_threads, lines
     alone = [Cavity(c, solver=solver, precision=precision) for c in cases]
     ha = [a.run(5) for a in alone]
     for c in range(len(cases)):
-        for f in "uvtp":
+        for f in ("u", "v", "t", "p", "resu"):
             assert np.array_equal(cav[c].get(f), alone[c].get(f)), f"cavity {c}: {f}"
         for k in HIST:
             assert np.array_equal(hb[c][k], ha[c][k]), f"cavity {c}: {k}"
@@ -154,7 +154,7 @@ def test_batch_shapes_match_single_cavity(solver, precision, line_threads, lines
     hb = b.run(3)
     for c in range(len(cases)):
         ha = alone[c].run(3)
-        for f in "uvtp":
+        for f in ("u", "v", "t", "p", "resu"):
             assert np.array_equal(cav[c].get(f), alone[c].get(f)), f"cavity {c} after ri_x: {f}"
         assert np.array_equal(hb[c]["residuo"], ha["residuo"]), f"cavity {c}: residuo"
     b.close()
