@@ -331,29 +331,38 @@ def cylinder_sources(case: Case, xc: float, yc: float, radius: float,
         put("t", (mi + 1, nj + 1), t_body)
 
 
-def upstream_body_sources(case: Case, opcion: str) -> None:
-    """The immersed bodies upstream's definir_cuerpo builds in
-    (source/mod_front_inmersa.f90:54-179): 'cuadr', the square of side 0.1
-    centred at (6.0, 0.05), and 'trian', the triangle 0 <= y <= 0.1 between
-    the lines x = 2y + 5.8 and x = 6.2 - 2y (recta_mxb1 / recta_mxb2,
-    :192-252).  Both force u and v only: inside, Sp = -1e51 and
-    Sc = 1e51 * 1e-11.  The mask is tested at the pressure-node coordinates
-    (xp(ii), yp(jj)) for both arrays, with the same floating-point
-    expressions as upstream's tests."""
-    m = case.mesh
-    mi, nj = m.mi, m.nj
-    X, Y = np.meshgrid(m.xp, m.yp, indexing="ij")  # (mi+1, nj+1)
-    if opcion == "cuadr":
-        yv, xv, hh = 0.05, 6.0, 0.1
-        inside = (yv - hh / 2.0 <= Y) & (Y <= yv + hh / 2.0) & (xv - hh / 2.0 <= X) & (X <= xv + hh / 2.0)
-    elif opcion == "trian":
+def upstream_body_mask(mesh: Mesh, opcion: str) -> np.ndarray:
+    """The (mi+1, nj+1) nodes inside one of upstream's immersed bodies
+    (source/mod_front_inmersa.f90:54-179), tested at the pressure-node
+    coordinates (xp(ii), yp(jj)) with the same floating-point expressions as
+    upstream's tests:
+      'cuadr': the square of side 0.1 centred at (6.0, 0.05) (:99-128);
+      'trian': the triangle 0 <= y <= 0.1 between the lines x = 2y + 5.8
+               and x = 6.2 - 2y (recta_mxb1 / recta_mxb2, :74-97, 192-252);
+      'chime': the rod of side 0.3 centred at (8.5, 0.5) (:130-175)."""
+    X, Y = np.meshgrid(mesh.xp, mesh.yp, indexing="ij")
+    if opcion in ("cuadr", "chime"):
+        yv, xv, hh = (0.05, 6.0, 0.1) if opcion == "cuadr" else (0.5, 8.5, 0.3)
+        return (yv - hh / 2.0 <= Y) & (Y <= yv + hh / 2.0) & (xv - hh / 2.0 <= X) & (X <= xv + hh / 2.0)
+    if opcion == "trian":
         yv, hh = 0.1, 0.1
         m1, m2 = 1.0 / 2.0, -1.0 / 2.0
         b1 = -(0.1 - m1 * 6.0) / m1
         b2 = -(0.1 - m2 * 6.0) / m2
-        inside = (yv - hh <= Y) & (Y <= yv) & (Y / m1 + b1 <= X) & (X <= Y / m2 + b2)
-    else:
-        raise ValueError(f"body {opcion!r}: 'cuadr' or 'trian'")
+        return (yv - hh <= Y) & (Y <= yv) & (Y / m1 + b1 <= X) & (X <= Y / m2 + b2)
+    raise ValueError(f"body {opcion!r}: 'cuadr', 'trian' or 'chime'")
+
+
+def upstream_body_sources(case: Case, opcion: str) -> None:
+    """The sources of upstream's immersed bodies (upstream_body_mask): u and
+    v are forced to rest inside, Sp = -1e51 and Sc = 1e51 * 1e-11; 'chime'
+    also holds T = 1 there, Sp = -1e51 and Sc = 1e51 * 1 (:164-165).  The
+    'chime' branch also sets a_u, a_v = 1e41 on the rod (:158, 162); every
+    x-assembly rewrites a_u, a_v on interior nodes before anything reads
+    them, so those values never take effect and are not set here."""
+    m = case.mesh
+    mi, nj = m.mi, m.nj
+    inside = upstream_body_mask(m, opcion)
     for var, shape in (("u", (mi, nj + 1)), ("v", (mi + 1, nj))):
         msk = inside[: shape[0], : shape[1]]
         sc = np.zeros(shape, order="F")
@@ -361,6 +370,12 @@ def upstream_body_sources(case: Case, opcion: str) -> None:
         sp[msk] = -PENALTY
         sc[msk] = PENALTY * 10.0e-12
         case.sources[var] = (np.asfortranarray(sc), np.asfortranarray(sp))
+    if opcion == "chime":
+        sc = np.zeros((mi + 1, nj + 1), order="F")
+        sp = np.zeros((mi + 1, nj + 1), order="F")
+        sp[inside] = -PENALTY
+        sc[inside] = PENALTY * 1.0
+        case.sources["t"] = (np.asfortranarray(sc), np.asfortranarray(sp))
 
 
 # --------------------------------------------------------------------------

@@ -96,14 +96,11 @@ def nc301x701(iters: int = 5000):
     return c
 
 
-def ib161x41():
-    """Mixed convection in a long lid-driven cavity (8 x 1, 161 x 41 uniform
-    mesh, Re = 100, Ri = 0.5, hot wall a, cold wall c, adiabatic b and d)
-    around upstream's square body 'cuadr' (side 0.1 centred at (6.0, 0.05),
-    on the bottom wall).  The case files carry no body: the reference builds
-    it in definir_cuerpo (its fixture comes from the binary built with BODY =
-    cuadr), and the oracle and the GPU get it from with_body()."""
-    mesh = C.make_mesh(161, 41, 8.0, 1.0, "unif")
+def _long_cavity(mi: int, nj: int, width: float, name: str):
+    """Mixed convection in a long lid-driven cavity (width x 1, uniform mesh,
+    Re = 100, Ri = 0.5, hot wall a, cold wall c, adiabatic b and d), 200
+    iterations from rest with T = 0.5 + U(-1e-3, 1e-3)."""
+    mesh = C.make_mesh(mi, nj, width, 1.0, "unif")
     prm = C.Params(adimen="mixta", Ra=100.0, Pr=0.71, Ri_1=0.5, dt=0.1, paq_itera=100, itermax=200,
                    rel_pres=0.3, rel_vel=0.7, rel_ener=0.7, flujo_ini="nulo", tempe_ini="archivo")
     bcs = {
@@ -115,11 +112,33 @@ def ib161x41():
     rng = np.random.default_rng(11)
     t0 = np.asfortranarray(0.5 + rng.uniform(-1e-3, 1e-3, size=(mi + 1, nj + 1)))
     z = lambda *sh: np.zeros(sh, order="F")  # noqa: E731
-    return C.Case(mesh, prm, bcs, z(mi, nj + 1), z(mi + 1, nj), t0, z(mi + 1, nj + 1), name="ib161x41")
+    return C.Case(mesh, prm, bcs, z(mi, nj + 1), z(mi + 1, nj), t0, z(mi + 1, nj + 1), name=name)
+
+
+# The cases around upstream's own immersed bodies.  Their case files carry no
+# body: the reference builds it in definir_cuerpo (each fixture comes from
+# the binary built with BODY = its option), and the oracle and the GPU get it
+# from with_body().
+def ib161x41():
+    """161 x 41, 8 x 1, around the square 'cuadr' (side 0.1 centred at
+    (6.0, 0.05), on the bottom wall)."""
+    return _long_cavity(161, 41, 8.0, "ib161x41")
+
+
+def ib161x41t():
+    """161 x 41, 8 x 1, around the triangle 'trian' (base 0.4 on the bottom
+    wall from x = 5.8 to 6.2, apex (6.0, 0.1))."""
+    return _long_cavity(161, 41, 8.0, "ib161x41t")
+
+
+def ib201x41c():
+    """201 x 41, 10 x 1, around the rod 'chime' (side 0.3 centred at
+    (8.5, 0.5)), which also holds T = 1."""
+    return _long_cavity(201, 41, 10.0, "ib201x41c")
 
 
 # the definir_cuerpo option of the reference build behind a fixture
-REF_BODY = {"ib161x41": "cuadr"}
+REF_BODY = {"ib161x41": "cuadr", "ib161x41t": "trian", "ib201x41c": "chime"}
 
 
 def with_body(case):
@@ -130,7 +149,8 @@ def with_body(case):
 
 
 CASES = {"nc18x14": nc18x14, "mixed22x17": mixed22x17, "ctrl18x14": ctrl18x14, "lid66": lid66,
-         "nc34": nc34, "nc301x701": nc301x701, "ib161x41": ib161x41}
+         "nc34": nc34, "nc301x701": nc301x701, "ib161x41": ib161x41, "ib161x41t": ib161x41t,
+         "ib201x41c": ib201x41c}
 
 # subsampling of the full-size fixture (every SUB_I-th node in x, SUB_J-th in y)
 SUB_I, SUB_J = 6, 10

@@ -210,17 +210,18 @@ def test_fortran_host_restart_round_trip(tmp_path):
         assert int(fh.readline().split()[2]) == 40
 
 
-def test_fortran_host_immersed_square_against_upstream(tmp_path):
-    """IXCHEL2D_CUERPO=cuadr: definir_cuerpo('cuadr') of the Fortran host
-    (ixchel2d_amd/fortran/mod_front_inmersa.f90) against the reference built
-    with the same option (mod_front_inmersa.f90:99-128, main.f90:327): with
-    the Thomas solver the fields, the restart files and the VTK sample equal
-    upstream's bit for bit."""
+@pytest.mark.parametrize("name", ["ib161x41", "ib161x41t", "ib201x41c"])
+def test_fortran_host_immersed_bodies_against_upstream(tmp_path, name):
+    """IXCHEL2D_CUERPO=cuadr | trian | chime: definir_cuerpo of the Fortran
+    host (ixchel2d_amd/fortran/mod_front_inmersa.f90) against the reference
+    built with the same option (mod_front_inmersa.f90:54-179, main.f90:327):
+    with the Thomas solver the fields, the restart files and the VTK sample
+    equal upstream's bit for bit."""
     assert os.path.exists(EXE), "Fortran host not built"
-    c = K.ib161x41()
-    gold = K.load_fixture("ib161x41")
+    c = K.CASES[name]()
+    gold = K.load_fixture(name)
     C.write_reference_case(c, str(tmp_path))
-    out = _run_host(c, tmp_path, IXCHEL2D_CUERPO="cuadr")
+    out = _run_host(c, tmp_path, IXCHEL2D_CUERPO=K.REF_BODY[name])
     for f in "uvtp":
         assert np.array_equal(out[f], gold[f]), f
     np.testing.assert_allclose(out["history"], gold["history"], rtol=1e-12, atol=1e-300)

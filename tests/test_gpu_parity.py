@@ -232,26 +232,23 @@ def test_immersed_cylinder_sources():
     assert np.max(np.abs(g.get("v")[mv])) <= 1e-10
 
 
-def _square_inside(mesh):
-    """p-nodes of upstream's 'cuadr' body (mod_front_inmersa.f90:99-128)."""
-    X, Y = np.meshgrid(mesh.xp, mesh.yp, indexing="ij")
-    return (0.05 - 0.05 <= Y) & (Y <= 0.05 + 0.05) & (6.0 - 0.05 <= X) & (X <= 6.0 + 0.05)
-
-
+@pytest.mark.parametrize("name", ["ib161x41", "ib161x41t", "ib201x41c"])
 @pytest.mark.parametrize("solver", ["thomas", "pcr"])
-def test_immersed_square_against_upstream(solver):
-    """Upstream's own immersed body: the reference built with
-    definir_cuerpo(..., 'cuadr') (main.f90:327), 200 iterations of a long
-    lid-driven mixed-convection cavity.  Thomas: every field bit for bit,
-    the penalty-amplified pressure inside the body included; PCR: 1e-10
-    relative, the pressure compared outside the body (inside it is rounding
-    noise amplified by the 1e51 penalty).  u and v inside sit at the
-    penalty value, about 1e-11."""
-    case = K.with_body(K.ib161x41())
-    gold = K.load_fixture("ib161x41")
+def test_immersed_bodies_against_upstream(solver, name):
+    """Upstream's own immersed bodies: the reference built with
+    definir_cuerpo(..., 'cuadr' | 'trian' | 'chime') (main.f90:327), 200
+    iterations of a long lid-driven mixed-convection cavity.  Thomas: every
+    field bit for bit, the penalty-amplified pressure inside the body
+    included; PCR: 1e-10 relative, the pressure compared outside the body
+    (inside it is rounding noise amplified by the 1e51 penalty).  u and v
+    inside sit at the penalty value, about 1e-11; 'chime' holds T = 1."""
+    body = K.REF_BODY[name]
+    case = K.with_body(K.CASES[name]())
+    gold = K.load_fixture(name)
     g = _cavity(case, solver)
     h = g.run(case.params.itermax)
-    outside = ~_square_inside(case.mesh)
+    inside = C.upstream_body_mask(case.mesh, body)
+    outside = ~inside
     for f in "uvtp":
         a, b = g.get(f), gold[f]
         if solver == "thomas":
@@ -264,15 +261,26 @@ def test_immersed_square_against_upstream(solver):
     mv = case.sources["v"][1] != 0.0
     assert np.max(np.abs(g.get("u")[mu])) <= 1e-10
     assert np.max(np.abs(g.get("v")[mv])) <= 1e-10
+    if body == "chime":
+        assert np.max(np.abs(g.get("t")[inside] - 1.0)) <= 1e-12
     logged = h["logged"].astype(bool)
     hist = np.c_[h["maxbo"], h["residuo"]][logged]
     assert hist.shape == gold["history"].shape
     np.testing.assert_allclose(hist[:, 0], gold["history"][:, 0], rtol=1e-9, atol=1e-15)
-    # residuo here (about 1e28) is the u residual of the penalised rows:
-    # the rounding of u inside the body times the 1e51 penalty, so the
-    # line solver's rounding shows in it (measured with PCR: 3.6e-7)
-    rtol = 1e-12 if solver == "thomas" else 1e-6
-    np.testing.assert_allclose(hist[:, 1], gold["history"][:, 1], rtol=rtol, atol=1e-15)
+    # residuo here (1e22 to 1e29) is the u residual of the penalised rows:
+    # the rounding of u inside the body times the 1e51 penalty, so the line
+    # solver's rounding shows in it.  With PCR it follows upstream's to 3.6e-7
+    # for the bodies on the wall, where u is small; the rod of 'chime' sits
+    # in the moving fluid, where the rounding noise itself differs between
+    # the solvers (up to 37%, at 1e22 against a physical residual below
+    # 1e-2), so there only its magnitude is compared.
+    if solver == "thomas":
+        np.testing.assert_allclose(hist[:, 1], gold["history"][:, 1], rtol=1e-12, atol=1e-15)
+    elif body != "chime":
+        np.testing.assert_allclose(hist[:, 1], gold["history"][:, 1], rtol=1e-6, atol=1e-15)
+    else:
+        ratio = hist[:, 1] / gold["history"][:, 1]
+        assert np.all((ratio > 0.5) & (ratio < 2.0)), ratio
 
 
 @pytest.mark.parametrize("solver", ["thomas", "pcr"])

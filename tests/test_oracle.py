@@ -10,14 +10,15 @@ import cases as K
 from ixchel2d_amd import case as C
 from oracle import pyoracle as O
 
-SMALL = ["nc18x14", "mixed22x17", "ctrl18x14", "lid66", "nc34", "ib161x41"]
+SMALL = ["nc18x14", "mixed22x17", "ctrl18x14", "lid66", "nc34", "ib161x41", "ib161x41t", "ib201x41c"]
 
 
 @pytest.mark.parametrize("name", SMALL)
 def test_oracle_matches_upstream_bit_for_bit(name):
-    """ib161x41 carries upstream's square body ('cuadr' in definir_cuerpo,
-    source/mod_front_inmersa.f90:54-179): the oracle gets it as sources
-    from case.upstream_body_sources, the reference from its own build."""
+    """ib161x41, ib161x41t and ib201x41c carry upstream's bodies 'cuadr',
+    'trian' and 'chime' (definir_cuerpo, source/mod_front_inmersa.f90:54-179;
+    'chime' also holds T = 1): the oracle gets them as sources from
+    case.upstream_body_sources, the reference from its own build."""
     c = K.with_body(K.CASES[name]())
     g = K.load_fixture(name)
     o = O.Oracle(c)
