@@ -1287,12 +1287,11 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     const int wv = (int)(threadIdx.x >> 6), t = (int)threadIdx.x;
     const int w = t & (LPW - 1), p0 = t / LPW;
     const int lw = line_s - wv + w;
-    const double *slice = lds + (w - wv) * kSlice;
-    const S *xw = reinterpret_cast<const S *>(reinterpret_cast<const char *>(tile_x) - reinterpret_cast<const char *>(lds) +
-                                              reinterpret_cast<const char *>(slice));
-    const R *aww = reinterpret_cast<const R *>(reinterpret_cast<const char *>(tile_aw) -
-                                               reinterpret_cast<const char *>(lds) +
-                                               reinterpret_cast<const char *>(slice));
+    // wave w's slice, and the tile slots at their offsets in every slice
+    const char *slice = reinterpret_cast<const char *>(lds + (w - wv) * kSlice);
+    const char *own = reinterpret_cast<const char *>(lds);
+    const S *xw = reinterpret_cast<const S *>(slice + (reinterpret_cast<const char *>(tile_x) - own));
+    const R *aww = reinterpret_cast<const R *>(slice + (reinterpret_cast<const char *>(tile_aw) - own));
     if (lw <= lastl) {
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
