@@ -17,6 +17,7 @@ Prints one JSON line.
   python bench.py --workload adjoint129 # hot-wall Nu gradient through the adjoint
   python bench.py --workload fp32_convergence  # FP64 vs both FP32 paths toward the steady state
   python bench.py --workload barrier --steps 2000  # grid barrier vs kernel boundary (probe)
+  python bench.py --workload fortran_host  # the 301x701 case through the Fortran host and its files
 """
 from __future__ import annotations
 
@@ -25,6 +26,7 @@ import json
 import os
 import subprocess
 import sys
+import shutil
 import tempfile
 import time
 
@@ -122,6 +124,66 @@ def reference_serial(case, budget_s: float = 15.0):
     return {"value": m.mi * m.nj / per / 1e6, "unit": "Mcell-SIMPLE-iters/s", "cores": 1, "kind": "reference",
             "sample": f"upstream serial flang build, {n} iterations of the 301x701 case minus a "
                       f"0-iteration run (setup and file reads)"}
+
+
+def fortran_host_bench(args):
+    """The 301x701 case through the Fortran host (ixchel2d_amd/fortran/
+    ixchel2d_gpu: upstream's driver over the C-ABI, reading and writing the
+    reference's files).  Two runs in packages of 1000 iterations, of 1000
+    and of 1000 + steps iterations (steps rounded up to whole packages); the
+    difference times the steps with their packages' outputs (fort.102, the
+    Nusselt record, out_*.dat, the .bin dumps and the VTK sample) and without
+    process start, file reads and set-up.  The same pair again without the
+    package outputs (postprocesar = .false.: fort.102 and the Nusselt record
+    only) shows what the outputs cost.  Upstream's serial build on the same
+    case beside it."""
+    from ixchel2d_amd import case as C
+    exe = os.path.join(ROOT, "ixchel2d_amd", "fortran", "ixchel2d_gpu")
+    if not os.path.exists(exe):
+        raise SystemExit("the Fortran host is not built (python -c 'import __graft_entry__ as g; g.build()')")
+    paq = 1000
+    steps = max(paq, -(-args.steps // paq) * paq)
+    case = build_case()
+    env = dict(os.environ, IXCHEL2D_SOLVER=args.solver, IXCHEL2D_PRECISION=args.precision)
+
+    def run(iters, outputs=True):
+        work = tempfile.mkdtemp(prefix="ixh_fhost_")
+        try:
+            case.params.itermax = iters
+            case.params.paq_itera = paq
+            C.write_reference_case(case, work, postprocesar=outputs)
+            t = time.perf_counter()
+            subprocess.run([exe], cwd=work, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                           check=True, timeout=600)
+            return time.perf_counter() - t
+        finally:
+            shutil.rmtree(work, ignore_errors=True)
+    t_base = run(paq)
+    t_full = run(paq + steps)
+    per = (t_full - t_base) / steps
+    t_base0 = run(paq, outputs=False)
+    t_full0 = run(paq + steps, outputs=False)
+    per0 = (t_full0 - t_base0) / steps
+    m = case.mesh
+    cpu = None if args.no_cpu else reference_serial(build_case())
+    out = {
+        "metric": "Mcell-SIMPLE-iters/s @301x701 through the Fortran host (reference files in and out)",
+        "value": round(m.mi * m.nj / per / 1e6, 2), "unit": "Mcell-SIMPLE-iters/s",
+        "n_gpus": 1, "steps": steps, "warmup": paq, "ms_per_step": round(per * 1e3, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64", "data": "synthetic",
+        "config": {"workload": "natural-convection tall cavity 301x701 (tanh beta=1.2, H/W=2.33, Ra=1e5, "
+                               "Pr=0.71, dt=0.003) through ixchel2d_gpu, packages of 1000 iterations with "
+                               "their outputs", "solver": args.solver, "path": args.precision,
+                   "process_seconds": {f"{paq} iterations": round(t_base, 3),
+                                       f"{paq + steps} iterations": round(t_full, 3)}},
+        "without_package_outputs": {"value": round(m.mi * m.nj / per0 / 1e6, 2), "ms_per_step": round(per0 * 1e3, 5),
+                                    "process_seconds": {f"{paq} iterations": round(t_base0, 3),
+                                                        f"{paq + steps} iterations": round(t_full0, 3)}},
+        "package_output_seconds": round((per - per0) * paq, 3),
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
 
 
 def cpu_baseline(case, budget_s: float = 15.0):
@@ -569,7 +631,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--solver", default="pcr", choices=["pcr", "thomas"])
     ap.add_argument("--workload", default="nc301x701",
-                    choices=["nc301x701", "batch256", "adjoint129", "fp32_convergence", "barrier"])
+                    choices=["nc301x701", "batch256", "adjoint129", "fp32_convergence", "barrier", "fortran_host"])
     ap.add_argument("--conv-chunk", type=int, default=30000, help="iterations per row of the convergence study")
     ap.add_argument("--conv-chunks", type=int, default=12, help="rows of the convergence study")
     ap.add_argument("--adj-dt", type=float, default=0.01, help="time step of the adjoint workload")
@@ -597,6 +659,9 @@ def main():
               "(a batch would shard across GPUs by cavity; this build runs on one GPU)", file=sys.stderr)
         sys.exit(2)
 
+    if args.workload == "fortran_host":
+        fortran_host_bench(args)
+        return
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
