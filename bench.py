@@ -9,13 +9,17 @@ with its speed-up and its error against FP64:
   - "fp32_mixed": u, v, T, p' stored and swept in FP32; mass residual, p'
     right-hand side, reductions, p and the Nusselt integral in FP64;
   - "fp32_refined": every field in FP64; the sweeps assemble in FP32 and
-    solve for FP32 increments against each line's FP64 residual.
+    solve for FP32 increments against each line's FP64 residual;
+  - "fp32_stochastic": the storage of fp32_mixed and the increment sweeps of
+    fp32_refined, every new u, v, T stored with stochastic rounding; with it
+    the FP32 rounding floor of the u residual and a run to residual 1e-8 that
+    finishes on fp32_refined.
 Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--solver pcr|thomas]
   python bench.py --workload batch256   # 256 mixed-size cavities, one launch per phase
   python bench.py --workload adjoint129 # hot-wall Nu gradient through the adjoint
-  python bench.py --workload fp32_convergence  # FP64 vs both FP32 paths toward the steady state
+  python bench.py --workload fp32_convergence  # FP64 vs the FP32 paths toward the steady state
   python bench.py --workload barrier --steps 2000  # grid barrier vs kernel boundary (probe)
   python bench.py --workload fortran_host  # the 301x701 case through the Fortran host and its files
 """
@@ -60,7 +64,7 @@ def alg_bytes(kernel: str, npts: int, precision: str, buoyancy: bool) -> int:
     n64, nr = ALG_OPERANDS[kernel]
     if not buoyancy and kernel in ("sweep_v_y", "sweep_v_x"):
         nr -= 1
-    return (8 * n64 + (4 if precision == "fp32_mixed" else 8) * nr) * npts
+    return (8 * n64 + (4 if precision in ("fp32_mixed", "fp32_stochastic") else 8) * nr) * npts
 
 
 def build_case():
@@ -455,7 +459,7 @@ def convergence_study(args):
     import torch
     from ixchel2d_amd.simple import Cavity
     case = build_case()
-    paths = ("fp64", "fp32_mixed", "fp32_refined")
+    paths = ("fp64", "fp32_mixed", "fp32_refined", "fp32_stochastic")
     cav = {p: Cavity(case, solver="pcr", precision=p) for p in paths}
     rows = []
     total = 0
@@ -479,7 +483,7 @@ def convergence_study(args):
         rows.append(row)
         print(json.dumps(row), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
-    print(json.dumps({"workload": "natural-convection tall cavity 301x701, FP64, FP32-mixed and FP32-refined "
+    print(json.dumps({"workload": "natural-convection tall cavity 301x701, FP64 and the FP32 paths "
                                   "toward the steady state", "chunk": args.conv_chunk, "rows": rows}))
 
 
@@ -611,6 +615,98 @@ def accuracy_converged(ref, case, solver, precision, tol, chunk, max_iters):
     return out
 
 
+STATE_FIELDS = ("u", "v", "t", "p", "u_ant", "v_ant", "t_ant")
+
+
+def rounding_floor(ref, case):
+    """What FP32 storage of u costs the FP64-measured residual: the steady u
+    residual (u_ant = u) of the FP64 path's converged state, and of the same
+    state with u, v, T rounded to FP32, to nearest and stochastically (u_ant
+    then rounded on its own, as two steps of a stochastic iteration would
+    leave it).  No FP32-stored u can go below the rounded figures."""
+    import numpy as np
+    from ixchel2d_amd.simple import Cavity
+    g64 = ref[0]
+    F = {n: g64.get(n) for n in STATE_FIELDS}
+    rng = np.random.default_rng(0)
+
+    def rn(a):
+        return a.astype(np.float32).astype(np.float64)
+
+    def sr(a):
+        f = a.astype(np.float32)
+        fd = f.astype(np.float64)
+        lo = np.where(fd > a, np.nextafter(f, np.float32(-np.inf)), f).astype(np.float64)
+        hi = np.where(fd < a, np.nextafter(f, np.float32(np.inf)), f).astype(np.float64)
+        w = np.where(hi > lo, (a - lo) / np.where(hi > lo, hi - lo, 1.0), 0.0)
+        return np.where(rng.random(a.shape) < w, hi, lo)
+
+    def resid(fields):
+        c = Cavity(case, solver="pcr", precision="fp64", upload=False)
+        for n, a in fields.items():
+            c.set(n, a)
+        r = c.residual_u()
+        c.close()
+        return r
+
+    exact = dict(F, u_ant=F["u"], v_ant=F["v"], t_ant=F["t"])
+    out = {"fp64_iterations": ref[1], "residuo_fp64_state": resid(exact)}
+    for name, fn in (("nearest", rn), ("stochastic", sr)):
+        m = {k: fn(F[k]) if k in ("u", "v", "t") else F[k] for k in ("u", "v", "t", "p")}
+        m.update(u_ant=fn(F["u"]) if name == "stochastic" else m["u"], v_ant=m["v"], t_ant=m["t"])
+        out[f"residuo_rounded_{name}"] = resid(m)
+    return out
+
+
+def hybrid_to_residual(ref, case, tol, chunk, max_iters):
+    """The FP32-stochastic path until its residual is within 3 tol (or has
+    not fallen 10% below its best for 3 chunks: FP32 storage of u keeps it
+    above a floor), then the FP32-refined path from its fields until the
+    FP64-measured residual is tol: FP32 arithmetic throughout, FP64 storage
+    only for the last stretch.  Against the FP64 path run to the same tol."""
+    import numpy as np
+    import torch
+    from ixchel2d_amd.simple import Cavity
+    g64, it64, r64, mb64, s64 = ref
+    t0 = time.perf_counter()
+    g = Cavity(case, solver="pcr", precision="fp32_stochastic")
+    it1, best, flat, r = 0, np.inf, 0, np.inf
+    while it1 < max_iters:
+        g.run(chunk - 1, cap=1)
+        r = float(g.run(1)["residuo"][-1])
+        it1 += chunk
+        flat = 0 if r < 0.9 * best else flat + 1
+        best = min(best, r)
+        if r <= 3.0 * tol or flat >= 3:
+            break
+    r1 = r
+    it2 = 0
+    if r > tol:
+        h = Cavity(case, solver="pcr", precision="fp32_refined", upload=False)
+        for n in STATE_FIELDS:
+            h.set(n, g.get(n))
+        g.close()
+        g = h
+        while it1 + it2 < max_iters:
+            g.run(chunk - 1, cap=1)
+            r = float(g.run(1)["residuo"][-1])
+            it2 += chunk
+            if r <= tol:
+                break
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    nu64, nu = g64.nusselt_x()[0], g.nusselt_x()[0]
+    out = {"tol": tol, "iterations_fp32_stochastic": it1, "residuo_at_switch": r1,
+           "iterations_fp32_refined": it2, "residuo": r, "reached": r <= tol, "seconds": round(sec, 3),
+           "fp64_iterations": it64, "fp64_seconds": round(s64, 3), "time_speedup_vs_fp64": round(s64 / sec, 3),
+           "nu_hot_fp64": nu64, "nu_hot": nu, "nu_rel_err": abs(nu - nu64) / abs(nu64)}
+    for f in "uvt":
+        a, b = g.get(f), g64.get(f)
+        out[f"linf_rel_err_{f}"] = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+    g.close()
+    return out
+
+
 def barrier_probe(args):
     """Cost of a grid-wide barrier in one cooperative launch against a kernel
     boundary, at the grid shapes of a persistent fused step
@@ -639,7 +735,7 @@ def main():
     ap.add_argument("--adj-reps", type=int, default=5, help="timed gradient evaluations")
     ap.add_argument("--batch", type=int, default=256, help="cavities of the batch workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed", "fp32_refined"],
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32_mixed", "fp32_refined", "fp32_stochastic"],
                     help="path of the batch workload")
     ap.add_argument("--fp64-only", action="store_true", help="skip the FP32 lines of the 301x701 workload")
     ap.add_argument("--accuracy-steps", type=int, default=5000,
@@ -690,7 +786,7 @@ def main():
     m = case.mesh
     cells = m.mi * m.nj
     r64 = time_path(case, args.solver, "fp64", args.steps, args.warmup, args.warm_seconds)
-    reduced = {"fp32_mixed": None, "fp32_refined": None}
+    reduced = {"fp32_mixed": None, "fp32_refined": None, "fp32_stochastic": None}
     if args.solver == "pcr" and not args.fp64_only:
         ref = None
         if args.converge_tol > 0:
@@ -704,6 +800,10 @@ def main():
                                                              args.converge_chunk, int(1.5 * ref[1]))
             reduced[prec] = r
         if ref is not None:
+            st = reduced["fp32_stochastic"]
+            st["fp32_rounding_floor"] = rounding_floor(ref, build_case())
+            st["stochastic_then_refined"] = hybrid_to_residual(ref, build_case(), args.converge_tol,
+                                                               max(1, args.converge_chunk // 2), int(1.5 * ref[1]))
             ref[0].close()
     cpu = None if args.no_cpu else cpu_baseline(build_case())
     out = {
@@ -728,6 +828,7 @@ def main():
         "launches": r64["launches"],
         "fp32_mixed": reduced["fp32_mixed"],
         "fp32_refined": reduced["fp32_refined"],
+        "fp32_stochastic": reduced["fp32_stochastic"],
     }
     print(json.dumps(out))
 

@@ -64,8 +64,14 @@ enum {
  *   FP64 steady state.
  * FP32_REFINED: every field stored in FP64; each sweep assembles its rows in
  *   FP32 and solves for the FP32 increment of the line against the line's
- *   FP64 residual, so the iteration converges to the FP64 steady state. */
-enum { IXH_FP64 = 0, IXH_FP32_MIXED = 1, IXH_FP32_REFINED = 2 };
+ *   FP64 residual, so the iteration converges to the FP64 steady state.
+ * FP32_STOCHASTIC: the storage of FP32_MIXED and the increment sweeps of
+ *   FP32_REFINED (the residual formed in FP64 from the FP32 fields); each new
+ *   u, v, T (sweeps and correction) is phi + delta formed in FP64 and stored
+ *   with stochastic rounding, so increments below the FP32 spacing are kept
+ *   on average instead of dropped.  Runs are deterministic: the rounding
+ *   bits are a hash of the value and its node. */
+enum { IXH_FP64 = 0, IXH_FP32_MIXED = 1, IXH_FP32_REFINED = 2, IXH_FP32_STOCHASTIC = 3 };
 
 typedef struct ixh_state ixh_state;
 
@@ -84,7 +90,7 @@ typedef struct {
   double conv_u, conv_t, conv_p, conv_resi, conv_paso;
   int32_t iter_ecuaci_max, iter_simple_max;
   int32_t solver;                  /* IXH_SOLVER_* */
-  int32_t precision;               /* IXH_FP64 | IXH_FP32_MIXED | IXH_FP32_REFINED */
+  int32_t precision;               /* IXH_FP64 | IXH_FP32_MIXED | IXH_FP32_REFINED | IXH_FP32_STOCHASTIC */
 } ixh_params;
 
 /* one record per SIMPLE iteration: sqrt of the squared-difference sums of

@@ -19,7 +19,7 @@ VAR_ID = {"u": 0, "v": 1, "t": 2}
 SIDE_ID = {"a": 0, "b": 1, "c": 2, "d": 3}
 DIR_Y, DIR_X = 0, 1
 SOLVER_ID = {"pcr": 0, "thomas": 1, "pcr_exact": 2}
-PRECISION_ID = {"fp64": 0, "fp32_mixed": 1, "fp32_refined": 2}
+PRECISION_ID = {"fp64": 0, "fp32_mixed": 1, "fp32_refined": 2, "fp32_stochastic": 3}
 
 # every symbol include/ixchel2d.h declares
 EXPORTS = [

@@ -90,6 +90,9 @@ struct ixh_state {
   // FP32-refined: every field stays FP64; the sweeps assemble in FP32 and
   // solve for FP32 increments against an FP64 residual
   bool delta = false;
+  // FP32-stochastic: FP32-mixed storage (f32 set), the increment sweeps of
+  // FP32-refined, and every new u, v, T stored with stochastic rounding
+  bool sr = false;
   // geometry (one allocation, 1-based arrays)
   double *d_geo = nullptr;
   Geo geo{};
@@ -182,12 +185,15 @@ int check_params(const ixh_params *p) {
     return fail(IXH_ERR_ARG, "diffusivities must be positive");
   if (p->solver != IXH_SOLVER_PCR && p->solver != IXH_SOLVER_THOMAS && p->solver != IXH_SOLVER_PCR_EXACT)
     return fail(IXH_ERR_ARG, "unknown solver %d", p->solver);
-  if (p->precision != IXH_FP64 && p->precision != IXH_FP32_MIXED && p->precision != IXH_FP32_REFINED)
+  if (p->precision != IXH_FP64 && p->precision != IXH_FP32_MIXED && p->precision != IXH_FP32_REFINED &&
+      p->precision != IXH_FP32_STOCHASTIC)
     return fail(IXH_ERR_ARG, "unknown precision %d", p->precision);
   if (p->precision == IXH_FP32_MIXED && p->solver != IXH_SOLVER_PCR)
     return fail(IXH_ERR_UNSUPP, "the FP32-mixed path uses the PCR solver only");
   if (p->precision == IXH_FP32_REFINED && p->solver != IXH_SOLVER_PCR)
     return fail(IXH_ERR_UNSUPP, "the FP32-refined path uses the PCR solver only");
+  if (p->precision == IXH_FP32_STOCHASTIC && p->solver != IXH_SOLVER_PCR)
+    return fail(IXH_ERR_UNSUPP, "the FP32-stochastic path uses the PCR solver only");
   return 0;
 }
 
@@ -232,6 +238,7 @@ int launch_pcorr_t(ixh_state *s, const SweepArgs &A, const CorrLine &K, int n) {
 
 // the p' x-sweep with the correction of its lines
 int launch_pcorr(ixh_state *s, const SweepArgs &A, const CorrLine &K, int n) {
+  if (s->sr) return launch_pcorr_t<SOLVE_DELTA_SR, float>(s, A, K, n);
   if (s->f32) return launch_pcorr_t<SOLVE_PCR, float>(s, A, K, n);
   if (s->delta) return launch_pcorr_t<SOLVE_DELTA, float>(s, A, K, n);
   if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_pcorr_t<SOLVE_THOMAS, double>(s, A, K, n);
@@ -248,6 +255,7 @@ int launch_sweep_pz_t(ixh_state *s, const SweepArgs &A, int n) {
 
 int launch_sweep_pz(ixh_state *s, const SweepArgs &A, int n) {
   if (!A.in_zero) return fail(IXH_ERR_ARG, "the zero-input p' y-sweep needs p' = 0 at entry");
+  if (s->sr) return launch_sweep_pz_t<SOLVE_DELTA_SR, float>(s, A, n);
   if (s->f32) return launch_sweep_pz_t<SOLVE_PCR, float>(s, A, n);
   if (s->delta) return launch_sweep_pz_t<SOLVE_DELTA, float>(s, A, n);
   if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_pz_t<SOLVE_THOMAS, double>(s, A, n);
@@ -257,6 +265,7 @@ int launch_sweep_pz(ixh_state *s, const SweepArgs &A, int n) {
 
 template <int EQ, int DIR>
 int launch_sweep(ixh_state *s, const SweepArgs &A, int n) {
+  if (s->sr) return launch_sweep_t<EQ, DIR, SOLVE_DELTA_SR, float>(s, A, n);
   if (s->f32) return launch_sweep_t<EQ, DIR, SOLVE_PCR, float>(s, A, n);
   if (s->delta) return launch_sweep_t<EQ, DIR, SOLVE_DELTA, float>(s, A, n);
   if (s->prm.solver == IXH_SOLVER_THOMAS) return launch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(s, A, n);
@@ -567,6 +576,7 @@ int prep_correct(ixh_state *s, CorrArgs &A, bool &active) {
   A.v = s->vb[s->v_cur];
   A.uT = s->ubT[s->u_cur];
   A.vT = s->vbT[s->v_cur];
+  A.sr = s->sr ? 1 : 0;
   return 0;
 }
 
@@ -694,6 +704,7 @@ int launch_multi_t(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn)
 
 template <int EQ0, int DIR0, int EQ1, int DIR1>
 int launch_multi(ixh_state *s, const MultiArgs &M, int nwg, int ept, int maxn) {
+  if (s->sr) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_DELTA_SR, float>(s, M, nwg, ept, maxn);
   if (s->f32) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_PCR, float>(s, M, nwg, ept, maxn);
   if (s->delta) return launch_multi_t<EQ0, DIR0, EQ1, DIR1, SOLVE_DELTA, float>(s, M, nwg, ept, maxn);
   if (s->prm.solver == IXH_SOLVER_THOMAS)
@@ -968,7 +979,8 @@ int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) 
   s->Q = ((nj + 1) + 15) / 16 * 16;  // so do the columns of the transposed copies
   s->fsize = std::max((size_t)s->P * (nj + 1), (size_t)s->Q * (mi + 1));
   s->prm = *params;
-  s->f32 = params->precision == IXH_FP32_MIXED;
+  s->f32 = params->precision == IXH_FP32_MIXED || params->precision == IXH_FP32_STOCHASTIC;
+  s->sr = params->precision == IXH_FP32_STOCHASTIC;
   s->delta = params->precision == IXH_FP32_REFINED;
   int rc = 0;
   auto bail = [&](int code) {
@@ -1607,7 +1619,7 @@ struct BatchStep {
 struct ixh_batch {
   std::vector<ixh_state *> st;
   int n = 0, solver = 0;
-  bool f32 = false, delta = false;
+  bool f32 = false, delta = false, sr = false;
   int ept[8] = {};
   // one 64-thread workgroup per line (PCR, every line at most 320 nodes):
   // four times as many lines resident per CU as with 256 threads
@@ -1771,6 +1783,7 @@ template <int EQ, int DIR>
 int batch_sweep(ixh_batch *b, int ph, const SweepArgs *D, hipStream_t st, const ResArgs *re = nullptr) {
   if (b->nwg[ph] == 0 && !re) return 0;
   D += (size_t)ph * b->n;
+  if (b->sr) return batch_sweep_t<EQ, DIR, SOLVE_DELTA_SR, float>(b, ph, D, st, re);
   if (b->f32) return batch_sweep_t<EQ, DIR, SOLVE_PCR, float>(b, ph, D, st, re);
   if (b->delta) return batch_sweep_t<EQ, DIR, SOLVE_DELTA, float>(b, ph, D, st, re);
   if (b->solver == IXH_SOLVER_THOMAS) return batch_sweep_t<EQ, DIR, SOLVE_THOMAS, double>(b, ph, D, st, re);
@@ -1920,6 +1933,7 @@ int ixh_batch_create(ixh_state *const *states, int32_t n, ixh_batch **out) {
   b->solver = solver;
   b->f32 = states[0]->f32;
   b->delta = states[0]->delta;
+  b->sr = states[0]->sr;
   int longest = 1;
   for (int ph = 0; ph < 8; ++ph) {
     int m = 1;
@@ -2215,7 +2229,7 @@ extern "C" {
 
 int ixh_adjoint_record(ixh_state *s, int32_t nsteps, int32_t every) {
   if (!s || nsteps < 1 || every < 1) return fail(IXH_ERR_ARG, "adjoint: bad arguments");
-  if (s->f32 || s->delta) return fail(IXH_ERR_UNSUPP, "adjoint: the FP64 path only");
+  if (s->f32 || s->delta || s->sr) return fail(IXH_ERR_UNSUPP, "adjoint: the FP64 path only");
   if (s->prm.iter_ecuaci_max >= 0 || s->prm.iter_simple_max >= 0)
     return fail(IXH_ERR_UNSUPP, "adjoint: single-pass loops only (iter_ecuaci_max < 0, iter_simple_max < 0)");
   for (int v = 0; v < 3; ++v) CHK(check_bcs(s, v));

@@ -30,8 +30,15 @@ enum Dir { DIR_Y = 0, DIR_X = 1 };
 // THOMAS: upstream's Thomas order with upstream-order assembly (bit-faithful);
 // PCR_EXACT: PCR with upstream-order assembly (isolates the solver rounding);
 // DELTA: the FP32-refined sweeps -- FP64 fields, the line's FP64 residual
-// against FP32 coefficients, and the FP32 PCR solve of the increment
-enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1, SOLVE_PCR_EXACT = 2, SOLVE_DELTA = 3 };
+// against FP32 coefficients, and the FP32 PCR solve of the increment;
+// DELTA_SR: the same increment sweeps over FP32 fields, each new value
+// phi + delta (formed in FP64) stored with stochastic rounding
+enum Solver { SOLVE_PCR = 0, SOLVE_THOMAS = 1, SOLVE_PCR_EXACT = 2, SOLVE_DELTA = 3, SOLVE_DELTA_SR = 4 };
+
+// the increment (residual-correction) sweeps
+constexpr bool delta_solver(int s) { return s == SOLVE_DELTA || s == SOLVE_DELTA_SR; }
+// solvers run by the wave-partitioned PCR, which also fits one-wave lines
+constexpr bool wave_solver(int s) { return s == SOLVE_PCR || delta_solver(s); }
 
 // storage type of the solved fields for a sweep of real type R
 template <int SOLVER, typename R>
@@ -42,6 +49,51 @@ template <typename R>
 struct StoreOf<SOLVE_DELTA, R> {
   using T = double;
 };
+
+// ------------------------------------------------------ stochastic rounding
+// Murmur3's 32-bit finaliser (Appleby, MurmurHash3, public domain): every
+// input bit reaches every output bit.
+__device__ __forceinline__ uint32_t fmix32(uint32_t h) {
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return h;
+}
+
+// d rounded to FP32 stochastically: to the FP32 neighbour above |d| with
+// probability (|d| - lo) / (hi - lo), else to the one below, so the rounding
+// is unbiased (Connolly, Higham & Mary, SIAM J. Sci. Comput. 43 (2021)
+// A566-A585).  FP32 keeps 23 of FP64's 52 fraction bits: adding a uniform
+// 29-bit integer to the magnitude's bit pattern carries into the kept bits
+// with exactly that probability, and truncation then picks the neighbour.
+// The 29 bits come from a hash of d's own bits and of key (node and stream),
+// so a run is deterministic, a cavity gives the same bits alone and in a
+// batch, and a value already representable in FP32 is returned unchanged.
+// Round-to-nearest would drop every increment below half an FP32 spacing:
+// near a steady state the per-step changes are that small, and the
+// iteration would stop short of it.
+__device__ __forceinline__ float sr_f32(double d, uint32_t key) {
+  const uint64_t b = (uint64_t)__double_as_longlong(d);
+  const uint32_t h = fmix32(key ^ fmix32((uint32_t)b ^ fmix32((uint32_t)(b >> 32))));
+  return __double2float_rz(__longlong_as_double((long long)(b + (uint64_t)(h >> 3))));
+}
+
+// key of node (ii, jj) in rounding stream s (one stream per field and writer)
+__device__ __forceinline__ uint32_t sr_key(int ii, int jj, uint32_t s) {
+  return (uint32_t)ii * 0x9e3779b1u ^ ((uint32_t)jj + (s << 20)) * 0x7f4a7c15u;
+}
+
+// a value formed in FP64 stored as S: stochastic rounding where sr is set
+// and S is FP32, a plain conversion otherwise
+template <typename S>
+__device__ __forceinline__ S store_as(double d, bool sr, uint32_t key) {
+  if constexpr (sizeof(S) == 4)
+    return sr ? sr_f32(d, key) : (float)d;
+  else
+    return d;
+}
 
 // Global-memory pointers.  A pointer the kernel reads from memory (the
 // argument tables of the batch kernels) is generic to the compiler, and a
@@ -536,25 +588,31 @@ __device__ __forceinline__ R fast_rhs(const SweepArgs &A, int ii, int jj, R head
 // where phi = phi_ant, so the iteration's fixed point is the steady state of
 // the FP64 fields for these neighbour coefficients; the increments only need
 // FP32 accuracy.
-template <int EQ, bool TR, typename R>
+// S: stored type of the fields (double, or float in the stochastic-rounding
+// sweeps, whose FP32 values enter the FP64 arithmetic exactly).
+template <int EQ, bool TR, typename R, typename S = double>
 __device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, const CoefT<R> &c) {
   const int P = A.P;
-  const double *phi = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
-  const double *ant = EQ == EQ_U ? A.uant : EQ == EQ_V ? A.vant : A.tant;
-  const double pP = AT(phi, ii, jj);
+  const S *phi = fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t);
+  const S *ant = fld<S>(EQ == EQ_U ? A.uant : EQ == EQ_V ? A.vant : A.tant);
+  const S *tf = fld<S>(A.t);
+  auto at = [&](const S *f, int i, int j) { return (double)AT(f, i, j); };
+  const double pP = at(phi, ii, jj);
   const double ar = EQ == EQ_U ? A.g.dxu[ii] * A.g.dyu[jj]
                   : EQ == EQ_V ? A.g.dxv[ii] * A.g.dyv[jj] : A.g.dxp[ii] * A.g.dyp[jj];
-  double r = -(double)c.ai * (AT(phi, ii - 1, jj) - pP) - (double)c.ad * (AT(phi, ii + 1, jj) - pP);
-  r = r - (double)c.bs * (AT(phi, ii, jj - 1) - pP) - (double)c.bn * (AT(phi, ii, jj + 1) - pP);
-  r = r + (double)c.td * (AT(ant, ii, jj) - pP);
+  double r = -(double)c.ai * (at(phi, ii - 1, jj) - pP) - (double)c.ad * (at(phi, ii + 1, jj) - pP);
+  r = r - (double)c.bs * (at(phi, ii, jj - 1) - pP) - (double)c.bn * (at(phi, ii, jj + 1) - pP);
+  r = r + (double)c.td * (at(ant, ii, jj) - pP);
   if (EQ == EQ_U) {
     const bool buoy = A.ph.ri_x != 0.0;
-    const double tint = A.g.fexp[ii] * ATZ(A.t, buoy, ii + 1, jj) + (1.0 - A.g.fexp[ii]) * ATZ(A.t, buoy, ii, jj);
+    const double tint = A.g.fexp[ii] * (double)ATZ(tf, buoy, ii + 1, jj) +
+                        (1.0 - A.g.fexp[ii]) * (double)ATZ(tf, buoy, ii, jj);
     r = r - ar * A.ph.ri_x * tint;
     r = r + (AT(A.p, ii, jj) - AT(A.p, ii + 1, jj)) * A.g.dyu[jj];
   } else if (EQ == EQ_V) {
     const bool buoy = A.ph.ri_y != 0.0;
-    const double tint = A.g.feyp[jj] * ATZ(A.t, buoy, ii, jj + 1) + (1.0 - A.g.feyp[jj]) * ATZ(A.t, buoy, ii, jj);
+    const double tint = A.g.feyp[jj] * (double)ATZ(tf, buoy, ii, jj + 1) +
+                        (1.0 - A.g.feyp[jj]) * (double)ATZ(tf, buoy, ii, jj);
     r = r - ar * A.ph.ri_y * tint;
     r = r + (AT(A.p, ii, jj) - AT(A.p, ii, jj + 1)) * A.g.dxv[ii];
   }
@@ -672,13 +730,25 @@ __device__ __forceinline__ void wpt_body(T *a, T *b, T *c, T *r, int n, const in
       C[i] = (in & (k != n - 1)) ? cv : T(0);
       R[i] = in ? rv : T(0);
     }
-    const bool hasL = A[0] != T(0);
-    // downward: L_i = spike on x[s-1].  The pivots B_i are final after this
+    // The spikes are kept normalised: L_i = A_0 l_i (coefficient of x[s-1]
+    // in row i) and Cp_i = C_{M-1} c_i (coefficient of x[e+1]), with l_0 = 1
+    // and c_{M-1} = 1 at the start of their passes.  The reduced rows below
+    // then combine rows s and e with factors of order one: multiplying by
+    // the couplings A_0 or C_{M-1} themselves, as a cross-multiplied form
+    // would, leaves a reduced pivot of their size, which for a vanishing
+    // coupling (strongly upwinded power-law faces at high Peclet numbers)
+    // underflows in FP32 and makes the whole line infinite.
+    const T a0 = A[0];
+    T cM = T(0);
+#pragma unroll
+    for (int i = 0; i < MMAX; ++i)
+      if (i == M - 1) cM = C[i];
+    // downward: l_i, spike on x[s-1].  The pivots B_i are final after this
     // pass, so their reciprocals serve the upward pass and the back
     // substitution too: only this pass has a reciprocal in its dependent
     // chain.
     T IB[MMAX];
-    L[0] = A[0];
+    L[0] = T(1);
     IB[0] = rcpn(B[0]);
 #pragma unroll
     for (int i = 1; i < MMAX; ++i) {
@@ -690,11 +760,11 @@ __device__ __forceinline__ void wpt_body(T *a, T *b, T *c, T *r, int n, const in
         IB[i] = rcpn(B[i]);
       }
     }
-    // upward: Cp_i = spike on x[e+1]
+    // upward: c_i, spike on x[e+1]
 #pragma unroll
     for (int i = MMAX - 1; i >= 0; --i) {
       if (i == M - 1) {
-        Cp[i] = C[i];
+        Cp[i] = T(1);
       } else if (i < M - 1) {
         const T g = C[i] * IB[i + 1];
         R[i] = R[i] - g * R[i + 1];
@@ -702,26 +772,19 @@ __device__ __forceinline__ void wpt_body(T *a, T *b, T *c, T *r, int n, const in
         Cp[i] = -g * Cp[i + 1];
       }
     }
-    T Ls = L[0], Bs = B[0], Cs = Cp[0], Rs = R[0];
-    T Le = T(0), Be = T(0), Ce = T(0), Re = T(0);
+    // rows s and e:  A_0 l x[s-1] + B x + C_{M-1} c x[e+1] = R, c_e = 1
+    const T Ls = L[0], Bs = B[0], Cs = Cp[0], Rs = R[0];
+    T Le = T(0), Be = T(0), Re = T(0);
 #pragma unroll
     for (int i = 0; i < MMAX; ++i)
       if (i == M - 1) {
-        Le = L[i]; Be = B[i]; Ce = Cp[i]; Re = R[i];
+        Le = L[i]; Be = B[i]; Re = R[i];
       }
-    const bool hasC = Ce != T(0);
-    // reduced rows: p = 0 -> z[2l] (couples 2l-1, 2l, 2l+1), p = 1 -> z[2l+1]
+    // reduced rows: p = 0 -> z[2l] = x[s] (row s - c_s row e: no x[e+1]),
+    // p = 1 -> z[2l+1] = x[e] (l_s row e - l_e row s: no x[s-1])
     T ra[2], rb[2], rc[2], rr[2];
-    if (hasC) {
-      ra[0] = Ce * Ls - Cs * Le; rb[0] = Ce * Bs; rc[0] = -(Cs * Be); rr[0] = Ce * Rs - Cs * Re;
-    } else {
-      ra[0] = Ls; rb[0] = Bs; rc[0] = T(0); rr[0] = Rs;
-    }
-    if (hasL) {
-      ra[1] = -(Le * Bs); rb[1] = Ls * Be; rc[1] = Ls * Ce - Le * Cs; rr[1] = Ls * Re - Le * Rs;
-    } else {
-      ra[1] = T(0); rb[1] = Be; rc[1] = Ce; rr[1] = Re;
-    }
+    ra[0] = a0 * (Ls - Cs * Le); rb[0] = Bs; rc[0] = -(Cs * Be); rr[0] = Rs - Cs * Re;
+    ra[1] = -(Le * Bs); rb[1] = Ls * Be; rc[1] = cM * (Ls - Le * Cs); rr[1] = Ls * Re - Le * Rs;
     // The 128 reduced rows: one cyclic-reduction step removes the odd
     // unknowns z[2l+1] from the even rows, PCR solves the 64 even rows (one
     // per lane), and the odd unknowns follow from their own rows.  Each row
@@ -765,7 +828,7 @@ __device__ __forceinline__ void wpt_body(T *a, T *b, T *c, T *r, int n, const in
         T x;
         if (i == 0) x = zs;
         else if (i == M - 1) x = ze;
-        else x = (R[i] - (l > 0 ? L[i] * xl : T(0)) - (l < 63 ? Cp[i] * xr : T(0))) * IB[i];
+        else x = (R[i] - (l > 0 ? L[i] * (a0 * xl) : T(0)) - (l < 63 ? Cp[i] * (cM * xr) : T(0))) * IB[i];
         if (k < n) r[k] = x;
       }
     }
@@ -944,13 +1007,15 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   const int lastl = Layout<EQ, DIR>::last_line(A.mi, A.nj);
   const bool active = LPW == 1 || line_s <= lastl;
   const int line = (LPW > 1 ? min(line_s, lastl) : line_s) + lane_zero();
-  static_assert(sizeof(R) == 8 || SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA,
-                "the FP32 sweeps use the PCR solver");
-  static_assert(SOLVER != SOLVE_DELTA || sizeof(R) == 4, "the increment sweeps are FP32");
+  static_assert(sizeof(R) == 8 || wave_solver(SOLVER), "the FP32 sweeps use the PCR solver");
+  static_assert(!delta_solver(SOLVER) || sizeof(R) == 4, "the increment sweeps are FP32");
   static_assert(BS == kBlock || BS == 64, "line workgroups of 256 threads or one wave");
   static_assert(BS == kBlock || SOLVER != SOLVE_THOMAS, "smaller line workgroups use the wave solver");
   using S = typename StoreOf<SOLVER, R>::T;      // stored type of the fields
-  constexpr bool DELTA = SOLVER == SOLVE_DELTA && EQ != EQ_P;
+  constexpr bool DELTA = delta_solver(SOLVER) && EQ != EQ_P;
+  // stochastic rounding of the stored results (FP32 increment sweeps, and
+  // the correction riding on their p' x-sweep)
+  constexpr bool SR = SOLVER == SOLVE_DELTA_SR;
   R *sa = reinterpret_cast<R *>(lds), *sb = sa + EPT * BS, *sc_ = sb + EPT * BS, *sr = sc_ + EPT * BS;
   double *red = lds + (4 * EPT * BS * (int)sizeof(R)) / 8;
   // line + 1 (CORR): after the first set and its reduction scratch
@@ -992,8 +1057,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   const S *src = fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : EQ == EQ_T ? A.t : A.pc);
   // the field at position q of this line (increment sweeps)
   auto along = [&](int q) -> double {
-    const double *f = EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t;
-    return DIR == DIR_X ? AT(f, q, line) : AT(f, line, q);
+    const S *f = fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t);
+    return DIR == DIR_X ? (double)AT(f, q, line) : (double)AT(f, line, q);
   };
 
   // the p' row at (ii, jj): stencil, right-hand side, mass residual b_o;
@@ -1065,14 +1130,14 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     R aw = R(0);
     if (EQ == EQ_P) {
       p_row(ii, jj, a0, a1, a2, rr, bval, false);
-    } else if constexpr (SOLVER == SOLVE_DELTA) {
+    } else if constexpr (delta_solver(SOLVER)) {
       R area;
       double pin;
-      const CoefT<R> c = fast_coef<EQ, TR, R, double>(A, ii, jj, area, pin);
+      const CoefT<R> c = fast_coef<EQ, TR, R, S>(A, ii, jj, area, pin);
       a0 = DIR == DIR_X ? c.ai : c.bs;
       a2 = DIR == DIR_X ? c.ad : c.bn;
       a1 = c.ac;
-      rr = delta_res<EQ, TR, R>(A, ii, jj, c);
+      rr = delta_res<EQ, TR, R, S>(A, ii, jj, c);
       aw = c.ac * R(A.ph.rel_vel);
       const double phic = along(pc);
       const bool pinned = pin == pin;
@@ -1249,13 +1314,14 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
     S x;
-    if constexpr (DELTA) x = keep_phi[e] + (double)sr[k];
+    if constexpr (DELTA) x = store_as<S>(keep_phi[e] + (double)sr[k], SR, sr_key(ii, jj, 2 * EQ + DIR));
     else x = sr[k];
     if (!PZ) AT(out, ii, jj) = x;
     if constexpr (LPW == 1) {
       AO(out2, ii, jj) = x;
     } else {
-      if constexpr (WIDE) tile_x[k] = x;
+      // the tile slot of x is this row's own (sr[k] held its increment)
+      if constexpr (WIDE || DELTA) tile_x[k] = x;
       if constexpr (MOM && DIR == DIR_X) tile_aw[k] = keep_aw[e];
     }
     if (NORM && pos >= 2 && pos <= mi) {
@@ -1327,7 +1393,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
         S un = S(keep_u[e]);
         if (pos >= 2 && pos <= mi - 1) {
           const double pcE = (double)S(sr[k + 1]);
-          un = S(keep_u[e] + dyu_line * (pcP - pcE) / keep_au[e]);
+          un = store_as<S>(keep_u[e] + dyu_line * (pcP - pcE) / keep_au[e], SR, sr_key(pos, line, 8));
         }
         AT(u1, pos, line) = un;
         AO(u1T, pos, line) = un;
@@ -1335,7 +1401,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       S vn = S(keep_v[e]);
       if (pos >= 2 && pos <= mi && line <= nj - 1) {
         const double pcN = (double)S(sr2[k]);
-        vn = S(keep_v[e] + keep_dxv[e] * (pcP - pcN) / keep_av[e]);
+        vn = store_as<S>(keep_v[e] + keep_dxv[e] * (pcP - pcN) / keep_av[e], SR, sr_key(pos, line, 9));
       }
       AT(v1, pos, line) = vn;
       AO(v1T, pos, line) = vn;
@@ -1429,6 +1495,7 @@ struct CorrArgs {
   double *u, *v, *p;          // row layout
   double *uT, *vT, *pT;       // transposed copies (pitch Q)
   const double *pc, *au, *av; // row layout
+  int sr;                     // FP32 u, v stored with stochastic rounding
 };
 
 constexpr int kTile = 32;
@@ -1456,9 +1523,11 @@ __device__ __forceinline__ void correct_tile(const CorrArgs &A, const int bx, co
     const double pcP = AT(pc, ic, jc);
     pn[m] = AT(A.p, ic, jc) + 0.6 * pcP;
     const double pcU = AT(pc, iu, jc);
-    un[m] = R(AT(u, iu, jc) + A.g.dyu[jc] * (pcU - (double)AT(pc, iu + 1, jc)) / (double)AT(au, iu, jc));
+    un[m] = store_as<R>(AT(u, iu, jc) + A.g.dyu[jc] * (pcU - (double)AT(pc, iu + 1, jc)) / (double)AT(au, iu, jc),
+                        A.sr != 0, sr_key(iu, jc, 8));
     const double pcV = AT(pc, ic, jv);
-    vn[m] = R(AT(v, ic, jv) + A.g.dxv[ic] * (pcV - (double)AT(pc, ic, jv + 1)) / (double)AT(av, ic, jv));
+    vn[m] = store_as<R>(AT(v, ic, jv) + A.g.dxv[ic] * (pcV - (double)AT(pc, ic, jv + 1)) / (double)AT(av, ic, jv),
+                        A.sr != 0, sr_key(ic, jv, 9));
   }
 #pragma unroll
   for (int m = 0; m < ROWS; ++m) {

@@ -35,8 +35,7 @@ constexpr int kWaveLine = 5 * 64;
 // nodes: one wave per line for sweeps only (no residual, no fold) on short
 // lines with the wave solver, else 256
 inline int multi_block(int solver, int maxn, bool residual, bool fold) {
-  const bool wave_solver = solver == SOLVE_PCR || solver == SOLVE_DELTA;
-  return wave_solver && !residual && !fold && maxn <= kWaveLine ? 64 : kBlock;
+  return wave_solver(solver) && !residual && !fold && maxn <= kWaveLine ? 64 : kBlock;
 }
 
 // one sweep of A.nlines lines of n nodes, one 256-thread workgroup per line
@@ -141,7 +140,7 @@ hipError_t launch_multi_k(const LaunchCtx &c, const MultiArgs &M, int nwg, int e
   const dim3 grid(nwg), block(kBlock);
   const int bs = multi_block(SOLVER, maxn, M.cnt[2] > 0, M.cnt[3] > 0);
   // sweeps only, every line short: one wave per line
-  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
+  if constexpr (wave_solver(SOLVER)) {
     if (bs == 64) {
       const dim3 wave(64);
       switch ((maxn + 63) / 64) {
@@ -172,7 +171,7 @@ template <class K, int SOLVER, typename... Args>
 hipError_t launch_batch_shaped(hipStream_t st, int nwg, int ept, int ept64, bool wave_lines, Args... args) {
   const dim3 grid(nwg), block(kBlock);
   const LaunchCtx c{st, nullptr, nullptr};
-  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
+  if constexpr (wave_solver(SOLVER)) {
     if (wave_lines) {
       const dim3 wave(64);
       switch (ept64) {
@@ -218,7 +217,7 @@ hipError_t launch_sweep_batch_k(hipStream_t st, const SweepArgs *D, const int2 *
 
 template <int EQ, int DIR, int SOLVER, typename R>
 hipError_t launch_sweep_group_batch_k(hipStream_t st, const SweepArgs *D, const int2 *wg, int ngroups, int ept64) {
-  if constexpr (SOLVER == SOLVE_PCR || SOLVER == SOLVE_DELTA) {
+  if constexpr (wave_solver(SOLVER)) {
     constexpr int G = kLinesPerGroup;
     constexpr bool PZ = EQ == EQ_P && DIR == DIR_Y;  // every batch step starts p' from zero
     const dim3 grid(ngroups), block(64 * G);

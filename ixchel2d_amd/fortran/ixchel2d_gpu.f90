@@ -140,6 +140,7 @@ program ixchel2d_gpu
   call get_environment_variable('IXCHEL2D_PRECISION', solver_env)
   if (trim(solver_env) == 'fp32_mixed') prm % precision = IXH_FP32_MIXED
   if (trim(solver_env) == 'fp32_refined') prm % precision = IXH_FP32_REFINED
+  if (trim(solver_env) == 'fp32_stochastic') prm % precision = IXH_FP32_STOCHASTIC
   call get_environment_variable('IXCHEL2D_LOOP', loop_env)
   por_fases = trim(loop_env) == 'phases'
   call ixh_check(ixh_create(grid, prm, estado), 'ixh_create')

@@ -17,7 +17,7 @@ module ixchel_hip
   integer(c_int), parameter :: IXH_VAR_U = 0, IXH_VAR_V = 1, IXH_VAR_T = 2
   integer(c_int), parameter :: IXH_DIR_Y = 0, IXH_DIR_X = 1
   integer(c_int), parameter :: IXH_SOLVER_PCR = 0, IXH_SOLVER_THOMAS = 1
-  integer(c_int), parameter :: IXH_FP64 = 0, IXH_FP32_MIXED = 1, IXH_FP32_REFINED = 2
+  integer(c_int), parameter :: IXH_FP64 = 0, IXH_FP32_MIXED = 1, IXH_FP32_REFINED = 2, IXH_FP32_STOCHASTIC = 3
 
   type, bind(C) :: ixh_grid_t
      integer(c_int32_t) :: mi, nj

@@ -11,7 +11,8 @@ as a captured HIP graph.
   - the hot-wall Nusselt number of every cavity with heat flow is within
     1e-4 (relative) of the same batch run on the FP64 path, on the
     FP32-mixed path (measured on MI355X after these 50 steps: largest 2.0e-5,
-    median 3.9e-6) and on the FP32-refined path (largest 6.4e-8);
+    median 3.9e-6), the FP32-refined path (largest 6.4e-8) and the
+    FP32-stochastic path;
   - the 16 smallest cavities (65^2) agree with the FP64 oracle to the FP32
     bound 2e-5 relative (pressure compared outside the bodies: inside, it is
     rounding noise amplified by the 1e51 penalty)."""
@@ -100,14 +101,16 @@ def test_batch256_nusselt_against_fp64(batch256):
     cases, cav, _, total = batch256
     ref = _run_batch(cases, "fp64", total)
     refined = _run_batch(cases, "fp32_refined", total)
-    for name, got, tol in (("fp32_mixed", cav, NU_TOL), ("fp32_refined", refined, NU_TOL)):
+    stochastic = _run_batch(cases, "fp32_stochastic", total)
+    for name, got, tol in (("fp32_mixed", cav, NU_TOL), ("fp32_refined", refined, NU_TOL),
+                           ("fp32_stochastic", stochastic, NU_TOL)):
         errs = _nu_errors(cases, got, ref)
         e = np.array([x for x, _ in errs])
         print(f"\n[batch256] {name}: hot-wall Nu vs FP64 over {len(e)} cavities: max {e.max():.2e} "
               f"(cavity {errs[-1][1]}), median {np.median(e):.2e}", flush=True)
         assert len(e) >= N // 2
         assert e.max() <= tol, (name, errs[-5:])
-    for r in ref + refined:
+    for r in ref + refined + stochastic:
         r.close()
 
 

@@ -307,6 +307,28 @@ def test_tridiagonal_against_dense_solve(solver):
                 assert np.array_equal(x[k], xo)
 
 
+def test_tridiagonal_vanishing_couplings():
+    """Sub- and super-diagonal entries of 1e-310 (subnormal) where the wave
+    solver's chunks meet (512 rows: 8 per lane), as strongly upwinded faces
+    give: the solve stays finite and equals a dense solve."""
+    from ixchel2d_amd.simple import tridiagonal
+    rng = np.random.default_rng(11)
+    n, nsys = 512, 4
+    a = rng.uniform(-1, 0, (nsys, n))
+    c = rng.uniform(-1, 0, (nsys, n))
+    b = 2.2 + rng.uniform(0, 1, (nsys, n))
+    r = rng.standard_normal((nsys, n))
+    a[:, 8::8] = -1e-310
+    c[:, 7::8] = -1e-310
+    a[1, 8::16] = 0.0  # exact zeros beside them
+    c[2, 15::16] = 0.0
+    x, _ = tridiagonal(a, b, c, r, solver="pcr")
+    assert np.all(np.isfinite(x))
+    for k in range(nsys):
+        A = np.diag(b[k]) + np.diag(a[k, 1:], -1) + np.diag(c[k, :-1], 1)
+        np.testing.assert_allclose(x[k], np.linalg.solve(A, r[k]), rtol=1e-12, atol=1e-13)
+
+
 def test_full_size_5000_iterations_against_upstream():
     """301x701, 5000 SIMPLE iterations (tanh mesh, Ra=1e5): PCR path vs the
     upstream CPU build, on the stored subsample, centre lines and norms, the
