@@ -14,6 +14,15 @@
 // two-level form spreads the arrivals over one counter per group of 32
 // workgroups before the root counter.
 //
+// A second line prices the synchronisation a persistent batch without grid
+// barriers would need: every workgroup that finishes a line group publishes
+// it to the next phase of its cavity (an agent-scope release, then an atomic
+// add on its cavity's counter), and every workgroup that starts one acquires
+// what other workgroups published (an agent-scope acquire).  Here every
+// workgroup writes its values, then does the arrival with the release
+// fence, the acquire fence, both or neither, and the launch time is compared
+// across the four forms.  Nothing waits on anything, so every form ends.
+//
 //   probe_barrier [reps] [per]
 #include <hip/hip_runtime.h>
 
@@ -108,6 +117,65 @@ __global__ void __launch_bounds__(kThreads) k_phases(unsigned *count, unsigned *
 
 __global__ void __launch_bounds__(kThreads) k_phase(double *sink, int per, int k) { phase_work(sink, per, k); }
 
+// mode bit 0: agent-scope release fence before the arrival, bit 1:
+// agent-scope acquire fence after it; the arrival is a relaxed atomic add on
+// the counter of "cavity" blockIdx.x % ncav (one 256-byte line each)
+constexpr int kCavities = 256;
+
+__global__ void __launch_bounds__(kThreads) k_publish(double *sink, int per, unsigned *counters, int mode) {
+  phase_work(sink, per, 0);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (mode & 1) __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(counters + 64 * (blockIdx.x % kCavities), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (mode & 2) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// times `launches` launches of k_publish of g workgroups in each mode and
+// prints the per-workgroup cost of the fences
+static void fence_probe(hipStream_t st, hipEvent_t e0, hipEvent_t e1, int per) {
+  const int launches = 20;
+  const int grids[] = {4096, 16384, 65536};
+  const size_t max_n = (size_t)65536 * kThreads * (size_t)per;
+  double *sink = nullptr;
+  unsigned *counters = nullptr;
+  CHECK(hipMalloc(&sink, max_n * sizeof(double)));
+  CHECK(hipMalloc(&counters, 64 * kCavities * sizeof(unsigned)));
+  CHECK(hipMemset(counters, 0, 64 * kCavities * sizeof(unsigned)));
+  printf("{\"probe\": \"per-workgroup publish: agent-scope release and acquire fences\", "
+         "\"launches\": %d, \"doubles_per_thread\": %d, \"cavity_counters\": %d, \"rows\": [",
+         launches, per, kCavities);
+  bool first = true;
+  for (int g : grids) {
+    float ms[4];
+    for (int mode = 0; mode < 4; ++mode) {
+      for (int w = 0; w < 3; ++w)
+        hipLaunchKernelGGL(k_publish, dim3(g), dim3(kThreads), 0, st, sink, per, counters, mode);
+      CHECK(hipGetLastError());
+      CHECK(hipEventRecord(e0, st));
+      for (int k = 0; k < launches; ++k)
+        hipLaunchKernelGGL(k_publish, dim3(g), dim3(kThreads), 0, st, sink, per, counters, mode);
+      CHECK(hipGetLastError());
+      CHECK(hipEventRecord(e1, st));
+      CHECK(hipEventSynchronize(e1));
+      CHECK(hipEventElapsedTime(&ms[mode], e0, e1));
+    }
+    const double us[4] = {ms[0] * 1e3 / launches, ms[1] * 1e3 / launches, ms[2] * 1e3 / launches,
+                          ms[3] * 1e3 / launches};
+    printf("%s{\"workgroups\": %d, \"us_per_launch\": {\"none\": %.2f, \"release\": %.2f, \"acquire\": %.2f, "
+           "\"both\": %.2f}, \"ns_per_workgroup_release\": %.2f, \"ns_per_workgroup_acquire\": %.2f, "
+           "\"ns_per_workgroup_both\": %.2f}",
+           first ? "" : ", ", g, us[0], us[1], us[2], us[3], (us[1] - us[0]) * 1e3 / g, (us[2] - us[0]) * 1e3 / g,
+           (us[3] - us[0]) * 1e3 / g);
+    first = false;
+  }
+  printf("]}\n");
+  CHECK(hipFree(sink));
+  CHECK(hipFree(counters));
+}
+
 int main(int argc, char **argv) {
   const int reps = argc > 1 ? atoi(argv[1]) : 2000;
   const int per = argc > 2 ? atoi(argv[2]) : 8;
@@ -180,6 +248,8 @@ int main(int argc, char **argv) {
     first = false;
   }
   printf("]}\n");
+  fflush(stdout);
+  if (per == 1) fence_probe(st, e0, e1, 1);
   CHECK(hipFree(count));
   CHECK(hipFree(gen));
   CHECK(hipFree(timeout));
