@@ -250,6 +250,8 @@ def batch_bench(args):
     dt = time.perf_counter() - t0
     cells = sum(c.mesh.mi * c.mesh.nj for c in cases)
     ms_per_step = dt * 1e3 / args.steps
+    import numpy as np
+    nonfinite = [c.case.name for c in cav if not all(np.all(np.isfinite(c.get(f))) for f in "uvtp")]
     # algorithmic bytes of one step: the per-kernel array counts over every
     # cavity; the v sweeps read T only with buoyancy; sources (cylinders) add
     # Sc, Sp to the u, v, T sweeps and to the residual
@@ -316,6 +318,7 @@ def batch_bench(args):
                      "iteration_alg_bytes": iter_bytes},
         "warm_untimed": {"steps": warm_steps, "seconds": round(warm_s, 3)},
         "nu_rel_err_vs_fp64": nu_err,
+        "nonfinite_cavities": nonfinite,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

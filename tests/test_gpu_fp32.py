@@ -9,7 +9,7 @@ a few times the errors measured on MI355X:
   - nc34 (1000 iterations, transient): 4e-5;
   - nc301x701 (5000 iterations, transient): fields 1.6e-4 (bound 4e-4),
     hot-wall Nu 4.5e-4 (bound 1e-3) -- above the 1e-4 Nu target, which the
-    FP32-refined path meets.
+    FP32-refined and FP32-stochastic paths meet (their own test files).
 Immersed bodies become rows phi = alpha Sc/-Sp + (1 - alpha) phi_old, the
 limit of upstream's under-relaxed 1e51 penalty row (which does not fit
 FP32): the body follows the FP64 path's values."""
@@ -67,7 +67,8 @@ def test_fp32_mixed_full_size_5000_iterations():
     # measured on MI355X: fields within 1.2e-4 .. 1.6e-4 of their maxima,
     # hot-wall Nu 4.5e-4 from FP64.  This path does not meet the 1e-4 Nu
     # bound (FP32 storage of fields that change by less than their FP32
-    # spacing per step); the FP32-refined path does and is tested to it
+    # spacing per step, rounded to nearest); the FP32-refined and
+    # FP32-stochastic paths do and are tested to it
     for f in "uvtp":
         a = g.get(f)
         assert relerr(K.subsample(a), gold[f + "_sub"]) <= 4e-4, f
