@@ -158,6 +158,29 @@ def test_against_upstream_fixture(name):
     np.testing.assert_allclose(h["maxbo"][:10], gold["history"][:10, 0], rtol=1e-4)
 
 
+@pytest.mark.parametrize("precision", ["fp32_mixed", "fp32_refined", "fp32_stochastic"])
+@pytest.mark.parametrize("name", ["nc18x14", "nc34", "nc200x24"])
+def test_fused_step_equals_kernel_by_kernel(name, precision):
+    """The fused 6-launch step (whose sweeps without a residual or fold run
+    one 64-thread wave per line) against the kernel-by-kernel step (256
+    threads per line): the same fields and norms bit for bit on every FP32
+    path.  nc200x24 has 201-node x-lines, 4 rows per lane."""
+    if name == "nc200x24":
+        case = C.natural_convection_case(200, 24, height=0.3, Ra=1e4, iters=12, dt=0.01, seed=2)
+    else:
+        case = K.CASES[name]()
+    case.params.itermax = 12
+    fused = _cavity(case, precision)
+    hf = fused.run(12)
+    case.params.iter_ecuaci_max, case.params.iter_simple_max = 3, 3
+    seq = _cavity(case, precision)
+    hs = seq.run(12)
+    for f in "uvtp":
+        assert np.array_equal(fused.get(f), seq.get(f)), f
+    for k in ("err_v", "err_p", "maxbo", "err_t", "residuo"):
+        assert np.array_equal(hf[k], hs[k]), k
+
+
 def test_deterministic():
     """The rounding bits come from the values and their nodes: two runs from
     the same start give the same bits."""
