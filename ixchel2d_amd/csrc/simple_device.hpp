@@ -1006,7 +1006,15 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
   const int tid = LPW > 1 ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   const int lastl = Layout<EQ, DIR>::last_line(A.mi, A.nj);
   const bool active = LPW == 1 || line_s <= lastl;
-  const int line = (LPW > 1 ? min(line_s, lastl) : line_s) + lane_zero();
+  // The line index is made a per-lane value, so that the line's uniform
+  // table lookups (geometry, face records, boundary values) go out as
+  // vector loads that can be waited for one by one.  The FP32-mixed group
+  // kernels of the batch keep it scalar: there the lookups go through the
+  // scalar cache instead of the vector memory path the sweeps saturate
+  // (7026 against 6694 Mcell-iters/s for the 256-cavity batch; FP64 4260
+  // against 4302 and FP32-stochastic 5360 against 5774 the other way).
+  constexpr bool kScalarLine = LPW > 1 && SOLVER == SOLVE_PCR && sizeof(R) == 4;
+  const int line = (LPW > 1 ? min(line_s, lastl) : line_s) + (kScalarLine ? 0 : lane_zero());
   static_assert(sizeof(R) == 8 || wave_solver(SOLVER), "the FP32 sweeps use the PCR solver");
   static_assert(!delta_solver(SOLVER) || sizeof(R) == 4, "the increment sweeps are FP32");
   static_assert(BS == kBlock || BS == 64, "line workgroups of 256 threads or one wave");
