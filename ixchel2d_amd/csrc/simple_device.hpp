@@ -590,14 +590,14 @@ __device__ __forceinline__ R fast_rhs(const SweepArgs &A, int ii, int jj, R head
 // FP32 accuracy.
 // S: stored type of the fields (double, or float in the stochastic-rounding
 // sweeps, whose FP32 values enter the FP64 arithmetic exactly).
+// pP: the field at (ii, jj), which the caller has loaded.
 template <int EQ, bool TR, typename R, typename S = double>
-__device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, const CoefT<R> &c) {
+__device__ __forceinline__ double delta_res(const SweepArgs &A, int ii, int jj, const CoefT<R> &c, double pP) {
   const int P = A.P;
   const S *phi = fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t);
   const S *ant = fld<S>(EQ == EQ_U ? A.uant : EQ == EQ_V ? A.vant : A.tant);
   const S *tf = fld<S>(A.t);
   auto at = [&](const S *f, int i, int j) { return (double)AT(f, i, j); };
-  const double pP = at(phi, ii, jj);
   const double ar = EQ == EQ_U ? A.g.dxu[ii] * A.g.dyu[jj]
                   : EQ == EQ_V ? A.g.dxv[ii] * A.g.dyv[jj] : A.g.dxp[ii] * A.g.dyp[jj];
   double r = -(double)c.ai * (at(phi, ii - 1, jj) - pP) - (double)c.ad * (at(phi, ii + 1, jj) - pP);
@@ -1110,7 +1110,8 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
 
   R keep_aw[EPT];
   S carry[EPT];
-  double keep_old[EPT], keep_b[EPT], keep_phi[EPT];
+  double keep_old[EPT], keep_b[EPT];
+  S keep_phi[EPT];  // the field at each row's own position (increment sweeps)
   // CORR: u, v, p, a_u, a_v and dx_v at this line's positions (pre-correction),
   // u and v of the carried boundary line
   double keep_u[EPT], keep_v[EPT], keep_p[EPT], keep_au[EPT], keep_av[EPT], keep_dxv[EPT];
@@ -1145,9 +1146,15 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
       a0 = DIR == DIR_X ? c.ai : c.bs;
       a2 = DIR == DIR_X ? c.ad : c.bn;
       a1 = c.ac;
-      rr = delta_res<EQ, TR, R, S>(A, ii, jj, c);
+      // the field at the row's own position, which serves the residual and
+      // the new value.  It is loaded at the unclamped position: an interior
+      // row's clamped position is its own, and the residual of an end row
+      // (assembled at its clamped neighbour) is replaced by its boundary row
+      keep_phi[e] = DIR == DIR_X ? AT(fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t), pr, line)
+                                 : AT(fld<S>(EQ == EQ_U ? A.u : EQ == EQ_V ? A.v : A.t), line, pr);
+      const double phic = (double)keep_phi[e];
+      rr = delta_res<EQ, TR, R, S>(A, ii, jj, c, phic);
       aw = c.ac * R(A.ph.rel_vel);
-      const double phic = along(pc);
       const bool pinned = pin == pin;
       a0 = pinned ? 0.0 : a0;
       a1 = pinned ? 1.0 : a1;
@@ -1209,9 +1216,9 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     rr = lo ? lbr : rr;
     if constexpr (DELTA) {
       // boundary rows in increments (a0 = 0 on the first row, a2 = 0 on the last)
-      keep_phi[e] = along(pr);
-      const double r_lo = rr - a1 * keep_phi[e] - a2 * phi_2;
-      const double r_hi = rr - a0 * phi_n1 - a1 * keep_phi[e];
+      const double ph = (double)keep_phi[e];
+      const double r_lo = rr - a1 * ph - a2 * phi_2;
+      const double r_hi = rr - a0 * phi_n1 - a1 * ph;
       rr = hi_row ? r_hi : rr;
       rr = lo ? r_lo : rr;
     }
@@ -1322,7 +1329,7 @@ __device__ __forceinline__ void sweep_line(const SweepArgs &A, const int line_s,
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
     S x;
-    if constexpr (DELTA) x = store_as<S>(keep_phi[e] + (double)sr[k], SR, sr_key(ii, jj, 2 * EQ + DIR));
+    if constexpr (DELTA) x = store_as<S>((double)keep_phi[e] + (double)sr[k], SR, sr_key(ii, jj, 2 * EQ + DIR));
     else x = sr[k];
     if (!PZ) AT(out, ii, jj) = x;
     if constexpr (LPW == 1) {
