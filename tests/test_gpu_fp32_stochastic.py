@@ -181,6 +181,25 @@ def test_fused_step_equals_kernel_by_kernel(name, precision):
         assert np.array_equal(hf[k], hs[k]), k
 
 
+@pytest.mark.parametrize("precision", ["fp32_mixed", "fp32_stochastic"])
+@pytest.mark.parametrize("mi,nj", [(20, 1100), (1100, 20), (700, 30)])
+def test_long_lines(precision, mi, nj):
+    """Lines past 1024 nodes (the LDS PCR solver in FP32) and 701-node
+    x-lines (the T and u x-sweeps at 256 threads): 30 fused iterations
+    against the FP64 path."""
+    case = C.natural_convection_case(mi, nj, height=nj / mi, Ra=1e4, iters=30, dt=0.05, seed=5)
+    g = _cavity(case, precision)
+    g.run(30)
+    g64 = _cavity(case, "fp64")
+    g64.run(30)
+    for f in "uvtp":
+        a = g.get(f)
+        assert np.all(np.isfinite(a)), f
+        e = relerr(a, g64.get(f))
+        print(f"[measure] {precision} {mi}x{nj} {f} {e:.2e}", flush=True)
+        assert e <= 1e-4, f
+
+
 def test_deterministic():
     """The rounding bits come from the values and their nodes: two runs from
     the same start give the same bits."""
