@@ -12,11 +12,11 @@
 // input w of the sweep is  wbar += -mu dR/dw : the right-hand-side stencil
 // (neighbours, (1-alpha) a_P phi_P, phi_ant, the pressure difference, the
 // buoyancy term, b_o) and the coefficients, whose power-law form depends on
-// the interpolated face velocities.  k_adj2_rows forms, per row, the adjoint
-// of the four neighbour coefficients (through a_P = -sum/alpha) times their
-// derivative with respect to their face velocity; k_adj2_gather collects
-// everything per node, so each node is written by one thread (deterministic,
-// no atomics).
+// the interpolated face velocities.  After its line's solve, k_adj2_solve
+// forms, per row, the adjoint of the four neighbour coefficients (through
+// a_P = -sum/alpha) times their derivative with respect to their face
+// velocity (adj2_row); k_adj2_gather collects everything per node, so each
+// node is written by one thread (deterministic, no atomics).
 #pragma once
 #include "simple_device.hpp"
 
@@ -122,10 +122,64 @@ struct Adj2Args {
   double *g_lo, *g_hi;  // (T) adjoints of the boundary values (+=, per node)
 };
 
+// workgroup of the per-node kernels: one wave, so that the 17k nodes of a
+// 129^2 cavity spread over every CU (their loads form long dependent chains)
+constexpr int kNodeBlock = 64;
+
 template <int EQ, int DIR>
 __device__ __forceinline__ int adj_n(int mi, int nj) { return Layout<EQ, DIR>::n(mi, nj); }
 
-// mu = A^-T ybar on every solved line (lines 2 .. last of the sweep)
+// per assembled row (ii, jj) with mu there: the adjoints of the four
+// neighbour coefficients, times their face-velocity derivative (U, V, T) or
+// as they are (p')
+template <int EQ, int DIR>
+__device__ __forceinline__ void adj2_row(const Adj2Args &G, int ii, int jj, double mu) {
+  constexpr bool TR = false;
+  const SweepArgs &A = G.S;
+  const int P = A.P;
+  const double *x = G.x, *phi = G.phi;
+  const double xP = AT(x, ii, jj);
+  const double fP = phi ? AT(phi, ii, jj) : 0.0;
+  // matrix neighbours (along the sweep) read x, right-hand-side neighbours phi
+  double xW, xE, xS, xN, fW, fE, fS, fN;
+  if (DIR == DIR_X) {
+    xW = AT(x, ii - 1, jj); xE = AT(x, ii + 1, jj);
+    fS = phi ? AT(phi, ii, jj - 1) : 0.0; fN = phi ? AT(phi, ii, jj + 1) : 0.0;
+    xS = xN = fW = fE = 0.0;
+  } else {
+    xS = AT(x, ii, jj - 1); xN = AT(x, ii, jj + 1);
+    fW = phi ? AT(phi, ii - 1, jj) : 0.0; fE = phi ? AT(phi, ii + 1, jj) : 0.0;
+    xW = xE = fS = fN = 0.0;
+  }
+  const double alpha = EQ == EQ_P ? A.ph.rel_pres : EQ == EQ_T ? A.ph.rel_ener : A.ph.rel_vel;
+  double jac = -mu * (xP - (1.0 - alpha) * fP);  // dJ/d a_P
+  if (G.acb) jac = jac + alpha * AT(G.acb, ii, jj);
+  // a_P = (-sum a_nb ...) / alpha
+  const double dsum = EQ == EQ_P ? -jac / alpha
+                                 : -jac * (EQ == EQ_T ? A.fg.inv_rel_ener : A.fg.inv_rel_vel);
+  double jai, jad, jbs, jbn;
+  if (DIR == DIR_X) {
+    jai = -mu * xW; jad = -mu * xE; jbs = -mu * fS; jbn = -mu * fN;
+  } else {
+    jai = -mu * fW; jad = -mu * fE; jbs = -mu * xS; jbn = -mu * xN;
+  }
+  jai = jai + dsum; jad = jad + dsum; jbs = jbs + dsum; jbn = jbn + dsum;
+  if (EQ == EQ_P) {
+    AT(G.fa[0], ii, jj) = jai;
+    AT(G.fa[1], ii, jj) = jad;
+    AT(G.fa[2], ii, jj) = jbs;
+    AT(G.fa[3], ii, jj) = jbn;
+  } else {
+    const CoefD c = coef_d<EQ == EQ_P ? EQ_T : EQ>(A, ii, jj);
+    AT(G.fa[0], ii, jj) = jai * c.dai;
+    AT(G.fa[1], ii, jj) = jad * c.dad;
+    AT(G.fa[2], ii, jj) = jbs * c.dbs;
+    AT(G.fa[3], ii, jj) = jbn * c.dbn;
+  }
+}
+
+// mu = A^-T ybar on every solved line (lines 2 .. last of the sweep), and
+// the row adjoints of the line's assembled rows
 template <int EQ, int DIR, int EPT>
 __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
   __shared__ double sa[EPT * kBlock], sb[EPT * kBlock], sc_[EPT * kBlock], sr[EPT * kBlock];
@@ -188,11 +242,15 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
   }
   __syncthreads();
   solve_line<SOLVE_PCR, EPT>(sa, sb, sc_, sr, n);
+  // mu, and the row adjoints of the line's assembled rows (positions 2 ..
+  // hi of lines 2 .. last: the rows the sweep assembles), which read mu at
+  // their own row only
   for (int k = threadIdx.x; k < n; k += kBlock) {
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
     AT(G.mu, ii, jj) = sr[k];
+    if (pos >= 2 && pos <= hi) adj2_row<EQ, DIR>(G, ii, jj, sr[k]);
   }
   if (EQ == EQ_T && threadIdx.x == 0) {
     G.g_lo[line] = G.g_lo[line] + sr[0];
@@ -200,69 +258,15 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
   }
 }
 
-// per assembled row: the adjoints of the four neighbour coefficients, times
-// their face-velocity derivative (U, V, T) or as they are (p')
-template <int EQ, int DIR>
-__global__ void __launch_bounds__(kBlock) k_adj2_rows(Adj2Args G) {
-  constexpr bool TR = false;
-  const SweepArgs &A = G.S;
-  const int P = A.P;
-  const int mi = A.mi, nj = A.nj;
-  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
-  if (idx >= (mi + 1) * (nj + 1)) return;
-  const int ii = 1 + idx % (mi + 1), jj = 1 + idx / (mi + 1);
-  if (!adj_row<EQ == EQ_P ? EQ_T : EQ>(ii, jj, mi, nj)) return;
-  const double mu = AT(G.mu, ii, jj);
-  const double *x = G.x, *phi = G.phi;
-  const double xP = AT(x, ii, jj);
-  const double fP = phi ? AT(phi, ii, jj) : 0.0;
-  // matrix neighbours (along the sweep) read x, right-hand-side neighbours phi
-  double xW, xE, xS, xN, fW, fE, fS, fN;
-  if (DIR == DIR_X) {
-    xW = AT(x, ii - 1, jj); xE = AT(x, ii + 1, jj);
-    fS = phi ? AT(phi, ii, jj - 1) : 0.0; fN = phi ? AT(phi, ii, jj + 1) : 0.0;
-    xS = xN = fW = fE = 0.0;
-  } else {
-    xS = AT(x, ii, jj - 1); xN = AT(x, ii, jj + 1);
-    fW = phi ? AT(phi, ii - 1, jj) : 0.0; fE = phi ? AT(phi, ii + 1, jj) : 0.0;
-    xW = xE = fS = fN = 0.0;
-  }
-  const double alpha = EQ == EQ_P ? A.ph.rel_pres : EQ == EQ_T ? A.ph.rel_ener : A.ph.rel_vel;
-  double jac = -mu * (xP - (1.0 - alpha) * fP);  // dJ/d a_P
-  if (G.acb) jac = jac + alpha * AT(G.acb, ii, jj);
-  // a_P = (-sum a_nb ...) / alpha
-  const double dsum = EQ == EQ_P ? -jac / alpha
-                                 : -jac * (EQ == EQ_T ? A.fg.inv_rel_ener : A.fg.inv_rel_vel);
-  double jai, jad, jbs, jbn;
-  if (DIR == DIR_X) {
-    jai = -mu * xW; jad = -mu * xE; jbs = -mu * fS; jbn = -mu * fN;
-  } else {
-    jai = -mu * fW; jad = -mu * fE; jbs = -mu * xS; jbn = -mu * xN;
-  }
-  jai = jai + dsum; jad = jad + dsum; jbs = jbs + dsum; jbn = jbn + dsum;
-  if (EQ == EQ_P) {
-    AT(G.fa[0], ii, jj) = jai;
-    AT(G.fa[1], ii, jj) = jad;
-    AT(G.fa[2], ii, jj) = jbs;
-    AT(G.fa[3], ii, jj) = jbn;
-  } else {
-    const CoefD c = coef_d<EQ == EQ_P ? EQ_T : EQ>(A, ii, jj);
-    AT(G.fa[0], ii, jj) = jai * c.dai;
-    AT(G.fa[1], ii, jj) = jad * c.dad;
-    AT(G.fa[2], ii, jj) = jbs * c.dbs;
-    AT(G.fa[3], ii, jj) = jbn * c.dbn;
-  }
-}
-
 // per node: adjoint of the sweep's input and of everything else the rows read
 template <int EQ, int DIR>
-__global__ void __launch_bounds__(kBlock) k_adj2_gather(Adj2Args G) {
+__global__ void __launch_bounds__(kNodeBlock) k_adj2_gather(Adj2Args G) {
   constexpr bool TR = false;
   constexpr int RQ = EQ == EQ_P ? EQ_T : EQ;  // row set of p' equals that of T
   const SweepArgs &A = G.S;
   const int P = A.P;
   const int mi = A.mi, nj = A.nj;
-  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
+  const int idx = (int)(blockIdx.x * kNodeBlock + threadIdx.x);
   if (idx >= (mi + 1) * (nj + 1)) return;
   const int I = 1 + idx % (mi + 1), J = 1 + idx / (mi + 1);
   const Geo &g = A.g;
@@ -401,11 +405,11 @@ struct AdjCorrArgs {
   double *aub, *avb;                // (+=)
 };
 
-__global__ void __launch_bounds__(kBlock) k_adj2_correct(AdjCorrArgs C) {
+__global__ void __launch_bounds__(kNodeBlock) k_adj2_correct(AdjCorrArgs C) {
   constexpr bool TR = false;
   const int P = C.P;
   const int mi = C.mi, nj = C.nj;
-  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
+  const int idx = (int)(blockIdx.x * kNodeBlock + threadIdx.x);
   if (idx >= (mi + 1) * (nj + 1)) return;
   const int I = 1 + idx % (mi + 1), J = 1 + idx / (mi + 1);
   const Geo &g = C.g;

@@ -2401,10 +2401,8 @@ int adj2_sweep(ixh_state *s, Adj2Args &G, int nlines, int n) {
   }
   HIPCHK(hipGetLastError());
   const int npts = (s->mi + 1) * (s->nj + 1);
-  const dim3 pg((npts + kBlock - 1) / kBlock);
-  hipLaunchKernelGGL((k_adj2_rows<EQ, DIR>), pg, block, 0, s->stream, G);
-  HIPCHK(hipGetLastError());
-  hipLaunchKernelGGL((k_adj2_gather<EQ, DIR>), pg, block, 0, s->stream, G);
+  const dim3 pg((npts + kNodeBlock - 1) / kNodeBlock);
+  hipLaunchKernelGGL((k_adj2_gather<EQ, DIR>), pg, dim3(kNodeBlock), 0, s->stream, G);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2448,7 +2446,7 @@ int adj2_step(ixh_state *s, double *const *tp, double *const *W) {
     C.ub3 = W[W_UB3]; C.vb3 = W[W_VB3]; C.pbn = W[W_PB];
     C.ub2 = W[W_UB2]; C.vb2 = W[W_VB2]; C.pcb = W[W_PCB2]; C.aub = W[W_AUB]; C.avb = W[W_AVB];
     const int npts = (mi + 1) * (nj + 1);
-    hipLaunchKernelGGL(k_adj2_correct, dim3((npts + kBlock - 1) / kBlock), dim3(kBlock), 0, s->stream, C);
+    hipLaunchKernelGGL(k_adj2_correct, dim3((npts + kNodeBlock - 1) / kNodeBlock), dim3(kNodeBlock), 0, s->stream, C);
     HIPCHK(hipGetLastError());
   }
   // p' x-sweep, then p' y-sweep (input zero)
