@@ -15,7 +15,7 @@
 // the interpolated face velocities.  After its line's solve, k_adj2_solve
 // forms, per row, the adjoint of the four neighbour coefficients (through
 // a_P = -sum/alpha) times their derivative with respect to their face
-// velocity (adj2_row); k_adj2_gather collects everything per node, so each
+// velocity (adj2_row_out); k_adj2_gather collects everything per node, so each
 // node is written by one thread (deterministic, no atomics).
 #pragma once
 #include "simple_device.hpp"
@@ -129,52 +129,58 @@ constexpr int kNodeBlock = 64;
 template <int EQ, int DIR>
 __device__ __forceinline__ int adj_n(int mi, int nj) { return Layout<EQ, DIR>::n(mi, nj); }
 
-// per assembled row (ii, jj) with mu there: the adjoints of the four
+// What the row adjoints of row (ii, jj) read besides mu: the row's own x
+// and phi, its four neighbour values in the order ai, ad, bs, bn (matrix
+// neighbours along the sweep from x, right-hand-side neighbours from phi),
+// the adjoint of its a_u / a_v store, and the derivatives of ai, ad, bs, bn
+// with respect to their face velocities (U, V, T; p' uses none).
+struct AdjRowIn {
+  double xP, fP, nb[4], acb, d[4];
+};
+
+template <int EQ, int DIR>
+__device__ __forceinline__ AdjRowIn adj2_row_in(const Adj2Args &G, int ii, int jj) {
+  constexpr bool TR = false;
+  const int P = G.S.P;
+  const double *x = G.x, *phi = G.phi;
+  AdjRowIn r;
+  r.xP = AT(x, ii, jj);
+  r.fP = phi ? AT(phi, ii, jj) : 0.0;
+  if (DIR == DIR_X) {
+    r.nb[0] = AT(x, ii - 1, jj); r.nb[1] = AT(x, ii + 1, jj);
+    r.nb[2] = phi ? AT(phi, ii, jj - 1) : 0.0; r.nb[3] = phi ? AT(phi, ii, jj + 1) : 0.0;
+  } else {
+    r.nb[0] = phi ? AT(phi, ii - 1, jj) : 0.0; r.nb[1] = phi ? AT(phi, ii + 1, jj) : 0.0;
+    r.nb[2] = AT(x, ii, jj - 1); r.nb[3] = AT(x, ii, jj + 1);
+  }
+  r.acb = G.acb ? AT(G.acb, ii, jj) : 0.0;
+  r.d[0] = r.d[1] = r.d[2] = r.d[3] = 1.0;
+  return r;
+}
+
+__device__ __forceinline__ void set_derivs(AdjRowIn &r, const CoefD &c) {
+  r.d[0] = c.dai; r.d[1] = c.dad; r.d[2] = c.dbs; r.d[3] = c.dbn;
+}
+
+// the row adjoints of row (ii, jj) from mu there: the adjoints of the four
 // neighbour coefficients, times their face-velocity derivative (U, V, T) or
 // as they are (p')
 template <int EQ, int DIR>
-__device__ __forceinline__ void adj2_row(const Adj2Args &G, int ii, int jj, double mu) {
+__device__ __forceinline__ void adj2_row_out(const Adj2Args &G, int ii, int jj, const AdjRowIn &r,
+                                             double mu) {
   constexpr bool TR = false;
   const SweepArgs &A = G.S;
   const int P = A.P;
-  const double *x = G.x, *phi = G.phi;
-  const double xP = AT(x, ii, jj);
-  const double fP = phi ? AT(phi, ii, jj) : 0.0;
-  // matrix neighbours (along the sweep) read x, right-hand-side neighbours phi
-  double xW, xE, xS, xN, fW, fE, fS, fN;
-  if (DIR == DIR_X) {
-    xW = AT(x, ii - 1, jj); xE = AT(x, ii + 1, jj);
-    fS = phi ? AT(phi, ii, jj - 1) : 0.0; fN = phi ? AT(phi, ii, jj + 1) : 0.0;
-    xS = xN = fW = fE = 0.0;
-  } else {
-    xS = AT(x, ii, jj - 1); xN = AT(x, ii, jj + 1);
-    fW = phi ? AT(phi, ii - 1, jj) : 0.0; fE = phi ? AT(phi, ii + 1, jj) : 0.0;
-    xW = xE = fS = fN = 0.0;
-  }
   const double alpha = EQ == EQ_P ? A.ph.rel_pres : EQ == EQ_T ? A.ph.rel_ener : A.ph.rel_vel;
-  double jac = -mu * (xP - (1.0 - alpha) * fP);  // dJ/d a_P
-  if (G.acb) jac = jac + alpha * AT(G.acb, ii, jj);
+  double jac = -mu * (r.xP - (1.0 - alpha) * r.fP);  // dJ/d a_P
+  if (G.acb) jac = jac + alpha * r.acb;
   // a_P = (-sum a_nb ...) / alpha
   const double dsum = EQ == EQ_P ? -jac / alpha
                                  : -jac * (EQ == EQ_T ? A.fg.inv_rel_ener : A.fg.inv_rel_vel);
-  double jai, jad, jbs, jbn;
-  if (DIR == DIR_X) {
-    jai = -mu * xW; jad = -mu * xE; jbs = -mu * fS; jbn = -mu * fN;
-  } else {
-    jai = -mu * fW; jad = -mu * fE; jbs = -mu * xS; jbn = -mu * xN;
-  }
-  jai = jai + dsum; jad = jad + dsum; jbs = jbs + dsum; jbn = jbn + dsum;
-  if (EQ == EQ_P) {
-    AT(G.fa[0], ii, jj) = jai;
-    AT(G.fa[1], ii, jj) = jad;
-    AT(G.fa[2], ii, jj) = jbs;
-    AT(G.fa[3], ii, jj) = jbn;
-  } else {
-    const CoefD c = coef_d<EQ == EQ_P ? EQ_T : EQ>(A, ii, jj);
-    AT(G.fa[0], ii, jj) = jai * c.dai;
-    AT(G.fa[1], ii, jj) = jad * c.dad;
-    AT(G.fa[2], ii, jj) = jbs * c.dbs;
-    AT(G.fa[3], ii, jj) = jbn * c.dbn;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double jq = -mu * r.nb[q] + dsum;
+    AT(G.fa[q], ii, jj) = EQ == EQ_P ? jq : jq * r.d[q];
   }
 }
 
@@ -192,12 +198,20 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
   const int line = 2 + (int)blockIdx.x;
   const Side &lo_side = DIR == DIR_X ? A.bc[0] : A.bc[1];
   const Side &hi_side = DIR == DIR_X ? A.bc[2] : A.bc[3];
-  for (int k = threadIdx.x; k < n; k += kBlock) {
+  // with at most 2 rows per thread the row adjoints' reads go out with the
+  // assembly's, before the solve; with more they follow the solve
+  constexpr bool kEarly = EPT <= 2;
+  AdjRowIn rin[kEarly ? EPT : 1];
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    if (k >= n) continue;
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
     double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
     if (pos >= 2 && pos <= hi) {
+      if constexpr (kEarly) rin[e] = adj2_row_in<EQ, DIR>(G, ii, jj);
       if (EQ == EQ_P) {
         const PCoefT<double> c = p_coef<TR, double>(A, ii, jj);
         a0 = DIR == DIR_X ? c.ai : c.bs;
@@ -208,6 +222,7 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
         a0 = DIR == DIR_X ? c.ai : c.bs;
         a2 = DIR == DIR_X ? c.ad : c.bn;
         a1 = c.ac;
+        if constexpr (kEarly) set_derivs(rin[e], c);
       }
     } else if (EQ != EQ_P) {
       const bool high = pos != 1;
@@ -245,12 +260,24 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
   // mu, and the row adjoints of the line's assembled rows (positions 2 ..
   // hi of lines 2 .. last: the rows the sweep assembles), which read mu at
   // their own row only
-  for (int k = threadIdx.x; k < n; k += kBlock) {
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int k = threadIdx.x + e * kBlock;
+    if (k >= n) continue;
     const int pos = k + 1;
     const int ii = DIR == DIR_X ? pos : line;
     const int jj = DIR == DIR_X ? line : pos;
-    AT(G.mu, ii, jj) = sr[k];
-    if (pos >= 2 && pos <= hi) adj2_row<EQ, DIR>(G, ii, jj, sr[k]);
+    const double mu = sr[k];
+    AT(G.mu, ii, jj) = mu;
+    if (pos >= 2 && pos <= hi) {
+      if constexpr (kEarly) {
+        adj2_row_out<EQ, DIR>(G, ii, jj, rin[e], mu);
+      } else {
+        AdjRowIn r = adj2_row_in<EQ, DIR>(G, ii, jj);
+        if (EQ != EQ_P) set_derivs(r, coef_d<EQ == EQ_P ? EQ_T : EQ>(A, ii, jj));
+        adj2_row_out<EQ, DIR>(G, ii, jj, r, mu);
+      }
+    }
   }
   if (EQ == EQ_T && threadIdx.x == 0) {
     G.g_lo[line] = G.g_lo[line] + sr[0];
