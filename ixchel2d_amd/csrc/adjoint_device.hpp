@@ -95,6 +95,14 @@ __device__ __forceinline__ CoefD coef_d(const SweepArgs &A, int ii, int jj) {
   return c;
 }
 
+// area of the control volume of row (ii, jj) of equation EQ, as coef_d forms it
+template <int EQ>
+__device__ __forceinline__ double row_area(const Geo &g, int ii, int jj) {
+  if (EQ == EQ_U) return g.dxu[ii] * g.dyu[jj];
+  if (EQ == EQ_V) return g.dxv[ii] * g.dyv[jj];
+  return g.dxp[ii] * g.dyp[jj];
+}
+
 // rows a sweep of EQ assembles (both directions): ii in [2, ihi], jj in [2, jhi]
 template <int EQ>
 __device__ __forceinline__ bool adj_row(int ii, int jj, int mi, int nj) {
@@ -112,6 +120,8 @@ struct Adj2Args {
   const double *acb;    // adjoint of a_u / a_v written by the x-sweep (nullptr: none)
   double *mu;           // transposed solve
   double *fa[4];        // per-row adjoints: face velocities (U, V, T) or coefficients (p')
+  double *cf[3];        // per row, from the solve: a_P and the two coefficients across
+                        // the sweep (bs, bn for x-sweeps, ai, ad for y-sweeps)
   double *phib;         // adjoint of the input: written (=)
   double *velub, *velvb;  // adjoints of the coefficient velocities (+=); the input's own
                           // array when it is one of them
@@ -212,18 +222,26 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
     double a0 = 0.0, a1 = 1.0, a2 = 0.0, rr = 0.0;
     if (pos >= 2 && pos <= hi) {
       if constexpr (kEarly) rin[e] = adj2_row_in<EQ, DIR>(G, ii, jj);
+      double x0, x1;  // the coefficients across the sweep
       if (EQ == EQ_P) {
         const PCoefT<double> c = p_coef<TR, double>(A, ii, jj);
         a0 = DIR == DIR_X ? c.ai : c.bs;
         a2 = DIR == DIR_X ? c.ad : c.bn;
         a1 = c.ac;
+        x0 = DIR == DIR_X ? c.bs : c.ai;
+        x1 = DIR == DIR_X ? c.bn : c.ad;
       } else {
         const CoefD c = coef_d<EQ>(A, ii, jj);
         a0 = DIR == DIR_X ? c.ai : c.bs;
         a2 = DIR == DIR_X ? c.ad : c.bn;
         a1 = c.ac;
+        x0 = DIR == DIR_X ? c.bs : c.ai;
+        x1 = DIR == DIR_X ? c.bn : c.ad;
         if constexpr (kEarly) set_derivs(rin[e], c);
       }
+      AT(G.cf[0], ii, jj) = a1;
+      AT(G.cf[1], ii, jj) = x0;
+      AT(G.cf[2], ii, jj) = x1;
     } else if (EQ != EQ_P) {
       const bool high = pos != 1;
       (void)bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
@@ -285,7 +303,10 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
   }
 }
 
-// per node: adjoint of the sweep's input and of everything else the rows read
+// per node: adjoint of the sweep's input and of everything else the rows read.
+// The coefficients come from the solve kernel (cf); every value is read
+// without a branch (at the node itself where the row it belongs to does not
+// exist, and then not selected), so a node's loads go out together.
 template <int EQ, int DIR>
 __global__ void __launch_bounds__(kNodeBlock) k_adj2_gather(Adj2Args G) {
   constexpr bool TR = false;
@@ -299,111 +320,121 @@ __global__ void __launch_bounds__(kNodeBlock) k_adj2_gather(Adj2Args G) {
   const Geo &g = A.g;
   const double *mu = G.mu;
   auto row = [&](int i, int j) { return adj_row<RQ>(i, j, mi, nj); };
+  // arr(i, j) where `on`, else arr(I, J) (a valid address, value not used)
+  auto at = [&](const double *arr, bool on, int i, int j) { return AT(arr, on ? i : I, on ? j : J); };
   const double alpha = EQ == EQ_P ? A.ph.rel_pres : EQ == EQ_T ? A.ph.rel_ener : A.ph.rel_vel;
   // extent of the swept variable
   const int im = EQ == EQ_U ? mi : mi + 1;
   const int jm = EQ == EQ_V ? nj : nj + 1;
   const bool in_phi = I <= im && J <= jm;
+  const bool rP = row(I, J), rW = row(I - 1, J), rE = row(I + 1, J), rS = row(I, J - 1), rN = row(I, J + 1);
+  const double muP = AT(mu, I, J);
+  // old values of the accumulators this node adds to
+  const double ub_old = EQ != EQ_U && G.velub ? AT(G.velub, I, J) : 0.0;
+  const double vb_old = EQ != EQ_V && G.velvb ? AT(G.velvb, I, J) : 0.0;
+  const double antb_old = EQ != EQ_P && G.antb ? AT(G.antb, I, J) : 0.0;
+  const double pb_old = (EQ == EQ_U || EQ == EQ_V) && G.pb ? AT(G.pb, I, J) : 0.0;
+  const double tb_old = EQ == EQ_V && G.tb ? AT(G.tb, I, J) : 0.0;
 
   // ---- input of the sweep
-  double pb_in = 0.0;
-  if (in_phi && G.phi) {
+  double pb_in;
+  {
     const int line = DIR == DIR_X ? J : I;
     const int last = Layout<EQ, DIR>::last_line(mi, nj);
-    if (line == 1 || line == last + 1) pb_in = AT(G.ybar, I, J);  // carried line
-    if (row(I, J)) {
-      double ac;
-      if (EQ == EQ_P) ac = p_coef<TR, double>(A, I, J).ac;
-      else ac = coef_d<RQ>(A, I, J).ac;
-      pb_in = pb_in + (1.0 - alpha) * ac * AT(mu, I, J);
-    }
+    const double yb = AT(G.ybar, I, J);
+    const double ac = AT(G.cf[0], I, J);
+    pb_in = line == 1 || line == last + 1 ? yb : 0.0;  // carried line
+    pb_in = rP ? pb_in + (1.0 - alpha) * ac * muP : pb_in;
     if (DIR == DIR_X) {
-      if (row(I, J + 1)) {
-        const double bs = EQ == EQ_P ? p_coef<TR, double>(A, I, J + 1).bs : coef_d<RQ>(A, I, J + 1).bs;
-        pb_in = pb_in - bs * AT(mu, I, J + 1);
-      }
-      if (row(I, J - 1)) {
-        const double bn = EQ == EQ_P ? p_coef<TR, double>(A, I, J - 1).bn : coef_d<RQ>(A, I, J - 1).bn;
-        pb_in = pb_in - bn * AT(mu, I, J - 1);
-      }
+      const double bs = at(G.cf[1], rN, I, J + 1), muN = at(mu, rN, I, J + 1);
+      const double bn = at(G.cf[2], rS, I, J - 1), muS = at(mu, rS, I, J - 1);
+      pb_in = rN ? pb_in - bs * muN : pb_in;
+      pb_in = rS ? pb_in - bn * muS : pb_in;
     } else {
-      if (row(I + 1, J)) {
-        const double ai = EQ == EQ_P ? p_coef<TR, double>(A, I + 1, J).ai : coef_d<RQ>(A, I + 1, J).ai;
-        pb_in = pb_in - ai * AT(mu, I + 1, J);
-      }
-      if (row(I - 1, J)) {
-        const double ad = EQ == EQ_P ? p_coef<TR, double>(A, I - 1, J).ad : coef_d<RQ>(A, I - 1, J).ad;
-        pb_in = pb_in - ad * AT(mu, I - 1, J);
-      }
+      const double ai = at(G.cf[1], rE, I + 1, J), muE = at(mu, rE, I + 1, J);
+      const double ad = at(G.cf[2], rW, I - 1, J), muW = at(mu, rW, I - 1, J);
+      pb_in = rE ? pb_in - ai * muE : pb_in;
+      pb_in = rW ? pb_in - ad * muW : pb_in;
     }
+    if (!G.phi) pb_in = 0.0;
   }
 
   // ---- face velocities of the coefficients (U, V, T)
   double gu = 0.0, gv = 0.0;  // adjoints of u(I, J), v(I, J)
   double *const *fa = G.fa;
   if (EQ == EQ_U) {
-    if (row(I, J)) gu = gu + (1.0 - g.fexu[I]) * AT(fa[1], I, J) + g.fexu[I - 1] * AT(fa[0], I, J);
-    if (row(I - 1, J)) gu = gu + g.fexu[I - 1] * AT(fa[1], I - 1, J);
-    if (row(I + 1, J)) gu = gu + (1.0 - g.fexu[I]) * AT(fa[0], I + 1, J);
-    if (row(I, J)) gv = gv + (1.0 - g.fexp[I]) * AT(fa[3], I, J);
-    if (row(I - 1, J)) gv = gv + g.fexp[I - 1] * AT(fa[3], I - 1, J);
-    if (row(I, J + 1)) gv = gv + (1.0 - g.fexp[I]) * AT(fa[2], I, J + 1);
-    if (row(I - 1, J + 1)) gv = gv + g.fexp[I - 1] * AT(fa[2], I - 1, J + 1);
+    const bool rNW = row(I - 1, J + 1);
+    const double f0P = AT(fa[0], I, J), f1P = AT(fa[1], I, J), f3P = AT(fa[3], I, J);
+    const double f1W = at(fa[1], rW, I - 1, J), f0E = at(fa[0], rE, I + 1, J);
+    const double f3W = at(fa[3], rW, I - 1, J), f2N = at(fa[2], rN, I, J + 1);
+    const double f2NW = at(fa[2], rNW, I - 1, J + 1);
+    gu = rP ? gu + (1.0 - g.fexu[I]) * f1P + g.fexu[I - 1] * f0P : gu;
+    gu = rW ? gu + g.fexu[I - 1] * f1W : gu;
+    gu = rE ? gu + (1.0 - g.fexu[I]) * f0E : gu;
+    gv = rP ? gv + (1.0 - g.fexp[I]) * f3P : gv;
+    gv = rW ? gv + g.fexp[I - 1] * f3W : gv;
+    gv = rN ? gv + (1.0 - g.fexp[I]) * f2N : gv;
+    gv = rNW ? gv + g.fexp[I - 1] * f2NW : gv;
   } else if (EQ == EQ_V) {
-    if (row(I, J)) gu = gu + (1.0 - g.feyp[J]) * AT(fa[1], I, J);
-    if (row(I, J - 1)) gu = gu + g.feyp[J - 1] * AT(fa[1], I, J - 1);
-    if (row(I + 1, J)) gu = gu + (1.0 - g.feyp[J]) * AT(fa[0], I + 1, J);
-    if (row(I + 1, J - 1)) gu = gu + g.feyp[J - 1] * AT(fa[0], I + 1, J - 1);
-    if (row(I, J)) gv = gv + (1.0 - g.feyv[J]) * AT(fa[3], I, J) + g.feyv[J - 1] * AT(fa[2], I, J);
-    if (row(I, J - 1)) gv = gv + g.feyv[J - 1] * AT(fa[3], I, J - 1);
-    if (row(I, J + 1)) gv = gv + (1.0 - g.feyv[J]) * AT(fa[2], I, J + 1);
+    const bool rSE = row(I + 1, J - 1);
+    const double f1P = AT(fa[1], I, J), f2P = AT(fa[2], I, J), f3P = AT(fa[3], I, J);
+    const double f1S = at(fa[1], rS, I, J - 1), f0E = at(fa[0], rE, I + 1, J);
+    const double f0SE = at(fa[0], rSE, I + 1, J - 1);
+    const double f3S = at(fa[3], rS, I, J - 1), f2N = at(fa[2], rN, I, J + 1);
+    gu = rP ? gu + (1.0 - g.feyp[J]) * f1P : gu;
+    gu = rS ? gu + g.feyp[J - 1] * f1S : gu;
+    gu = rE ? gu + (1.0 - g.feyp[J]) * f0E : gu;
+    gu = rSE ? gu + g.feyp[J - 1] * f0SE : gu;
+    gv = rP ? gv + (1.0 - g.feyv[J]) * f3P + g.feyv[J - 1] * f2P : gv;
+    gv = rS ? gv + g.feyv[J - 1] * f3S : gv;
+    gv = rN ? gv + (1.0 - g.feyv[J]) * f2N : gv;
   } else if (EQ == EQ_T) {
-    if (row(I, J)) gu = gu + AT(fa[1], I, J);
-    if (row(I + 1, J)) gu = gu + AT(fa[0], I + 1, J);
-    if (row(I, J)) gv = gv + AT(fa[3], I, J);
-    if (row(I, J + 1)) gv = gv + AT(fa[2], I, J + 1);
+    const double f1P = AT(fa[1], I, J), f3P = AT(fa[3], I, J);
+    const double f0E = at(fa[0], rE, I + 1, J), f2N = at(fa[2], rN, I, J + 1);
+    gu = rP ? gu + f1P : gu;
+    gu = rE ? gu + f0E : gu;
+    gv = rP ? gv + f3P : gv;
+    gv = rN ? gv + f2N : gv;
   } else {
     // p': b_o = (u_w - u_e) dyp + (v_s - v_n) dxp and the coefficients
     // -dyp^2 / a_u, -dxp^2 / a_v
-    if (row(I + 1, J)) gu = gu + AT(mu, I + 1, J) * g.dyp[J];
-    if (row(I, J)) gu = gu - AT(mu, I, J) * g.dyp[J];
-    if (row(I, J + 1)) gv = gv + AT(mu, I, J + 1) * g.dxp[I];
-    if (row(I, J)) gv = gv - AT(mu, I, J) * g.dxp[I];
+    const double muE = at(mu, rE, I + 1, J), muN = at(mu, rN, I, J + 1);
+    const double f0E = at(fa[0], rE, I + 1, J), f1P = AT(fa[1], I, J);
+    const double f2N = at(fa[2], rN, I, J + 1), f3P = AT(fa[3], I, J);
+    const double au = AT(A.au, I, J), av = AT(A.av, I, J);
+    const double aub = AT(G.aub, I, J), avb = AT(G.avb, I, J);
+    gu = rE ? gu + muE * g.dyp[J] : gu;
+    gu = rP ? gu - muP * g.dyp[J] : gu;
+    gv = rN ? gv + muN * g.dxp[I] : gv;
+    gv = rP ? gv - muP * g.dxp[I] : gv;
     if (I <= mi && J <= nj + 1) {
       double s = 0.0;
-      if (row(I, J)) s = s + AT(fa[1], I, J);
-      if (row(I + 1, J)) s = s + AT(fa[0], I + 1, J);
-      if (s != 0.0) {
-        const double au = AT(A.au, I, J);
-        AT(G.aub, I, J) = AT(G.aub, I, J) + s * g.dyp[J] * g.dyp[J] / (au * au);
-      }
+      s = rP ? s + f1P : s;
+      s = rE ? s + f0E : s;
+      if (s != 0.0) AT(G.aub, I, J) = aub + s * g.dyp[J] * g.dyp[J] / (au * au);
     }
     if (I <= mi + 1 && J <= nj) {
       double s = 0.0;
-      if (row(I, J)) s = s + AT(fa[3], I, J);
-      if (row(I, J + 1)) s = s + AT(fa[2], I, J + 1);
-      if (s != 0.0) {
-        const double av = AT(A.av, I, J);
-        AT(G.avb, I, J) = AT(G.avb, I, J) + s * g.dxp[I] * g.dxp[I] / (av * av);
-      }
+      s = rP ? s + f3P : s;
+      s = rN ? s + f2N : s;
+      if (s != 0.0) AT(G.avb, I, J) = avb + s * g.dxp[I] * g.dxp[I] / (av * av);
     }
   }
 
   // ---- phi_ant, pressure and buoyancy terms of the momentum and energy rows
   double ant = 0.0, pp = 0.0, tt = 0.0;
-  if (EQ != EQ_P && row(I, J)) {
-    const CoefD c = coef_d<RQ>(A, I, J);
-    ant = c.area * A.fg.inv_dt * AT(mu, I, J);
-  }
+  if (EQ != EQ_P) ant = rP ? row_area<RQ>(g, I, J) * A.fg.inv_dt * muP : ant;
   if (EQ == EQ_U) {
-    if (row(I, J)) pp = pp + AT(mu, I, J) * g.dyu[J];
-    if (row(I - 1, J)) pp = pp - AT(mu, I - 1, J) * g.dyu[J];
+    const double muW = at(mu, rW, I - 1, J);
+    pp = rP ? pp + muP * g.dyu[J] : pp;
+    pp = rW ? pp - muW * g.dyu[J] : pp;
   } else if (EQ == EQ_V) {
-    if (row(I, J)) pp = pp + AT(mu, I, J) * g.dxv[I];
-    if (row(I, J - 1)) pp = pp - AT(mu, I, J - 1) * g.dxv[I];
+    const double muS = at(mu, rS, I, J - 1);
+    pp = rP ? pp + muP * g.dxv[I] : pp;
+    pp = rS ? pp - muS * g.dxv[I] : pp;
     if (A.ph.ri_y != 0.0) {
-      if (row(I, J)) tt = tt - AT(mu, I, J) * g.dxv[I] * g.dyv[J] * A.ph.ri_y * (1.0 - g.feyp[J]);
-      if (row(I, J - 1)) tt = tt - AT(mu, I, J - 1) * g.dxv[I] * g.dyv[J - 1] * A.ph.ri_y * g.feyp[J - 1];
+      tt = rP ? tt - muP * g.dxv[I] * g.dyv[J] * A.ph.ri_y * (1.0 - g.feyp[J]) : tt;
+      tt = rS ? tt - muS * g.dxv[I] * g.dyv[J - 1] * A.ph.ri_y * g.feyp[J - 1] : tt;
     }
   }
 
@@ -414,11 +445,11 @@ __global__ void __launch_bounds__(kNodeBlock) k_adj2_gather(Adj2Args G) {
     if (EQ == EQ_V) v = v + gv;
     AT(G.phib, I, J) = v;
   }
-  if (EQ != EQ_U && G.velub && I <= mi && J <= nj + 1) AT(G.velub, I, J) = AT(G.velub, I, J) + gu;
-  if (EQ != EQ_V && G.velvb && I <= mi + 1 && J <= nj) AT(G.velvb, I, J) = AT(G.velvb, I, J) + gv;
-  if (EQ != EQ_P && G.antb && in_phi) AT(G.antb, I, J) = AT(G.antb, I, J) + ant;
-  if ((EQ == EQ_U || EQ == EQ_V) && G.pb) AT(G.pb, I, J) = AT(G.pb, I, J) + pp;
-  if (EQ == EQ_V && G.tb && tt != 0.0) AT(G.tb, I, J) = AT(G.tb, I, J) + tt;
+  if (EQ != EQ_U && G.velub && I <= mi && J <= nj + 1) AT(G.velub, I, J) = ub_old + gu;
+  if (EQ != EQ_V && G.velvb && I <= mi + 1 && J <= nj) AT(G.velvb, I, J) = vb_old + gv;
+  if (EQ != EQ_P && G.antb && in_phi) AT(G.antb, I, J) = antb_old + ant;
+  if ((EQ == EQ_U || EQ == EQ_V) && G.pb) AT(G.pb, I, J) = pb_old + pp;
+  if (EQ == EQ_V && G.tb && tt != 0.0) AT(G.tb, I, J) = tb_old + tt;
 }
 
 // adjoint of the correction (main.f90:931-962):  p += 0.6 p',

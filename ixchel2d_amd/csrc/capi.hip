@@ -2333,7 +2333,8 @@ namespace {
 enum Tape { TP_U0, TP_V0, TP_T0, TP_P0, TP_U1, TP_U2, TP_AU, TP_V1, TP_V2, TP_AV, TP_PC1, TP_PC2, TP_U3, TP_V3,
             TP_T1, TP_T2, TP_N };
 enum Work { W_UB3, W_VB3, W_TB2, W_PB, W_TB1, W_TB0, W_UB2, W_VB2, W_PCB2, W_PCB1, W_AUB, W_AVB, W_VB1, W_VB0,
-            W_UB1, W_UB0, W_UANTB, W_VANTB, W_TANTB, W_MU, W_FA0, W_FA1, W_FA2, W_FA3, W_N };
+            W_UB1, W_UB0, W_UANTB, W_VANTB, W_TANTB, W_MU, W_FA0, W_FA1, W_FA2, W_FA3, W_CF0, W_CF1, W_CF2,
+            W_N };
 
 int dcopy(ixh_state *s, double *dst, const double *src) {
   HIPCHK(hipMemcpyAsync(dst, src, s->fsize * sizeof(double), hipMemcpyDeviceToDevice, s->stream));
@@ -2415,6 +2416,7 @@ Adj2Args adj2_args(ixh_state *s, int var, double *const *W) {
   if (var >= 0) set_bc(s, G.S, var, IXH_DIR_X);
   G.mu = W[W_MU];
   for (int k = 0; k < 4; ++k) G.fa[k] = W[W_FA0 + k];
+  for (int k = 0; k < 3; ++k) G.cf[k] = W[W_CF0 + k];
   return G;
 }
 
