@@ -249,6 +249,7 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
     sa[k] = a0;
     sb[k] = a1;
     sc_[k] = a2;
+    sr[k] = AT(G.ybar, ii, jj);
   }
   __syncthreads();
   double ta[EPT], tc[EPT];
@@ -265,12 +266,8 @@ __global__ void __launch_bounds__(kBlock) k_adj2_solve(Adj2Args G) {
   for (int e = 0; e < EPT; ++e) {
     const int k = threadIdx.x + e * kBlock;
     if (k < n) {
-      const int pos = k + 1;
-      const int ii = DIR == DIR_X ? pos : line;
-      const int jj = DIR == DIR_X ? line : pos;
       sa[k] = ta[e];
       sc_[k] = tc[e];
-      sr[k] = AT(G.ybar, ii, jj);
     }
   }
   __syncthreads();
@@ -473,30 +470,30 @@ __global__ void __launch_bounds__(kNodeBlock) k_adj2_correct(AdjCorrArgs C) {
   const Geo &g = C.g;
   auto ucor = [&](int i, int j) { return i >= 2 && i <= mi - 1 && j >= 2 && j <= nj; };
   auto vcor = [&](int i, int j) { return i >= 2 && i <= mi && j >= 2 && j <= nj - 1; };
+  // arr(i, j) where `on`, else arr(I, J) (a valid address, value not used):
+  // every load of the node goes out without a branch
+  auto at = [&](const double *arr, bool on, int i, int j) { return AT(arr, on ? i : I, on ? j : J); };
+  const bool uP = ucor(I, J), uW = ucor(I - 1, J), vP = vcor(I, J), vS = vcor(I, J - 1);
+  const double ubP = AT(C.ub3, I, J), vbP = AT(C.vb3, I, J);
+  const double auP = AT(C.au, I, J), avP = AT(C.av, I, J);
+  const double pcP = AT(C.pc, I, J), pcE = at(C.pc, uP, I + 1, J), pcN = at(C.pc, vP, I, J + 1);
+  const double aub = AT(C.aub, I, J), avb = AT(C.avb, I, J), pbn = AT(C.pbn, I, J);
+  const double ubW = at(C.ub3, uW, I - 1, J), auW = at(C.au, uW, I - 1, J);
+  const double vbS = at(C.vb3, vS, I, J - 1), avS = at(C.av, vS, I, J - 1);
   if (I <= mi) {
-    const double ub = AT(C.ub3, I, J);
-    AT(C.ub2, I, J) = ub;
-    if (ucor(I, J)) {
-      const double au = AT(C.au, I, J);
-      const double d = AT(C.pc, I, J) - AT(C.pc, I + 1, J);
-      AT(C.aub, I, J) = AT(C.aub, I, J) - ub * g.dyu[J] * d / (au * au);
-    }
+    AT(C.ub2, I, J) = ubP;
+    if (uP) AT(C.aub, I, J) = aub - ubP * g.dyu[J] * (pcP - pcE) / (auP * auP);
   }
   if (J <= nj) {
-    const double vb = AT(C.vb3, I, J);
-    AT(C.vb2, I, J) = vb;
-    if (vcor(I, J)) {
-      const double av = AT(C.av, I, J);
-      const double d = AT(C.pc, I, J) - AT(C.pc, I, J + 1);
-      AT(C.avb, I, J) = AT(C.avb, I, J) - vb * g.dxv[I] * d / (av * av);
-    }
+    AT(C.vb2, I, J) = vbP;
+    if (vP) AT(C.avb, I, J) = avb - vbP * g.dxv[I] * (pcP - pcN) / (avP * avP);
   }
   double s = 0.0;
-  if (I >= 2 && I <= mi && J >= 2 && J <= nj) s = 0.6 * AT(C.pbn, I, J);
-  if (ucor(I, J)) s = s + AT(C.ub3, I, J) * g.dyu[J] / AT(C.au, I, J);
-  if (ucor(I - 1, J)) s = s - AT(C.ub3, I - 1, J) * g.dyu[J] / AT(C.au, I - 1, J);
-  if (vcor(I, J)) s = s + AT(C.vb3, I, J) * g.dxv[I] / AT(C.av, I, J);
-  if (vcor(I, J - 1)) s = s - AT(C.vb3, I, J - 1) * g.dxv[I] / AT(C.av, I, J - 1);
+  if (I >= 2 && I <= mi && J >= 2 && J <= nj) s = 0.6 * pbn;
+  s = uP ? s + ubP * g.dyu[J] / auP : s;
+  s = uW ? s - ubW * g.dyu[J] / auW : s;
+  s = vP ? s + vbP * g.dxv[I] / avP : s;
+  s = vS ? s - vbS * g.dxv[I] / avS : s;
   AT(C.pcb, I, J) = s;
 }
 
