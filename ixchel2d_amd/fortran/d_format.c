@@ -1,0 +1,172 @@
+/*
+ * Records of Fortran's D23.15 edit descriptor, formatted without the Fortran
+ * runtime's per-item I/O: the host's out_*.dat writer (postproceso_dat, after
+ * upstream's mod_postproceso.f90 postproceso_dat with its "(3D23.15)" and
+ * "(4D23.15)" records) hands whole arrays of records to d23_records.
+ *
+ * A value is written as flang writes it: 23 characters, right-justified;
+ * a minus sign when the sign bit is set (-0.0 included); "0." and the 15
+ * significant digits of |x| correctly rounded to nearest (ties to even);
+ * the exponent of the 0.ddd form as "D+dd" / "D-dd", or "+ddd" / "-ddd"
+ * when it has three digits; zero as 0.000000000000000D+00.
+ *
+ * The 15 digits come from y = |x| 10^(14-E), E the decimal exponent, with
+ * 10^(14-E) as a double-double (hi + lo, built once in binary128) and the
+ * product a * hi split exactly (Dekker), so y is known to about 1e-16 at
+ * y < 1e15.  When y lies within 1e-6 of halfway between two integers, and
+ * for |x| outside [1e-280, 1e290] (where a split could overflow), the C
+ * library's exact conversion (snprintf "%.14e") decides instead.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define KMAX 300 /* |14 - E| for 1e-280 <= |x| <= 1e290, with room for the search; 2^27 10^KMAX
+                       * must not overflow */
+
+/* 10^k = hi + lo; hh + hl = hi, each half with at most 26 significant bits */
+typedef struct {
+  double hi, lo, hh, hl;
+} Pow10;
+
+static Pow10 p10[2 * KMAX + 1];
+static int p10_ready = 0;
+
+static void split(double x, double *h, double *l) {
+  const double c = 134217729.0 * x; /* 2^27 + 1 */
+  *h = c - (c - x);
+  *l = x - *h;
+}
+
+static void init_p10(void) {
+  __float128 up = 1, q;
+  for (int k = 0; k <= KMAX; ++k) {
+    for (int sgn = -1; sgn <= 1; sgn += 2) {
+      if (k == 0 && sgn < 0) continue;
+      q = sgn > 0 ? up : 1 / up;
+      Pow10 *p = &p10[KMAX + sgn * k];
+      p->hi = (double)q;
+      p->lo = (double)(q - (__float128)p->hi);
+      split(p->hi, &p->hh, &p->hl);
+    }
+    up = up * 10;
+  }
+  p10_ready = 1;
+}
+
+/* d.dddddddddddddde+XX from the exact conversion: n and E */
+static void digits_exact(double a, int64_t *n_out, int *e_out) {
+  char buf[40];
+  snprintf(buf, sizeof buf, "%.14e", a);
+  int64_t n = 0;
+  const char *c = buf;
+  for (; *c && *c != 'e'; ++c)
+    if (*c >= '0' && *c <= '9') n = n * 10 + (*c - '0');
+  *n_out = n;
+  *e_out = (int)strtol(c + 1, NULL, 10);
+}
+
+/* 15 significant digits of a > 0 (finite) as n in [1e14, 1e15) and the
+ * exponent q of the 0.ddd form: a ~ 0.n x 10^q */
+static void digits15(double a, int64_t *n_out, int *q_out) {
+  int64_t n = 0;
+  int E = 0;
+  int settled = 0;
+  if (a >= 1e-280 && a <= 1e290) {
+    uint64_t bits;
+    memcpy(&bits, &a, sizeof bits);
+    const int e2 = (int)((bits >> 52) & 0x7ff) - 1023; /* 2^e2 <= a < 2^(e2+1) */
+    E = (e2 * 78913) >> 18; /* floor(e2 log10 2), within one of floor(log10 a) */
+    double ah, al;
+    split(a, &ah, &al);
+    for (int t = 0; t < 4 && !settled; ++t) {
+      const Pow10 *p = &p10[14 - E + KMAX];
+      const double yh = a * p->hi;
+      const double yl = (((ah * p->hh - yh) + ah * p->hl) + al * p->hh) + al * p->hl + a * p->lo;
+      if (yh < 1.0e14) {
+        --E;
+      } else if (yh >= 1.0e15) {
+        ++E;
+      } else {
+        settled = 1;
+        n = (int64_t)yh;
+        const double f = (yh - (double)n) + yl; /* yh - n exact: yh has a unit of 2^-6 or more */
+        if (fabs(f - 0.5) < 1.0e-6) settled = 0; /* too close to call */
+        else if (f > 0.5) ++n;
+        break;
+      }
+    }
+  }
+  if (!settled) digits_exact(a, &n, &E);
+  if (n == 1000000000000000LL) {
+    n = 100000000000000LL;
+    ++E;
+  }
+  *n_out = n;
+  *q_out = E + 1;
+}
+
+static const char kTwo[] =
+    "00010203040506070809101112131415161718192021222324252627282930313233343536373839"
+    "40414243444546474849505152535455565758596061626364656667686970717273747576777879"
+    "8081828384858687888990919293949596979899";
+
+/* one value, 23 characters at out; 0 on success, 1 for a non-finite value.
+ * Both exponent forms leave the same layout: two blanks or a blank and the
+ * sign, "0.", 15 digits, then "D+dd" or "+ddd". */
+static int d23_15(double x, char *out) {
+  if (!isfinite(x)) return 1;
+  const double a = fabs(x);
+  int64_t n = 0;
+  int q = 0;
+  if (a != 0.0) digits15(a, &n, &q);
+  out[0] = ' ';
+  out[1] = signbit(x) ? '-' : ' ';
+  out[2] = '0';
+  out[3] = '.';
+  /* the 15 digits (out[4..18]) as 7 + 8, two at a time */
+  uint32_t hi = (uint32_t)(n / 100000000), lo = (uint32_t)(n % 100000000);
+  for (int d = 18; d >= 12; d -= 2) {
+    const uint32_t two = lo % 100;
+    lo /= 100;
+    out[d] = kTwo[2 * two + 1];
+    out[d - 1] = kTwo[2 * two];
+  }
+  for (int d = 10; d >= 5; d -= 2) {
+    const uint32_t two = hi % 100;
+    hi /= 100;
+    out[d] = kTwo[2 * two + 1];
+    out[d - 1] = kTwo[2 * two];
+  }
+  out[4] = (char)('0' + hi);
+  const int aq = q < 0 ? -q : q;
+  const char sign = q < 0 ? '-' : '+';
+  if (aq <= 99) {
+    out[19] = 'D';
+    out[20] = sign;
+  } else {
+    out[19] = sign;
+    out[20] = (char)('0' + aq / 100);
+  }
+  out[21] = (char)('0' + (aq / 10) % 10);
+  out[22] = (char)('0' + aq % 10);
+  return 0;
+}
+
+/* nrec records of ncol values each (vals: record after record), each record
+ * ncol fields of 23 characters and a newline, into out (nrec * (23 ncol + 1)
+ * characters).  Returns 0, or 1 if a value is not finite (out is then
+ * incomplete and the caller writes the records itself). */
+int d23_records(const double *vals, int64_t nrec, int32_t ncol, char *out) {
+  if (!p10_ready) init_p10();
+  const int64_t width = 23 * (int64_t)ncol + 1;
+  for (int64_t r = 0; r < nrec; ++r) {
+    char *line = out + r * width;
+    for (int32_t c = 0; c < ncol; ++c)
+      if (d23_15(vals[r * ncol + c], line + 23 * c)) return 1;
+    line[23 * ncol] = '\n';
+  }
+  return 0;
+}

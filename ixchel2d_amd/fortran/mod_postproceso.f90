@@ -9,6 +9,18 @@ module postproceso
   use malla, only : mi, nj, DBL, xu, yv, xp, yp, form24, form25
   use ixchel_hip
   implicit none
+
+  interface
+     ! d_format.c: nrec records of ncol values as (ncolD23.15) lines
+     integer(c_int) function d23_records(vals, nrec, ncol, out) bind(C, name='d23_records')
+       import :: c_double, c_int64_t, c_int32_t, c_int, c_char
+       real(c_double), intent(in) :: vals(*)
+       integer(c_int64_t), value :: nrec
+       integer(c_int32_t), value :: ncol
+       character(kind=c_char), intent(out) :: out(*)
+     end function d23_records
+  end interface
+
 contains
 
   ! nusselt_promedio_y (mod_postproceso.f90:100-215) on the device's T
@@ -33,35 +45,70 @@ contains
   end function entero_caracter
 
   ! out_n<nj>m<mi>_R<Re>{u,v,p}.dat, readable by lectura_mallas_escalonadas
+  ! (upstream's postproceso_dat: a list-directed header, then one (3D23.15)
+  ! or (4D23.15) record per node).  The records are formatted by d23_records
+  ! (d_format.c) into one buffer and written in one stream write; a field
+  ! with a non-finite value goes through Fortran's formatted write instead.
   subroutine postproceso_dat(prefijo, u, v, temp, pres, placa_min, placa_max, itera_total, ao)
     character(len=*), intent(in) :: prefijo
     real(DBL), intent(in) :: u(mi, nj+1), v(mi+1, nj), temp(mi+1, nj+1), pres(mi+1, nj+1), ao
     integer, intent(in) :: placa_min, placa_max, itera_total
-    integer :: ii, jj, un
-    open(newunit=un, file=prefijo//'u.dat')
-    write(un, *) placa_min, placa_max, itera_total, ao
+    real(DBL), allocatable :: rec(:, :)
+    integer :: ii, jj, k
+    allocate(rec(3, mi * (nj + 1)))
+    k = 0
     do jj = 1, nj + 1
        do ii = 1, mi
-          write(un, form24) xu(ii), yp(jj), u(ii, jj)
+          k = k + 1
+          rec(:, k) = [xu(ii), yp(jj), u(ii, jj)]
        end do
     end do
-    close(un)
-    open(newunit=un, file=prefijo//'v.dat')
-    write(un, *) placa_min, placa_max, itera_total, ao
+    call write_dat(prefijo//'u.dat', rec)
+    deallocate(rec)
+    allocate(rec(3, (mi + 1) * nj))
+    k = 0
     do jj = 1, nj
        do ii = 1, mi + 1
-          write(un, form24) xp(ii), yv(jj), v(ii, jj)
+          k = k + 1
+          rec(:, k) = [xp(ii), yv(jj), v(ii, jj)]
        end do
     end do
-    close(un)
-    open(newunit=un, file=prefijo//'p.dat')
-    write(un, *) placa_min, placa_max, itera_total, ao
+    call write_dat(prefijo//'v.dat', rec)
+    deallocate(rec)
+    allocate(rec(4, (mi + 1) * (nj + 1)))
+    k = 0
     do jj = 1, nj + 1
        do ii = 1, mi + 1
-          write(un, form25) xp(ii), yp(jj), temp(ii, jj), pres(ii, jj)
+          k = k + 1
+          rec(:, k) = [xp(ii), yp(jj), temp(ii, jj), pres(ii, jj)]
        end do
     end do
-    close(un)
+    call write_dat(prefijo//'p.dat', rec)
+  contains
+    subroutine write_dat(archivo, rec)
+      character(len=*), intent(in) :: archivo
+      real(DBL), intent(in) :: rec(:, :)
+      character(kind=c_char), allocatable :: buf(:)
+      integer :: un, r
+      open(newunit=un, file=archivo, status='replace')
+      write(un, *) placa_min, placa_max, itera_total, ao
+      allocate(buf(size(rec, 2) * (23 * size(rec, 1) + 1)))
+      if (d23_records(rec, int(size(rec, 2), c_int64_t), int(size(rec, 1), c_int32_t), buf) == 0) then
+         close(un)
+         open(newunit=un, file=archivo, access='stream', form='unformatted', position='append', &
+              status='old')
+         write(un) buf
+      else if (size(rec, 1) == 3) then
+         do r = 1, size(rec, 2)
+            write(un, form24) rec(:, r)
+         end do
+      else
+         do r = 1, size(rec, 2)
+            write(un, form25) rec(:, r)
+         end do
+      end if
+      close(un)
+    end subroutine write_dat
   end subroutine postproceso_dat
 
   ! postproceso_bin (mod_postproceso.f90:310-365): exact FP64 dumps

@@ -16,7 +16,7 @@ from ixchel2d_amd import case as C
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FDIR = os.path.join(ROOT, "ixchel2d_amd", "fortran")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
-MODS = ("mod_ixchel_hip", "mod_malla", "mod_postproceso")
+MODS = ("mod_ixchel_hip", "mod_malla", "mod_postproceso", "d_format")
 
 
 @pytest.fixture(scope="module")
@@ -30,7 +30,7 @@ def driver(tmp_path_factory):
     out = tmp_path_factory.mktemp("vtk") / "vtk_driver"
     subprocess.run([FLANG, "-O2", "-I", FDIR, "-o", str(out), os.path.join(ROOT, "tests", "vtk_driver.f90")]
                    + [os.path.join(FDIR, m + ".o") for m in MODS]
-                   + [f"-L{os.path.dirname(lib)}", "-lixchel2d", f"-Wl,-rpath,{os.path.dirname(lib)}"],
+                   + [f"-L{os.path.dirname(lib)}", "-lixchel2d", "-lm", f"-Wl,-rpath,{os.path.dirname(lib)}"],
                    check=True, cwd=str(out.parent))
     return str(out)
 
