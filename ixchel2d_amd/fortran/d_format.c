@@ -18,10 +18,12 @@
  * library's exact conversion (snprintf "%.14e") decides instead.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #define KMAX 300 /* |14 - E| for 1e-280 <= |x| <= 1e290, with room for the search; 2^27 10^KMAX
                        * must not overflow */
@@ -32,7 +34,7 @@ typedef struct {
 } Pow10;
 
 static Pow10 p10[2 * KMAX + 1];
-static int p10_ready = 0;
+static pthread_once_t p10_once = PTHREAD_ONCE_INIT;
 
 static void split(double x, double *h, double *l) {
   const double c = 134217729.0 * x; /* 2^27 + 1 */
@@ -53,7 +55,6 @@ static void init_p10(void) {
     }
     up = up * 10;
   }
-  p10_ready = 1;
 }
 
 /* d.dddddddddddddde+XX from the exact conversion: n and E */
@@ -155,18 +156,82 @@ static int d23_15(double x, char *out) {
   return 0;
 }
 
+typedef struct {
+  const double *vals;
+  int64_t r0, r1;
+  int32_t ncol;
+  char *out;
+  int status;
+} Chunk;
+
+static void *format_chunk(void *arg) {
+  Chunk *c = (Chunk *)arg;
+  const int64_t width = 23 * (int64_t)c->ncol + 1;
+  c->status = 0;
+  for (int64_t r = c->r0; r < c->r1 && !c->status; ++r) {
+    char *line = c->out + r * width;
+    for (int32_t k = 0; k < c->ncol; ++k)
+      if (d23_15(c->vals[r * c->ncol + k], line + 23 * k)) c->status = 1;
+    line[23 * c->ncol] = '\n';
+  }
+  return NULL;
+}
+
+#define MAX_THREADS 8
+#define MIN_RECORDS 16384 /* per thread */
+
 /* nrec records of ncol values each (vals: record after record), each record
  * ncol fields of 23 characters and a newline, into out (nrec * (23 ncol + 1)
- * characters).  Returns 0, or 1 if a value is not finite (out is then
- * incomplete and the caller writes the records itself). */
+ * characters), on up to 8 threads (each a contiguous range of records).
+ * Returns 0, or 1 if a value is not finite (out is then incomplete and the
+ * caller writes the records itself). */
 int d23_records(const double *vals, int64_t nrec, int32_t ncol, char *out) {
-  if (!p10_ready) init_p10();
-  const int64_t width = 23 * (int64_t)ncol + 1;
-  for (int64_t r = 0; r < nrec; ++r) {
-    char *line = out + r * width;
-    for (int32_t c = 0; c < ncol; ++c)
-      if (d23_15(vals[r * ncol + c], line + 23 * c)) return 1;
-    line[23 * ncol] = '\n';
+  pthread_once(&p10_once, init_p10);
+  const long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+  int64_t nt = nrec / MIN_RECORDS;
+  if (nt > MAX_THREADS) nt = MAX_THREADS;
+  if (nt > ncpu) nt = ncpu;
+  if (nt < 1) nt = 1;
+  Chunk ch[MAX_THREADS];
+  pthread_t th[MAX_THREADS];
+  int started[MAX_THREADS] = {0};
+  for (int64_t t = 0; t < nt; ++t) {
+    ch[t] = (Chunk){vals, nrec * t / nt, nrec * (t + 1) / nt, ncol, out, 0};
+    if (t > 0) started[t] = pthread_create(&th[t], NULL, format_chunk, &ch[t]) == 0;
   }
-  return 0;
+  format_chunk(&ch[0]);
+  int status = ch[0].status;
+  for (int64_t t = 1; t < nt; ++t) {
+    if (started[t])
+      pthread_join(th[t], NULL);
+    else
+      format_chunk(&ch[t]); /* no thread: this one does the chunk */
+    status |= ch[t].status;
+  }
+  return status;
+}
+
+/* the same records appended to the file named by path (pathlen characters,
+ * not terminated, as Fortran passes them).  Returns 0; 1 if a value is not
+ * finite (nothing is appended); 2 if the file cannot be written. */
+int d23_append(const char *path, int32_t pathlen, const double *vals, int64_t nrec, int32_t ncol) {
+  const size_t bytes = (size_t)nrec * (size_t)(23 * ncol + 1);
+  char *buf = (char *)malloc(bytes + 1);
+  char *name = (char *)malloc((size_t)pathlen + 1);
+  if (!buf || !name) {
+    free(buf);
+    free(name);
+    return 2;
+  }
+  memcpy(name, path, (size_t)pathlen);
+  name[pathlen] = 0;
+  int status = d23_records(vals, nrec, ncol, buf);
+  if (status == 0) {
+    FILE *f = fopen(name, "ab");
+    if (!f || fwrite(buf, 1, bytes, f) != bytes) status = 2;
+    if (f && fclose(f) != 0) status = 2;
+  }
+  free(buf);
+  free(name);
+  return status;
 }

@@ -51,9 +51,9 @@ def test_d23_records_equal_flang_formatted_write():
     v = _values()
     v = v[: (len(v) // 12) * 12]
     with tempfile.TemporaryDirectory(prefix="ixh_dfmt_") as d:
-        subprocess.run(["gcc", "-O2", "-Wall", "-c", os.path.join(ROOT, "ixchel2d_amd", "fortran", "d_format.c"),
-                        "-o", "d_format.o"], cwd=d, check=True)
-        subprocess.run([FLANG, "-O2", os.path.join(ROOT, "tests", "dfmt_driver.f90"), "d_format.o", "-lm",
+        src = os.path.join(ROOT, "ixchel2d_amd", "fortran", "d_format.c")
+        subprocess.run(["gcc", "-O2", "-Wall", "-pthread", "-c", src, "-o", "d_format.o"], cwd=d, check=True)
+        subprocess.run([FLANG, "-O2", os.path.join(ROOT, "tests", "dfmt_driver.f90"), "d_format.o", "-lm", "-lpthread",
                         "-o", "dfmt_driver"], cwd=d, check=True)
         with open(os.path.join(d, "valores.raw"), "wb") as fh:
             fh.write(np.int32(len(v)).tobytes())

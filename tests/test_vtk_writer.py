@@ -30,7 +30,7 @@ def driver(tmp_path_factory):
     out = tmp_path_factory.mktemp("vtk") / "vtk_driver"
     subprocess.run([FLANG, "-O2", "-I", FDIR, "-o", str(out), os.path.join(ROOT, "tests", "vtk_driver.f90")]
                    + [os.path.join(FDIR, m + ".o") for m in MODS]
-                   + [f"-L{os.path.dirname(lib)}", "-lixchel2d", "-lm", f"-Wl,-rpath,{os.path.dirname(lib)}"],
+                   + [f"-L{os.path.dirname(lib)}", "-lixchel2d", "-lm", "-lpthread", f"-Wl,-rpath,{os.path.dirname(lib)}"],
                    check=True, cwd=str(out.parent))
     return str(out)
 
