@@ -134,10 +134,11 @@ def fortran_host_bench(args):
     """The 301x701 case through the Fortran host (ixchel2d_amd/fortran/
     ixchel2d_gpu: upstream's driver over the C-ABI, reading and writing the
     reference's files).  Two runs in packages of 1000 iterations, of 1000
-    and of 1000 + steps iterations (steps rounded up to whole packages); the
-    difference times the steps with their packages' outputs (fort.102, the
-    Nusselt record, out_*.dat, the .bin dumps and the VTK sample) and without
-    process start, file reads and set-up.  The same pair again without the
+    and of 1000 + steps iterations (steps rounded up to whole packages, at
+    least 4000), each the fastest of three; the difference times the steps
+    with their packages' outputs (fort.102, the Nusselt record, out_*.dat,
+    the .bin dumps and the VTK sample) and without process start, file reads
+    and set-up.  The same pair again without the
     package outputs (postprocesar = .false.: fort.102 and the Nusselt record
     only) shows what the outputs cost.  Upstream's serial build on the same
     case beside it."""
@@ -146,7 +147,7 @@ def fortran_host_bench(args):
     if not os.path.exists(exe):
         raise SystemExit("the Fortran host is not built (python -c 'import __graft_entry__ as g; g.build()')")
     paq = 1000
-    steps = max(paq, -(-args.steps // paq) * paq)
+    steps = max(4 * paq, -(-args.steps // paq) * paq)
     case = build_case()
     env = dict(os.environ, IXCHEL2D_SOLVER=args.solver, IXCHEL2D_PRECISION=args.precision)
 
@@ -162,11 +163,13 @@ def fortran_host_bench(args):
             return time.perf_counter() - t
         finally:
             shutil.rmtree(work, ignore_errors=True)
-    t_base = run(paq)
-    t_full = run(paq + steps)
+    # each process time is the least of three runs: process start and file
+    # reads vary by tens of ms from run to run
+    t_base = min(run(paq) for _ in range(3))
+    t_full = min(run(paq + steps) for _ in range(3))
     per = (t_full - t_base) / steps
-    t_base0 = run(paq, outputs=False)
-    t_full0 = run(paq + steps, outputs=False)
+    t_base0 = min(run(paq, outputs=False) for _ in range(3))
+    t_full0 = min(run(paq + steps, outputs=False) for _ in range(3))
     per0 = (t_full0 - t_base0) / steps
     m = case.mesh
     cpu = None if args.no_cpu else reference_serial(build_case())
