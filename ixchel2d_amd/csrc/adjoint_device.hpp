@@ -132,10 +132,6 @@ struct Adj2Args {
   double *g_lo, *g_hi;  // (T) adjoints of the boundary values (+=, per node)
 };
 
-// workgroup of the per-node kernels: one wave, so that the 17k nodes of a
-// 129^2 cavity spread over every CU (their loads form long dependent chains)
-constexpr int kNodeBlock = 64;
-
 template <int EQ, int DIR>
 __device__ __forceinline__ int adj_n(int mi, int nj) { return Layout<EQ, DIR>::n(mi, nj); }
 

@@ -129,7 +129,7 @@ struct ixh_state {
   int adj_steps = 0, adj_every = 0;
   std::vector<double *> adj_ck;
   std::vector<double *> adj_tape;
-  double *adj_w[4]{};  // lambda, lambda', mu, adjoint of T_ant
+  double *adj_w[7]{};  // lambda, lambda', mu, adjoint of T_ant, the sweep's row coefficients (3)
   double *adj_g[4]{};  // adjoints of the boundary values of T, sides a..d
   std::vector<double *> adj2_tape;  // coupled adjoint: 16 fields per step of a segment
   std::vector<double *> adj2_w;     // coupled adjoint: work arrays
@@ -1103,10 +1103,10 @@ int ixh_destroy(ixh_state *s) {
   for (double *p : s->adj_tape) (void)hipFree(p);
   for (double *p : s->adj2_tape) (void)hipFree(p);
   for (double *p : s->adj2_w) (void)hipFree(p);
-  for (int k = 0; k < 4; ++k) {
-    if (s->adj_w[k]) (void)hipFree(s->adj_w[k]);
+  for (double *p : s->adj_w)
+    if (p) (void)hipFree(p);
+  for (int k = 0; k < 4; ++k)
     if (s->adj_g[k]) (void)hipFree(s->adj_g[k]);
-  }
   if (s->ev0) (void)hipEventDestroy(s->ev0);
   if (s->ev1) (void)hipEventDestroy(s->ev1);
   for (hipEvent_t e : s->lev) (void)hipEventDestroy(e);
@@ -2201,8 +2201,9 @@ int adj_step(ixh_state *s, const double *u, const double *v) {
   G.S.u = u;
   G.S.v = v;
   double *lam = s->adj_w[0], *lamp = s->adj_w[1], *mu = s->adj_w[2], *tant = s->adj_w[3];
+  for (int k = 0; k < 3; ++k) G.cf[k] = s->adj_w[4 + k];
   const int npts = (s->mi + 1) * (s->nj + 1);
-  const dim3 pg((npts + kBlock - 1) / kBlock), block(kBlock);
+  const dim3 pg((npts + kNodeBlock - 1) / kNodeBlock), block(kNodeBlock);
   // x-sweep (applied last in the step)
   G.lam = lam; G.mu = mu; G.g_lo = s->adj_g[IXH_SIDE_A]; G.g_hi = s->adj_g[IXH_SIDE_C];
   CHK(adj_sweep_launch<DIR_X>(s, G, s->nj - 1, s->mi + 1));
@@ -2272,8 +2273,9 @@ int ixh_adjoint_energy(ixh_state *s, const double *tbar, int32_t ld, double *gra
     return fail(IXH_ERR_ARG, "adjoint: bad arguments");
   if (s->adj_steps < 1) return fail(IXH_ERR_ARG, "adjoint: no recorded run (ixh_adjoint_record)");
   const int K = s->adj_steps, C = s->adj_every;
+  for (double *&p : s->adj_w)
+    if (!p) CHK(dmalloc(&p, s->fsize));
   for (int k = 0; k < 4; ++k) {
-    if (!s->adj_w[k]) CHK(dmalloc(&s->adj_w[k], s->fsize));
     if (!s->adj_g[k]) CHK(dmalloc(&s->adj_g[k], (size_t)std::max(s->mi, s->nj) + 4));
     HIPCHK(hipMemsetAsync(s->adj_g[k], 0, ((size_t)std::max(s->mi, s->nj) + 4) * sizeof(double), s->stream));
   }

@@ -1831,7 +1831,14 @@ struct AdjArgs {
   double *out;             // (gather) adjoint of the sweep's input
   double *tant;            // (gather) adjoint of T_ant: written by X, added by Y
   double *g_lo, *g_hi;     // boundary-value adjoints of the low / high side (per node)
+  double *cf[3];           // per row, from the sweep: a_P and the two coefficients across
+                           // it (bs, bn for x-sweeps, ai, ad for y-sweeps)
 };
+
+// workgroup of the adjoints' per-node kernels: one wave, so that the nodes
+// of a small cavity (17k at 129^2) spread over every CU (their loads form
+// long dependent chains)
+constexpr int kNodeBlock = 64;
 
 template <int DIR, int EPT>
 __global__ void __launch_bounds__(kBlock) k_adj_sweep(AdjArgs G) {
@@ -1857,6 +1864,9 @@ __global__ void __launch_bounds__(kBlock) k_adj_sweep(AdjArgs G) {
       a0 = DIR == DIR_X ? c.ai : c.bs;
       a2 = DIR == DIR_X ? c.ad : c.bn;
       a1 = c.ac;
+      AT(G.cf[0], ii, jj) = c.ac;
+      AT(G.cf[1], ii, jj) = DIR == DIR_X ? c.bs : c.ai;
+      AT(G.cf[2], ii, jj) = DIR == DIR_X ? c.bn : c.ad;
     } else {
       const bool high = pos != 1;
       (void)bc_row(high ? hi_side : lo_side, line, high, a0, a1, a2, rr);
@@ -1907,54 +1917,46 @@ __global__ void __launch_bounds__(kBlock) k_adj_sweep(AdjArgs G) {
 // adjoint of the sweep's right-hand side at node (i, j): out = d/d(input);
 // X also writes tant, Y adds its own T_ant part and tant into out
 template <int DIR>
-__global__ void __launch_bounds__(kBlock) k_adj_gather(AdjArgs G) {
+__global__ void __launch_bounds__(kNodeBlock) k_adj_gather(AdjArgs G) {
   constexpr bool TR = false;
   const SweepArgs &A = G.S;
   const int P = A.P;
   const int mi = A.mi, nj = A.nj;
-  const int idx = (int)(blockIdx.x * kBlock + threadIdx.x);
+  const int idx = (int)(blockIdx.x * kNodeBlock + threadIdx.x);
   if (idx >= (mi + 1) * (nj + 1)) return;
   const int ii = 1 + idx % (mi + 1), jj = 1 + idx / (mi + 1);
   const double keep = 1.0 - A.ph.rel_ener;
   const double *mu = G.mu_in;
-  double area, pin;
+  // the rows of the sweep, and every value read without a branch: arr(i, j)
+  // where `on`, else arr(ii, jj) (a valid address, value not used), so the
+  // node's loads go out together
+  auto row = [&](int i, int j) { return i >= 2 && i <= mi && j >= 2 && j <= nj; };
+  auto at = [&](const double *arr, bool on, int i, int j) { return AT(arr, on ? i : ii, on ? j : jj); };
+  const bool rP = row(ii, jj);
+  const double lam = AT(G.lam, ii, jj), muP = AT(mu, ii, jj), ac = AT(G.cf[0], ii, jj);
+  const double tant_old = DIR == DIR_X ? 0.0 : AT(G.tant, ii, jj);
   double o = 0.0, ta = 0.0;
   // lines the sweep does not solve are copied from its input
   const bool passthrough = DIR == DIR_X ? (jj == 1 || jj == nj + 1) : (ii == 1 || ii == mi + 1);
-  if (passthrough) {
-    o = AT(G.lam, ii, jj);
-  } else if (ii >= 2 && ii <= mi && jj >= 2 && jj <= nj) {
-    // the row of (ii, jj) reads its own node through (1 - alpha) a_P phi
-    const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii, jj, area, pin);
-    o = keep * c.ac * AT(mu, ii, jj);
-    ta = area * A.fg.inv_dt * AT(mu, ii, jj);
-  }
+  // the row of (ii, jj) reads its own node through (1 - alpha) a_P phi
+  o = passthrough ? lam : rP ? keep * ac * muP : o;
+  ta = rP ? A.g.dxp[ii] * A.g.dyp[jj] * A.fg.inv_dt * muP : ta;  // the T cell's area, as fast_coef forms it
   // neighbouring rows read this node in their head term
   if (DIR == DIR_X) {
-    if (ii >= 2 && ii <= mi) {
-      if (jj + 1 >= 2 && jj + 1 <= nj) {
-        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii, jj + 1, area, pin);
-        o = o - c.bs * AT(mu, ii, jj + 1);
-      }
-      if (jj - 1 >= 2 && jj - 1 <= nj) {
-        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii, jj - 1, area, pin);
-        o = o - c.bn * AT(mu, ii, jj - 1);
-      }
-    }
+    const bool rN = row(ii, jj + 1), rS = row(ii, jj - 1);
+    const double bs = at(G.cf[1], rN, ii, jj + 1), muN = at(mu, rN, ii, jj + 1);
+    const double bn = at(G.cf[2], rS, ii, jj - 1), muS = at(mu, rS, ii, jj - 1);
+    o = rN ? o - bs * muN : o;
+    o = rS ? o - bn * muS : o;
     AT(G.out, ii, jj) = o;
     AT(G.tant, ii, jj) = ta;
   } else {
-    if (jj >= 2 && jj <= nj) {
-      if (ii + 1 >= 2 && ii + 1 <= mi) {
-        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii + 1, jj, area, pin);
-        o = o - c.ai * AT(mu, ii + 1, jj);
-      }
-      if (ii - 1 >= 2 && ii - 1 <= mi) {
-        const CoefT<double> c = fast_coef<EQ_T, TR, double>(A, ii - 1, jj, area, pin);
-        o = o - c.ad * AT(mu, ii - 1, jj);
-      }
-    }
-    AT(G.out, ii, jj) = o + ta + AT(G.tant, ii, jj);
+    const bool rE = row(ii + 1, jj), rW = row(ii - 1, jj);
+    const double ai = at(G.cf[1], rE, ii + 1, jj), muE = at(mu, rE, ii + 1, jj);
+    const double ad = at(G.cf[2], rW, ii - 1, jj), muW = at(mu, rW, ii - 1, jj);
+    o = rE ? o - ai * muE : o;
+    o = rW ? o - ad * muW : o;
+    AT(G.out, ii, jj) = o + ta + tant_old;
   }
 }
 
