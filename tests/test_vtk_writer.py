@@ -1,9 +1,11 @@
-"""The Fortran host's VTK writer (postproceso_vtk, after upstream's
-mod_postproceso.f90:225-300 and the node velocities of main.f90:1295-1313)
-against the VTK files upstream itself wrote: the final fields of each
-golden case, written by the host's writer, give upstream's file byte for
-byte (header, FP32 big-endian points, pressure, temperature, velocities).
-Runs on the CPU: the writer is host code and the driver makes no GPU call."""
+"""The Fortran host's writers against the files upstream itself wrote: the
+VTK sample (postproceso_vtk, after upstream's mod_postproceso.f90:225-300
+and the node velocities of main.f90:1295-1313) and the restart files
+(postproceso_dat, postproceso_bin).  The final fields of each golden case,
+written by the host's writers, give upstream's files byte for byte (VTK:
+header, FP32 big-endian points, pressure, temperature, velocities; .dat:
+the list-directed header and the D23.15 records).  Runs on the CPU: the
+writers are host code and the driver makes no GPU call."""
 import os
 import subprocess
 
@@ -53,6 +55,60 @@ def test_vtk_matches_upstream_bytes(driver, name, tmp_path):
     ref = gold["vtk"].tobytes()
     assert len(data) == len(ref)
     assert data == ref
+
+
+def write_dat(driver, name, fields, header, tmp_path):
+    """out{u,v,p}.dat and out{u,v,p}.bin from the host's writers"""
+    c = K.CASES[name]()
+    for f in "uvtp":
+        np.ascontiguousarray(fields[f].T, dtype="<f8").tofile(os.path.join(tmp_path, f + ".raw"))
+    names = C.write_mesh_files(c, str(tmp_path))
+    subprocess.run([driver, *names, header], check=True, cwd=str(tmp_path), stdout=subprocess.PIPE,
+                   stderr=subprocess.STDOUT)
+    out = {}
+    for f in "uvp":
+        for ext in ("dat", "bin"):
+            with open(os.path.join(tmp_path, f"out{f}.{ext}"), "rb") as fh:
+                out[f + ext] = fh.read()
+    return out, c
+
+
+@pytest.mark.parametrize("name", ["nc18x14", "mixed22x17", "ctrl18x14", "lid66", "nc34"])
+def test_restart_files_match_upstream_bytes(driver, name, tmp_path):
+    """postproceso_dat (the D23.15 records formatted in C) and
+    postproceso_bin on the reference's final fields give the reference's
+    out_*.dat bytes (mod_postproceso.f90 postproceso_dat, main.f90:1329-1354)
+    and .bin records (x, y, u / x, y, v / T, p per node, FP64)."""
+    gold = K.load_fixture(name)
+    head = gold["dat_u"].tobytes().split(b"\n", 1)[0].decode()
+    out, c = write_dat(driver, name, {f: gold[f] for f in "uvtp"}, head, tmp_path)
+    for f in "uvp":
+        assert out[f + "dat"] == gold["dat_" + f].tobytes(), f + ".dat"
+    mi, nj = c.mesh.mi, c.mesh.nj
+    ub = np.frombuffer(out["ubin"], "<f8").reshape(nj + 1, mi, 3)
+    vb = np.frombuffer(out["vbin"], "<f8").reshape(nj, mi + 1, 3)
+    pb = np.frombuffer(out["pbin"], "<f8").reshape(nj + 1, mi + 1, 2)
+    assert np.array_equal(ub[:, :, 2].T, gold["u"]) and np.array_equal(vb[:, :, 2].T, gold["v"])
+    assert np.array_equal(pb[:, :, 0].T, gold["t"]) and np.array_equal(pb[:, :, 1].T, gold["p"])
+
+
+def test_restart_file_with_a_nonfinite_value(driver, tmp_path):
+    """A field holding a NaN goes through Fortran's formatted write instead:
+    every other record as before, the NaN record as flang writes it."""
+    name = "nc18x14"
+    gold = K.load_fixture(name)
+    head = gold["dat_u"].tobytes().split(b"\n", 1)[0].decode()
+    fields = {f: gold[f].copy() for f in "uvtp"}
+    fields["u"][3, 4] = np.nan
+    out, c = write_dat(driver, name, fields, head, tmp_path)
+    ref = gold["dat_u"].tobytes().split(b"\n")
+    got = out["udat"].split(b"\n")
+    assert len(got) == len(ref)
+    k = 1 + 4 * c.mesh.mi + 3  # header, then records by rows of mi
+    bad = [i for i, (a, b) in enumerate(zip(got, ref)) if a != b]
+    assert bad == [k]
+    assert got[k][:46] == ref[k][:46] and got[k][46:].strip() == b"NaN" and len(got[k]) == 69
+    assert out["vdat"] == gold["dat_v"].tobytes() and out["pdat"] == gold["dat_p"].tobytes()
 
 
 def test_vtk_layout_and_node_velocities(driver, tmp_path):
