@@ -1,8 +1,11 @@
 /*
- * Records of Fortran's D23.15 edit descriptor, formatted without the Fortran
- * runtime's per-item I/O: the host's out_*.dat writer (postproceso_dat, after
- * upstream's mod_postproceso.f90 postproceso_dat with its "(3D23.15)" and
- * "(4D23.15)" records) hands whole arrays of records to d23_records.
+ * Records of Fortran's D23.15 edit descriptor, formatted and read without
+ * the Fortran runtime's per-item I/O: the host's out_*.dat writer
+ * (postproceso_dat, after upstream's mod_postproceso.f90 postproceso_dat
+ * with its "(3D23.15)" and "(4D23.15)" records) hands whole arrays of records
+ * to d23_records / d23_append, and its reader of the mesh and restart files
+ * (lectura_mallas_escalonadas, after upstream's mod_malla.f90:137-162) takes
+ * them from d23_read (reading: below).
  *
  * A value is written as flang writes it: 23 characters, right-justified;
  * a minus sign when the sign bit is set (-0.0 included); "0." and the 15
@@ -19,6 +22,7 @@
  */
 #include <math.h>
 #include <pthread.h>
+#include <stddef.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -234,4 +238,186 @@ int d23_append(const char *path, int32_t pathlen, const double *vals, int64_t nr
   free(buf);
   free(name);
   return status;
+}
+
+/* ------------------------------------------------------------------ reading
+ * The host's mesh and restart files (upstream's mod_malla.f90:137-162 reads
+ * them with "(3D23.15)" / "(4D23.15)" after a list-directed header line)
+ * parsed without the Fortran runtime's per-item input.  A field is read as
+ * Fortran's D23.15 edit descriptor reads it when it holds an optional sign,
+ * digits with a decimal point, and an optional exponent (D, d, E or e and
+ * an optional sign, or a sign alone), with blanks only around it; the value
+ * is the correctly rounded one (strtod).  Anything else -- a blank field,
+ * blanks inside, no decimal point (where Fortran would scale by 10^-15), a
+ * short line, too few lines, a value out of range -- makes the call return
+ * 1, and the caller reads the file itself. */
+
+static char *read_file(const char *path, int32_t pathlen, size_t *len) {
+  char name[4096];
+  if (pathlen < 0 || (size_t)pathlen >= sizeof name) return NULL;
+  memcpy(name, path, (size_t)pathlen);
+  name[pathlen] = 0;
+  FILE *f = fopen(name, "rb");
+  if (!f) return NULL;
+  char *buf = NULL;
+  size_t cap = 0, n = 0;
+  for (;;) {
+    if (n == cap) {
+      cap = cap ? 2 * cap : (size_t)1 << 20;
+      char *nb = (char *)realloc(buf, cap + 1);
+      if (!nb) {
+        free(buf);
+        fclose(f);
+        return NULL;
+      }
+      buf = nb;
+    }
+    const size_t got = fread(buf + n, 1, cap - n, f);
+    n += got;
+    if (got == 0) break;
+  }
+  fclose(f);
+  buf[n] = 0;
+  *len = n;
+  return buf;
+}
+
+/* one 23-character field; 0 on success */
+static int parse_d23(const char *f, double *out) {
+  int i = 0, end = 23;
+  while (i < end && f[i] == ' ') ++i;
+  while (end > i && f[end - 1] == ' ') --end;
+  if (i == end) return 1;
+  char b[32];
+  int k = 0, dot = 0, digit = 0, expo = 0;
+  for (; i < end; ++i) {
+    const char c = f[i];
+    if (c >= '0' && c <= '9') {
+      digit = 1;
+      b[k++] = c;
+    } else if (c == '.') {
+      if (dot || expo) return 1;
+      dot = 1;
+      b[k++] = c;
+    } else if (c == 'D' || c == 'd' || c == 'E' || c == 'e') {
+      if (expo || !digit) return 1;
+      expo = 1;
+      b[k++] = 'e';
+    } else if (c == '+' || c == '-') {
+      if (k == 0 || b[k - 1] == 'e') {
+        b[k++] = c;
+      } else if (!expo && digit) {
+        expo = 1;  /* an exponent given by its sign alone */
+        b[k++] = 'e';
+        b[k++] = c;
+      } else {
+        return 1;
+      }
+    } else {
+      return 1;
+    }
+    if (k > 28) return 1;
+  }
+  if (!dot || !digit) return 1;
+  b[k] = 0;
+  char *e;
+  const double v = strtod(b, &e);
+  if (*e || !isfinite(v)) return 1;
+  *out = v;
+  return 0;
+}
+
+typedef struct {
+  const char *const *line;
+  int64_t r0, r1;
+  int32_t ncol;
+  double *vals;
+  int status;
+} ParseChunk;
+
+static void *parse_chunk(void *arg) {
+  ParseChunk *c = (ParseChunk *)arg;
+  c->status = 0;
+  for (int64_t r = c->r0; r < c->r1 && !c->status; ++r)
+    for (int32_t k = 0; k < c->ncol && !c->status; ++k)
+      c->status = parse_d23(c->line[r] + 23 * k, &c->vals[r * c->ncol + k]);
+  return NULL;
+}
+
+/* the nrec records of ncol values after the header line of the file named
+ * by path (pathlen characters, not terminated) into vals, record after
+ * record, parsed on up to 8 threads.  Returns 0, or 1 if the file is not
+ * of the form described above (vals is then incomplete). */
+int d23_read(const char *path, int32_t pathlen, int64_t nrec, int32_t ncol, double *vals) {
+  size_t len = 0;
+  char *buf = read_file(path, pathlen, &len);
+  if (!buf) return 1;
+  const char **line = (const char **)malloc((size_t)(nrec > 0 ? nrec : 1) * sizeof *line);
+  int status = line ? 0 : 1;
+  char *p = buf, *endb = buf + len;
+  char *nl = status ? NULL : memchr(p, '\n', (size_t)(endb - p)); /* end of the header */
+  if (!nl) status = 1;
+  p = nl ? nl + 1 : endb;
+  for (int64_t r = 0; r < nrec && !status; ++r) {
+    char *q = memchr(p, '\n', (size_t)(endb - p));
+    char *e = q ? q : endb;
+    if (e - p < 23 * (ptrdiff_t)ncol) {
+      status = 1;
+      break;
+    }
+    line[r] = p;
+    p = q ? q + 1 : endb;
+  }
+  if (!status) {
+    const long ncpu = sysconf(_SC_NPROCESSORS_ONLN);
+    int64_t nt = nrec / MIN_RECORDS;
+    if (nt > MAX_THREADS) nt = MAX_THREADS;
+    if (nt > ncpu) nt = ncpu;
+    if (nt < 1) nt = 1;
+    ParseChunk ch[MAX_THREADS];
+    pthread_t th[MAX_THREADS];
+    int started[MAX_THREADS] = {0};
+    for (int64_t t = 0; t < nt; ++t) {
+      ch[t] = (ParseChunk){line, nrec * t / nt, nrec * (t + 1) / nt, ncol, vals, 0};
+      if (t > 0) started[t] = pthread_create(&th[t], NULL, parse_chunk, &ch[t]) == 0;
+    }
+    parse_chunk(&ch[0]);
+    status = ch[0].status;
+    for (int64_t t = 1; t < nt; ++t) {
+      if (started[t])
+        pthread_join(th[t], NULL);
+      else
+        parse_chunk(&ch[t]);
+      status |= ch[t].status;
+    }
+  }
+  free(line);
+  free(buf);
+  return status;
+}
+
+/* the lines after the first of the file named by path that hold a
+ * character other than a blank, into *n (the record count of a mesh file);
+ * 0 on success, 1 if the file cannot be read */
+int d23_count_lines(const char *path, int32_t pathlen, int64_t *n) {
+  size_t len = 0;
+  char *buf = read_file(path, pathlen, &len);
+  if (!buf) return 1;
+  int64_t count = 0;
+  const char *p = memchr(buf, '\n', len);
+  p = p ? p + 1 : buf + len;
+  const char *endb = buf + len;
+  while (p < endb) {
+    const char *q = memchr(p, '\n', (size_t)(endb - p));
+    const char *e = q ? q : endb;
+    for (const char *c = p; c < e; ++c)
+      if (*c != ' ') {
+        ++count;
+        break;
+      }
+    p = q ? q + 1 : endb;
+  }
+  free(buf);
+  *n = count;
+  return 0;
 }

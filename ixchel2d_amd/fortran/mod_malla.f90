@@ -16,13 +16,38 @@ module malla
   character(len=20) :: mic, njc
   real(DBL), allocatable, target :: xu(:), yv(:), xp(:), yp(:)
 
+  interface
+     ! d_format.c: the records after the header line read as (ncolD23.15);
+     ! 1 when the file needs Fortran's own formatted read
+     integer(c_int) function d23_read(path, pathlen, nrec, ncol, vals) bind(C, name='d23_read')
+       import :: c_char, c_int32_t, c_int64_t, c_double, c_int
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: pathlen
+       integer(c_int64_t), value :: nrec
+       integer(c_int32_t), value :: ncol
+       real(c_double), intent(out) :: vals(*)
+     end function d23_read
+     ! d_format.c: the non-blank lines after the first
+     integer(c_int) function d23_count_lines(path, pathlen, n) bind(C, name='d23_count_lines')
+       import :: c_char, c_int32_t, c_int64_t, c_int
+       character(kind=c_char), intent(in) :: path(*)
+       integer(c_int32_t), value :: pathlen
+       integer(c_int64_t), intent(out) :: n
+     end function d23_count_lines
+  end interface
+
 contains
 
   integer function cuenta_lineas(archivo) result(n)
     character(len=*), intent(in) :: archivo
     integer :: ios, u
+    integer(c_int64_t) :: nc
     character(len=256) :: linea
     n = 0
+    if (d23_count_lines(trim(archivo), int(len_trim(archivo), c_int32_t), nc) == 0) then
+       n = int(nc)
+       return
+    end if
     open(newunit=u, file=trim(archivo), status='old', action='read')
     read(u, '(A)', iostat=ios) linea          ! header
     do
@@ -46,39 +71,71 @@ contains
   end subroutine dimensiones_malla
 
   ! lectura_mallas_escalonadas (source/mod_malla.f90:62-162): coordinates and
-  ! the initial u, v, T, p; the geometry factors are built by the library
+  ! the initial u, v, T, p; the geometry factors are built by the library.
+  ! Each file's header is read here; its records by d23_read (d_format.c),
+  ! or by Fortran's formatted read when d23_read declines the file.
   subroutine lectura_mallas_escalonadas(entrada_u, entrada_v, entrada_tp, u_ant, v_ant, pres, temp_ant, &
        ao, placa_min, placa_max, itera_inicial)
     character(len=*), intent(in) :: entrada_u, entrada_v, entrada_tp
     real(DBL), intent(out) :: u_ant(:,:), v_ant(:,:), pres(:,:), temp_ant(:,:)
     real(DBL), intent(out) :: ao
     integer, intent(out) :: placa_min, placa_max, itera_inicial
-    integer :: ii, jj, un
+    real(DBL), allocatable :: rec(:, :)
+    integer :: ii, jj, k
     if (.not. allocated(xu)) allocate(xu(mi), yv(nj), xp(mi+1), yp(nj+1))
-    open(newunit=un, file=trim(entrada_u), status='old', action='read')
-    read(un, *) placa_min, placa_max, itera_inicial, ao
+    call registros(entrada_u, 3, mi * (nj + 1))
+    k = 0
     do jj = 1, nj + 1
        do ii = 1, mi
-          read(un, form24) xu(ii), yp(jj), u_ant(ii, jj)
+          k = k + 1
+          xu(ii) = rec(1, k)
+          yp(jj) = rec(2, k)
+          u_ant(ii, jj) = rec(3, k)
        end do
     end do
-    close(un)
-    open(newunit=un, file=trim(entrada_v), status='old', action='read')
-    read(un, *) placa_min, placa_max, itera_inicial, ao
+    call registros(entrada_v, 3, (mi + 1) * nj)
+    k = 0
     do jj = 1, nj
        do ii = 1, mi + 1
-          read(un, form24) xp(ii), yv(jj), v_ant(ii, jj)
+          k = k + 1
+          xp(ii) = rec(1, k)
+          yv(jj) = rec(2, k)
+          v_ant(ii, jj) = rec(3, k)
        end do
     end do
-    close(un)
-    open(newunit=un, file=trim(entrada_tp), status='old', action='read')
-    read(un, *) placa_min, placa_max, itera_inicial, ao
+    call registros(entrada_tp, 4, (mi + 1) * (nj + 1))
+    k = 0
     do jj = 1, nj + 1
        do ii = 1, mi + 1
-          read(un, form25) xp(ii), yp(jj), temp_ant(ii, jj), pres(ii, jj)
+          k = k + 1
+          xp(ii) = rec(1, k)
+          yp(jj) = rec(2, k)
+          temp_ant(ii, jj) = rec(3, k)
+          pres(ii, jj) = rec(4, k)
        end do
     end do
-    close(un)
+  contains
+    ! the header and the nrec records of ncol values of one file into rec
+    subroutine registros(archivo, ncol, nrec)
+      character(len=*), intent(in) :: archivo
+      integer, intent(in) :: ncol, nrec
+      integer :: un, r
+      if (allocated(rec)) deallocate(rec)
+      allocate(rec(ncol, nrec))
+      open(newunit=un, file=trim(archivo), status='old', action='read')
+      read(un, *) placa_min, placa_max, itera_inicial, ao
+      if (d23_read(trim(archivo), int(len_trim(archivo), c_int32_t), int(nrec, c_int64_t), &
+           int(ncol, c_int32_t), rec) /= 0) then
+         do r = 1, nrec
+            if (ncol == 3) then
+               read(un, form24) rec(:, r)
+            else
+               read(un, form25) rec(:, r)
+            end if
+         end do
+      end if
+      close(un)
+    end subroutine registros
   end subroutine lectura_mallas_escalonadas
 
 end module malla
