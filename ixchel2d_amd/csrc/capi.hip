@@ -1309,8 +1309,10 @@ int ixh_set_sources(ixh_state *s, int32_t var, const double *sc, const double *s
   const int fid = var == 0 ? IXH_U : var == 1 ? IXH_V : IXH_T;
   int m, n;
   field_extent(s, fid, &m, &n);
-  HIPCHK(hipMemcpy2D(s->src_c[var], (size_t)s->P * 8, sc, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy2D(s->src_p[var], (size_t)s->P * 8, sp, (size_t)m * 8, (size_t)m * 8, n, hipMemcpyHostToDevice));
+  // whole pitched arrays in one copy each (a 2-D copy from pageable memory
+  // goes row by row)
+  CHK(upload_as<double>(s, s->src_c[var], sc, m, m, n, false));
+  CHK(upload_as<double>(s, s->src_p[var], sp, m, m, n, false));
   CHK(upload_as<double>(s, s->src_cT[var], sc, m, m, n, true));
   CHK(upload_as<double>(s, s->src_pT[var], sp, m, m, n, true));
   s->has_src[var] = true;
