@@ -106,15 +106,16 @@ def test_run_against_upstream_fixture(name, solver):
     np.testing.assert_allclose(hist, gold["history"], rtol=1e-9, atol=1e-15)
 
 
-@pytest.mark.parametrize("mi,nj", [(20, 900), (20, 1100), (1100, 20), (700, 30), (1800, 12), (12, 1800)])
+@pytest.mark.parametrize("mi,nj", [(20, 900), (20, 1100), (1100, 20), (700, 30), (1800, 12), (12, 1800),
+                                   (2047, 12), (12, 2047)])
 def test_long_lines_against_oracle(mi, nj):
     """Line lengths the 301x701 run does not reach, each through its own
     launch shapes: 901-node y-lines (4 rows per thread), 1101-node y- and
     x-lines (256 threads, the LDS PCR solver for lines past 1024 nodes),
     701-node x-lines (the T/u x launch with 256 threads), 1801-node x- and
     y-lines (8 rows per thread; the p' x-sweep with the correction then
-    holds two lines in 135 KB of LDS).  30 fused iterations against the
-    oracle."""
+    holds two lines in 135 KB of LDS), and 2048-node x- and y-lines, the
+    longest the library builds.  30 fused iterations against the oracle."""
     case = C.natural_convection_case(mi, nj, height=nj / mi, Ra=1e4, iters=30, dt=0.05, seed=5)
     case.params.paq_itera = 10
     g = _cavity(case, "pcr")
@@ -123,6 +124,16 @@ def test_long_lines_against_oracle(mi, nj):
     o.run(30)
     for f in "uvtp":
         assert relerr(g.get(f), o.get(f)) <= REL, f"{f}: rel err {relerr(g.get(f), o.get(f)):.2e}"
+
+
+@pytest.mark.parametrize("mi,nj", [(2048, 12), (12, 2048)])
+def test_lines_past_the_longest_are_rejected(mi, nj):
+    """A mesh whose lines would have 2049 nodes is refused at creation with
+    the library's error, not run."""
+    from ixchel2d_amd import _lib
+    case = C.natural_convection_case(mi, nj, height=nj / mi, Ra=1e4, iters=1, dt=0.05, seed=5)
+    with pytest.raises(_lib.IxhError, match="lines longer than 2048"):
+        _cavity(case, "pcr")
 
 
 @pytest.mark.parametrize("precision,tol", [("fp32_mixed", 5e-5), ("fp32_refined", 5e-6)])
