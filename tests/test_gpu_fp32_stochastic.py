@@ -182,11 +182,11 @@ def test_fused_step_equals_kernel_by_kernel(name, precision):
 
 
 @pytest.mark.parametrize("precision", ["fp32_mixed", "fp32_stochastic"])
-@pytest.mark.parametrize("mi,nj", [(20, 1100), (1100, 20), (700, 30)])
+@pytest.mark.parametrize("mi,nj", [(20, 1100), (1100, 20), (700, 30), (2047, 12), (12, 2047)])
 def test_long_lines(precision, mi, nj):
-    """Lines past 1024 nodes (the LDS PCR solver in FP32) and 701-node
-    x-lines (the T and u x-sweeps at 256 threads): 30 fused iterations
-    against the FP64 path."""
+    """Lines past 1024 nodes (the LDS PCR solver in FP32) up to the longest
+    built (2048 nodes) and 701-node x-lines (the T and u x-sweeps at 256
+    threads): 30 fused iterations against the FP64 path."""
     case = C.natural_convection_case(mi, nj, height=nj / mi, Ra=1e4, iters=30, dt=0.05, seed=5)
     g = _cavity(case, precision)
     g.run(30)

@@ -137,7 +137,7 @@ def test_lines_past_the_longest_are_rejected(mi, nj):
 
 
 @pytest.mark.parametrize("precision,tol", [("fp32_mixed", 5e-5), ("fp32_refined", 5e-6)])
-@pytest.mark.parametrize("mi,nj", [(20, 900), (20, 1100), (1100, 20), (700, 30)])
+@pytest.mark.parametrize("mi,nj", [(20, 900), (20, 1100), (1100, 20), (700, 30), (2047, 12), (12, 2047)])
 def test_long_lines_fp32_paths(mi, nj, precision, tol):
     """The FP32 paths through the same long-line launch shapes (FP32 wave
     solver up to 1024 nodes, FP32 LDS PCR beyond), 30 fused iterations
