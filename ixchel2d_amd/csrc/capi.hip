@@ -963,7 +963,7 @@ int tri_scratch(double **out) {
 extern "C" {
 
 const char *ixh_last_error(void) { return g_err.c_str(); }
-const char *ixh_version(void) { return "ixchel2d-amd 0.2 (gfx950, fp64 and fp32-mixed)"; }
+const char *ixh_version(void) { return "ixchel2d-amd 0.3 (gfx950; FP64, FP32-mixed, FP32-refined, FP32-stochastic)"; }
 const char *ixh_kernel_name(int32_t id) { return (id >= 0 && id < K_NUM) ? kKernelNames[id] : ""; }
 
 int ixh_create(const ixh_grid *grid, const ixh_params *params, ixh_state **out) {

@@ -48,3 +48,23 @@ def test_errors_come_back_as_codes_without_a_gpu():
     # NULL grid: rejected before any device work
     assert L.ixh_create(None, None, ctypes.byref(h)) == -1
     assert b"NULL" in L.ixh_last_error()
+
+
+@pytest.mark.parametrize("mi,nj", [(3, 10), (10, 3)])
+def test_meshes_below_the_smallest_are_rejected(mi, nj):
+    """ixh_create refuses mi or nj below 4 with its error code and message
+    before it touches the GPU (the grid is handed over directly: the Python
+    mesh builder refuses such meshes itself)."""
+    import numpy as np
+    from ixchel2d_amd import case as C
+    from ixchel2d_amd.simple import params_of
+    case = C.natural_convection_case(10, 10, height=1.0, Ra=1e4, iters=1, dt=0.05, seed=5)
+    m = case.mesh
+    geo = [np.ascontiguousarray(x, dtype=np.float64) for x in (m.xu, m.yv, m.xp, m.yp)]
+    grid = _lib.Grid(mi, nj, *[x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) for x in geo])
+    prm = params_of(case, "pcr", "fp64")
+    h = ctypes.c_void_p()
+    L = _lib.load()
+    assert L.ixh_create(ctypes.byref(grid), ctypes.byref(prm), ctypes.byref(h)) == -1  # IXH_ERR_ARG
+    assert not h.value
+    assert "mesh needs mi, nj >= 4" in L.ixh_last_error().decode()

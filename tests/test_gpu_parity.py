@@ -126,6 +126,25 @@ def test_long_lines_against_oracle(mi, nj):
         assert relerr(g.get(f), o.get(f)) <= REL, f"{f}: rel err {relerr(g.get(f), o.get(f)):.2e}"
 
 
+@pytest.mark.parametrize("mi,nj", [(4, 4), (4, 30), (30, 4), (5, 7)])
+@pytest.mark.parametrize("solver", ["thomas", "pcr"])
+def test_smallest_meshes_against_oracle(mi, nj, solver):
+    """The smallest meshes the library takes (mi, nj >= 4: u x-lines of 4
+    nodes, v y-lines of 4): 30 iterations of the fused step against the
+    oracle, bit for bit with upstream's Thomas order."""
+    case = C.natural_convection_case(mi, nj, height=nj / mi, Ra=1e4, iters=30, dt=0.05, seed=5)
+    case.params.paq_itera = 10
+    g = _cavity(case, solver)
+    g.run(30)
+    o = Oracle(case)
+    o.run(30)
+    for f in "uvtp":
+        if solver == "thomas":
+            assert np.array_equal(g.get(f), o.get(f)), f
+        else:
+            assert relerr(g.get(f), o.get(f)) <= REL, f"{f}: rel err {relerr(g.get(f), o.get(f)):.2e}"
+
+
 @pytest.mark.parametrize("mi,nj", [(2048, 12), (12, 2048)])
 def test_lines_past_the_longest_are_rejected(mi, nj):
     """A mesh whose lines would have 2049 nodes is refused at creation with
