@@ -1411,9 +1411,12 @@ int ixh_simple_iterations(ixh_state *s, int32_t nsteps, ixh_hist *hist, int32_t 
       CHK(run_fused(s, nsteps));
     }
     const int keep = nsteps < cap ? nsteps : cap;
-    std::vector<double> h((size_t)8 * (keep > 0 ? keep : 1));
+    // the records through the pinned staging buffer (a small copy to
+    // pageable memory is staged by the runtime at extra cost)
+    CHK(stage_bytes(s, (size_t)8 * (keep > 0 ? keep : 1) * sizeof(double)));
+    const double *h = static_cast<const double *>(s->h_stage);
     if (keep > 0)
-      HIPCHK(hipMemcpyAsync(h.data(), s->d_hist, (size_t)8 * keep * sizeof(double), hipMemcpyDeviceToHost,
+      HIPCHK(hipMemcpyAsync(s->h_stage, s->d_hist, (size_t)8 * keep * sizeof(double), hipMemcpyDeviceToHost,
                             s->stream));
     HIPCHK(hipStreamSynchronize(s->stream));
     for (int k = 0; k < keep; ++k) {
