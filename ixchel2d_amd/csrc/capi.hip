@@ -1231,9 +1231,12 @@ int ixh_upload_field(ixh_state *s, int32_t f, const double *src, int32_t ld) {
   return 0;
 }
 
-// a pinned staging buffer of at least `bytes` for downloads
+// a pinned staging buffer of at least `bytes` for downloads and history
+// records; never smaller than one field, so that the short copies of a run's
+// records do not reallocate it (a pinned allocation takes hundreds of us)
 static int stage_bytes(ixh_state *s, size_t bytes) {
   if (bytes <= s->h_stage_bytes) return 0;
+  bytes = std::max(bytes, s->fsize * sizeof(double));
   if (s->h_stage) HIPCHK(hipHostFree(s->h_stage));
   s->h_stage = nullptr;
   s->h_stage_bytes = 0;
