@@ -515,12 +515,20 @@ __global__ void __launch_bounds__(kBlock) k_zero_many(CopyList L) {
     gp(dst)[i] = 0.0;
 }
 
-// dst[c] += src[c] for every listed pair c, in one launch
-__global__ void __launch_bounds__(kBlock) k_add_many(CopyList L) {
-  const double *src = L.src[blockIdx.y];
+// dst[c] += src[c], then src[c] = 0, for every listed pair c, and dst[c] = 0
+// where src[c] is null, in one launch (the sources are the caller's work
+// arrays)
+__global__ void __launch_bounds__(kBlock) k_add_clear_many(CopyList L) {
+  double *src = const_cast<double *>(L.src[blockIdx.y]);
   double *dst = L.dst[blockIdx.y];
-  for (int i = (int)(blockIdx.x * kBlock + threadIdx.x); i < L.n; i += (int)(gridDim.x * kBlock))
-    gp(dst)[i] = gp(dst)[i] + gp(src)[i];
+  for (int i = (int)(blockIdx.x * kBlock + threadIdx.x); i < L.n; i += (int)(gridDim.x * kBlock)) {
+    if (src) {
+      gp(dst)[i] = gp(dst)[i] + gp(src)[i];
+      gp(src)[i] = 0.0;
+    } else {
+      gp(dst)[i] = 0.0;
+    }
+  }
 }
 
 

@@ -2129,10 +2129,10 @@ int ixh_batch_autotune(ixh_batch *b, int32_t trial_steps, int32_t *order, int32_
 // steps from its checkpoint.
 namespace {
 
-enum ManyOp { MANY_COPY, MANY_ZERO, MANY_ADD };
+enum ManyOp { MANY_COPY, MANY_ZERO, MANY_ADD_CLEAR };
 
-// one launch over whole buffers: dst = src, dst = 0 or dst += src for every
-// listed (dst, src) pair
+// one launch over whole buffers: dst = src, dst = 0, or dst += src then
+// src = 0 (dst = 0 where src is null) for every listed (dst, src) pair
 int many(ixh_state *s, ManyOp op, std::initializer_list<std::pair<double *, const double *>> pairs) {
   CopyList L;
   memset(&L, 0, sizeof L);
@@ -2150,7 +2150,7 @@ int many(ixh_state *s, ManyOp op, std::initializer_list<std::pair<double *, cons
   else if (op == MANY_ZERO)
     hipLaunchKernelGGL(k_zero_many, grid, dim3(kBlock), 0, s->stream, L);
   else
-    hipLaunchKernelGGL(k_add_many, grid, dim3(kBlock), 0, s->stream, L);
+    hipLaunchKernelGGL(k_add_clear_many, grid, dim3(kBlock), 0, s->stream, L);
   HIPCHK(hipGetLastError());
   return 0;
 }
@@ -2451,10 +2451,10 @@ Adj2Args adj2_args(ixh_state *s, int var, double *const *W) {
 
 // adjoint of one step: (ubar, vbar, Tbar, pbar) of its end, held in
 // W_UB3, W_VB3, W_TB2, W_PB, become those of its start in W_UB0, W_VB0,
-// W_TB0, W_PB; boundary adjoints of T accumulate in adj_g
+// W_TB0, W_PB; boundary adjoints of T accumulate in adj_g.  The
+// accumulators W_AUB, W_AVB, W_UANTB, W_VANTB, W_TANTB are zero on entry
+// and are left zero for the next step.
 int adj2_step(ixh_state *s, double *const *tp, double *const *W) {
-  CHK(many(s, MANY_ZERO, {{W[W_AUB], nullptr}, {W[W_AVB], nullptr}, {W[W_UANTB], nullptr},
-                          {W[W_VANTB], nullptr}, {W[W_TANTB], nullptr}}));
   const int mi = s->mi, nj = s->nj;
   // T x-sweep, then T y-sweep
   {
@@ -2512,8 +2512,10 @@ int adj2_step(ixh_state *s, double *const *tp, double *const *W) {
     G.x = tp[TP_U1]; G.phi = tp[TP_U0]; G.ybar = W[W_UB1]; G.acb = nullptr; G.phib = W[W_UB0];
     CHK((adj2_sweep<EQ_U, DIR_Y>(s, G, mi - 2, nj + 1)));
   }
-  // the lagged copies are the fields of the step's start
-  CHK(many(s, MANY_ADD, {{W[W_UB0], W[W_UANTB]}, {W[W_VB0], W[W_VANTB]}, {W[W_TB0], W[W_TANTB]}}));
+  // the lagged copies are the fields of the step's start; the accumulators
+  // cleared for the next step in the same launch
+  CHK(many(s, MANY_ADD_CLEAR, {{W[W_UB0], W[W_UANTB]}, {W[W_VB0], W[W_VANTB]}, {W[W_TB0], W[W_TANTB]},
+                               {W[W_AUB], nullptr}, {W[W_AVB], nullptr}}));
   return 0;
 }
 
@@ -2545,7 +2547,8 @@ int ixh_adjoint_coupled(ixh_state *s, const double *tbar, int32_t ld, double *gr
   }
   std::vector<double *> W(s->adj2_w.begin(), s->adj2_w.begin() + W_N);
   const size_t fb = s->fsize * sizeof(double);
-  for (int k : {W_UB3, W_VB3, W_PB}) HIPCHK(hipMemsetAsync(W[k], 0, fb, s->stream));
+  for (int k : {W_UB3, W_VB3, W_PB, W_AUB, W_AVB, W_UANTB, W_VANTB, W_TANTB})
+    HIPCHK(hipMemsetAsync(W[k], 0, fb, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
   CHK(upload_as<double>(s, W[W_TB2], tbar, ld, s->mi + 1, s->nj + 1, false));
   const int nseg = (K + C - 1) / C;
