@@ -2374,14 +2374,14 @@ int dcopy(ixh_state *s, double *dst, const double *src) {
 
 // one time step in the order of fast_step, every produced field copied to
 // the tape (the residual and the norms do not change fields and are
-// skipped).  The copies go out in three launches, each before any of its
+// skipped).  The copies go out in two launches, each before any of its
 // buffers can be overwritten: the sweeps write buffers other than the ones
 // they read (u0 is held out of the rotation for the u sweeps: after a
 // checkpoint is loaded u_ant is another buffer, and the u x-sweep could
 // otherwise pick u0's), so u0, v0 (the v snapshot), T0, u1, u2, v1, v2,
 // a_u, a_v and p0 (which the correction updates in place) are intact after
-// the v x-sweep; p'1 and p'2 after the p' x-sweep and correction; u3, v3
-// and T1, T2 after the T x-sweep.
+// the v x-sweep; p'1, p'2 (which the T sweeps do not touch), u3, v3 and
+// T1, T2 after the T x-sweep.
 int tape_step(ixh_state *s, double *const *tp) {
   double *u0 = s->ub[s->u_cur], *v0 = s->vb[s->v_cur], *t0 = s->tb[s->t_cur];
   s->u_hold = s->u_cur;
@@ -2406,12 +2406,14 @@ int tape_step(ixh_state *s, double *const *tp) {
   CorrLine corr;
   prep_correct_line(s, corr);
   CHK(launch_pcorr(s, px, corr, line_len(s, EQ_P, IXH_DIR_X)));
-  CHK(copy_many(s, {{tp[TP_PC1], pc1}, {tp[TP_PC2], s->pcb[s->pc_cur]}}));
+  double *pc2 = s->pcb[s->pc_cur];
   double *u3 = s->ub[s->u_cur], *v3 = s->vb[s->v_cur];
   CHK(sweep_t(s, IXH_DIR_Y));
   double *t1 = s->tb[s->t_cur];
   CHK(sweep_t(s, IXH_DIR_X));
-  CHK(copy_many(s, {{tp[TP_U3], u3}, {tp[TP_V3], v3}, {tp[TP_T1], t1}, {tp[TP_T2], s->tb[s->t_cur]}}));
+  // the T sweeps leave the p' buffers alone: p'1 and p'2 go with the rest
+  CHK(copy_many(s, {{tp[TP_PC1], pc1}, {tp[TP_PC2], pc2}, {tp[TP_U3], u3}, {tp[TP_V3], v3}, {tp[TP_T1], t1},
+                    {tp[TP_T2], s->tb[s->t_cur]}}));
   s->u_ant = s->u_cur;
   s->v_ant = s->v_cur;
   s->t_ant = s->t_cur;
